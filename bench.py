@@ -1,0 +1,180 @@
+"""Benchmark of the batched alignment hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--fill interpair|striped]
+
+A step is one score_and_grad() pass over the whole batch, inputs already resident in HBM.
+Prints ONE JSON line: GCUPS (cells = sum of len(A)*len(B), the reference's convention), the
+roofline of the pass's kernels (measured with HIP events on the stream they run on), the same
+pass with plain per-bucket launches for comparison, and the CPU baseline: the reference engine
+(oracle/_ref, when built) or the repository's C restatement, timed on this host's cores over a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic lane-operations per cell (one max / add / sub / mul / FMA = 1), counted from the
+# reference recurrences: affine hard / score 13, linear 5 (local 6), soft affine 32 (forward 11,
+# backward with the fused gradient 21), soft linear 14.
+OPS_PER_CELL = {("affine", "hard"): 13, ("affine", "none"): 13, ("linear", "hard"): 5, ("linear", "none"): 5,
+                ("affine", "soft"): 32, ("linear", "soft"): 14}
+FP32_PEAK_TFLOPS = 157.3        # MI355X published FP32 vector peak (packed FMA = 4 flops / lane / clock)
+VALU_ISSUE_PEAK_TOPS = 39.3     # 256 CUs x 64 lanes x 2.4 GHz non-packed VALU lane-ops
+
+WORKLOADS = ("protein_soft_homologs", "dna_local_short", "dna_local_short_score", "protein_global_hard")
+
+
+def make_workload(name):
+    from nwgrad_amd import workloads as wl
+    if name == "protein_global_hard":
+        return wl.protein_global_hard(), "hard"
+    if name == "dna_local_short":
+        return wl.dna_local_short(), "hard"
+    if name == "dna_local_short_score":
+        return wl.dna_local_short(), "none"
+    if name == "protein_soft_homologs":
+        return wl.protein_soft_homologs(), "soft"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def make_batch(w, grad_mode, fill):
+    import nwgrad_amd as nw
+    p = nw.AlignParams(w.matrix, alphabet=w.alphabet, gap_open_a=w.gaps[0], gap_extend_a=w.gaps[1],
+                       gap_open_b=w.gaps[2], gap_extend_b=w.gaps[3])
+    b = nw.SeqPairBatch.from_codes(w.codes_a, w.off_a, w.codes_b, w.off_b, p, gap_model=w.gap_model, mode=w.mode,
+                                   grad_mode=grad_mode)
+    b.fill = fill
+    return b
+
+
+def time_steps(b, steps, warmup, sync):
+    for _ in range(warmup):
+        b.score_and_grad()
+    sync()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.score_and_grad()
+        kms.append(b.last_kernel_ms)
+    sync()
+    dt = time.perf_counter() - t0
+    return dt, float(np.mean(kms))
+
+
+def cpu_baseline(w, grad_mode, budget_s=8.0):
+    """The reference engine (oracle/_ref) on all host cores, else the C oracle on one: a
+    bounded sample of the workload's pairs, best of 2 timed passes after a warm-up."""
+    import oracle
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    rng = np.random.default_rng(0)
+    if oracle.have_ref():
+        from oracle import ref
+        kind = "reference"
+
+        def run(sub):
+            sa, sb = sub.strings()
+            rb = ref.RefBatch(sa, sb, sub.alphabet, sub.matrix, sub.gaps, gap_model=sub.gap_model, mode=sub.mode,
+                              grad_mode=grad_mode, n_threads=cores)
+            if grad_mode == "soft":
+                rb.set_soft(precision="float32")
+            rb.score_and_grad()  # warm-up
+            best = 1e30
+            for _ in range(2):
+                t0 = time.perf_counter()
+                rb.score_and_grad()
+                best = min(best, time.perf_counter() - t0)
+            return best
+    else:
+        kind = "port"
+        cores = 1
+
+        def run(sub):
+            t0 = time.perf_counter()
+            if grad_mode == "soft":
+                oracle.soft_log(sub.matrix, sub.gaps, sub.codes_a, sub.off_a, sub.codes_b, sub.off_b,
+                                gap_model=sub.gap_model, mode=sub.mode)
+            else:
+                oracle.hard(sub.matrix, sub.gaps, sub.codes_a, sub.off_a, sub.codes_b, sub.off_b,
+                            gap_model=sub.gap_model, mode=sub.mode, with_grad=grad_mode == "hard")
+            return time.perf_counter() - t0
+    n = w.n_pairs
+    k = min(n, 256)
+    idx = np.sort(rng.choice(n, size=k, replace=False))
+    sub = w.subset(idx)
+    t = run(sub)
+    per_cell = t / max(sub.cells, 1)
+    target = int(min(n, max(k, k * (budget_s / 3.0) / max(t, 1e-6))))
+    if target > k:
+        idx = np.sort(rng.choice(n, size=target, replace=False))
+        sub = w.subset(idx)
+        t = run(sub)
+        per_cell = t / max(sub.cells, 1)
+    return {"value": round(1.0 / per_cell / 1e9, 4), "unit": "GCUPS", "cores": cores, "kind": kind,
+            "sample": f"{sub.n_pairs} of {n} pairs ({sub.cells} cells), uniform without replacement, "
+                      f"best of 2 passes after a warm-up"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default=os.environ.get("NWG_BENCH_WORKLOAD", "dna_local_short"),
+                    choices=WORKLOADS)
+    ap.add_argument("--fill", default="interpair", choices=("interpair", "striped"))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(pairs shard across GPUs with one gradient all-reduce per step; see DESIGN.md)", file=sys.stderr)
+        return 2
+    import torch
+    sync = torch.cuda.synchronize
+    print(f"bench: workload {args.workload}", file=sys.stderr, flush=True)
+    w, grad_mode = make_workload(args.workload)
+    b = make_batch(w, grad_mode, args.fill)
+    cells = b.cells
+    print(f"bench: {w.n_pairs} pairs, {cells} cells, {b.tiles} tiles; timing {args.steps} steps", file=sys.stderr,
+          flush=True)
+    dt, kms = time_steps(b, args.steps, args.warmup, sync)
+    gcups = cells * args.steps / dt / 1e9
+    # the other launch mode, same batch shape
+    other = "striped" if args.fill == "interpair" else "interpair"
+    b.fill = other
+    dt2, kms2 = time_steps(b, max(2, args.steps // 2), 1, sync)
+    gcups_other = cells * max(2, args.steps // 2) / dt2 / 1e9
+    b.fill = args.fill
+    ops = OPS_PER_CELL[(w.gap_model, grad_mode)] + (1 if (w.mode == "local" and w.gap_model == "linear") else 0)
+    achieved = ops * cells / (kms * 1e-3) / 1e12
+    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "valu_issue_peak": VALU_ISSUE_PEAK_TOPS, "valu_issue_frac": round(achieved / VALU_ISSUE_PEAK_TOPS, 4),
+            "ops_per_cell": ops, "kernel_ms": round(kms, 4)}
+    cpu = None if args.no_cpu else cpu_baseline(w, grad_mode)
+    dtype = "f32"
+    line = {"metric": "GCUPS soft-grad, batched vs plain GPU fill @ bit-exact/1e-5; p99 latency",
+            "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+            "config": {"workload": args.workload, "pairs": w.n_pairs, "cells": cells, "grad_mode": grad_mode,
+                       "gap_model": w.gap_model, "mode": w.mode, "fill": args.fill},
+            "kernel_gcups": round(cells / (kms * 1e-3) / 1e9, 3),
+            "plain_vs_batched": {"fill": other, "gcups": round(gcups_other, 3),
+                                 "kernel_gcups": round(cells / (kms2 * 1e-3) / 1e9, 3),
+                                 "batched_over_plain": round(gcups / gcups_other, 3)},
+            "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

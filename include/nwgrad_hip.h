@@ -1,0 +1,118 @@
+/* nwgrad_hip.h -- the C ABI of the MI355X (gfx950) alignment hot path.
+ *
+ * A plain-C boundary (no HIP or torch types in any signature): device-resident batches of
+ * encoded sequence pairs, the hard / score-only / soft passes over them, and the
+ * reductions of their per-pair gradients.  The header-only C++ engine
+ * (nwgrad_amd/cpp/nwgrad_amd/) and the Python module sit above it; nothing below it runs
+ * on the CPU except argument checks and the batch planner.
+ *
+ * Reference interface replaced (paths relative to the reference's src/nwgrad/cpp/nwgrad/):
+ * the reference's kernels are noexcept functions taking one plain-data job struct each,
+ * called through LevelKernels function pointers (simd_levels.hpp:1291-1393): ViterbiJob
+ * (:188-219), ScoreJob (:253-267), InterJobT (:497-530), InterSoftJobT (:542-562),
+ * SoftPairJobT (:581-627).  Their callers are BatchEngine's group and pair runners
+ * (batch_engine.hpp:2446-2456, 997-1012; aligner.hpp:3427-3465, 6946-6956).  This ABI
+ * moves the boundary up one level, to the batch, so that pairs and results stay in HBM
+ * between calls: each entry below names the BatchEngine / SeqPairBatchT member whose work
+ * it does.
+ *
+ * Conventions.
+ *  - Every call returns NWG_OK (0) or an NWG_E* status; nwg_last_error() describes the
+ *    last failure of the context.  The C++ engine turns statuses into the reference's
+ *    exceptions (batch: ValueError for NWG_EINVAL and NWG_ERANGE, RuntimeError for
+ *    NWG_ESTATE, IndexError for NWG_EINDEX).
+ *  - Host pointers unless a name ends in _dev.  Calls that take host output buffers
+ *    synchronise the context's stream before returning; *_dev calls are asynchronous on it.
+ *  - One caller per context at a time (the reference's batch has the same rule,
+ *    batch_engine.hpp:2005-2014).
+ */
+#ifndef NWGRAD_HIP_H
+#define NWGRAD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NWG_OK 0
+#define NWG_EINVAL 1   /* invalid argument (std::invalid_argument) */
+#define NWG_ESTATE 2   /* results requested before they exist (std::logic_error) */
+#define NWG_ERANGE 3   /* soft range failure (std::domain_error) */
+#define NWG_EINDEX 4   /* index out of range (std::out_of_range) */
+#define NWG_EDEVICE 5  /* HIP runtime failure */
+#define NWG_ENOMEM 6   /* device or host allocation failure */
+
+typedef struct nwg_ctx nwg_ctx;
+typedef struct nwg_batch nwg_batch;
+
+/* Problem type of a batch, fixed at creation (SeqPairBatchT ctor, seq_pair_batch.hpp:73-78). */
+typedef struct {
+    int32_t precision; /* 0: float32 Viterbi/soft (SeqPairBatch), 1: float64 (SeqPairBatchDouble) */
+    int32_t gap_model; /* 0 linear, 1 affine */
+    int32_t mode;      /* 0 global, 1 local, 2 glocal */
+    int32_t grad_mode; /* 0 none (score only), 1 hard, 2 soft */
+} nwg_problem;
+
+/* Context on one device.  Replaces the reference's thread-pool setup
+ * (parallel.hpp:165-243): the GPU is the pool. */
+int nwg_init(int device, nwg_ctx **out);
+void nwg_destroy(nwg_ctx *ctx);
+const char *nwg_last_error(const nwg_ctx *ctx);
+/* Work of this context goes to `hip_stream` (a hipStream_t; NULL: the null stream). */
+int nwg_set_stream(nwg_ctx *ctx, void *hip_stream);
+
+/* Upload N pairs (codes in [0, n_alpha), offsets of N+1 entries) and plan them into
+ * tiles.  Replaces BatchEngine's constructor and planner (batch_engine.hpp:284-350,
+ * 2610-2676): codes stay device-resident for the life of the batch. */
+int nwg_batch_create(nwg_ctx *ctx, const nwg_problem *prob, int n_alpha, int64_t n_pairs,
+                     const uint8_t *codes_a, const int64_t *off_a, const uint8_t *codes_b,
+                     const int64_t *off_b, nwg_batch **out);
+void nwg_batch_free(nwg_batch *b);
+
+/* Params: matrix n_alpha*n_alpha row-major, gaps4 = go_a, ge_a, go_b, ge_b.  Cast to the
+ * batch's precision on upload as static_cast<T>(double).  Invalidates cached results.
+ * Replaces BatchEngine::set_params (batch_engine.hpp:597-606). */
+int nwg_batch_set_params(nwg_batch *b, const double *matrix, const double *gaps4);
+
+/* fill 0: one persistent launch pulling tiles from a device work queue (default);
+ * fill 1: plain per-bucket launches.  Results do not depend on it
+ * (the reference's `fill=` knob, py_exports.cpp:1180-1207). */
+int nwg_batch_set_fill(nwg_batch *b, int fill);
+
+/* Soft settings (py_exports.cpp soft_temperature / soft_precision).  precision 0: float32
+ * scaled pass (float32 batches only), 1: float64 scaled pass. */
+int nwg_batch_set_soft(nwg_batch *b, double temperature, int precision);
+
+/* The full pass: score, and gradient unless grad_mode is none.  Per-pair results stay on
+ * the device; *total_out (may be NULL) receives the sum of scores in pair order.
+ * Replaces BatchEngine::score_and_grad / soft_score_and_grad (batch_engine.hpp:615-629,
+ * batch_soft_defer.inl:167-240). */
+int nwg_batch_run(nwg_batch *b, double *total_out);
+/* Enqueue the pass on the context's stream and return (no total, no synchronisation). */
+int nwg_batch_run_async(nwg_batch *b);
+
+/* Copies of the cached results (BatchEngine::scores / grads_into,
+ * batch_engine.hpp:689-695, 722-746).  mats: N*n_alpha*n_alpha, gaps: N*4. */
+int nwg_batch_scores(nwg_batch *b, double *out);
+int nwg_batch_grads(nwg_batch *b, double *mats, double *gaps);
+
+/* sum_i w_i * grad_i in 4096-pair blocks (each summed in pair order, then the block sums
+ * in block order): the reference's blocked_sum_ (batch_engine.hpp:702-718, 1788-1827),
+ * bit for bit.  w NULL: compute_grad.  out: n_alpha*n_alpha + 4 doubles. */
+int nwg_batch_weighted_grad(nwg_batch *b, const double *w, int64_t n, double *out);
+/* The same on device buffers, asynchronous on the context's stream (for autograd). */
+int nwg_batch_weighted_grad_dev(nwg_batch *b, const double *w_dev, double *out_dev);
+/* Device pointer to the per-pair scores (N doubles), valid until the batch is freed. */
+int nwg_batch_scores_dev(nwg_batch *b, const double **out_dev);
+
+/* Introspection for tests and the bench: number of tiles and the lane-group width used. */
+int nwg_batch_plan_info(nwg_batch *b, int64_t *n_tiles, int64_t *n_cells);
+/* Device-time of the last pass's kernels in milliseconds (HIP events on the context's
+ * stream around the kernels only). */
+int nwg_batch_last_kernel_ms(nwg_batch *b, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NWGRAD_HIP_H */

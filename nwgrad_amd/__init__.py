@@ -1,0 +1,36 @@
+"""nwgrad_amd: the batched alignment hot path of nwgrad, native to AMD Instinct MI355X (gfx950).
+
+Drop-in surface (the reference's ``nwgrad`` names and argument meanings):
+
+* ``SeqPairBatch`` (float32 Viterbi / soft) and ``SeqPairBatchDouble`` (float64):
+  ``score_and_grad``, ``compute_grad``, ``scores``, ``grads``, ``weighted_grad``,
+  ``set_params``, ``fill``, soft settings.
+* ``AlignParams`` / ``SubstMatrix``: parameters and gradients.
+
+Every alignment runs on the GPU through ``libnwgrad_hip.so`` (HIP kernels behind the C ABI of
+``include/nwgrad_hip.h``).  There is no CPU fallback: importing works anywhere the native module
+was built, but constructing a batch without a usable gfx950 device raises ``RuntimeError``.
+"""
+from __future__ import annotations
+
+import os as _os
+
+_here = _os.path.dirname(_os.path.abspath(__file__))
+
+try:
+    from ._nwgrad_amd import AlignParams, SeqPairBatch, SeqPairBatchDouble, SubstMatrix, device_available
+except ImportError as _e:  # the native module is the product: fail loudly
+    raise ImportError(
+        "nwgrad_amd: the native module is not built (run `python nwgrad_amd/_build.py` or "
+        "`python -c 'import __graft_entry__ as g; g.build()'`): " + str(_e)) from _e
+
+AA_ORDER = "ACDEFGHIKLMNPQRSTVWY"
+DNA = "ACGT"
+
+__all__ = ["AlignParams", "SeqPairBatch", "SeqPairBatchDouble", "SubstMatrix", "device_available",
+           "AA_ORDER", "DNA", "lib_path"]
+
+
+def lib_path() -> str:
+    """Path of the HIP library this package runs on."""
+    return _os.path.join(_here, "libnwgrad_hip.so")

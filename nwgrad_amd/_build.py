@@ -1,0 +1,79 @@
+"""Build the package's native parts in-tree (no GPU needed; hipcc cross-compiles gfx950).
+
+* ``libnwgrad_hip.so``: every HIP kernel plus the C ABI of include/nwgrad_hip.h, built by
+  hipcc for gfx950.  Compiled with -ffp-contract=off (the hard path is bit-exact only if no
+  multiply-add is fused, as in the reference build, CMakeLists.txt:75) and with packed FP32
+  instructions disabled.
+* ``_nwgrad_amd*.so``: the pybind11 module over the header-only C++ engine; host code only,
+  linked against libnwgrad_hip.so through an $ORIGIN rpath.
+
+    python nwgrad_amd/_build.py
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+HIP_SOURCES = ["nwg_api.hip"]
+HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def lib_path() -> str:
+    return os.path.join(HERE, "libnwgrad_hip.so")
+
+
+def ext_path() -> str:
+    suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    return os.path.join(HERE, "_nwgrad_amd" + suffix)
+
+
+def build_hip(verbose: bool = False) -> str:
+    out = lib_path()
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    cmd = [HIPCC, *HIP_FLAGS, "-o", out + ".tmp", *srcs]
+    _run(cmd, verbose)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_ext(verbose: bool = False) -> str:
+    import pybind11
+
+    out = ext_path()
+    cmd = ["g++", "-O2", "-std=c++20", "-fPIC", "-shared", "-fvisibility=hidden",
+           "-I" + pybind11.get_include(), "-I" + sysconfig.get_paths()["include"],
+           os.path.join(CSRC, "py_module.cpp"), "-o", out + ".tmp",
+           "-L" + HERE, "-lnwgrad_hip", "-Wl,-rpath,$ORIGIN"]
+    _run(cmd, verbose)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=CSRC, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    # hipcc's host pass does not know the packed-fp32 target feature and says so: harmless.
+    lines = [l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l]
+    if r.returncode != 0:
+        sys.stderr.write("\n".join(lines) + "\n")
+        raise RuntimeError("build failed: " + " ".join(cmd))
+    if verbose and lines:
+        print("\n".join(lines))
+
+
+def build(verbose: bool = False) -> None:
+    build_hip(verbose)
+    build_ext(verbose)
+
+
+if __name__ == "__main__":
+    build(verbose=True)

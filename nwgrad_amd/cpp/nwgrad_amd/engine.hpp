@@ -1,0 +1,403 @@
+// nwgrad_amd engine: the header-only C++20 core above the C ABI (include/nwgrad_hip.h).
+//
+// It mirrors the reference's drop-in surface for the batched hot path (relative to the
+// reference's src/nwgrad/cpp/nwgrad/):
+//   Alphabet      alphabet.hpp:33-198    (interned symbol sets, encode with validation)
+//   SubstMatrix   subst_matrix.hpp:35-183 (one N x N block per set, row-major)
+//   AlignParams   align_params.hpp:45-203 (matrix + go_a, ge_a, go_b, ge_b; + - * as a gradient)
+//   SeqPairBatchT seq_pair_batch.hpp:56-192 and BatchEngine (batch_engine.hpp:244-3737):
+//                 ctor, set_params, score_and_grad, scores, grads_into, weighted_grad,
+//                 compute_grad, fill / soft settings
+// with the same argument meanings and the same exception types.  All alignment work runs
+// on the GPU through the C ABI; this layer only validates, encodes and converts.
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <string_view>
+#include <vector>
+
+#include "../../../include/nwgrad_hip.h"
+
+namespace nwgrad_amd {
+
+inline constexpr std::string_view kAAOrder = "ACDEFGHIKLMNPQRSTVWY";
+
+// Interned, immutable symbol set: two alphabets are equal iff they are the same object.
+class Alphabet {
+public:
+    static const Alphabet& get(std::string_view symbols) {
+        static std::mutex mu;
+        static auto* registry = new std::map<std::string, const Alphabet*, std::less<>>();
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = registry->find(symbols);
+        if (it != registry->end()) return *it->second;
+        const Alphabet* a = new Alphabet(symbols);
+        registry->emplace(std::string(symbols), a);
+        return *a;
+    }
+    int size() const noexcept { return n_; }
+    const std::string& symbols() const noexcept { return symbols_; }
+    int index_of(char c) const noexcept { return lut_[static_cast<unsigned char>(c)]; }
+    // Validate and encode s into out (s.size() bytes); throws naming the character.
+    void encode_into(std::string_view s, uint8_t* out) const {
+        for (size_t k = 0; k < s.size(); ++k) {
+            const int i = index_of(s[k]);
+            if (i < 0) throw_bad_char(s[k], k);
+            out[k] = static_cast<uint8_t>(i);
+        }
+    }
+    Alphabet(const Alphabet&) = delete;
+    Alphabet& operator=(const Alphabet&) = delete;
+
+private:
+    explicit Alphabet(std::string_view symbols) : symbols_(symbols), n_(static_cast<int>(symbols.size())) {
+        if (symbols.empty()) throw std::invalid_argument("nwgrad: alphabet must be non-empty");
+        if (symbols.size() > 127) throw std::invalid_argument("nwgrad: alphabet may have at most 127 symbols");
+        for (int i = 0; i < 256; ++i) lut_[i] = -1;
+        for (int i = 0; i < n_; ++i) {
+            const char ch = symbols[static_cast<size_t>(i)];
+            const auto c = static_cast<unsigned char>(ch);
+            if (c >= 128)
+                throw std::invalid_argument("nwgrad: alphabet symbols must be ASCII; got a byte >= 0x80 "
+                                            "(a non-ASCII character encoded as UTF-8?)");
+            if (ch == '-')
+                throw std::invalid_argument("nwgrad: '-' cannot be an alphabet symbol; it is reserved as "
+                                            "the gap marker in aligned sequences");
+            if (lut_[c] >= 0)
+                throw std::invalid_argument(std::string("nwgrad: duplicate symbol '") + ch + "' in alphabet \"" +
+                                            symbols_ + "\"");
+            lut_[c] = static_cast<int16_t>(i);
+        }
+    }
+    [[noreturn]] void throw_bad_char(char c, size_t pos) const {
+        static const char* hex = "0123456789abcdef";
+        const auto u = static_cast<unsigned char>(c);
+        std::string msg = "nwgrad: character '";
+        msg += c;
+        msg += "' (0x";
+        msg += hex[u >> 4];
+        msg += hex[u & 0xf];
+        msg += ") at position " + std::to_string(pos) + " is not in alphabet \"" + symbols_ + "\"";
+        throw std::invalid_argument(msg);
+    }
+    std::string symbols_;
+    int n_;
+    int16_t lut_[256];
+};
+
+// An N x N substitution matrix over an interned alphabet, row-major (subst_matrix.hpp:29-33).
+class SubstMatrix {
+public:
+    SubstMatrix(const double* src, const Alphabet& a) : alpha_(&a), blk_(src, src + a.size() * a.size()) {}
+    explicit SubstMatrix(const Alphabet& a) : alpha_(&a), blk_(static_cast<size_t>(a.size() * a.size()), 0.0) {}
+    const Alphabet& alphabet() const noexcept { return *alpha_; }
+    int size() const noexcept { return alpha_->size(); }
+    const double* data() const noexcept { return blk_.data(); }
+    double* data() noexcept { return blk_.data(); }
+    double at(int i, int j) const { return blk_[static_cast<size_t>(i * size() + j)]; }
+    double score(char a, char b) const {
+        const int ia = alpha_->index_of(a), ib = alpha_->index_of(b);
+        if (ia < 0 || ib < 0) {
+            const char bad = ia < 0 ? a : b;
+            throw std::invalid_argument(std::string("nwgrad: character '") + bad + "' is not in alphabet \"" +
+                                        alpha_->symbols() + "\"");
+        }
+        return at(ia, ib);
+    }
+    void check_same(const SubstMatrix& o) const {
+        if (alpha_ != o.alpha_)
+            throw std::invalid_argument("nwgrad: cannot combine matrices over different alphabets (\"" +
+                                        alpha_->symbols() + "\" vs \"" + o.alpha_->symbols() + "\")");
+    }
+    SubstMatrix& operator+=(const SubstMatrix& o) {
+        check_same(o);
+        for (size_t k = 0; k < blk_.size(); ++k) blk_[k] += o.blk_[k];
+        return *this;
+    }
+    SubstMatrix& operator*=(double s) noexcept {
+        for (auto& v : blk_) v *= s;
+        return *this;
+    }
+
+private:
+    const Alphabet* alpha_;
+    std::vector<double> blk_;
+};
+
+// Parameter point and gradient (align_params.hpp:10-43: gap fields of a gradient are <= 0).
+struct AlignParams {
+    SubstMatrix matrix;
+    double gap_open_a = 0.0, gap_extend_a = 0.0, gap_open_b = 0.0, gap_extend_b = 0.0;
+
+    explicit AlignParams(const SubstMatrix& m, double goa = 0, double gea = 0, double gob = 0, double geb = 0)
+        : matrix(m), gap_open_a(goa), gap_extend_a(gea), gap_open_b(gob), gap_extend_b(geb) {}
+    explicit AlignParams(const Alphabet& a) : matrix(a) {}
+
+    const Alphabet& alphabet() const noexcept { return matrix.alphabet(); }
+    void gaps4(double* g) const { g[0] = gap_open_a; g[1] = gap_extend_a; g[2] = gap_open_b; g[3] = gap_extend_b; }
+
+    AlignParams& operator+=(const AlignParams& o) {
+        matrix += o.matrix;
+        gap_open_a += o.gap_open_a; gap_extend_a += o.gap_extend_a;
+        gap_open_b += o.gap_open_b; gap_extend_b += o.gap_extend_b;
+        return *this;
+    }
+    AlignParams& operator*=(double s) noexcept {
+        matrix *= s;
+        gap_open_a *= s; gap_extend_a *= s; gap_open_b *= s; gap_extend_b *= s;
+        return *this;
+    }
+    AlignParams operator+(const AlignParams& o) const { AlignParams r(*this); r += o; return r; }
+    AlignParams operator*(double s) const { AlignParams r(*this); r *= s; return r; }
+    AlignParams operator-() const { return *this * -1.0; }
+    AlignParams operator-(const AlignParams& o) const { return *this + (-1.0 * o); }
+    AlignParams& operator-=(const AlignParams& o) { return *this += (-1.0 * o); }
+    friend AlignParams operator*(double s, const AlignParams& p) { return p * s; }
+};
+
+enum class GapModel { Linear = 0, Affine = 1 };
+enum class AlignMode { Global = 0, Local = 1, Glocal = 2 };
+enum class GradMode { None = 0, Hard = 1, Soft = 2 };
+
+// C ABI status -> the reference's exception types (py_exports / batch_engine conventions).
+[[noreturn]] inline void throw_status(int rc, const char* msg) {
+    const std::string m = msg ? msg : "nwgrad_hip: error";
+    switch (rc) {
+        case NWG_EINVAL: throw std::invalid_argument(m);
+        case NWG_ESTATE: throw std::logic_error(m);
+        case NWG_ERANGE: throw std::domain_error(m);
+        case NWG_EINDEX: throw std::out_of_range(m);
+        default: throw std::runtime_error(m);
+    }
+}
+
+// One context per process and device, created on first use.
+class Device {
+public:
+    static nwg_ctx* ctx(int device = 0) {
+        static std::mutex mu;
+        static std::map<int, nwg_ctx*> ctxs;
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = ctxs.find(device);
+        if (it != ctxs.end()) return it->second;
+        nwg_ctx* c = nullptr;
+        const int rc = nwg_init(device, &c);
+        if (rc != NWG_OK)
+            throw std::runtime_error("nwgrad_amd: no usable HIP device " + std::to_string(device) +
+                                     " (nwg_init status " + std::to_string(rc) +
+                                     "); this build runs its alignment kernels on an MI355X (gfx950) only");
+        ctxs[device] = c;
+        return c;
+    }
+};
+
+// SeqPairBatchT<T>: T = float (SeqPairBatch) or double (SeqPairBatchDouble).
+template <class T>
+class SeqPairBatchT {
+public:
+    SeqPairBatchT(const std::vector<std::string_view>& seqs_a, const std::vector<std::string_view>& seqs_b,
+                  const AlignParams& params, GapModel gm, AlignMode am, GradMode gd, int n_threads = 0,
+                  int device = 0)
+        : alpha_(&params.alphabet()), params_(params), gm_(gm), am_(am), gd_(gd), n_threads_(n_threads) {
+        if (n_threads < 0) throw std::invalid_argument("nwgrad: n_threads must be >= 0");
+        if (seqs_a.size() != seqs_b.size())
+            throw std::invalid_argument("nwgrad: seqs_a and seqs_b must have the same length");
+        const size_t N = seqs_a.size();
+        std::vector<int64_t> oa(N + 1, 0), ob(N + 1, 0);
+        for (size_t i = 0; i < N; ++i) {
+            oa[i + 1] = oa[i] + static_cast<int64_t>(seqs_a[i].size());
+            ob[i + 1] = ob[i] + static_cast<int64_t>(seqs_b[i].size());
+        }
+        std::vector<uint8_t> ca(static_cast<size_t>(oa[N])), cb(static_cast<size_t>(ob[N]));
+        for (size_t i = 0; i < N; ++i) {
+            alpha_->encode_into(seqs_a[i], ca.data() + oa[i]);
+            alpha_->encode_into(seqs_b[i], cb.data() + ob[i]);
+        }
+        create_(ca.data(), oa.data(), cb.data(), ob.data(), static_cast<int64_t>(N), device);
+    }
+    // Pre-encoded pairs (codes in [0, alphabet size)).
+    SeqPairBatchT(const uint8_t* ca, const int64_t* oa, const uint8_t* cb, const int64_t* ob, int64_t N,
+                  const AlignParams& params, GapModel gm, AlignMode am, GradMode gd, int device = 0)
+        : alpha_(&params.alphabet()), params_(params), gm_(gm), am_(am), gd_(gd) {
+        create_(ca, oa, cb, ob, N, device);
+    }
+    SeqPairBatchT(const SeqPairBatchT&) = delete;
+    SeqPairBatchT& operator=(const SeqPairBatchT&) = delete;
+    ~SeqPairBatchT() {
+        if (b_) nwg_batch_free(b_);
+    }
+
+    size_t size() const noexcept { return static_cast<size_t>(N_); }
+    const Alphabet& alphabet() const {
+        if (N_ == 0) throw std::logic_error("nwgrad: SeqPairBatch is empty; it has no alphabet");
+        return *alpha_;
+    }
+    GapModel gap_model() const noexcept { return gm_; }
+    AlignMode align_mode() const noexcept { return am_; }
+    GradMode grad_mode() const noexcept { return gd_; }
+    int n_threads() const noexcept { return n_threads_; }
+    void set_n_threads(int v) {
+        if (v < 0) throw std::invalid_argument("nwgrad: n_threads must be >= 0");
+        n_threads_ = v;
+    }
+
+    // batch_engine.hpp:597-606: a copy of params, over the batch's alphabet.
+    void set_params(const AlignParams& p) {
+        if (&p.alphabet() != alpha_ && N_ > 0)
+            throw std::invalid_argument("nwgrad: params alphabet \"" + p.alphabet().symbols() +
+                                        "\" does not match the batch's \"" + alpha_->symbols() + "\"");
+        params_ = p;
+        upload_params_();
+    }
+    const AlignParams& params() const noexcept { return params_; }
+
+    // fill: "interpair" = one persistent launch over a device work queue; "striped" /
+    // "rowwise" = plain per-bucket launches.  Results are the same whichever runs.
+    void set_fill(const std::string& f) {
+        int v;
+        if (f == "interpair") v = 0;
+        else if (f == "striped" || f == "rowwise") v = 1;
+        else throw std::invalid_argument("nwgrad: unknown fill \"" + f +
+                                         "\" (expected \"interpair\", \"striped\" or \"rowwise\")");
+        check_(nwg_batch_set_fill(b_, v));
+        fill_ = f;
+    }
+    const std::string& fill() const noexcept { return fill_; }
+
+    void set_soft_temperature(double t) {
+        if (!(t > 0.0)) throw std::invalid_argument("nwgrad: soft_temperature must be > 0");
+        soft_temp_ = t;
+        check_(nwg_batch_set_soft(b_, soft_temp_, soft_f32_ ? 0 : 1));
+    }
+    double soft_temperature() const noexcept { return soft_temp_; }
+    void set_soft_precision(const std::string& v) {
+        if (v == "float32") {
+            if (!std::is_same<T, float>::value)
+                throw std::invalid_argument("nwgrad: soft_precision=\"float32\" needs a float32 SeqPair / "
+                                            "SeqPairBatch (the *Double classes run double)");
+            soft_f32_ = true;
+        } else if (v == "double") {
+            soft_f32_ = false;
+        } else {
+            throw std::invalid_argument("nwgrad: soft_precision must be \"float32\" or \"double\", got \"" + v + "\"");
+        }
+        check_(nwg_batch_set_soft(b_, soft_temp_, soft_f32_ ? 0 : 1));
+    }
+    std::string soft_precision() const { return soft_f32_ ? "float32" : "double"; }
+
+    // batch_engine.hpp:615-629: every pair's score (and gradient); the sum in pair order.
+    double score_and_grad() {
+        double total = 0.0;
+        check_(nwg_batch_run(b_, &total));
+        valid_ = true;
+        return total;
+    }
+    // Enqueue only (no total, no synchronisation): for device-side pipelines.
+    void score_and_grad_async() {
+        check_(nwg_batch_run_async(b_));
+        valid_ = true;
+    }
+
+    std::vector<double> scores() const {
+        std::vector<double> out(static_cast<size_t>(N_));
+        if (!valid_) throw std::logic_error("nwgrad: score not valid; compute an alignment first");
+        check_(nwg_batch_scores(b_, out.data()));
+        return out;
+    }
+    void grads_into(double* mats, double* gaps) const {
+        alphabet();
+        if (gd_ == GradMode::None) throw std::logic_error("nwgrad: grad_mode \"none\" computes no gradient");
+        if (!valid_) throw std::logic_error("nwgrad: gradient not valid; compute an alignment first");
+        check_(nwg_batch_grads(b_, mats, gaps));
+    }
+    AlignParams weighted_grad(const double* w, size_t n) const {
+        const Alphabet& a = alphabet();
+        if (n != size())
+            throw std::invalid_argument("nwgrad: weighted_grad() needs one weight per pair (got " + std::to_string(n) +
+                                        " weights for " + std::to_string(size()) + " pairs)");
+        return reduce_(a, w);
+    }
+    AlignParams compute_grad() const { return reduce_(alphabet(), nullptr); }
+
+    // Device-side plumbing for autograd and serving.
+    nwg_batch* handle() const noexcept { return b_; }
+    const double* scores_dev() const {
+        const double* p = nullptr;
+        check_(nwg_batch_scores_dev(b_, &p));
+        return p;
+    }
+    void weighted_grad_dev(const double* w_dev, double* out_dev) const {
+        if (gd_ == GradMode::None) throw std::logic_error("nwgrad: grad_mode \"none\" computes no gradient");
+        check_(nwg_batch_weighted_grad_dev(b_, w_dev, out_dev));
+    }
+    double last_kernel_ms() const {
+        double ms = 0.0;
+        check_(nwg_batch_last_kernel_ms(b_, &ms));
+        return ms;
+    }
+    int64_t cells() const {
+        int64_t t = 0, c = 0;
+        check_(nwg_batch_plan_info(b_, &t, &c));
+        return c;
+    }
+    int64_t tiles() const {
+        int64_t t = 0, c = 0;
+        check_(nwg_batch_plan_info(b_, &t, &c));
+        return t;
+    }
+
+private:
+    void create_(const uint8_t* ca, const int64_t* oa, const uint8_t* cb, const int64_t* ob, int64_t N, int device) {
+        N_ = N;
+        ctx_ = Device::ctx(device);
+        nwg_problem pr{};
+        pr.precision = std::is_same<T, float>::value ? 0 : 1;
+        pr.gap_model = static_cast<int32_t>(gm_);
+        pr.mode = static_cast<int32_t>(am_);
+        pr.grad_mode = static_cast<int32_t>(gd_);
+        soft_f32_ = std::is_same<T, float>::value;
+        check_(nwg_batch_create(ctx_, &pr, alpha_->size(), N, ca, oa, cb, ob, &b_));
+        check_(nwg_batch_set_soft(b_, soft_temp_, soft_f32_ ? 0 : 1));
+        upload_params_();
+    }
+    void upload_params_() {
+        double g[4];
+        params_.gaps4(g);
+        check_(nwg_batch_set_params(b_, params_.matrix.data(), g));
+        valid_ = false;
+    }
+    AlignParams reduce_(const Alphabet& a, const double* w) const {
+        if (gd_ == GradMode::None) throw std::logic_error("nwgrad: grad_mode \"none\" computes no gradient");
+        if (!valid_) throw std::logic_error("nwgrad: gradient not valid; compute an alignment first");
+        const int nn = a.size() * a.size();
+        std::vector<double> out(static_cast<size_t>(nn + 4));
+        check_(nwg_batch_weighted_grad(b_, w, w ? static_cast<int64_t>(size()) : 0, out.data()));
+        AlignParams r(SubstMatrix(out.data(), a), out[nn], out[nn + 1], out[nn + 2], out[nn + 3]);
+        return r;
+    }
+    void check_(int rc) const {
+        if (rc != NWG_OK) throw_status(rc, nwg_last_error(ctx_));
+    }
+
+    const Alphabet* alpha_;
+    AlignParams params_;
+    GapModel gm_;
+    AlignMode am_;
+    GradMode gd_;
+    int n_threads_ = 0;
+    int64_t N_ = 0;
+    nwg_ctx* ctx_ = nullptr;
+    nwg_batch* b_ = nullptr;
+    bool valid_ = false;
+    std::string fill_ = "interpair";
+    double soft_temp_ = 1.0;
+    bool soft_f32_ = true;
+};
+
+}  // namespace nwgrad_amd

@@ -1,0 +1,54 @@
+// Shared device-side definitions of the gfx950 alignment kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nwg {
+
+constexpr int kWave = 64;
+
+// Problem-type codes, matching include/nwgrad_hip.h.
+enum : int { kLinear = 0, kAffine = 1 };
+enum : int { kGlobal = 0, kLocal = 1, kGlocal = 2 };
+
+// One tile = the pairs one wavefront fills together.  G lanes per pair (a power of two
+// dividing 64), npairs <= 64 / G pairs, their indices at order[first .. first + npairs).
+// steps = max_m + G - 1 row steps of the skewed lane-group pipeline.
+struct TileDesc {
+    int32_t first;
+    int16_t G;
+    int16_t npairs;
+    int32_t max_m;
+    int32_t max_n;
+};
+static_assert(sizeof(TileDesc) == 16, "TileDesc is 16 bytes");
+
+template <typename T>
+__device__ __forceinline__ T ninf();
+template <>
+__device__ __forceinline__ float ninf<float>() { return -__builtin_huge_valf(); }
+template <>
+__device__ __forceinline__ double ninf<double>() { return -__builtin_huge_val(); }
+
+// IEEE max of two finite-or-(-inf) values.  It equals std::max on every value the DP can
+// hold, and differs from it only in the sign bit of a +-0 tie, which no comparison sees.
+__device__ __forceinline__ float dmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double dmax(double a, double b) { return __builtin_fmax(a, b); }
+
+// Lane-shift by one towards higher lanes across the whole wavefront (lane 0 keeps its own
+// value, which the caller overwrites).  ds_bpermute based: works for any group width.
+__device__ __forceinline__ float shift_up(float v) { return __shfl_up(v, 1, kWave); }
+__device__ __forceinline__ double shift_up(double v) { return __shfl_up(v, 1, kWave); }
+__device__ __forceinline__ int shift_up(int v) { return __shfl_up(v, 1, kWave); }
+__device__ __forceinline__ float shift_down(float v) { return __shfl_down(v, 1, kWave); }
+__device__ __forceinline__ double shift_down(double v) { return __shfl_down(v, 1, kWave); }
+__device__ __forceinline__ int shift_down(int v) { return __shfl_down(v, 1, kWave); }
+
+// Wavefront-local memory ordering for LDS handed between lanes of one wave.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+}  // namespace nwg

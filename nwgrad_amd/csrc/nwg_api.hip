@@ -1,0 +1,569 @@
+// The C ABI of include/nwgrad_hip.h: contexts, device-resident batches, the batch
+// planner, kernel launches and the gradient reductions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/nwgrad_hip.h"
+#include "hard_kernels.hpp"
+#include "reduce_kernels.hpp"
+
+using nwg::TileDesc;
+
+struct nwg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 256;
+    std::string err;
+};
+
+namespace {
+
+constexpr int kS = 16;                 // columns per lane of the lane-group kernels
+constexpr int kMaxG = 64;              // lanes per pair at most: one wavefront
+constexpr int kMaxN = kS * kMaxG;      // longest B a lane group holds
+constexpr int kHistBytes = 16384;      // LDS histogram budget per wave (hard walk)
+constexpr int kBlocksPerCU = 8;        // persistent grid: waves per CU
+constexpr int64_t kSumBlock = 4096;    // the reference's WEIGHTED_GRAD_BLOCK
+
+int fail(nwg_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define NWG_HIP(ctx, call)                                                                       \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail((ctx), NWG_EDEVICE, "%s failed: %s", #call, hipGetErrorString(e_));       \
+    } while (0)
+
+template <typename P>
+struct DevPtr {
+    P* p = nullptr;
+    size_t n = 0;
+    DevPtr() = default;
+    DevPtr(const DevPtr&) = delete;
+    DevPtr& operator=(const DevPtr&) = delete;
+    ~DevPtr() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        reset();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(P));
+        if (e == hipSuccess) n = count;
+        else p = nullptr;
+        return e;
+    }
+};
+
+int next_pow2(int x) {
+    int g = 1;
+    while (g < x) g <<= 1;
+    return g;
+}
+
+}  // namespace
+
+struct Bucket {
+    int G;
+    int tile_begin, tile_end;  // tiles [begin, end) of the plain order
+};
+
+struct nwg_batch {
+    nwg_ctx* ctx = nullptr;
+    nwg_problem prob{};
+    int na = 0;
+    int64_t N = 0;
+    std::vector<int32_t> mlen, nlen;
+    // device-resident pairs and results
+    DevPtr<uint8_t> ca, cb;
+    DevPtr<int64_t> oa, ob;
+    DevPtr<double> scores, grads;
+    // plan
+    std::vector<TileDesc> tiles;          // plain (bucket) order
+    std::vector<Bucket> buckets;
+    std::vector<int32_t> persist_order;   // tile indices, longest first
+    DevPtr<int32_t> order;                // pair indices, tile-major
+    DevPtr<TileDesc> d_tiles;             // plain order
+    DevPtr<TileDesc> d_tiles_p;           // persistent order
+    DevPtr<int32_t> queue;
+    DevPtr<uint32_t> table;
+    int64_t slot_dwords = 0;
+    int64_t table_slots = 0;
+    int hist_slots = 1;
+    int64_t cells = 0;
+    // soft plan scratch
+    DevPtr<float> soft_f32;
+    DevPtr<double> soft_f64;
+    DevPtr<int32_t> soft_exp;
+    DevPtr<uint8_t> soft_ok;
+    // params
+    std::vector<double> matrix;
+    double gaps[4] = {0, 0, 0, 0};
+    bool have_params = false;
+    DevPtr<float> sub_f32;
+    DevPtr<double> sub_f64;
+    DevPtr<double> soft_w;                // exp'd soft weights (double master)
+    // reduction scratch
+    DevPtr<double> partial, wdev, outdev;
+    double* host_scores = nullptr;        // pinned
+    // settings
+    int fill = 0;
+    double temperature = 1.0;
+    int soft_precision = 0;
+    bool results_valid = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+
+    ~nwg_batch() {
+        if (host_scores) (void)hipHostFree(host_scores);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    }
+};
+
+extern "C" {
+
+int nwg_init(int device, nwg_ctx** out) {
+    if (!out) return NWG_EINVAL;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return NWG_EDEVICE;
+    if (device < 0 || device >= count) return NWG_EINVAL;
+    auto* c = new (std::nothrow) nwg_ctx();
+    if (!c) return NWG_ENOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete c; return NWG_EDEVICE; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->n_cu = prop.multiProcessorCount;
+    *out = c;
+    return NWG_OK;
+}
+
+void nwg_destroy(nwg_ctx* ctx) { delete ctx; }
+
+const char* nwg_last_error(const nwg_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int nwg_set_stream(nwg_ctx* ctx, void* s) {
+    if (!ctx) return NWG_EINVAL;
+    ctx->stream = static_cast<hipStream_t>(s);
+    return NWG_OK;
+}
+
+int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t N, const uint8_t* codes_a,
+                     const int64_t* off_a, const uint8_t* codes_b, const int64_t* off_b, nwg_batch** out) {
+    if (!ctx || !prob || !out) return NWG_EINVAL;
+    *out = nullptr;
+    if (prob->precision < 0 || prob->precision > 1 || prob->gap_model < 0 || prob->gap_model > 1 ||
+        prob->mode < 0 || prob->mode > 2 || prob->grad_mode < 0 || prob->grad_mode > 2)
+        return fail(ctx, NWG_EINVAL, "nwgrad_hip: invalid problem type");
+    if (n_alpha < 1 || n_alpha > 64)
+        return fail(ctx, NWG_EINVAL, "nwgrad_hip: alphabet size %d outside [1, 64]", n_alpha);
+    if (N < 0 || N > INT32_MAX) return fail(ctx, NWG_EINVAL, "nwgrad_hip: invalid pair count");
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    std::unique_ptr<nwg_batch> b(new (std::nothrow) nwg_batch());
+    if (!b) return fail(ctx, NWG_ENOMEM, "nwgrad_hip: out of host memory");
+    b->ctx = ctx;
+    b->prob = *prob;
+    b->na = n_alpha;
+    b->N = N;
+    b->mlen.resize((size_t)N);
+    b->nlen.resize((size_t)N);
+    const int64_t la = N ? off_a[N] : 0, lb = N ? off_b[N] : 0;
+    for (int64_t i = 0; i < N; ++i) {
+        const int64_t m = off_a[i + 1] - off_a[i], n = off_b[i + 1] - off_b[i];
+        if (m < 0 || n < 0) return fail(ctx, NWG_EINVAL, "nwgrad_hip: offsets must be non-decreasing");
+        if (n > kMaxN)
+            return fail(ctx, NWG_EINVAL,
+                        "nwgrad_hip: pair %lld has len(B) = %lld; this build's lane-group kernels hold "
+                        "at most %d columns",
+                        (long long)i, (long long)n, kMaxN);
+        if (m > (1 << 24)) return fail(ctx, NWG_EINVAL, "nwgrad_hip: pair %lld is too long", (long long)i);
+        b->mlen[(size_t)i] = (int32_t)m;
+        b->nlen[(size_t)i] = (int32_t)n;
+        b->cells += m * n;
+    }
+    for (int64_t k = 0; k < la; ++k)
+        if (codes_a[k] >= n_alpha) return fail(ctx, NWG_EINVAL, "nwgrad_hip: code out of range in A");
+    for (int64_t k = 0; k < lb; ++k)
+        if (codes_b[k] >= n_alpha) return fail(ctx, NWG_EINVAL, "nwgrad_hip: code out of range in B");
+
+    // Planner: lane-group width from len(B), then tiles of 64/G pairs with similar len(A).
+    const int nn = n_alpha * n_alpha;
+    int gmin = 1;
+    while (gmin < kMaxG && (64 / gmin) * nn * 2 > kHistBytes) gmin <<= 1;
+    b->hist_slots = 64 / gmin;
+    std::vector<int32_t> gsel((size_t)N);
+    for (int64_t i = 0; i < N; ++i) {
+        int G = next_pow2((b->nlen[(size_t)i] + kS - 1) / kS);
+        gsel[(size_t)i] = std::max(G, gmin);
+    }
+    std::vector<int32_t> idx((size_t)N);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) {
+        if (gsel[(size_t)x] != gsel[(size_t)y]) return gsel[(size_t)x] < gsel[(size_t)y];
+        return b->mlen[(size_t)x] > b->mlen[(size_t)y];
+    });
+    int64_t pos = 0;
+    int64_t max_steps = 1;
+    while (pos < N) {
+        const int G = gsel[(size_t)idx[(size_t)pos]];
+        Bucket bk{G, (int)b->tiles.size(), 0};
+        while (pos < N && gsel[(size_t)idx[(size_t)pos]] == G) {
+            const int per = 64 / G;
+            TileDesc td{};
+            td.first = (int32_t)pos;
+            td.G = (int16_t)G;
+            int np = 0, mm = 0, mn = 0;
+            while (np < per && pos < N && gsel[(size_t)idx[(size_t)pos]] == G) {
+                mm = std::max(mm, b->mlen[(size_t)idx[(size_t)pos]]);
+                mn = std::max(mn, b->nlen[(size_t)idx[(size_t)pos]]);
+                ++np;
+                ++pos;
+            }
+            td.npairs = (int16_t)np;
+            td.max_m = mm;
+            td.max_n = mn;
+            max_steps = std::max<int64_t>(max_steps, (int64_t)mm + G - 1);
+            b->tiles.push_back(td);
+        }
+        bk.tile_end = (int)b->tiles.size();
+        b->buckets.push_back(bk);
+    }
+    b->persist_order.resize(b->tiles.size());
+    std::iota(b->persist_order.begin(), b->persist_order.end(), 0);
+    std::stable_sort(b->persist_order.begin(), b->persist_order.end(), [&](int32_t x, int32_t y) {
+        const TileDesc& a = b->tiles[(size_t)x];
+        const TileDesc& c = b->tiles[(size_t)y];
+        return (int64_t)(a.max_m + a.G - 1) * a.G > (int64_t)(c.max_m + c.G - 1) * c.G;
+    });
+    std::vector<TileDesc> ptiles(b->tiles.size());
+    for (size_t k = 0; k < ptiles.size(); ++k) ptiles[k] = b->tiles[(size_t)b->persist_order[k]];
+    b->slot_dwords = max_steps * (kS / 4) * 64;
+
+    // Device residency.
+    NWG_HIP(ctx, b->ca.alloc((size_t)std::max<int64_t>(la, 1)));
+    NWG_HIP(ctx, b->cb.alloc((size_t)std::max<int64_t>(lb, 1)));
+    NWG_HIP(ctx, b->oa.alloc((size_t)N + 1));
+    NWG_HIP(ctx, b->ob.alloc((size_t)N + 1));
+    NWG_HIP(ctx, b->order.alloc((size_t)std::max<int64_t>(N, 1)));
+    NWG_HIP(ctx, b->d_tiles.alloc(std::max<size_t>(b->tiles.size(), 1)));
+    NWG_HIP(ctx, b->d_tiles_p.alloc(std::max<size_t>(b->tiles.size(), 1)));
+    NWG_HIP(ctx, b->queue.alloc(4));
+    NWG_HIP(ctx, b->scores.alloc((size_t)std::max<int64_t>(N, 1)));
+    if (prob->grad_mode != 0) NWG_HIP(ctx, b->grads.alloc((size_t)std::max<int64_t>(N, 1) * (nn + 4)));
+    if (la) NWG_HIP(ctx, hipMemcpy(b->ca.p, codes_a, (size_t)la, hipMemcpyHostToDevice));
+    if (lb) NWG_HIP(ctx, hipMemcpy(b->cb.p, codes_b, (size_t)lb, hipMemcpyHostToDevice));
+    NWG_HIP(ctx, hipMemcpy(b->oa.p, off_a, sizeof(int64_t) * (size_t)(N + 1), hipMemcpyHostToDevice));
+    NWG_HIP(ctx, hipMemcpy(b->ob.p, off_b, sizeof(int64_t) * (size_t)(N + 1), hipMemcpyHostToDevice));
+    if (N) NWG_HIP(ctx, hipMemcpy(b->order.p, idx.data(), sizeof(int32_t) * (size_t)N, hipMemcpyHostToDevice));
+    if (!b->tiles.empty()) {
+        NWG_HIP(ctx, hipMemcpy(b->d_tiles.p, b->tiles.data(), sizeof(TileDesc) * b->tiles.size(),
+                               hipMemcpyHostToDevice));
+        NWG_HIP(ctx, hipMemcpy(b->d_tiles_p.p, ptiles.data(), sizeof(TileDesc) * ptiles.size(),
+                               hipMemcpyHostToDevice));
+    }
+    NWG_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&b->host_scores),
+                               sizeof(double) * (size_t)std::max<int64_t>(N, 1), hipHostMallocDefault));
+    NWG_HIP(ctx, hipEventCreate(&b->ev0));
+    NWG_HIP(ctx, hipEventCreate(&b->ev1));
+    *out = b.release();
+    return NWG_OK;
+}
+
+void nwg_batch_free(nwg_batch* b) { delete b; }
+
+int nwg_batch_set_params(nwg_batch* b, const double* matrix, const double* gaps4) {
+    if (!b || !matrix || !gaps4) return NWG_EINVAL;
+    nwg_ctx* ctx = b->ctx;
+    const int nn = b->na * b->na;
+    b->matrix.assign(matrix, matrix + nn);
+    std::memcpy(b->gaps, gaps4, sizeof b->gaps);
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    if (b->prob.precision == 0) {
+        std::vector<float> s((size_t)nn);
+        for (int k = 0; k < nn; ++k) s[(size_t)k] = static_cast<float>(matrix[k]);
+        if (!b->sub_f32.p) NWG_HIP(ctx, b->sub_f32.alloc((size_t)nn));
+        NWG_HIP(ctx, hipMemcpy(b->sub_f32.p, s.data(), sizeof(float) * (size_t)nn, hipMemcpyHostToDevice));
+    } else {
+        if (!b->sub_f64.p) NWG_HIP(ctx, b->sub_f64.alloc((size_t)nn));
+        NWG_HIP(ctx, hipMemcpy(b->sub_f64.p, matrix, sizeof(double) * (size_t)nn, hipMemcpyHostToDevice));
+    }
+    b->have_params = true;
+    b->results_valid = false;
+    return NWG_OK;
+}
+
+int nwg_batch_set_fill(nwg_batch* b, int fill) {
+    if (!b || fill < 0 || fill > 1) return NWG_EINVAL;
+    b->fill = fill;
+    return NWG_OK;
+}
+
+int nwg_batch_set_soft(nwg_batch* b, double temperature, int precision) {
+    if (!b) return NWG_EINVAL;
+    if (!(temperature > 0.0)) return fail(b->ctx, NWG_EINVAL, "nwgrad_hip: soft temperature must be > 0");
+    if (precision < 0 || precision > 1) return fail(b->ctx, NWG_EINVAL, "nwgrad_hip: soft precision is 0 or 1");
+    if (precision == 0 && b->prob.precision == 1)
+        return fail(b->ctx, NWG_EINVAL,
+                    "nwgrad: soft_precision=\"float32\" needs a float32 SeqPair / SeqPairBatch (the *Double "
+                    "classes run double)");
+    b->temperature = temperature;
+    b->soft_precision = precision;
+    b->results_valid = false;
+    return NWG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+int launch_soft(nwg_batch* b) {
+    return fail(b->ctx, NWG_EINVAL, "nwgrad_hip: the soft pass is not built yet");
+}
+int soft_check_ok(nwg_batch*) { return NWG_OK; }
+
+template <typename T>
+int launch_hard(nwg_batch* b) {
+    nwg_ctx* ctx = b->ctx;
+    const bool hard = b->prob.grad_mode == 1;
+    if (hard && b->table.n == 0) {
+        const int64_t slots = b->fill == 0 ? std::min<int64_t>((int64_t)b->tiles.size(),
+                                                               (int64_t)ctx->n_cu * kBlocksPerCU)
+                                           : (int64_t)b->tiles.size();
+        b->table_slots = std::max<int64_t>(slots, 1);
+        NWG_HIP(ctx, b->table.alloc((size_t)(b->table_slots * b->slot_dwords)));
+    }
+    nwg::HardArgs<T> A{};
+    A.codes_a = b->ca.p;
+    A.codes_b = b->cb.p;
+    A.off_a = b->oa.p;
+    A.off_b = b->ob.p;
+    A.order = b->order.p;
+    A.n_tiles = (int32_t)b->tiles.size();
+    A.sub = std::is_same<T, float>::value ? reinterpret_cast<const T*>(b->sub_f32.p)
+                                           : reinterpret_cast<const T*>(b->sub_f64.p);
+    A.na = b->na;
+    A.hist_slots = b->hist_slots;
+    A.go_a = static_cast<T>(b->gaps[0]);
+    A.ge_a = static_cast<T>(b->gaps[1]);
+    A.go_b = static_cast<T>(b->gaps[2]);
+    A.ge_b = static_cast<T>(b->gaps[3]);
+    A.scores = b->scores.p;
+    A.grads = b->grads.p;
+    A.table = b->table.p;
+    A.slot_dwords = b->slot_dwords;
+    const int nn = b->na * b->na;
+    const size_t lds = (size_t)((nn * (int)sizeof(T) + 15) & ~15) + (hard ? (size_t)b->hist_slots * nn * 2 : 0);
+    const int gm = b->prob.gap_model, am = b->prob.mode;
+    auto pick = [&](auto gmc, auto amc, auto hardc) -> void (*)(nwg::HardArgs<T>) {
+        return nwg::hard_fill_kernel<T, decltype(gmc)::value, decltype(amc)::value, kS, decltype(hardc)::value>;
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using BT = std::true_type;
+    using BF = std::false_type;
+    void (*kern)(nwg::HardArgs<T>) = nullptr;
+#define NWG_PICK(GMV, AMV)                                                        \
+    if (gm == GMV::value && am == AMV::value) kern = hard ? pick(GMV{}, AMV{}, BT{}) : pick(GMV{}, AMV{}, BF{});
+    NWG_PICK(I0, I0) NWG_PICK(I0, I1) NWG_PICK(I0, I2) NWG_PICK(I1, I0) NWG_PICK(I1, I1) NWG_PICK(I1, I2)
+#undef NWG_PICK
+    if (!kern) return fail(ctx, NWG_EINVAL, "nwgrad_hip: unsupported problem type");
+    if (b->tiles.empty()) return NWG_OK;
+    if (b->fill == 0) {
+        NWG_HIP(ctx, hipMemsetAsync(b->queue.p, 0, sizeof(int32_t), ctx->stream));
+        A.tiles = b->d_tiles_p.p;
+        A.queue = b->queue.p;
+        A.tile_base = 0;
+        const int grid = (int)std::min<int64_t>((int64_t)b->tiles.size(), (int64_t)ctx->n_cu * kBlocksPerCU);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, ctx->stream, A);
+        NWG_HIP(ctx, hipGetLastError());
+    } else {
+        A.tiles = b->d_tiles.p;
+        A.queue = nullptr;
+        for (const Bucket& bk : b->buckets) {
+            A.tile_base = bk.tile_begin;
+            A.n_tiles = bk.tile_end;
+            hipLaunchKernelGGL(kern, dim3(bk.tile_end - bk.tile_begin), dim3(64), lds, ctx->stream, A);
+            NWG_HIP(ctx, hipGetLastError());
+        }
+    }
+    return NWG_OK;
+}
+
+int run_async(nwg_batch* b) {
+    nwg_ctx* ctx = b->ctx;
+    if (!b->have_params) return fail(ctx, NWG_ESTATE, "nwgrad_hip: set_params() has not been called");
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    NWG_HIP(ctx, hipEventRecord(b->ev0, ctx->stream));
+    int rc;
+    if (b->prob.grad_mode == 2) {
+        rc = b->prob.precision == 0 && b->soft_precision == 0 ? launch_soft<float>(b) : launch_soft<double>(b);
+    } else {
+        rc = b->prob.precision == 0 ? launch_hard<float>(b) : launch_hard<double>(b);
+    }
+    if (rc) return rc;
+    NWG_HIP(ctx, hipEventRecord(b->ev1, ctx->stream));
+    b->results_valid = true;
+    return NWG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nwg_batch_run_async(nwg_batch* b) {
+    if (!b) return NWG_EINVAL;
+    return run_async(b);
+}
+
+int nwg_batch_run(nwg_batch* b, double* total_out) {
+    if (!b) return NWG_EINVAL;
+    nwg_ctx* ctx = b->ctx;
+    int rc = run_async(b);
+    if (rc) return rc;
+    if (b->N)
+        NWG_HIP(ctx, hipMemcpyAsync(b->host_scores, b->scores.p, sizeof(double) * (size_t)b->N,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, b->ev0, b->ev1) == hipSuccess) b->last_ms = ms;
+    if (b->prob.grad_mode == 2) {
+        rc = soft_check_ok(b);
+        if (rc) return rc;
+    }
+    if (total_out) {
+        // Pair-index order, as the reference sums (batch_engine.hpp:1754-1757).
+        double t = 0.0;
+        for (int64_t i = 0; i < b->N; ++i) t += b->host_scores[i];
+        *total_out = t;
+    }
+    return NWG_OK;
+}
+
+int nwg_batch_scores(nwg_batch* b, double* out) {
+    if (!b || !out) return NWG_EINVAL;
+    nwg_ctx* ctx = b->ctx;
+    if (!b->results_valid) return fail(ctx, NWG_ESTATE, "nwgrad: score not valid; compute an alignment first");
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    if (b->N)
+        NWG_HIP(ctx, hipMemcpyAsync(out, b->scores.p, sizeof(double) * (size_t)b->N, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return NWG_OK;
+}
+
+int nwg_batch_scores_dev(nwg_batch* b, const double** out_dev) {
+    if (!b || !out_dev) return NWG_EINVAL;
+    *out_dev = b->scores.p;
+    return NWG_OK;
+}
+
+int nwg_batch_grads(nwg_batch* b, double* mats, double* gaps) {
+    if (!b || !mats || !gaps) return NWG_EINVAL;
+    nwg_ctx* ctx = b->ctx;
+    if (b->prob.grad_mode == 0)
+        return fail(ctx, NWG_ESTATE, "nwgrad: grad_mode \"none\" computes no gradient");
+    if (!b->results_valid) return fail(ctx, NWG_ESTATE, "nwgrad: gradient not valid; compute an alignment first");
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    const int nn = b->na * b->na, gs = nn + 4;
+    if (b->N == 0) return NWG_OK;
+    std::vector<double> tmp((size_t)b->N * gs);
+    NWG_HIP(ctx, hipMemcpyAsync(tmp.data(), b->grads.p, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < b->N; ++i) {
+        std::memcpy(mats + i * nn, tmp.data() + i * gs, sizeof(double) * (size_t)nn);
+        std::memcpy(gaps + i * 4, tmp.data() + i * gs + nn, sizeof(double) * 4);
+    }
+    return NWG_OK;
+}
+
+static int weighted_dev(nwg_batch* b, const double* w_dev, double* out_dev) {
+    nwg_ctx* ctx = b->ctx;
+    const int gs = b->na * b->na + 4;
+    const int64_t nblocks = (b->N + kSumBlock - 1) / kSumBlock;
+    if (b->partial.n < (size_t)(nblocks * gs)) NWG_HIP(ctx, b->partial.alloc((size_t)(nblocks * gs)));
+    {
+        const int threads = 64;
+        const int64_t total = nblocks * gs;
+        const int grid = (int)((total + threads - 1) / threads);
+        hipLaunchKernelGGL(nwg::block_sum_kernel, dim3(std::max(grid, 1)), dim3(threads), 0, ctx->stream,
+                           b->grads.p, w_dev, b->N, gs, kSumBlock, b->partial.p);
+        NWG_HIP(ctx, hipGetLastError());
+    }
+    hipLaunchKernelGGL(nwg::block_total_kernel, dim3((gs + 63) / 64), dim3(64), 0, ctx->stream, b->partial.p,
+                       nblocks, gs, out_dev);
+    NWG_HIP(ctx, hipGetLastError());
+    return NWG_OK;
+}
+
+int nwg_batch_weighted_grad(nwg_batch* b, const double* w, int64_t n, double* out) {
+    if (!b || !out) return NWG_EINVAL;
+    nwg_ctx* ctx = b->ctx;
+    if (b->N == 0) return fail(ctx, NWG_EINVAL, "nwgrad: SeqPairBatch is empty");
+    if (w && n != b->N)
+        return fail(ctx, NWG_EINVAL, "nwgrad: weighted_grad() needs one weight per pair (got %lld weights for %lld pairs)",
+                    (long long)n, (long long)b->N);
+    if (b->prob.grad_mode == 0)
+        return fail(ctx, NWG_ESTATE, "nwgrad: grad_mode \"none\" computes no gradient");
+    if (!b->results_valid) return fail(ctx, NWG_ESTATE, "nwgrad: gradient not valid; compute an alignment first");
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    const int gs = b->na * b->na + 4;
+    const double* wd = nullptr;
+    if (w) {
+        if (b->wdev.n < (size_t)b->N) NWG_HIP(ctx, b->wdev.alloc((size_t)b->N));
+        NWG_HIP(ctx, hipMemcpyAsync(b->wdev.p, w, sizeof(double) * (size_t)b->N, hipMemcpyHostToDevice, ctx->stream));
+        wd = b->wdev.p;
+    }
+    if (b->outdev.n < (size_t)gs) NWG_HIP(ctx, b->outdev.alloc((size_t)gs));
+    int rc = weighted_dev(b, wd, b->outdev.p);
+    if (rc) return rc;
+    NWG_HIP(ctx, hipMemcpyAsync(out, b->outdev.p, sizeof(double) * (size_t)gs, hipMemcpyDeviceToHost, ctx->stream));
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return NWG_OK;
+}
+
+int nwg_batch_weighted_grad_dev(nwg_batch* b, const double* w_dev, double* out_dev) {
+    if (!b || !out_dev) return NWG_EINVAL;
+    if (b->N == 0) return fail(b->ctx, NWG_EINVAL, "nwgrad: SeqPairBatch is empty");
+    if (!b->results_valid) return fail(b->ctx, NWG_ESTATE, "nwgrad: gradient not valid; compute an alignment first");
+    return weighted_dev(b, w_dev, out_dev);
+}
+
+int nwg_batch_plan_info(nwg_batch* b, int64_t* n_tiles, int64_t* n_cells) {
+    if (!b) return NWG_EINVAL;
+    if (n_tiles) *n_tiles = (int64_t)b->tiles.size();
+    if (n_cells) *n_cells = b->cells;
+    return NWG_OK;
+}
+
+int nwg_batch_last_kernel_ms(nwg_batch* b, double* ms) {
+    if (!b || !ms) return NWG_EINVAL;
+    *ms = b->last_ms;
+    return NWG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,155 @@
+"""TEST INFRASTRUCTURE ONLY: the CPU oracle of the alignment hot path.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg may import this
+package, and only as the checker.  The product (``nwgrad_amd``) never imports it.
+
+Two oracles live here:
+
+* ``liboracle.so`` (built from ``nwg_oracle.c``): a plain-C restatement of the reference's
+  hard fill / walk / counts, score-only fill, log-space soft forward-backward and blocked
+  gradient sum.  Each function cites the reference lines it follows.
+* ``_ref/libnwgref.so`` (built by ``Makefile.ref`` from the reference's own sources, when
+  they are present): the reference engine itself behind a small C ABI (``ref_harness.cpp``).
+
+Pairs are passed as encoded ``uint8`` codes with ``int64`` offsets (``codes[off[i]:off[i+1]]``
+is pair i's sequence), the layout the product uses too.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libnwgref.so")
+
+GAP_MODELS = {"linear": 0, "affine": 1}
+MODES = {"global": 0, "local": 1, "glocal": 2}
+
+
+def build(force: bool = False) -> str:
+    """Compile the C restatement (seconds).  -ffp-contract=off keeps every hard-path
+    operation a separate rounding, as the reference's build does (CMakeLists.txt:75)."""
+    src = os.path.join(HERE, "nwg_oracle.c")
+    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
+        subprocess.check_call(["gcc", "-O2", "-std=c11", "-ffp-contract=off", "-fPIC", "-shared",
+                               "-o", LIB_PATH, src, "-lm"])
+    return LIB_PATH
+
+
+def build_ref() -> str | None:
+    """Build oracle/_ref from the reference's sources when they are present (this
+    container only; the GPU box receives the built library)."""
+    if not os.path.isdir("/root/reference/src/nwgrad/cpp/nwgrad"):
+        return REF_PATH if os.path.exists(REF_PATH) else None
+    root = os.path.dirname(HERE)
+    subprocess.check_call(["make", "-s", "-f", "oracle/Makefile.ref"], cwd=root)
+    return REF_PATH
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("gap_model", ctypes.c_int), ("mode", ctypes.c_int), ("n_alpha", ctypes.c_int),
+                ("matrix", ctypes.POINTER(ctypes.c_double)),
+                ("go_a", ctypes.c_double), ("ge_a", ctypes.c_double),
+                ("go_b", ctypes.c_double), ("ge_b", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        L.nwo_hard.argtypes = [P(_Params), ctypes.c_int, ctypes.c_int64, P(ctypes.c_uint8), P(ctypes.c_int64),
+                               P(ctypes.c_uint8), P(ctypes.c_int64), ctypes.c_int, P(ctypes.c_double),
+                               P(ctypes.c_double)]
+        L.nwo_hard.restype = ctypes.c_int
+        L.nwo_soft_log.argtypes = [P(_Params), ctypes.c_double, ctypes.c_int64, P(ctypes.c_uint8),
+                                   P(ctypes.c_int64), P(ctypes.c_uint8), P(ctypes.c_int64),
+                                   P(ctypes.c_double), P(ctypes.c_double)]
+        L.nwo_soft_log.restype = ctypes.c_int
+        L.nwo_weighted_sum.argtypes = [ctypes.c_int64, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double),
+                                       P(ctypes.c_double)]
+        L.nwo_weighted_sum.restype = None
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def _params(matrix, gaps, gap_model, mode):
+    mat = np.ascontiguousarray(matrix, dtype=np.float64)
+    na = mat.shape[0]
+    p = _Params(GAP_MODELS[gap_model], MODES[mode], na, _ptr(mat, ctypes.c_double),
+                float(gaps[0]), float(gaps[1]), float(gaps[2]), float(gaps[3]))
+    return p, mat
+
+
+def _pairs(codes_a, off_a, codes_b, off_b):
+    ca = np.ascontiguousarray(codes_a, dtype=np.uint8)
+    cb = np.ascontiguousarray(codes_b, dtype=np.uint8)
+    oa = np.ascontiguousarray(off_a, dtype=np.int64)
+    ob = np.ascontiguousarray(off_b, dtype=np.int64)
+    if ca.size == 0:
+        ca = np.zeros(1, np.uint8)
+    if cb.size == 0:
+        cb = np.zeros(1, np.uint8)
+    return ca, oa, cb, ob
+
+
+def hard(matrix, gaps, codes_a, off_a, codes_b, off_b, *, gap_model="affine", mode="global",
+         precision="float32", with_grad=True):
+    """Scores (N,) and per-pair gradients (N, na*na+4) of the hard path (or score-only)."""
+    p, mat = _params(matrix, gaps, gap_model, mode)
+    ca, oa, cb, ob = _pairs(codes_a, off_a, codes_b, off_b)
+    N = len(oa) - 1
+    na = mat.shape[0]
+    scores = np.zeros(N, np.float64)
+    grads = np.zeros((N, na * na + 4), np.float64) if with_grad else np.zeros((1, 1))
+    rc = lib().nwo_hard(ctypes.byref(p), 1 if precision in ("float64", "double") else 0, N,
+                        _ptr(ca, ctypes.c_uint8), _ptr(oa, ctypes.c_int64), _ptr(cb, ctypes.c_uint8),
+                        _ptr(ob, ctypes.c_int64), 1 if with_grad else 0, _ptr(scores, ctypes.c_double),
+                        _ptr(grads, ctypes.c_double))
+    if rc:
+        raise MemoryError("oracle: allocation failed")
+    return scores, (grads if with_grad else None)
+
+
+def soft_log(matrix, gaps, codes_a, off_a, codes_b, off_b, *, gap_model="affine", mode="global",
+             temperature=1.0):
+    """Log-space double forward-backward: scores T*logZ (N,) and expected counts."""
+    p, mat = _params(matrix, gaps, gap_model, mode)
+    ca, oa, cb, ob = _pairs(codes_a, off_a, codes_b, off_b)
+    N = len(oa) - 1
+    na = mat.shape[0]
+    scores = np.zeros(N, np.float64)
+    grads = np.zeros((N, na * na + 4), np.float64)
+    rc = lib().nwo_soft_log(ctypes.byref(p), float(temperature), N, _ptr(ca, ctypes.c_uint8),
+                            _ptr(oa, ctypes.c_int64), _ptr(cb, ctypes.c_uint8), _ptr(ob, ctypes.c_int64),
+                            _ptr(scores, ctypes.c_double), _ptr(grads, ctypes.c_double))
+    if rc:
+        raise MemoryError("oracle: allocation failed")
+    return scores, grads
+
+
+def weighted_sum(grads, weights=None):
+    """The reference's blocked sum (4096-pair blocks) of per-pair gradient rows."""
+    g = np.ascontiguousarray(grads, dtype=np.float64)
+    N, gs = g.shape
+    out = np.zeros(gs, np.float64)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    lib().nwo_weighted_sum(N, gs, _ptr(g, ctypes.c_double),
+                           None if w is None else _ptr(w, ctypes.c_double), _ptr(out, ctypes.c_double))
+    return out
+
+
+def have_ref() -> bool:
+    return os.path.exists(REF_PATH)

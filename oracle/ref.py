@@ -1,0 +1,125 @@
+"""TEST INFRASTRUCTURE ONLY: the reference engine built from its own sources
+(``oracle/_ref/libnwgref.so``, see ``Makefile.ref``), driven through ``ref_harness.cpp``.
+
+``RefBatch`` mirrors the reference's ``SeqPairBatch`` / ``SeqPairBatchDouble`` Python class
+(py_exports.cpp:787-1263) closely enough for parity tests and the CPU baseline of bench.py.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import REF_PATH
+
+_lib = None
+_TB = {"auto": 0, "pointers": 1, "scores": 2, "hirschberg": 3, "hirschberg_pmax": 4}
+_GM = {"linear": 0, "affine": 1}
+_MODE = {"global": 0, "local": 1, "glocal": 2}
+_GRAD = {"none": 0, "hard": 1, "soft": 2}
+_FILL = {"interpair": 0, "striped": 1, "rowwise": 2}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(REF_PATH)
+        P = ctypes.POINTER
+        vp = ctypes.c_void_p
+        L.nwgref_last_error.restype = ctypes.c_char_p
+        L.nwgref_create.argtypes = [ctypes.c_int, ctypes.c_char_p, P(ctypes.c_double), P(ctypes.c_double),
+                                    ctypes.c_long, ctypes.c_char_p, P(ctypes.c_long), ctypes.c_char_p,
+                                    P(ctypes.c_long), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, P(vp)]
+        L.nwgref_destroy.argtypes = [vp]
+        L.nwgref_set_fill.argtypes = [vp, ctypes.c_int]
+        L.nwgref_set_soft.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.nwgref_set_params.argtypes = [vp, ctypes.c_char_p, P(ctypes.c_double), P(ctypes.c_double)]
+        L.nwgref_score_and_grad.argtypes = [vp, P(ctypes.c_double)]
+        L.nwgref_scores.argtypes = [vp, P(ctypes.c_double)]
+        L.nwgref_grads.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_double)]
+        L.nwgref_weighted_grad.argtypes = [vp, P(ctypes.c_double), ctypes.c_long, P(ctypes.c_double)]
+        L.nwgref_compute_grad.argtypes = [vp, P(ctypes.c_double)]
+        _lib = L
+    return _lib
+
+
+_ERR = {1: ValueError, 2: RuntimeError, 3: ValueError, 4: IndexError, 5: RuntimeError}
+
+
+def _check(rc):
+    if rc:
+        raise _ERR.get(rc, RuntimeError)(lib().nwgref_last_error().decode())
+
+
+def _d(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class RefBatch:
+    def __init__(self, seqs_a, seqs_b, alphabet, matrix, gaps, *, double=False, gap_model="affine",
+                 mode="global", grad_mode="hard", n_threads=0, traceback="auto"):
+        L = lib()
+        self.alphabet = alphabet
+        self.n_alpha = len(alphabet)
+        self.N = len(seqs_a)
+        sa = "".join(seqs_a).encode()
+        sb = "".join(seqs_b).encode()
+        oa = np.zeros(self.N + 1, np.int64)
+        ob = np.zeros(self.N + 1, np.int64)
+        oa[1:] = np.cumsum([len(s) for s in seqs_a])
+        ob[1:] = np.cumsum([len(s) for s in seqs_b])
+        self._keep = (sa, sb, oa, ob)
+        mat = np.ascontiguousarray(matrix, np.float64)
+        g = np.ascontiguousarray(gaps, np.float64)
+        h = ctypes.c_void_p()
+        _check(L.nwgref_create(1 if double else 0, alphabet.encode(), _d(mat), _d(g), self.N, sa,
+                               oa.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), sb,
+                               ob.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), _GM[gap_model], _MODE[mode],
+                               _GRAD[grad_mode], n_threads, _TB[traceback], ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().nwgref_destroy(self._h)
+            self._h = None
+
+    def set_fill(self, fill):
+        _check(lib().nwgref_set_fill(self._h, _FILL[fill]))
+
+    def set_soft(self, temperature=1.0, precision="float32", impl="scaled"):
+        _check(lib().nwgref_set_soft(self._h, float(temperature), 0 if precision == "float32" else 1,
+                                     {"scaled": 0, "scaled_or_log": 1, "log": 2}[impl]))
+
+    def set_params(self, matrix, gaps):
+        mat = np.ascontiguousarray(matrix, np.float64)
+        g = np.ascontiguousarray(gaps, np.float64)
+        _check(lib().nwgref_set_params(self._h, self.alphabet.encode(), _d(mat), _d(g)))
+
+    def score_and_grad(self):
+        t = ctypes.c_double()
+        _check(lib().nwgref_score_and_grad(self._h, ctypes.byref(t)))
+        return t.value
+
+    def scores(self):
+        out = np.zeros(self.N, np.float64)
+        _check(lib().nwgref_scores(self._h, _d(out)))
+        return out
+
+    def grads(self):
+        na = self.n_alpha
+        mats = np.zeros((self.N, na, na), np.float64)
+        gaps = np.zeros((self.N, 4), np.float64)
+        _check(lib().nwgref_grads(self._h, _d(mats), _d(gaps)))
+        return mats, gaps
+
+    def weighted_grad(self, w):
+        w = np.ascontiguousarray(w, np.float64)
+        out = np.zeros(self.n_alpha ** 2 + 4, np.float64)
+        _check(lib().nwgref_weighted_grad(self._h, _d(w), len(w), _d(out)))
+        return out
+
+    def compute_grad(self):
+        out = np.zeros(self.n_alpha ** 2 + 4, np.float64)
+        _check(lib().nwgref_compute_grad(self._h, _d(out)))
+        return out

@@ -1,0 +1,186 @@
+// TEST INFRASTRUCTURE ONLY.  A C-ABI driver over the reference nwgrad engine
+// (SeqPairBatchT<T>, seq_pair_batch.hpp), compiled from the reference's own sources by
+// oracle/Makefile.ref into oracle/_ref/libnwgref.so.  It is the second oracle (next to the
+// restatement in oracle/nwg_oracle.c) and the CPU baseline of bench.py.  Nothing in the
+// product (nwgrad_amd/) links, loads or calls it.
+//
+// The calls mirror the reference's Python surface (py_exports.cpp:787-1263):
+//   SeqPairBatch(seqs_a, seqs_b, params, n_threads=, traceback=, gap_model=, mode=,
+//                grad_mode=)  -> nwgref_create
+//   .fill / .soft_temperature / .soft_precision / .soft_impl        -> nwgref_set_*
+//   .score_and_grad() (soft: nwgrad_soft_score_and_grad)             -> nwgref_score_and_grad
+//   .scores() / .grads() / .weighted_grad(w) / .compute_grad()       -> nwgref_scores / ...
+//   .set_params(p)                                                   -> nwgref_set_params
+// Exceptions are mapped to status codes: 1 invalid_argument (ValueError), 2 logic_error
+// (RuntimeError), 3 domain_error (ValueError), 4 out_of_range (IndexError), 5 other.
+#include "seq_pair_batch.hpp"
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <string_view>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Handle {
+    int dbl = 0;
+    std::string chars_a, chars_b;
+    std::unique_ptr<SeqPairBatchT<float>> f;
+    std::unique_ptr<SeqPairBatchT<double>> d;
+    int grad_mode = 1;
+};
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::domain_error& e) {
+        g_err = e.what();
+        return 3;
+    } catch (const std::invalid_argument& e) {
+        g_err = e.what();
+        return 1;
+    } catch (const std::out_of_range& e) {
+        g_err = e.what();
+        return 4;
+    } catch (const std::logic_error& e) {
+        g_err = e.what();
+        return 2;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 5;
+    }
+}
+
+AlignParams make_params(const char* alphabet, const double* matrix, const double* gaps4) {
+    const Alphabet& al = Alphabet::get(alphabet);
+    return AlignParams(SubstMatrix(matrix, al), gaps4[0], gaps4[1], gaps4[2], gaps4[3]);
+}
+
+template <class B, class F>
+auto with_batch(Handle* h, F&& f) {
+    if (h->dbl) return f(*h->d);
+    return f(*h->f);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nwgref_last_error(void) { return g_err.c_str(); }
+
+// gap_model 0 linear / 1 affine; mode 0 global / 1 local / 2 glocal; grad_mode 0 none /
+// 1 hard / 2 soft; traceback 0 auto / 1 pointers / 2 scores / 3 hirschberg /
+// 4 hirschberg_pmax.  Sequences are given as characters, concatenated, with N+1 offsets.
+int nwgref_create(int dbl, const char* alphabet, const double* matrix, const double* gaps4,
+                  long n_pairs, const char* seq_a, const long* off_a, const char* seq_b,
+                  const long* off_b, int gap_model, int mode, int grad_mode, int n_threads,
+                  int traceback, void** out) {
+    return guarded([&] {
+        auto h = std::make_unique<Handle>();
+        h->dbl = dbl;
+        h->grad_mode = grad_mode;
+        h->chars_a.assign(seq_a, static_cast<size_t>(off_a[n_pairs]));
+        h->chars_b.assign(seq_b, static_cast<size_t>(off_b[n_pairs]));
+        std::vector<std::string_view> va, vb;
+        va.reserve(static_cast<size_t>(n_pairs));
+        vb.reserve(static_cast<size_t>(n_pairs));
+        for (long i = 0; i < n_pairs; ++i) {
+            va.emplace_back(h->chars_a.data() + off_a[i], static_cast<size_t>(off_a[i + 1] - off_a[i]));
+            vb.emplace_back(h->chars_b.data() + off_b[i], static_cast<size_t>(off_b[i + 1] - off_b[i]));
+        }
+        const AlignParams p = make_params(alphabet, matrix, gaps4);
+        const GapModel gm = gap_model ? GapModel::Affine : GapModel::Linear;
+        const AlignMode am = mode == 1 ? AlignMode::Local : mode == 2 ? AlignMode::Glocal : AlignMode::Global;
+        const GradMode gd = grad_mode == 0 ? GradMode::None : grad_mode == 2 ? GradMode::Soft : GradMode::Hard;
+        const TracebackMode tbs[] = {TracebackMode::Default, TracebackMode::Pointers, TracebackMode::Scores,
+                                     TracebackMode::Hirschberg, TracebackMode::HirschbergPmax};
+        const TracebackMode tb = tbs[(traceback >= 0 && traceback <= 4) ? traceback : 0];
+        if (dbl)
+            h->d = std::make_unique<SeqPairBatchT<double>>(va, vb, p, gm, am, gd, n_threads, tb);
+        else
+            h->f = std::make_unique<SeqPairBatchT<float>>(va, vb, p, gm, am, gd, n_threads, tb);
+        *out = h.release();
+    });
+}
+
+void nwgref_destroy(void* hp) { delete static_cast<Handle*>(hp); }
+
+// fill 0 interpair / 1 striped / 2 rowwise
+int nwgref_set_fill(void* hp, int fill) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guarded([&] { with_batch<void>(h, [&](auto& b) { b.set_fill(fill == 2, fill == 0); }); });
+}
+
+// precision 0 float32 / 1 double; impl 0 scaled / 1 scaled_or_log / 2 log
+int nwgref_set_soft(void* hp, double temperature, int precision, int impl) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guarded([&] {
+        with_batch<void>(h, [&](auto& b) {
+            b.set_soft_temperature(temperature);
+            b.set_soft_impl(impl == 2 ? SoftImpl::Log : impl == 1 ? SoftImpl::ScaledOrLog : SoftImpl::Scaled);
+        });
+        if (!h->dbl) h->f->set_soft_f32(precision == 0);
+    });
+}
+
+int nwgref_set_params(void* hp, const char* alphabet, const double* matrix, const double* gaps4) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guarded([&] {
+        const AlignParams p = make_params(alphabet, matrix, gaps4);
+        with_batch<void>(h, [&](auto& b) { b.set_params(p); });
+    });
+}
+
+int nwgref_score_and_grad(void* hp, double* total) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guarded([&] {
+        *total = with_batch<double>(h, [&](auto& b) -> double {
+            if (h->grad_mode == 2) return nwgrad_soft_score_and_grad(b, false);
+            return b.score_and_grad(false);
+        });
+    });
+}
+
+int nwgref_scores(void* hp, double* out) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guarded([&] {
+        const std::vector<double> s = with_batch<std::vector<double>>(h, [](auto& b) { return b.scores(); });
+        std::memcpy(out, s.data(), s.size() * sizeof(double));
+    });
+}
+
+// mats: N*n*n, gaps: N*4
+int nwgref_grads(void* hp, double* mats, double* gaps) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guarded([&] { with_batch<void>(h, [&](auto& b) { b.grads_into(mats, gaps); }); });
+}
+
+static void params_out(const AlignParams& g, double* out) {
+    const size_t nn = static_cast<size_t>(g.matrix.size()) * static_cast<size_t>(g.matrix.size());
+    std::memcpy(out, g.matrix.data(), nn * sizeof(double));
+    out[nn] = g.gap_open_a;
+    out[nn + 1] = g.gap_extend_a;
+    out[nn + 2] = g.gap_open_b;
+    out[nn + 3] = g.gap_extend_b;
+}
+
+// out: n*n + 4 doubles (matrix row-major, then go_a, ge_a, go_b, ge_b)
+int nwgref_weighted_grad(void* hp, const double* w, long n, double* out) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guarded([&] {
+        with_batch<void>(h, [&](auto& b) { params_out(b.weighted_grad(w, static_cast<size_t>(n)), out); });
+    });
+}
+
+int nwgref_compute_grad(void* hp, double* out) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guarded([&] { with_batch<void>(h, [&](auto& b) { params_out(b.compute_grad(), out); }); });
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+"""Writes the golden fixtures of tests/golden/ from the reference engine built from its own
+sources (oracle/_ref/libnwgref.so; see oracle/Makefile.ref).  Run in the build container:
+
+    make -f oracle/Makefile.ref && python tests/golden/make_golden.py
+
+Each fixture is an .npz (no pickles) holding the inputs -- alphabet, matrix, gaps, problem
+type, encoded pairs -- and the reference's outputs: per-pair scores and gradients, and for the
+hard sets also compute_grad() and weighted_grad(w) for seeded weights.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import ref  # noqa: E402
+from nwgrad_amd import workloads as wl  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def run_ref(w, precision, grad_mode, temperature=1.0, soft_precision="float32"):
+    sa, sb = w.strings()
+    b = ref.RefBatch(sa, sb, w.alphabet, w.matrix, w.gaps, double=(precision == "float64"),
+                     gap_model=w.gap_model, mode=w.mode, grad_mode=grad_mode, n_threads=1)
+    if grad_mode == "soft":
+        b.set_soft(temperature=temperature, precision=soft_precision)
+    total = b.score_and_grad()
+    scores = b.scores()
+    out = {"total": np.array(total), "scores": scores}
+    if grad_mode != "none":
+        mats, gaps = b.grads()
+        out["grads"] = np.concatenate([mats.reshape(len(scores), -1), gaps], axis=1)
+        out["compute_grad"] = b.compute_grad()
+        wts = wl.rng(w.name, "weights").normal(size=len(scores))
+        out["weights"] = wts
+        out["weighted_grad"] = b.weighted_grad(wts)
+    return out
+
+
+def save(name, w, precision, grad_mode, res, **extra):
+    path = os.path.join(OUT, name + ".npz")
+    np.savez_compressed(path, alphabet=np.array(w.alphabet), matrix=w.matrix, gaps=w.gaps,
+                        gap_model=np.array(w.gap_model), mode=np.array(w.mode), precision=np.array(precision),
+                        grad_mode=np.array(grad_mode), codes_a=w.codes_a, off_a=w.off_a, codes_b=w.codes_b,
+                        off_b=w.off_b, **res, **{k: np.array(v) for k, v in extra.items()})
+    print("wrote", path, len(w.off_a) - 1, "pairs")
+
+
+def edge_pairs(w):
+    """Append the reference tests' edge cases: empty A, empty B, both empty, length 1."""
+    extra_a = [[], [1], [], [0], [2, 2, 2]]
+    extra_b = [[3, 1], [], [], [0], [2]]
+    ca = [w.codes_a[w.off_a[i]:w.off_a[i + 1]] for i in range(w.n_pairs)]
+    cb = [w.codes_b[w.off_b[i]:w.off_b[i + 1]] for i in range(w.n_pairs)]
+    ca += [np.array(x, np.uint8) for x in extra_a]
+    cb += [np.array(x, np.uint8) for x in extra_b]
+    pa, oa = wl._pack(ca)
+    pb, ob = wl._pack(cb)
+    w.codes_a, w.off_a, w.codes_b, w.off_b = pa, oa, pb, ob
+    return w
+
+
+def main():
+    for gm in ("affine", "linear"):
+        for mode in ("global", "local", "glocal"):
+            for prec in ("float32", "float64"):
+                # DNA, +2/-1, small gaps: tie-heavy (the decisive case for the tie rules)
+                w = wl.random_pairs(f"golden_dna_{gm}_{mode}", 48, "ACGT", (0, 30), (0, 70),
+                                    gaps=(2.0, 1.0, 3.0, 1.0) if gm == "affine" else (0.0, 1.0, 0.0, 2.0),
+                                    gap_model=gm, mode=mode, related=0.5)
+                w = edge_pairs(w)
+                save(f"hard_dna_{gm}_{mode}_{prec}", w, prec, "hard", run_ref(w, prec, "hard"))
+                # protein, random integer matrix, non-integral gaps
+                w = wl.random_pairs(f"golden_prot_{gm}_{mode}", 32, wl.AA, (1, 120), (1, 200),
+                                    gaps=(10.5, 0.75, 9.25, 1.5), gap_model=gm, mode=mode, related=0.5)
+                save(f"hard_prot_{gm}_{mode}_{prec}", w, prec, "hard", run_ref(w, prec, "hard"))
+            # score-only
+            w = wl.random_pairs(f"golden_score_{gm}_{mode}", 32, "ACGT", (1, 30), (1, 70),
+                                gaps=(2.0, 1.0, 2.0, 1.0), gap_model=gm, mode=mode, related=0.5)
+            save(f"score_dna_{gm}_{mode}_float32", w, "float32", "none", run_ref(w, "float32", "none"))
+            # soft (float32 scaled pass and double)
+            w = wl.random_pairs(f"golden_soft_{gm}_{mode}", 24, wl.AA, (1, 90), (1, 120),
+                                gaps=(11.0, 1.0, 11.0, 1.0), gap_model=gm, mode=mode, related=0.5)
+            save(f"soft_prot_{gm}_{mode}_float32", w, "float32", "soft",
+                 run_ref(w, "float32", "soft", soft_precision="float32"))
+            save(f"soft_prot_{gm}_{mode}_float64", w, "float64", "soft",
+                 run_ref(w, "float64", "soft", soft_precision="double"))
+    # temperature != 1
+    w = wl.random_pairs("golden_soft_temp", 16, "ACGT", (1, 40), (1, 60), gaps=(3.0, 1.0, 3.0, 1.0),
+                        gap_model="affine", mode="global", related=0.5)
+    save("soft_dna_affine_global_float64_T2", w, "float64", "soft",
+         run_ref(w, "float64", "soft", temperature=2.0, soft_precision="double"), temperature=2.0)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,117 @@
+"""GPU parity of the hard and score-only paths: bit-exact against the reference's golden
+vectors and against the oracle, through the C ABI (via the engine and its module)."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from nwgrad_amd import workloads as wl
+from tests.gpu_util import batch, from_workload, grads_rows, say
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = sorted(p for p in glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz"))
+                if "/hard_" in p or "/score_" in p)
+
+
+def _golden_batch(z, fill):
+    return batch(str(z["alphabet"]), z["matrix"], z["gaps"], z["codes_a"], z["off_a"], z["codes_b"], z["off_b"],
+                 precision=str(z["precision"]), gap_model=str(z["gap_model"]), mode=str(z["mode"]),
+                 grad_mode=str(z["grad_mode"]), fill=fill)
+
+
+@pytest.mark.parametrize("fill", ["interpair", "striped"])
+@pytest.mark.parametrize("path", GOLDEN, ids=lambda p: os.path.basename(p)[:-4])
+def test_golden_bit_exact(gpu, path, fill):
+    z = np.load(path)
+    say("case", os.path.basename(path), fill)
+    b = _golden_batch(z, fill)
+    total = b.score_and_grad()
+    np.testing.assert_array_equal(b.scores(), z["scores"])
+    assert total == float(z["total"])
+    if str(z["grad_mode"]) == "hard":
+        np.testing.assert_array_equal(grads_rows(b), z["grads"])
+        cg = b.compute_grad().to_dict()
+        flat = np.concatenate([cg["matrix"].ravel(), [cg["gap_open_a"], cg["gap_extend_a"], cg["gap_open_b"],
+                                                       cg["gap_extend_b"]]])
+        np.testing.assert_array_equal(flat, z["compute_grad"])
+        wg = b.weighted_grad(np.ascontiguousarray(z["weights"])).to_dict()
+        flat = np.concatenate([wg["matrix"].ravel(), [wg["gap_open_a"], wg["gap_extend_a"], wg["gap_open_b"],
+                                                       wg["gap_extend_b"]]])
+        np.testing.assert_array_equal(flat, z["weighted_grad"])
+
+
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+def test_protein_global_hard_config(gpu, precision):
+    """4,096 protein pairs, lengths 100-300, global affine 11/1: every score and count."""
+    w = wl.protein_global_hard()
+    say("case protein_global_hard", precision, w.n_pairs, "pairs", w.cells, "cells")
+    s_ref, g_ref = oracle.hard(w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, precision=precision)
+    for fill in ("interpair", "striped"):
+        b = from_workload(w, precision=precision, fill=fill)
+        total = b.score_and_grad()
+        np.testing.assert_array_equal(b.scores(), s_ref)
+        np.testing.assert_array_equal(grads_rows(b), g_ref)
+        assert total == float(np.cumsum(s_ref)[-1])
+
+
+def test_dna_local_short_config(gpu):
+    """1,048,576 DNA pairs (half with a planted reverse complement), local linear: hard and
+    score-only, persistent and per-bucket launches, every pair against the oracle."""
+    w = wl.dna_local_short()
+    say("case dna_local_short", w.n_pairs, "pairs", w.cells, "cells")
+    s_ref, g_ref = oracle.hard(w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b)
+    for fill in ("interpair", "striped"):
+        b = from_workload(w, fill=fill)
+        b.score_and_grad()
+        np.testing.assert_array_equal(b.scores(), s_ref)
+        np.testing.assert_array_equal(grads_rows(b), g_ref)
+        np.testing.assert_array_equal(b.compute_grad().to_dict()["matrix"].ravel(),
+                                      oracle.weighted_sum(g_ref)[:16])
+        del b
+        b = from_workload(w, fill=fill, grad_mode="none")
+        b.score_and_grad()
+        np.testing.assert_array_equal(b.scores(), s_ref)
+
+
+@pytest.mark.parametrize("gm", ["affine", "linear"])
+@pytest.mark.parametrize("mode", ["global", "local", "glocal"])
+def test_ragged_lane_groups(gpu, gm, mode):
+    """Every lane-group width (len(B) from 0 to 1024) and ragged len(A) in one batch."""
+    r = wl.rng(f"ragged_{gm}_{mode}", "lens")
+    nb = np.concatenate([np.arange(0, 40), r.integers(0, 1025, size=200), [1023, 1024, 16, 17, 32, 33]])
+    na_ = r.integers(0, 300, size=len(nb))
+    A = [r.integers(0, 20, size=int(x)).astype(np.uint8) for x in na_]
+    B = [r.integers(0, 20, size=int(x)).astype(np.uint8) for x in nb]
+    ca, oa = wl._pack(A)
+    cb, ob = wl._pack(B)
+    mat = wl.random_matrix(r)
+    gaps = np.array([4.5, 0.5, 3.25, 1.0])
+    say("case ragged", gm, mode, len(nb), "pairs")
+    for precision in ("float32", "float64"):
+        s_ref, g_ref = oracle.hard(mat, gaps, ca, oa, cb, ob, gap_model=gm, mode=mode, precision=precision)
+        b = batch(wl.AA, mat, gaps, ca, oa, cb, ob, precision=precision, gap_model=gm, mode=mode)
+        b.score_and_grad()
+        np.testing.assert_array_equal(b.scores(), s_ref)
+        np.testing.assert_array_equal(grads_rows(b), g_ref)
+
+
+def test_set_params_and_errors(gpu):
+    w = wl.random_pairs("set_params", 50, "ACGT", (5, 30), (5, 60), gaps=(2, 1, 2, 1))
+    b = from_workload(w)
+    with pytest.raises(RuntimeError):
+        b.scores()
+    b.score_and_grad()
+    with pytest.raises(ValueError):
+        b.weighted_grad(np.ones(3))
+    from tests.gpu_util import params_of
+    mat2 = w.matrix * 2 - 0.5
+    b.set_params(params_of(w.alphabet, mat2, [3, 0.5, 1, 2]))
+    with pytest.raises(RuntimeError):
+        b.compute_grad()
+    b.score_and_grad()
+    s_ref, g_ref = oracle.hard(mat2, [3, 0.5, 1, 2], w.codes_a, w.off_a, w.codes_b, w.off_b)
+    np.testing.assert_array_equal(b.scores(), s_ref)
+    np.testing.assert_array_equal(grads_rows(b), g_ref)

@@ -1,0 +1,179 @@
+"""The oracle pinned: the reference's golden vectors, its known answers, brute-force
+enumeration and closed forms.  CPU only."""
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from nwgrad_amd import workloads as wl
+from tests import bruteforce as bf
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+
+def _args(z):
+    return (z["matrix"], z["gaps"], z["codes_a"], z["off_a"], z["codes_b"], z["off_b"])
+
+
+@pytest.mark.parametrize("path", [p for p in GOLDEN if "/hard_" in p or "/score_" in p],
+                         ids=lambda p: os.path.basename(p))
+def test_hard_matches_reference_golden(path):
+    """Bit-exact: scores, per-pair counts, compute_grad and weighted_grad of the reference."""
+    z = np.load(path)
+    gd = str(z["grad_mode"])
+    s, g = oracle.hard(*_args(z), gap_model=str(z["gap_model"]), mode=str(z["mode"]),
+                       precision=str(z["precision"]), with_grad=gd == "hard")
+    np.testing.assert_array_equal(s, z["scores"])
+    assert float(np.sum(s)) == float(z["total"]) or np.isclose(np.sum(s), z["total"])
+    if gd == "hard":
+        np.testing.assert_array_equal(g, z["grads"])
+        np.testing.assert_array_equal(oracle.weighted_sum(g), z["compute_grad"])
+        np.testing.assert_array_equal(oracle.weighted_sum(g, z["weights"]), z["weighted_grad"])
+
+
+@pytest.mark.parametrize("path", [p for p in GOLDEN if "/soft_" in p], ids=lambda p: os.path.basename(p))
+def test_soft_matches_reference_golden(path):
+    """Log-space double vs the reference's scaled pass: 1e-12 against its double pass, and
+    against its float32 pass within the reference's own float32 tolerance (2e-4 rel / 1e-5 abs,
+    tests/Python/test_soft_float32.py:27-31)."""
+    z = np.load(path)
+    T = float(z["temperature"]) if "temperature" in z else 1.0
+    s, g = oracle.soft_log(*_args(z), gap_model=str(z["gap_model"]), mode=str(z["mode"]), temperature=T)
+    if str(z["precision"]) == "float64":
+        np.testing.assert_allclose(s, z["scores"], rtol=1e-12, atol=1e-10)
+        np.testing.assert_allclose(g, z["grads"], rtol=1e-10, atol=1e-11)
+    else:
+        np.testing.assert_allclose(s, z["scores"], rtol=2e-4, atol=1e-5)
+        np.testing.assert_allclose(g, z["grads"], rtol=2e-4, atol=1e-5)
+
+
+def _enc(s, alphabet=wl.AA):
+    return np.array([alphabet.index(c) for c in s], np.uint8)
+
+
+def _one(a, b, matrix, gaps, **kw):
+    ca, cb = _enc(a), _enc(b)
+    return oracle.hard(matrix, gaps, ca, np.array([0, len(ca)]), cb, np.array([0, len(cb)]), **kw)
+
+
+UNIT = np.where(np.eye(20) > 0, 1.0, -1.0)
+
+
+def test_known_answers_reference_cpp():
+    """Literal counts from the reference's tests/cpp/test_gradient.cpp:35-154."""
+    na = 20
+    s, g = _one("ACDE", "ACDE", UNIT, [0, 1, 0, 1], gap_model="linear", mode="global")
+    assert s[0] == 4.0
+    for c in "ACDE":
+        assert g[0, wl.AA.index(c) * na + wl.AA.index(c)] == 1.0
+    assert g[0, :400].sum() == 4.0
+    s, g = _one("A", "", UNIT, [0, 1, 0, 1], gap_model="linear", mode="global")
+    assert g[0, :400].sum() == 0.0
+    s, g = _one("ADE", "ACDE", UNIT, [0, 0.5, 0, 0.5], gap_model="linear", mode="global")
+    assert g[0, :400].sum() == 3.0
+    s, g = _one("ADE", "MMMADEM", UNIT, [0, 1, 0, 1], gap_model="linear", mode="local")
+    assert g[0, :400].sum() == 3.0 and g[0, wl.AA.index("M") * na + wl.AA.index("M")] == 0.0
+    s, g = _one("AAAA", "CCCC", UNIT, [0, 1, 0, 1], gap_model="linear", mode="local")
+    assert g[0].sum() == 0.0 and s[0] == 0.0
+    s, g = _one("ACDE", "ACDE", UNIT, [10, 1, 10, 1], gap_model="affine", mode="global")
+    assert s[0] == 4.0 and g[0, :400].sum() == 4.0
+    s, g = _one("A", "", UNIT, [10, 1, 10, 1], gap_model="affine", mode="global")
+    assert g[0, :400].sum() == 0.0 and g[0, 402] == -1.0 and g[0, 403] == -1.0
+    s, g = _one("ADE", "MMMADEM", UNIT, [10, 1, 10, 1], gap_model="affine", mode="local")
+    assert g[0, :400].sum() == 3.0
+
+
+def test_soft_closed_forms():
+    """tests/Python/test_soft_gradient_known.py: single character global linear, Z = e^s + 2e^{-2g};
+    "AA" vs "AA" zero matrix: E[diagonal steps] = (2 + 6e^{-2g}) / (1 + 6e^{-2g} + 6e^{-4g})."""
+    r = np.random.default_rng(1)
+    mat = r.integers(-4, 5, size=(20, 20)).astype(float)
+    for a, b in [("A", "A"), ("A", "C"), ("W", "Y")]:
+        ge = 1.0
+        s = mat[wl.AA.index(a), wl.AA.index(b)]
+        denom = math.exp(s) + 2 * math.exp(-2 * ge)
+        ca, cb = _enc(a), _enc(b)
+        lz, g = oracle.soft_log(mat, [0, ge, 0, ge], ca, [0, 1], cb, [0, 1], gap_model="linear", mode="global")
+        assert lz[0] == pytest.approx(math.log(denom), rel=1e-12)
+        assert g[0, wl.AA.index(a) * 20 + wl.AA.index(b)] == pytest.approx(math.exp(s) / denom, rel=1e-12)
+    g_ = 0.7
+    ca = _enc("AA")
+    lz, g = oracle.soft_log(np.zeros((20, 20)), [0, g_, 0, g_], ca, [0, 2], ca, [0, 2], gap_model="linear")
+    e2, e4 = math.exp(-2 * g_), math.exp(-4 * g_)
+    assert g[0, 0] == pytest.approx((2 + 6 * e2) / (1 + 6 * e2 + 6 * e4), rel=1e-12)
+    assert lz[0] == pytest.approx(math.log(1 + 6 * e2 + 6 * e4), rel=1e-12)
+
+
+CASES = [(gm, mode) for gm in ("affine", "linear") for mode in ("global", "local", "glocal")]
+
+
+@pytest.mark.parametrize("gm,mode", CASES)
+def test_hard_vs_bruteforce(gm, mode):
+    """Best score equals exhaustive enumeration; where the best alignment is unique, so are its
+    counts; and always score == counts . params (the path the walk returns scores the score)."""
+    r = np.random.default_rng(hash((gm, mode)) % 2**32)
+    na = 4
+    for trial in range(60):
+        m, n = int(r.integers(0, 5)), int(r.integers(0, 5))
+        a = r.integers(0, na, size=m).astype(np.uint8)
+        b = r.integers(0, na, size=n).astype(np.uint8)
+        mat = np.round(r.normal(0, 2, size=(na, na)), 3)
+        gaps = np.round(r.uniform(0.1, 2.5, size=4), 3)
+        s, g = oracle.hard(mat, gaps, a, [0, m], b, [0, n], gap_model=gm, mode=mode, precision="float64")
+        best, uniq = bf.hard(list(a), list(b), mat, gaps, gap_model=gm, mode=mode)
+        assert s[0] == pytest.approx(best, abs=1e-9), (trial, m, n)
+        theta = np.concatenate([mat.ravel(), gaps])
+        assert float(g[0] @ theta) == pytest.approx(s[0], abs=1e-9)
+        if uniq is not None:
+            np.testing.assert_array_equal(g[0], uniq)
+
+
+@pytest.mark.parametrize("gm,mode", CASES)
+def test_soft_vs_bruteforce(gm, mode):
+    r = np.random.default_rng(hash(("soft", gm, mode)) % 2**32)
+    na = 3
+    for trial in range(25):
+        m, n = int(r.integers(0, 4)), int(r.integers(0, 4))
+        a = r.integers(0, na, size=m).astype(np.uint8)
+        b = r.integers(0, na, size=n).astype(np.uint8)
+        mat = r.normal(0, 1.5, size=(na, na))
+        gaps = r.uniform(0.2, 2.0, size=4)
+        T = float(r.choice([0.5, 1.0, 2.0]))
+        s, g = oracle.soft_log(mat, gaps, a, [0, m], b, [0, n], gap_model=gm, mode=mode, temperature=T)
+        es, eg = bf.soft(list(a), list(b), mat, gaps, T, gap_model=gm, mode=mode)
+        assert s[0] == pytest.approx(es, rel=1e-11, abs=1e-11)
+        np.testing.assert_allclose(g[0], eg, rtol=1e-9, atol=1e-11)
+
+
+def test_planted_mistake_is_caught():
+    """A plausible slip -- the open charged on extension moves too -- fails the identity check:
+    the checks above are not restatements of the oracle's own formula."""
+    mat = np.where(np.eye(4) > 0, 2.0, -1.0)
+    a = np.array([0, 1, 2, 3, 0, 1], np.uint8)
+    b = np.array([0, 1, 3, 0, 1], np.uint8)
+    s, g = oracle.hard(mat, [2, 1, 2, 1], a, [0, 6], b, [0, 5], precision="float64")
+    best, uniq = bf.hard(list(a), list(b), mat, [2, 1, 2, 1])
+    assert s[0] == best
+    wrong = g[0].copy()
+    wrong[16] = wrong[17]  # opens := extends: the slip
+    theta = np.concatenate([mat.ravel(), [2, 1, 2, 1]])
+    assert float(g[0] @ theta) == s[0]
+    assert float(wrong @ theta) != s[0] or np.array_equal(wrong, g[0])
+
+
+@pytest.mark.skipif(not oracle.have_ref(), reason="oracle/_ref not built (needs the reference sources)")
+def test_config_sample_matches_reference_engine():
+    """A seeded sample of the protein global affine configuration against the reference engine."""
+    from oracle import ref
+    w = wl.protein_global_hard(64)
+    sa, sb = w.strings()
+    for prec in ("float32", "float64"):
+        rb = ref.RefBatch(sa, sb, w.alphabet, w.matrix, w.gaps, double=prec == "float64", n_threads=2)
+        rb.score_and_grad()
+        s, g = oracle.hard(w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, precision=prec)
+        np.testing.assert_array_equal(s, rb.scores())
+        mats, gaps = rb.grads()
+        np.testing.assert_array_equal(g, np.concatenate([mats.reshape(64, -1), gaps], 1))
