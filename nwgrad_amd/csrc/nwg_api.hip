@@ -348,12 +348,18 @@ template <typename T>
 int launch_hard(nwg_batch* b) {
     nwg_ctx* ctx = b->ctx;
     const bool hard = b->prob.grad_mode == 1;
-    if (hard && b->table.n == 0) {
-        const int64_t slots = b->fill == 0 ? std::min<int64_t>((int64_t)b->tiles.size(),
-                                                               (int64_t)ctx->n_cu * kBlocksPerCU)
-                                           : (int64_t)b->tiles.size();
-        b->table_slots = std::max<int64_t>(slots, 1);
-        NWG_HIP(ctx, b->table.alloc((size_t)(b->table_slots * b->slot_dwords)));
+    if (hard) {
+        // One slot per persistent block, or one per tile for plain launches; grown (never
+        // shrunk) when the fill mode needs more slots than the table holds.
+        const int64_t slots = std::max<int64_t>(
+            b->fill == 0 ? std::min<int64_t>((int64_t)b->tiles.size(), (int64_t)ctx->n_cu * kBlocksPerCU)
+                         : (int64_t)b->tiles.size(),
+            1);
+        if (b->table.n < (size_t)(slots * b->slot_dwords)) {
+            NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            NWG_HIP(ctx, b->table.alloc((size_t)(slots * b->slot_dwords)));
+            b->table_slots = slots;
+        }
     }
     nwg::HardArgs<T> A{};
     A.codes_a = b->ca.p;
