@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <limits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -15,6 +17,7 @@
 #include "../../include/nwgrad_hip.h"
 #include "hard_kernels.hpp"
 #include "reduce_kernels.hpp"
+#include "soft_kernels.hpp"
 
 using nwg::TileDesc;
 
@@ -31,6 +34,7 @@ constexpr int kS = 16;                 // columns per lane of the lane-group ker
 constexpr int kMaxG = 64;              // lanes per pair at most: one wavefront
 constexpr int kMaxN = kS * kMaxG;      // longest B a lane group holds
 constexpr int kHistBytes = 16384;      // LDS histogram budget per wave (hard walk)
+constexpr int kSoftHistBytes = 13312;  // LDS histogram budget per wave (soft expected counts)
 constexpr int kBlocksPerCU = 8;        // persistent grid: waves per CU
 constexpr int64_t kSumBlock = 4096;    // the reference's WEIGHTED_GRAD_BLOCK
 
@@ -108,11 +112,14 @@ struct nwg_batch {
     DevPtr<uint32_t> table;
     int64_t slot_dwords = 0;
     int64_t table_slots = 0;
+    int64_t max_rows = 0;                 // max over tiles of max_m + G (soft row segments)
     int hist_slots = 1;
     int64_t cells = 0;
-    // soft plan scratch
+    // soft scratch: forward row segments and the exp'd profile, per working precision
     DevPtr<float> soft_f32;
     DevPtr<double> soft_f64;
+    DevPtr<float> prof_f32;
+    DevPtr<double> prof_f64;
     DevPtr<int32_t> soft_exp;
     DevPtr<uint8_t> soft_ok;
     // params
@@ -209,9 +216,13 @@ int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t
         if (codes_b[k] >= n_alpha) return fail(ctx, NWG_EINVAL, "nwgrad_hip: code out of range in B");
 
     // Planner: lane-group width from len(B), then tiles of 64/G pairs with similar len(A).
+    // The hard walk keeps uint16 match counts per pair slot in LDS, the soft pass double
+    // expected counts: the slot count, hence the narrowest lane group, follows the budget.
     const int nn = n_alpha * n_alpha;
+    const int hist_entry = prob->grad_mode == 1 ? 2 : prob->grad_mode == 2 ? 8 : 0;
+    const int hist_budget = prob->grad_mode == 2 ? kSoftHistBytes : kHistBytes;
     int gmin = 1;
-    while (gmin < kMaxG && (64 / gmin) * nn * 2 > kHistBytes) gmin <<= 1;
+    while (gmin < kMaxG && (64 / gmin) * nn * hist_entry > hist_budget) gmin <<= 1;
     b->hist_slots = 64 / gmin;
     std::vector<int32_t> gsel((size_t)N);
     for (int64_t i = 0; i < N; ++i) {
@@ -260,6 +271,7 @@ int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t
     std::vector<TileDesc> ptiles(b->tiles.size());
     for (size_t k = 0; k < ptiles.size(); ++k) ptiles[k] = b->tiles[(size_t)b->persist_order[k]];
     b->slot_dwords = max_steps * (kS / 4) * 64;
+    b->max_rows = max_steps + 1;
 
     // Device residency.
     NWG_HIP(ctx, b->ca.alloc((size_t)std::max<int64_t>(la, 1)));
@@ -338,11 +350,117 @@ int nwg_batch_set_soft(nwg_batch* b, double temperature, int precision) {
 
 namespace {
 
-template <typename T>
+template <typename SF> DevPtr<SF>& soft_table(nwg_batch* b);
+template <> DevPtr<float>& soft_table<float>(nwg_batch* b) { return b->soft_f32; }
+template <> DevPtr<double>& soft_table<double>(nwg_batch* b) { return b->soft_f64; }
+template <typename SF> DevPtr<SF>& soft_profile(nwg_batch* b);
+template <> DevPtr<float>& soft_profile<float>(nwg_batch* b) { return b->prof_f32; }
+template <> DevPtr<double>& soft_profile<double>(nwg_batch* b) { return b->prof_f64; }
+
+template <typename SF>
 int launch_soft(nwg_batch* b) {
-    return fail(b->ctx, NWG_EINVAL, "nwgrad_hip: the soft pass is not built yet");
+    nwg_ctx* ctx = b->ctx;
+    const int nn = b->na * b->na;
+    const bool lin = b->prob.gap_model == 0;
+    const double it = 1.0 / b->temperature;
+    // Weights in double, then cast once (aligner.hpp:6870-6885): the profile exp(s / T) and
+    // the gap weights exp(-(go + ge) / T), exp(-ge / T); linear gaps have no open.
+    std::vector<SF> ps((size_t)nn);
+    const SF big = std::numeric_limits<SF>::max();
+    for (int k = 0; k < nn; ++k) {
+        const double v = std::exp(b->matrix[(size_t)k] * it);
+        if (!(v <= (double)big)) return fail(ctx, NWG_ERANGE, "nwgrad: soft path: exp(score / T) overflows");
+        ps[(size_t)k] = static_cast<SF>(v);
+    }
+    const double ea = std::exp(-b->gaps[1] * it), eb = std::exp(-b->gaps[3] * it);
+    const double oa = lin ? ea : std::exp(-(b->gaps[0] + b->gaps[1]) * it);
+    const double ob = lin ? eb : std::exp(-(b->gaps[2] + b->gaps[3]) * it);
+    for (double v : {ea, eb, oa, ob})
+        if (!(v <= (double)big)) return fail(ctx, NWG_ERANGE, "nwgrad: soft path: a gap weight overflows");
+    DevPtr<SF>& dps = soft_profile<SF>(b);
+    if (dps.n < (size_t)nn) NWG_HIP(ctx, dps.alloc((size_t)nn));
+    NWG_HIP(ctx, hipMemcpyAsync(dps.p, ps.data(), sizeof(SF) * (size_t)nn, hipMemcpyHostToDevice, ctx->stream));
+    const int64_t slots = std::max<int64_t>(
+        std::min<int64_t>((int64_t)b->tiles.size(), (int64_t)ctx->n_cu * kBlocksPerCU), 1);
+    constexpr int P = nwg::SegLayout<kS>::padded;
+    const size_t fneed = (size_t)(slots * b->max_rows * P * 64);
+    DevPtr<SF>& ftab = soft_table<SF>(b);
+    if (ftab.n < fneed) {
+        NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        NWG_HIP(ctx, ftab.alloc(fneed));
+    }
+    if (b->soft_exp.n < (size_t)(slots * b->max_rows * 64))
+        NWG_HIP(ctx, b->soft_exp.alloc((size_t)(slots * b->max_rows * 64)));
+    if (b->soft_ok.n < (size_t)std::max<int64_t>(b->N, 1)) NWG_HIP(ctx, b->soft_ok.alloc((size_t)std::max<int64_t>(b->N, 1)));
+    nwg::SoftArgs<SF> A{};
+    A.codes_a = b->ca.p;
+    A.codes_b = b->cb.p;
+    A.off_a = b->oa.p;
+    A.off_b = b->ob.p;
+    A.order = b->order.p;
+    A.psub = dps.p;
+    A.na = b->na;
+    A.hist_slots = b->hist_slots;
+    A.oa = static_cast<SF>(oa);
+    A.ea = static_cast<SF>(ea);
+    A.ob = static_cast<SF>(ob);
+    A.eb = static_cast<SF>(eb);
+    A.linear = lin ? 1 : 0;
+    A.temperature = b->temperature;
+    A.scores = b->scores.p;
+    A.grads = b->grads.p;
+    A.ok = b->soft_ok.p;
+    A.ftab = ftab.p;
+    A.etab = b->soft_exp.p;
+    A.slot_rows = b->max_rows;
+    void (*kern)(nwg::SoftArgs<SF>) = b->prob.mode == 1   ? nwg::soft_kernel<SF, nwg::kLocal, kS>
+                                      : b->prob.mode == 2 ? nwg::soft_kernel<SF, nwg::kGlocal, kS>
+                                                          : nwg::soft_kernel<SF, nwg::kGlobal, kS>;
+    const size_t lds = (size_t)b->hist_slots * nn * sizeof(double) + (size_t)nn * sizeof(SF);
+    if (b->tiles.empty()) return NWG_OK;
+    if (b->fill == 0) {
+        NWG_HIP(ctx, hipMemsetAsync(b->queue.p, 0, sizeof(int32_t), ctx->stream));
+        A.tiles = b->d_tiles_p.p;
+        A.n_tiles = (int32_t)b->tiles.size();
+        A.queue = b->queue.p;
+        A.tile_base = 0;
+        hipLaunchKernelGGL(kern, dim3((unsigned)slots), dim3(64), lds, ctx->stream, A);
+        NWG_HIP(ctx, hipGetLastError());
+    } else {
+        // plain: one launch per lane-group bucket, at most `slots` tiles per launch
+        A.tiles = b->d_tiles.p;
+        A.queue = nullptr;
+        for (const Bucket& bk : b->buckets) {
+            for (int t0 = bk.tile_begin; t0 < bk.tile_end; t0 += (int)slots) {
+                const int t1 = std::min<int>(bk.tile_end, t0 + (int)slots);
+                A.tile_base = t0;
+                A.n_tiles = t1;
+                hipLaunchKernelGGL(kern, dim3(t1 - t0), dim3(64), lds, ctx->stream, A);
+                NWG_HIP(ctx, hipGetLastError());
+            }
+        }
+    }
+    return NWG_OK;
 }
-int soft_check_ok(nwg_batch*) { return NWG_OK; }
+
+// After a soft pass: every pair must have stayed in range (the reference reruns a float32
+// pair that does not in double, then fails with std::domain_error; aligner.hpp:6554-6586).
+int soft_check_ok(nwg_batch* b) {
+    nwg_ctx* ctx = b->ctx;
+    if (b->N == 0) return NWG_OK;
+    std::vector<uint8_t> ok((size_t)b->N);
+    NWG_HIP(ctx, hipMemcpy(ok.data(), b->soft_ok.p, (size_t)b->N, hipMemcpyDeviceToHost));
+    int64_t bad = 0, first = -1;
+    for (int64_t i = 0; i < b->N; ++i)
+        if (!ok[(size_t)i]) { if (first < 0) first = i; ++bad; }
+    if (bad) {
+        b->results_valid = false;
+        return fail(ctx, NWG_ERANGE,
+                    "nwgrad: soft path: %lld pair(s) (first: %lld) do not fit the scaled probability-space "
+                    "forward-backward (a weight or row overflowed)", (long long)bad, (long long)first);
+    }
+    return NWG_OK;
+}
 
 template <typename T>
 int launch_hard(nwg_batch* b) {

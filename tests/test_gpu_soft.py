@@ -1,0 +1,107 @@
+"""GPU parity of the soft path: expected counts and T*log Z against the oracle's log-space
+double forward-backward and the reference's golden vectors.
+
+Tolerance (the target is 1e-5 relative): scores |s - s_ref| <= 1e-5 * max(1, |s_ref|);
+gradients max_k |g_k - g_ref_k| <= 1e-5 * max_k |g_ref_k| per pair (the norm-relative form:
+an expected count far below the largest one is held to the same absolute error).  The
+float64 pass is held to 1e-10.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from nwgrad_amd import workloads as wl
+from tests.gpu_util import batch, from_workload, grads_rows, say
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = sorted(p for p in glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")) if "/soft_" in p)
+
+
+def rel_errors(s, g, s_ref, g_ref):
+    es = np.max(np.abs(s - s_ref) / np.maximum(1.0, np.abs(s_ref))) if len(s) else 0.0
+    scale = np.maximum(np.max(np.abs(g_ref), axis=1), 1e-300)
+    eg = np.max(np.max(np.abs(g - g_ref), axis=1) / scale) if len(s) else 0.0
+    return float(es), float(eg)
+
+
+def check(b, s_ref, g_ref, tol):
+    b.score_and_grad()
+    es, eg = rel_errors(b.scores(), grads_rows(b), s_ref, g_ref)
+    say(f"  max rel error: score {es:.3e}, gradient {eg:.3e} (tol {tol:g})")
+    assert es <= tol and eg <= tol, (es, eg)
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=lambda p: os.path.basename(p)[:-4])
+def test_golden(gpu, path):
+    z = np.load(path)
+    T = float(z["temperature"]) if "temperature" in z else 1.0
+    say("case", os.path.basename(path))
+    for prec, tol in (("float32", 1e-5), ("float64", 1e-10)):
+        if prec == "float64" and str(z["precision"]) == "float32":
+            continue
+        b = batch(str(z["alphabet"]), z["matrix"], z["gaps"], z["codes_a"], z["off_a"], z["codes_b"], z["off_b"],
+                  precision=prec, gap_model=str(z["gap_model"]), mode=str(z["mode"]), grad_mode="soft")
+        b.soft_temperature = T
+        # the golden file holds the reference's own float32 or double result; hold float32 to it at
+        # 1e-5, and both to the log-space oracle
+        s_log, g_log = oracle.soft_log(z["matrix"], z["gaps"], z["codes_a"], z["off_a"], z["codes_b"], z["off_b"],
+                                       gap_model=str(z["gap_model"]), mode=str(z["mode"]), temperature=T)
+        check(b, z["scores"], z["grads"], 1e-5 if prec == "float32" else max(tol, 1e-9))
+        check(b, s_log, g_log, tol)
+
+
+@pytest.mark.parametrize("gm", ["affine", "linear"])
+@pytest.mark.parametrize("mode", ["global", "local", "glocal"])
+def test_random_vs_oracle(gpu, gm, mode):
+    """Lane-group widths 1..64 (len(B) 0..1024), ragged len(A), related and unrelated pairs."""
+    r = wl.rng(f"soft_{gm}_{mode}", "lens")
+    nb = np.concatenate([np.arange(0, 24), r.integers(0, 1025, size=24), [1024, 16, 17, 256, 257]])
+    na_ = np.concatenate([r.integers(0, 200, size=len(nb) - 4), [0, 1, 2, 700]])
+    A = [r.integers(0, 20, size=int(x)).astype(np.uint8) for x in na_]
+    B = [r.integers(0, 20, size=int(x)).astype(np.uint8) for x in nb]
+    for k in range(0, len(A), 3):   # a third related
+        if len(A[k]):
+            B[k] = wl.mutate(r, A[k], 20, 0.2, 0.03, 3.0)
+    ca, oa = wl._pack(A)
+    cb, ob = wl._pack(B)
+    mat = wl.random_matrix(r)
+    gaps = np.array([11.0, 1.0, 10.0, 1.5])
+    say("case soft random", gm, mode, len(nb), "pairs")
+    s_ref, g_ref = oracle.soft_log(mat, gaps, ca, oa, cb, ob, gap_model=gm, mode=mode)
+    for prec, tol in (("float32", 1e-5), ("float64", 1e-10)):
+        for fill in ("interpair", "striped"):
+            b = batch(wl.AA, mat, gaps, ca, oa, cb, ob, precision=prec, gap_model=gm, mode=mode, grad_mode="soft",
+                      fill=fill)
+            check(b, s_ref, g_ref, tol)
+
+
+def test_protein_soft_homologs_sample(gpu):
+    """A seeded sample of the soft configuration (homologs and decoys, lengths 50-1500)."""
+    w = wl.protein_soft_homologs(256, 256)
+    idx = np.concatenate([np.arange(0, 256, 4), np.arange(256, 512, 4)])
+    w = w.subset(idx)
+    say("case protein_soft_homologs sample", w.n_pairs, "pairs", w.cells, "cells")
+    s_ref, g_ref = oracle.soft_log(w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b)
+    b = from_workload(w, grad_mode="soft")
+    check(b, s_ref, g_ref, 1e-5)
+    b = from_workload(w, grad_mode="soft", precision="float64")
+    check(b, s_ref, g_ref, 1e-10)
+
+
+def test_weighted_grad_soft(gpu):
+    w = wl.random_pairs("soft_weighted", 300, wl.AA, (30, 200), (30, 200), related=0.5)
+    s_ref, g_ref = oracle.soft_log(w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b)
+    b = from_workload(w, grad_mode="soft")
+    b.score_and_grad()
+    wts = wl.rng("soft_weighted", "w").normal(size=w.n_pairs)
+    got = b.weighted_grad(wts).to_dict()
+    flat = np.concatenate([got["matrix"].ravel(), [got["gap_open_a"], got["gap_extend_a"], got["gap_open_b"],
+                                                   got["gap_extend_b"]]])
+    want = oracle.weighted_sum(g_ref, wts)
+    assert np.max(np.abs(flat - want)) <= 1e-5 * np.max(np.abs(want))
+    # the device reduction equals the reference's blocked sum of the device rows, bit for bit
+    np.testing.assert_array_equal(flat, oracle.weighted_sum(grads_rows(b), wts))
