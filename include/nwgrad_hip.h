@@ -111,6 +111,10 @@ int nwg_batch_plan_info(nwg_batch *b, int64_t *n_tiles, int64_t *n_cells);
 /* Device-time of the last pass's kernels in milliseconds (HIP events on the context's
  * stream around the kernels only). */
 int nwg_batch_last_kernel_ms(nwg_batch *b, double *ms);
+/* Pairs of the last soft pass that the float32 pass refused (lost range or mass beyond
+ * 2^-20) and that were rerun in double -- the reference's kF32Refused fallback
+ * (aligner.hpp:6554-6572, batch_engine.hpp:1285-1306). */
+int nwg_batch_soft_reruns(nwg_batch *b, int64_t *n);
 
 #ifdef __cplusplus
 }

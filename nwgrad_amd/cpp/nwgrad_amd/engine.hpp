@@ -346,6 +346,11 @@ public:
         check_(nwg_batch_plan_info(b_, &t, &c));
         return c;
     }
+    int64_t soft_reruns() const {
+        int64_t n = 0;
+        check_(nwg_batch_soft_reruns(b_, &n));
+        return n;
+    }
     int64_t tiles() const {
         int64_t t = 0, c = 0;
         check_(nwg_batch_plan_info(b_, &t, &c));

@@ -4,10 +4,10 @@
 
 #include <algorithm>
 #include <cmath>
-#include <limits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <new>
 #include <numeric>
@@ -33,9 +33,10 @@ namespace {
 constexpr int kS = 16;                 // columns per lane of the lane-group kernels
 constexpr int kMaxG = 64;              // lanes per pair at most: one wavefront
 constexpr int kMaxN = kS * kMaxG;      // longest B a lane group holds
-constexpr int kHistBytes = 16384;      // LDS histogram budget per wave (hard walk)
-constexpr int kSoftHistBytes = 13312;  // LDS histogram budget per wave (soft expected counts)
-constexpr int kBlocksPerCU = 8;        // persistent grid: waves per CU
+constexpr int kHistBytes = 16384;      // LDS histogram budget per wave (hard walk, uint16 counts)
+constexpr int kSoftHistBytes = 13312;  // LDS histogram budget per wave (soft, double expected counts)
+constexpr int kBlocksPerCU = 8;        // persistent grid of the hard / score fills: waves per CU
+constexpr int kSoftBlocksPerCU = 4;    // the soft pass holds 256 VGPRs: one wave per SIMD
 constexpr int64_t kSumBlock = 4096;    // the reference's WEIGHTED_GRAD_BLOCK
 
 int fail(nwg_ctx* c, int code, const char* fmt, ...) {
@@ -76,7 +77,19 @@ struct DevPtr {
         else p = nullptr;
         return e;
     }
+    // grow to at least `count` elements (contents are not kept)
+    hipError_t reserve(size_t count) { return n >= count && p ? hipSuccess : alloc(count); }
 };
+
+// Persistent grid: as many one-wave blocks as the kernel keeps resident per CU (registers
+// and LDS permitting), times the CUs -- never more blocks than tiles.
+template <typename K>
+int64_t resident_blocks(nwg_ctx* ctx, K kern, size_t lds, int fallback_per_cu) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds) != hipSuccess || per_cu <= 0)
+        per_cu = fallback_per_cu;
+    return (int64_t)per_cu * ctx->n_cu;
+}
 
 int next_pow2(int x) {
     int g = 1;
@@ -84,12 +97,25 @@ int next_pow2(int x) {
     return g;
 }
 
-}  // namespace
-
 struct Bucket {
     int G;
     int tile_begin, tile_end;  // tiles [begin, end) of the plain order
 };
+
+// The planner's output for a set of pairs: tiles of 64/G pairs with one lane-group width
+// G each (G from len(B), at least gmin), pairs sorted by len(A) within a width so a
+// tile's rows are even; the plain order (bucket by bucket) and the persistent order
+// (longest tile first) of the same tiles.
+struct Plan {
+    std::vector<TileDesc> tiles;
+    std::vector<Bucket> buckets;
+    std::vector<int32_t> order;
+    int64_t max_steps = 1;   // max over tiles of max_m + G - 1
+    DevPtr<int32_t> d_order;
+    DevPtr<TileDesc> d_tiles, d_ptiles;
+};
+
+}  // namespace
 
 struct nwg_batch {
     nwg_ctx* ctx = nullptr;
@@ -101,37 +127,29 @@ struct nwg_batch {
     DevPtr<uint8_t> ca, cb;
     DevPtr<int64_t> oa, ob;
     DevPtr<double> scores, grads;
-    // plan
-    std::vector<TileDesc> tiles;          // plain (bucket) order
-    std::vector<Bucket> buckets;
-    std::vector<int32_t> persist_order;   // tile indices, longest first
-    DevPtr<int32_t> order;                // pair indices, tile-major
-    DevPtr<TileDesc> d_tiles;             // plain order
-    DevPtr<TileDesc> d_tiles_p;           // persistent order
-    DevPtr<int32_t> queue;
-    DevPtr<uint32_t> table;
-    int64_t slot_dwords = 0;
-    int64_t table_slots = 0;
-    int64_t max_rows = 0;                 // max over tiles of max_m + G (soft row segments)
-    int hist_slots = 1;
+    // plans: every pair, and the pairs a float32 soft pass refused
+    int gmin = 1, hist_slots = 64;
+    Plan plan, rerun;
     int64_t cells = 0;
-    // soft scratch: forward row segments and the exp'd profile, per working precision
-    DevPtr<float> soft_f32;
-    DevPtr<double> soft_f64;
-    DevPtr<float> prof_f32;
-    DevPtr<double> prof_f64;
-    DevPtr<int32_t> soft_exp;
+    DevPtr<int32_t> queue;
+    // hard: pointer codes per wave slot
+    DevPtr<uint32_t> table;
+    // soft: forward row segments, exponents, maxima, scale-up factors (per precision)
+    DevPtr<float> ft32, mt32, lt32, pf32;
+    DevPtr<double> ft64, mt64, lt64, pf64;
+    DevPtr<int32_t> etab;
     DevPtr<uint8_t> soft_ok;
+    std::vector<uint8_t> host_ok;
+    int64_t soft_reruns = 0;
     // params
     std::vector<double> matrix;
     double gaps[4] = {0, 0, 0, 0};
     bool have_params = false;
     DevPtr<float> sub_f32;
     DevPtr<double> sub_f64;
-    DevPtr<double> soft_w;                // exp'd soft weights (double master)
     // reduction scratch
     DevPtr<double> partial, wdev, outdev;
-    double* host_scores = nullptr;        // pinned
+    double* host_scores = nullptr;   // pinned
     // settings
     int fill = 0;
     double temperature = 1.0;
@@ -147,19 +165,83 @@ struct nwg_batch {
     }
 };
 
+namespace {
+
+int build_plan(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& pl) {
+    nwg_ctx* ctx = b->ctx;
+    pl.tiles.clear();
+    pl.buckets.clear();
+    pl.max_steps = 1;
+    const int64_t N = (int64_t)pairs.size();
+    std::vector<int32_t> gsel((size_t)b->N > 0 ? (size_t)b->N : 1);
+    for (int32_t i : pairs) gsel[(size_t)i] = std::max(next_pow2((b->nlen[(size_t)i] + kS - 1) / kS), b->gmin);
+    pl.order = pairs;
+    std::stable_sort(pl.order.begin(), pl.order.end(), [&](int32_t x, int32_t y) {
+        if (gsel[(size_t)x] != gsel[(size_t)y]) return gsel[(size_t)x] < gsel[(size_t)y];
+        return b->mlen[(size_t)x] > b->mlen[(size_t)y];
+    });
+    int64_t pos = 0;
+    while (pos < N) {
+        const int G = gsel[(size_t)pl.order[(size_t)pos]];
+        Bucket bk{G, (int)pl.tiles.size(), 0};
+        while (pos < N && gsel[(size_t)pl.order[(size_t)pos]] == G) {
+            const int per = 64 / G;
+            TileDesc td{};
+            td.first = (int32_t)pos;
+            td.G = (int16_t)G;
+            int np = 0, mm = 0, mn = 0;
+            while (np < per && pos < N && gsel[(size_t)pl.order[(size_t)pos]] == G) {
+                mm = std::max(mm, b->mlen[(size_t)pl.order[(size_t)pos]]);
+                mn = std::max(mn, b->nlen[(size_t)pl.order[(size_t)pos]]);
+                ++np;
+                ++pos;
+            }
+            td.npairs = (int16_t)np;
+            td.max_m = mm;
+            td.max_n = mn;
+            pl.max_steps = std::max<int64_t>(pl.max_steps, (int64_t)mm + G - 1);
+            pl.tiles.push_back(td);
+        }
+        bk.tile_end = (int)pl.tiles.size();
+        pl.buckets.push_back(bk);
+    }
+    std::vector<TileDesc> ptiles(pl.tiles);
+    std::stable_sort(ptiles.begin(), ptiles.end(), [](const TileDesc& a, const TileDesc& c) {
+        return (int64_t)(a.max_m + a.G - 1) * a.G > (int64_t)(c.max_m + c.G - 1) * c.G;
+    });
+    NWG_HIP(ctx, pl.d_order.reserve(std::max<size_t>(pl.order.size(), 1)));
+    NWG_HIP(ctx, pl.d_tiles.reserve(std::max<size_t>(pl.tiles.size(), 1)));
+    NWG_HIP(ctx, pl.d_ptiles.reserve(std::max<size_t>(pl.tiles.size(), 1)));
+    if (N)
+        NWG_HIP(ctx, hipMemcpyAsync(pl.d_order.p, pl.order.data(), sizeof(int32_t) * (size_t)N, hipMemcpyHostToDevice,
+                                    ctx->stream));
+    if (!pl.tiles.empty()) {
+        NWG_HIP(ctx, hipMemcpyAsync(pl.d_tiles.p, pl.tiles.data(), sizeof(TileDesc) * pl.tiles.size(),
+                                    hipMemcpyHostToDevice, ctx->stream));
+        NWG_HIP(ctx, hipMemcpyAsync(pl.d_ptiles.p, ptiles.data(), sizeof(TileDesc) * ptiles.size(),
+                                    hipMemcpyHostToDevice, ctx->stream));
+    }
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return NWG_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int nwg_init(int device, nwg_ctx** out) {
     if (!out) return NWG_EINVAL;
     *out = nullptr;
     int count = 0;
-    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
-        return NWG_EDEVICE;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return NWG_EDEVICE;
     if (device < 0 || device >= count) return NWG_EINVAL;
     auto* c = new (std::nothrow) nwg_ctx();
     if (!c) return NWG_ENOMEM;
     c->device = device;
-    if (hipSetDevice(device) != hipSuccess) { delete c; return NWG_EDEVICE; }
+    if (hipSetDevice(device) != hipSuccess) {
+        delete c;
+        return NWG_EDEVICE;
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         c->n_cu = prop.multiProcessorCount;
@@ -181,8 +263,8 @@ int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t
                      const int64_t* off_a, const uint8_t* codes_b, const int64_t* off_b, nwg_batch** out) {
     if (!ctx || !prob || !out) return NWG_EINVAL;
     *out = nullptr;
-    if (prob->precision < 0 || prob->precision > 1 || prob->gap_model < 0 || prob->gap_model > 1 ||
-        prob->mode < 0 || prob->mode > 2 || prob->grad_mode < 0 || prob->grad_mode > 2)
+    if (prob->precision < 0 || prob->precision > 1 || prob->gap_model < 0 || prob->gap_model > 1 || prob->mode < 0 ||
+        prob->mode > 2 || prob->grad_mode < 0 || prob->grad_mode > 2)
         return fail(ctx, NWG_EINVAL, "nwgrad_hip: invalid problem type");
     if (n_alpha < 1 || n_alpha > 64)
         return fail(ctx, NWG_EINVAL, "nwgrad_hip: alphabet size %d outside [1, 64]", n_alpha);
@@ -202,8 +284,8 @@ int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t
         if (m < 0 || n < 0) return fail(ctx, NWG_EINVAL, "nwgrad_hip: offsets must be non-decreasing");
         if (n > kMaxN)
             return fail(ctx, NWG_EINVAL,
-                        "nwgrad_hip: pair %lld has len(B) = %lld; this build's lane-group kernels hold "
-                        "at most %d columns",
+                        "nwgrad_hip: pair %lld has len(B) = %lld; this build's lane-group kernels hold at most %d "
+                        "columns",
                         (long long)i, (long long)n, kMaxN);
         if (m > (1 << 24)) return fail(ctx, NWG_EINVAL, "nwgrad_hip: pair %lld is too long", (long long)i);
         b->mlen[(size_t)i] = (int32_t)m;
@@ -215,7 +297,6 @@ int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t
     for (int64_t k = 0; k < lb; ++k)
         if (codes_b[k] >= n_alpha) return fail(ctx, NWG_EINVAL, "nwgrad_hip: code out of range in B");
 
-    // Planner: lane-group width from len(B), then tiles of 64/G pairs with similar len(A).
     // The hard walk keeps uint16 match counts per pair slot in LDS, the soft pass double
     // expected counts: the slot count, hence the narrowest lane group, follows the budget.
     const int nn = n_alpha * n_alpha;
@@ -223,64 +304,13 @@ int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t
     const int hist_budget = prob->grad_mode == 2 ? kSoftHistBytes : kHistBytes;
     int gmin = 1;
     while (gmin < kMaxG && (64 / gmin) * nn * hist_entry > hist_budget) gmin <<= 1;
+    b->gmin = gmin;
     b->hist_slots = 64 / gmin;
-    std::vector<int32_t> gsel((size_t)N);
-    for (int64_t i = 0; i < N; ++i) {
-        int G = next_pow2((b->nlen[(size_t)i] + kS - 1) / kS);
-        gsel[(size_t)i] = std::max(G, gmin);
-    }
-    std::vector<int32_t> idx((size_t)N);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) {
-        if (gsel[(size_t)x] != gsel[(size_t)y]) return gsel[(size_t)x] < gsel[(size_t)y];
-        return b->mlen[(size_t)x] > b->mlen[(size_t)y];
-    });
-    int64_t pos = 0;
-    int64_t max_steps = 1;
-    while (pos < N) {
-        const int G = gsel[(size_t)idx[(size_t)pos]];
-        Bucket bk{G, (int)b->tiles.size(), 0};
-        while (pos < N && gsel[(size_t)idx[(size_t)pos]] == G) {
-            const int per = 64 / G;
-            TileDesc td{};
-            td.first = (int32_t)pos;
-            td.G = (int16_t)G;
-            int np = 0, mm = 0, mn = 0;
-            while (np < per && pos < N && gsel[(size_t)idx[(size_t)pos]] == G) {
-                mm = std::max(mm, b->mlen[(size_t)idx[(size_t)pos]]);
-                mn = std::max(mn, b->nlen[(size_t)idx[(size_t)pos]]);
-                ++np;
-                ++pos;
-            }
-            td.npairs = (int16_t)np;
-            td.max_m = mm;
-            td.max_n = mn;
-            max_steps = std::max<int64_t>(max_steps, (int64_t)mm + G - 1);
-            b->tiles.push_back(td);
-        }
-        bk.tile_end = (int)b->tiles.size();
-        b->buckets.push_back(bk);
-    }
-    b->persist_order.resize(b->tiles.size());
-    std::iota(b->persist_order.begin(), b->persist_order.end(), 0);
-    std::stable_sort(b->persist_order.begin(), b->persist_order.end(), [&](int32_t x, int32_t y) {
-        const TileDesc& a = b->tiles[(size_t)x];
-        const TileDesc& c = b->tiles[(size_t)y];
-        return (int64_t)(a.max_m + a.G - 1) * a.G > (int64_t)(c.max_m + c.G - 1) * c.G;
-    });
-    std::vector<TileDesc> ptiles(b->tiles.size());
-    for (size_t k = 0; k < ptiles.size(); ++k) ptiles[k] = b->tiles[(size_t)b->persist_order[k]];
-    b->slot_dwords = max_steps * (kS / 4) * 64;
-    b->max_rows = max_steps + 1;
 
-    // Device residency.
     NWG_HIP(ctx, b->ca.alloc((size_t)std::max<int64_t>(la, 1)));
     NWG_HIP(ctx, b->cb.alloc((size_t)std::max<int64_t>(lb, 1)));
     NWG_HIP(ctx, b->oa.alloc((size_t)N + 1));
     NWG_HIP(ctx, b->ob.alloc((size_t)N + 1));
-    NWG_HIP(ctx, b->order.alloc((size_t)std::max<int64_t>(N, 1)));
-    NWG_HIP(ctx, b->d_tiles.alloc(std::max<size_t>(b->tiles.size(), 1)));
-    NWG_HIP(ctx, b->d_tiles_p.alloc(std::max<size_t>(b->tiles.size(), 1)));
     NWG_HIP(ctx, b->queue.alloc(4));
     NWG_HIP(ctx, b->scores.alloc((size_t)std::max<int64_t>(N, 1)));
     if (prob->grad_mode != 0) NWG_HIP(ctx, b->grads.alloc((size_t)std::max<int64_t>(N, 1) * (nn + 4)));
@@ -288,13 +318,10 @@ int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t
     if (lb) NWG_HIP(ctx, hipMemcpy(b->cb.p, codes_b, (size_t)lb, hipMemcpyHostToDevice));
     NWG_HIP(ctx, hipMemcpy(b->oa.p, off_a, sizeof(int64_t) * (size_t)(N + 1), hipMemcpyHostToDevice));
     NWG_HIP(ctx, hipMemcpy(b->ob.p, off_b, sizeof(int64_t) * (size_t)(N + 1), hipMemcpyHostToDevice));
-    if (N) NWG_HIP(ctx, hipMemcpy(b->order.p, idx.data(), sizeof(int32_t) * (size_t)N, hipMemcpyHostToDevice));
-    if (!b->tiles.empty()) {
-        NWG_HIP(ctx, hipMemcpy(b->d_tiles.p, b->tiles.data(), sizeof(TileDesc) * b->tiles.size(),
-                               hipMemcpyHostToDevice));
-        NWG_HIP(ctx, hipMemcpy(b->d_tiles_p.p, ptiles.data(), sizeof(TileDesc) * ptiles.size(),
-                               hipMemcpyHostToDevice));
-    }
+    std::vector<int32_t> all((size_t)N);
+    std::iota(all.begin(), all.end(), 0);
+    int rc = build_plan(b.get(), all, b->plan);
+    if (rc) return rc;
     NWG_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&b->host_scores),
                                sizeof(double) * (size_t)std::max<int64_t>(N, 1), hipHostMallocDefault));
     NWG_HIP(ctx, hipEventCreate(&b->ev0));
@@ -315,10 +342,10 @@ int nwg_batch_set_params(nwg_batch* b, const double* matrix, const double* gaps4
     if (b->prob.precision == 0) {
         std::vector<float> s((size_t)nn);
         for (int k = 0; k < nn; ++k) s[(size_t)k] = static_cast<float>(matrix[k]);
-        if (!b->sub_f32.p) NWG_HIP(ctx, b->sub_f32.alloc((size_t)nn));
+        NWG_HIP(ctx, b->sub_f32.reserve((size_t)nn));
         NWG_HIP(ctx, hipMemcpy(b->sub_f32.p, s.data(), sizeof(float) * (size_t)nn, hipMemcpyHostToDevice));
     } else {
-        if (!b->sub_f64.p) NWG_HIP(ctx, b->sub_f64.alloc((size_t)nn));
+        NWG_HIP(ctx, b->sub_f64.reserve((size_t)nn));
         NWG_HIP(ctx, hipMemcpy(b->sub_f64.p, matrix, sizeof(double) * (size_t)nn, hipMemcpyHostToDevice));
     }
     b->have_params = true;
@@ -350,154 +377,11 @@ int nwg_batch_set_soft(nwg_batch* b, double temperature, int precision) {
 
 namespace {
 
-template <typename SF> DevPtr<SF>& soft_table(nwg_batch* b);
-template <> DevPtr<float>& soft_table<float>(nwg_batch* b) { return b->soft_f32; }
-template <> DevPtr<double>& soft_table<double>(nwg_batch* b) { return b->soft_f64; }
-template <typename SF> DevPtr<SF>& soft_profile(nwg_batch* b);
-template <> DevPtr<float>& soft_profile<float>(nwg_batch* b) { return b->prof_f32; }
-template <> DevPtr<double>& soft_profile<double>(nwg_batch* b) { return b->prof_f64; }
-
-template <typename SF>
-int launch_soft(nwg_batch* b) {
-    nwg_ctx* ctx = b->ctx;
-    const int nn = b->na * b->na;
-    const bool lin = b->prob.gap_model == 0;
-    const double it = 1.0 / b->temperature;
-    // Weights in double, then cast once (aligner.hpp:6870-6885): the profile exp(s / T) and
-    // the gap weights exp(-(go + ge) / T), exp(-ge / T); linear gaps have no open.
-    std::vector<SF> ps((size_t)nn);
-    const SF big = std::numeric_limits<SF>::max();
-    for (int k = 0; k < nn; ++k) {
-        const double v = std::exp(b->matrix[(size_t)k] * it);
-        if (!(v <= (double)big)) return fail(ctx, NWG_ERANGE, "nwgrad: soft path: exp(score / T) overflows");
-        ps[(size_t)k] = static_cast<SF>(v);
-    }
-    const double ea = std::exp(-b->gaps[1] * it), eb = std::exp(-b->gaps[3] * it);
-    const double oa = lin ? ea : std::exp(-(b->gaps[0] + b->gaps[1]) * it);
-    const double ob = lin ? eb : std::exp(-(b->gaps[2] + b->gaps[3]) * it);
-    for (double v : {ea, eb, oa, ob})
-        if (!(v <= (double)big)) return fail(ctx, NWG_ERANGE, "nwgrad: soft path: a gap weight overflows");
-    DevPtr<SF>& dps = soft_profile<SF>(b);
-    if (dps.n < (size_t)nn) NWG_HIP(ctx, dps.alloc((size_t)nn));
-    NWG_HIP(ctx, hipMemcpyAsync(dps.p, ps.data(), sizeof(SF) * (size_t)nn, hipMemcpyHostToDevice, ctx->stream));
-    const int64_t slots = std::max<int64_t>(
-        std::min<int64_t>((int64_t)b->tiles.size(), (int64_t)ctx->n_cu * kBlocksPerCU), 1);
-    constexpr int P = nwg::SegLayout<kS>::padded;
-    const size_t fneed = (size_t)(slots * b->max_rows * P * 64);
-    DevPtr<SF>& ftab = soft_table<SF>(b);
-    if (ftab.n < fneed) {
-        NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        NWG_HIP(ctx, ftab.alloc(fneed));
-    }
-    if (b->soft_exp.n < (size_t)(slots * b->max_rows * 64))
-        NWG_HIP(ctx, b->soft_exp.alloc((size_t)(slots * b->max_rows * 64)));
-    if (b->soft_ok.n < (size_t)std::max<int64_t>(b->N, 1)) NWG_HIP(ctx, b->soft_ok.alloc((size_t)std::max<int64_t>(b->N, 1)));
-    nwg::SoftArgs<SF> A{};
-    A.codes_a = b->ca.p;
-    A.codes_b = b->cb.p;
-    A.off_a = b->oa.p;
-    A.off_b = b->ob.p;
-    A.order = b->order.p;
-    A.psub = dps.p;
-    A.na = b->na;
-    A.hist_slots = b->hist_slots;
-    A.oa = static_cast<SF>(oa);
-    A.ea = static_cast<SF>(ea);
-    A.ob = static_cast<SF>(ob);
-    A.eb = static_cast<SF>(eb);
-    A.linear = lin ? 1 : 0;
-    A.temperature = b->temperature;
-    A.scores = b->scores.p;
-    A.grads = b->grads.p;
-    A.ok = b->soft_ok.p;
-    A.ftab = ftab.p;
-    A.etab = b->soft_exp.p;
-    A.slot_rows = b->max_rows;
-    void (*kern)(nwg::SoftArgs<SF>) = b->prob.mode == 1   ? nwg::soft_kernel<SF, nwg::kLocal, kS>
-                                      : b->prob.mode == 2 ? nwg::soft_kernel<SF, nwg::kGlocal, kS>
-                                                          : nwg::soft_kernel<SF, nwg::kGlobal, kS>;
-    const size_t lds = (size_t)b->hist_slots * nn * sizeof(double) + (size_t)nn * sizeof(SF);
-    if (b->tiles.empty()) return NWG_OK;
-    if (b->fill == 0) {
-        NWG_HIP(ctx, hipMemsetAsync(b->queue.p, 0, sizeof(int32_t), ctx->stream));
-        A.tiles = b->d_tiles_p.p;
-        A.n_tiles = (int32_t)b->tiles.size();
-        A.queue = b->queue.p;
-        A.tile_base = 0;
-        hipLaunchKernelGGL(kern, dim3((unsigned)slots), dim3(64), lds, ctx->stream, A);
-        NWG_HIP(ctx, hipGetLastError());
-    } else {
-        // plain: one launch per lane-group bucket, at most `slots` tiles per launch
-        A.tiles = b->d_tiles.p;
-        A.queue = nullptr;
-        for (const Bucket& bk : b->buckets) {
-            for (int t0 = bk.tile_begin; t0 < bk.tile_end; t0 += (int)slots) {
-                const int t1 = std::min<int>(bk.tile_end, t0 + (int)slots);
-                A.tile_base = t0;
-                A.n_tiles = t1;
-                hipLaunchKernelGGL(kern, dim3(t1 - t0), dim3(64), lds, ctx->stream, A);
-                NWG_HIP(ctx, hipGetLastError());
-            }
-        }
-    }
-    return NWG_OK;
-}
-
-// After a soft pass: every pair must have stayed in range (the reference reruns a float32
-// pair that does not in double, then fails with std::domain_error; aligner.hpp:6554-6586).
-int soft_check_ok(nwg_batch* b) {
-    nwg_ctx* ctx = b->ctx;
-    if (b->N == 0) return NWG_OK;
-    std::vector<uint8_t> ok((size_t)b->N);
-    NWG_HIP(ctx, hipMemcpy(ok.data(), b->soft_ok.p, (size_t)b->N, hipMemcpyDeviceToHost));
-    int64_t bad = 0, first = -1;
-    for (int64_t i = 0; i < b->N; ++i)
-        if (!ok[(size_t)i]) { if (first < 0) first = i; ++bad; }
-    if (bad) {
-        b->results_valid = false;
-        return fail(ctx, NWG_ERANGE,
-                    "nwgrad: soft path: %lld pair(s) (first: %lld) do not fit the scaled probability-space "
-                    "forward-backward (a weight or row overflowed)", (long long)bad, (long long)first);
-    }
-    return NWG_OK;
-}
-
+// ── hard and score-only fills ───────────────────────────────────────────────
 template <typename T>
-int launch_hard(nwg_batch* b) {
+int launch_hard(nwg_batch* b, const Plan& pl) {
     nwg_ctx* ctx = b->ctx;
     const bool hard = b->prob.grad_mode == 1;
-    if (hard) {
-        // One slot per persistent block, or one per tile for plain launches; grown (never
-        // shrunk) when the fill mode needs more slots than the table holds.
-        const int64_t slots = std::max<int64_t>(
-            b->fill == 0 ? std::min<int64_t>((int64_t)b->tiles.size(), (int64_t)ctx->n_cu * kBlocksPerCU)
-                         : (int64_t)b->tiles.size(),
-            1);
-        if (b->table.n < (size_t)(slots * b->slot_dwords)) {
-            NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            NWG_HIP(ctx, b->table.alloc((size_t)(slots * b->slot_dwords)));
-            b->table_slots = slots;
-        }
-    }
-    nwg::HardArgs<T> A{};
-    A.codes_a = b->ca.p;
-    A.codes_b = b->cb.p;
-    A.off_a = b->oa.p;
-    A.off_b = b->ob.p;
-    A.order = b->order.p;
-    A.n_tiles = (int32_t)b->tiles.size();
-    A.sub = std::is_same<T, float>::value ? reinterpret_cast<const T*>(b->sub_f32.p)
-                                           : reinterpret_cast<const T*>(b->sub_f64.p);
-    A.na = b->na;
-    A.hist_slots = b->hist_slots;
-    A.go_a = static_cast<T>(b->gaps[0]);
-    A.ge_a = static_cast<T>(b->gaps[1]);
-    A.go_b = static_cast<T>(b->gaps[2]);
-    A.ge_b = static_cast<T>(b->gaps[3]);
-    A.scores = b->scores.p;
-    A.grads = b->grads.p;
-    A.table = b->table.p;
-    A.slot_dwords = b->slot_dwords;
     const int nn = b->na * b->na;
     const size_t lds = (size_t)((nn * (int)sizeof(T) + 15) & ~15) + (hard ? (size_t)b->hist_slots * nn * 2 : 0);
     const int gm = b->prob.gap_model, am = b->prob.mode;
@@ -510,24 +394,51 @@ int launch_hard(nwg_batch* b) {
     using BT = std::true_type;
     using BF = std::false_type;
     void (*kern)(nwg::HardArgs<T>) = nullptr;
-#define NWG_PICK(GMV, AMV)                                                        \
+#define NWG_PICK(GMV, AMV) \
     if (gm == GMV::value && am == AMV::value) kern = hard ? pick(GMV{}, AMV{}, BT{}) : pick(GMV{}, AMV{}, BF{});
     NWG_PICK(I0, I0) NWG_PICK(I0, I1) NWG_PICK(I0, I2) NWG_PICK(I1, I0) NWG_PICK(I1, I1) NWG_PICK(I1, I2)
 #undef NWG_PICK
     if (!kern) return fail(ctx, NWG_EINVAL, "nwgrad_hip: unsupported problem type");
-    if (b->tiles.empty()) return NWG_OK;
+    const int64_t slot_dwords = pl.max_steps * (kS / 4) * 64;
+    const int64_t slots = std::max<int64_t>(
+        b->fill == 0 ? std::min<int64_t>((int64_t)pl.tiles.size(), resident_blocks(ctx, kern, lds, kBlocksPerCU))
+                     : (int64_t)pl.tiles.size(),
+        1);
+    if (hard && b->table.n < (size_t)(slots * slot_dwords)) {
+        NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        NWG_HIP(ctx, b->table.alloc((size_t)(slots * slot_dwords)));
+    }
+    nwg::HardArgs<T> A{};
+    A.codes_a = b->ca.p;
+    A.codes_b = b->cb.p;
+    A.off_a = b->oa.p;
+    A.off_b = b->ob.p;
+    A.order = pl.d_order.p;
+    A.sub = std::is_same<T, float>::value ? reinterpret_cast<const T*>(b->sub_f32.p)
+                                           : reinterpret_cast<const T*>(b->sub_f64.p);
+    A.na = b->na;
+    A.hist_slots = b->hist_slots;
+    A.go_a = static_cast<T>(b->gaps[0]);
+    A.ge_a = static_cast<T>(b->gaps[1]);
+    A.go_b = static_cast<T>(b->gaps[2]);
+    A.ge_b = static_cast<T>(b->gaps[3]);
+    A.scores = b->scores.p;
+    A.grads = b->grads.p;
+    A.table = b->table.p;
+    A.slot_dwords = slot_dwords;
+    if (pl.tiles.empty()) return NWG_OK;
     if (b->fill == 0) {
         NWG_HIP(ctx, hipMemsetAsync(b->queue.p, 0, sizeof(int32_t), ctx->stream));
-        A.tiles = b->d_tiles_p.p;
+        A.tiles = pl.d_ptiles.p;
+        A.n_tiles = (int32_t)pl.tiles.size();
         A.queue = b->queue.p;
         A.tile_base = 0;
-        const int grid = (int)std::min<int64_t>((int64_t)b->tiles.size(), (int64_t)ctx->n_cu * kBlocksPerCU);
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, ctx->stream, A);
+        hipLaunchKernelGGL(kern, dim3((unsigned)slots), dim3(64), lds, ctx->stream, A);
         NWG_HIP(ctx, hipGetLastError());
     } else {
-        A.tiles = b->d_tiles.p;
+        A.tiles = pl.d_tiles.p;
         A.queue = nullptr;
-        for (const Bucket& bk : b->buckets) {
+        for (const Bucket& bk : pl.buckets) {
             A.tile_base = bk.tile_begin;
             A.n_tiles = bk.tile_end;
             hipLaunchKernelGGL(kern, dim3(bk.tile_end - bk.tile_begin), dim3(64), lds, ctx->stream, A);
@@ -537,20 +448,180 @@ int launch_hard(nwg_batch* b) {
     return NWG_OK;
 }
 
+// ── soft pass ──────────────────────────────────────────────────────────────
+template <typename SF> struct SoftScratch;
+template <> struct SoftScratch<float> {
+    static DevPtr<float>& ft(nwg_batch* b) { return b->ft32; }
+    static DevPtr<float>& mt(nwg_batch* b) { return b->mt32; }
+    static DevPtr<float>& lt(nwg_batch* b) { return b->lt32; }
+    static DevPtr<float>& pf(nwg_batch* b) { return b->pf32; }
+};
+template <> struct SoftScratch<double> {
+    static DevPtr<double>& ft(nwg_batch* b) { return b->ft64; }
+    static DevPtr<double>& mt(nwg_batch* b) { return b->mt64; }
+    static DevPtr<double>& lt(nwg_batch* b) { return b->lt64; }
+    static DevPtr<double>& pf(nwg_batch* b) { return b->pf64; }
+};
+
+template <typename SF>
+int launch_soft(nwg_batch* b, const Plan& pl) {
+    using SS = SoftScratch<SF>;
+    nwg_ctx* ctx = b->ctx;
+    const int nn = b->na * b->na;
+    const bool lin = b->prob.gap_model == 0;
+    const double it = 1.0 / b->temperature;
+    // Weights in double, cast once (aligner.hpp:6870-6885): the profile exp(s / T) and the
+    // gap weights exp(-(go + ge) / T), exp(-ge / T); linear gaps have no open.
+    std::vector<SF> ps((size_t)nn);
+    const double big = (double)std::numeric_limits<SF>::max();
+    for (int k = 0; k < nn; ++k) {
+        const double v = std::exp(b->matrix[(size_t)k] * it);
+        if (!(v <= big)) return fail(ctx, NWG_ERANGE, "nwgrad: soft path: exp(score / T) overflows");
+        ps[(size_t)k] = static_cast<SF>(v);
+    }
+    const double ea = std::exp(-b->gaps[1] * it), eb = std::exp(-b->gaps[3] * it);
+    const double oa = lin ? ea : std::exp(-(b->gaps[0] + b->gaps[1]) * it);
+    const double ob = lin ? eb : std::exp(-(b->gaps[2] + b->gaps[3]) * it);
+    for (double v : {ea, eb, oa, ob})
+        if (!(v <= big)) return fail(ctx, NWG_ERANGE, "nwgrad: soft path: a gap weight overflows");
+    DevPtr<SF>& dps = SS::pf(b);
+    NWG_HIP(ctx, dps.reserve((size_t)nn));
+    NWG_HIP(ctx, hipMemcpyAsync(dps.p, ps.data(), sizeof(SF) * (size_t)nn, hipMemcpyHostToDevice, ctx->stream));
+    void (*kern)(nwg::SoftArgs<SF>) = b->prob.mode == 1   ? nwg::soft_kernel<SF, nwg::kLocal, kS>
+                                      : b->prob.mode == 2 ? nwg::soft_kernel<SF, nwg::kGlocal, kS>
+                                                          : nwg::soft_kernel<SF, nwg::kGlobal, kS>;
+    const size_t lds = (size_t)b->hist_slots * nn * sizeof(double) + (size_t)nn * sizeof(SF);
+    const int64_t slots = std::max<int64_t>(
+        std::min<int64_t>((int64_t)pl.tiles.size(), resident_blocks(ctx, kern, lds, kSoftBlocksPerCU)), 1);
+    constexpr int P = nwg::SegLayout<kS>::padded;
+    const int64_t rows = pl.max_steps + 1;
+    const size_t seg = (size_t)(slots * rows * 64);
+    if (SS::ft(b).n < seg * P || b->etab.n < seg || SS::mt(b).n < seg || SS::lt(b).n < seg) {
+        NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        NWG_HIP(ctx, SS::ft(b).reserve(seg * P));
+        NWG_HIP(ctx, b->etab.reserve(seg));
+        NWG_HIP(ctx, SS::mt(b).reserve(seg));
+        NWG_HIP(ctx, SS::lt(b).reserve(seg));
+    }
+    NWG_HIP(ctx, b->soft_ok.reserve((size_t)std::max<int64_t>(b->N, 1)));
+    nwg::SoftArgs<SF> A{};
+    A.codes_a = b->ca.p;
+    A.codes_b = b->cb.p;
+    A.off_a = b->oa.p;
+    A.off_b = b->ob.p;
+    A.order = pl.d_order.p;
+    A.psub = dps.p;
+    A.na = b->na;
+    A.hist_slots = b->hist_slots;
+    A.oa = static_cast<SF>(oa);
+    A.ea = static_cast<SF>(ea);
+    A.ob = static_cast<SF>(ob);
+    A.eb = static_cast<SF>(eb);
+    A.linear = lin ? 1 : 0;
+    A.temperature = b->temperature;
+    A.scores = b->scores.p;
+    A.grads = b->grads.p;
+    A.ok = b->soft_ok.p;
+    A.ftab = SS::ft(b).p;
+    A.etab = b->etab.p;
+    A.mtab = SS::mt(b).p;
+    A.ltab = SS::lt(b).p;
+    A.tol = std::is_same<SF, float>::value ? 0x1p-20 : 0x1p-45;   // the reference's SoftRange<F>::tol
+    A.vmin = (double)std::numeric_limits<SF>::min();
+    A.slot_rows = rows;
+    if (pl.tiles.empty()) return NWG_OK;
+    if (b->fill == 0) {
+        NWG_HIP(ctx, hipMemsetAsync(b->queue.p, 0, sizeof(int32_t), ctx->stream));
+        A.tiles = pl.d_ptiles.p;
+        A.n_tiles = (int32_t)pl.tiles.size();
+        A.queue = b->queue.p;
+        A.tile_base = 0;
+        hipLaunchKernelGGL(kern, dim3((unsigned)slots), dim3(64), lds, ctx->stream, A);
+        NWG_HIP(ctx, hipGetLastError());
+    } else {
+        // plain: one launch per lane-group bucket, at most `slots` tiles per launch
+        A.tiles = pl.d_tiles.p;
+        A.queue = nullptr;
+        for (const Bucket& bk : pl.buckets) {
+            for (int t0 = bk.tile_begin; t0 < bk.tile_end; t0 += (int)slots) {
+                const int t1 = std::min<int>(bk.tile_end, t0 + (int)slots);
+                A.tile_base = t0;
+                A.n_tiles = t1;
+                hipLaunchKernelGGL(kern, dim3(t1 - t0), dim3(64), lds, ctx->stream, A);
+                NWG_HIP(ctx, hipGetLastError());
+            }
+        }
+    }
+    return NWG_OK;
+}
+
+// After a soft pass: a pair whose float32 pass lost range or mass beyond 2^-20 reruns in
+// double, and one that fails in double too fails the call (the reference's policy,
+// aligner.hpp:6554-6586: float32, then double, then std::domain_error under "scaled").
+int soft_finish(nwg_batch* b) {
+    nwg_ctx* ctx = b->ctx;
+    b->soft_reruns = 0;
+    if (b->N == 0) return NWG_OK;
+    b->host_ok.resize((size_t)b->N);
+    NWG_HIP(ctx, hipMemcpyAsync(b->host_ok.data(), b->soft_ok.p, (size_t)b->N, hipMemcpyDeviceToHost, ctx->stream));
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<int32_t> refused;
+    for (int64_t i = 0; i < b->N; ++i)
+        if (!b->host_ok[(size_t)i]) refused.push_back((int32_t)i);
+    if (refused.empty()) return NWG_OK;
+    const bool ran_f32 = b->prob.precision == 0 && b->soft_precision == 0;
+    if (ran_f32) {
+        b->soft_reruns = (int64_t)refused.size();
+        int rc = build_plan(b, refused, b->rerun);
+        if (rc) return rc;
+        rc = launch_soft<double>(b, b->rerun);
+        if (rc) return rc;
+        NWG_HIP(ctx, hipMemcpyAsync(b->host_ok.data(), b->soft_ok.p, (size_t)b->N, hipMemcpyDeviceToHost, ctx->stream));
+        NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        refused.clear();
+        for (int64_t i = 0; i < b->N; ++i)
+            if (!b->host_ok[(size_t)i]) refused.push_back((int32_t)i);
+        if (refused.empty()) return NWG_OK;
+    }
+    b->results_valid = false;
+    return fail(ctx, NWG_ERANGE,
+                "nwgrad: soft path: %lld pair(s) (first: %lld) do not fit the scaled probability-space "
+                "forward-backward (mass lost to underflow could move the result by more than 2^-45, or a weight "
+                "overflows -- typically a very low temperature or extreme scores)",
+                (long long)refused.size(), (long long)refused[0]);
+}
+
 int run_async(nwg_batch* b) {
     nwg_ctx* ctx = b->ctx;
     if (!b->have_params) return fail(ctx, NWG_ESTATE, "nwgrad_hip: set_params() has not been called");
     NWG_HIP(ctx, hipSetDevice(ctx->device));
     NWG_HIP(ctx, hipEventRecord(b->ev0, ctx->stream));
     int rc;
-    if (b->prob.grad_mode == 2) {
-        rc = b->prob.precision == 0 && b->soft_precision == 0 ? launch_soft<float>(b) : launch_soft<double>(b);
-    } else {
-        rc = b->prob.precision == 0 ? launch_hard<float>(b) : launch_hard<double>(b);
-    }
+    if (b->prob.grad_mode == 2)
+        rc = (b->prob.precision == 0 && b->soft_precision == 0) ? launch_soft<float>(b, b->plan)
+                                                                : launch_soft<double>(b, b->plan);
+    else
+        rc = b->prob.precision == 0 ? launch_hard<float>(b, b->plan) : launch_hard<double>(b, b->plan);
     if (rc) return rc;
     NWG_HIP(ctx, hipEventRecord(b->ev1, ctx->stream));
     b->results_valid = true;
+    return NWG_OK;
+}
+
+int weighted_dev(nwg_batch* b, const double* w_dev, double* out_dev) {
+    nwg_ctx* ctx = b->ctx;
+    const int gs = b->na * b->na + 4;
+    const int64_t nblocks = (b->N + kSumBlock - 1) / kSumBlock;
+    NWG_HIP(ctx, b->partial.reserve((size_t)(nblocks * gs)));
+    const int threads = 64;
+    const int64_t total = nblocks * gs;
+    const int grid = (int)((total + threads - 1) / threads);
+    hipLaunchKernelGGL(nwg::block_sum_kernel, dim3(std::max(grid, 1)), dim3(threads), 0, ctx->stream, b->grads.p,
+                       w_dev, b->N, gs, kSumBlock, b->partial.p);
+    NWG_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(nwg::block_total_kernel, dim3((gs + 63) / 64), dim3(64), 0, ctx->stream, b->partial.p, nblocks,
+                       gs, out_dev);
+    NWG_HIP(ctx, hipGetLastError());
     return NWG_OK;
 }
 
@@ -568,17 +639,18 @@ int nwg_batch_run(nwg_batch* b, double* total_out) {
     nwg_ctx* ctx = b->ctx;
     int rc = run_async(b);
     if (rc) return rc;
-    if (b->N)
-        NWG_HIP(ctx, hipMemcpyAsync(b->host_scores, b->scores.p, sizeof(double) * (size_t)b->N,
-                                    hipMemcpyDeviceToHost, ctx->stream));
     NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, b->ev0, b->ev1) == hipSuccess) b->last_ms = ms;
     if (b->prob.grad_mode == 2) {
-        rc = soft_check_ok(b);
+        rc = soft_finish(b);
         if (rc) return rc;
     }
     if (total_out) {
+        if (b->N)
+            NWG_HIP(ctx, hipMemcpyAsync(b->host_scores, b->scores.p, sizeof(double) * (size_t)b->N,
+                                        hipMemcpyDeviceToHost, ctx->stream));
+        NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
         // Pair-index order, as the reference sums (batch_engine.hpp:1754-1757).
         double t = 0.0;
         for (int64_t i = 0; i < b->N; ++i) t += b->host_scores[i];
@@ -608,8 +680,7 @@ int nwg_batch_scores_dev(nwg_batch* b, const double** out_dev) {
 int nwg_batch_grads(nwg_batch* b, double* mats, double* gaps) {
     if (!b || !mats || !gaps) return NWG_EINVAL;
     nwg_ctx* ctx = b->ctx;
-    if (b->prob.grad_mode == 0)
-        return fail(ctx, NWG_ESTATE, "nwgrad: grad_mode \"none\" computes no gradient");
+    if (b->prob.grad_mode == 0) return fail(ctx, NWG_ESTATE, "nwgrad: grad_mode \"none\" computes no gradient");
     if (!b->results_valid) return fail(ctx, NWG_ESTATE, "nwgrad: gradient not valid; compute an alignment first");
     NWG_HIP(ctx, hipSetDevice(ctx->device));
     const int nn = b->na * b->na, gs = nn + 4;
@@ -625,44 +696,25 @@ int nwg_batch_grads(nwg_batch* b, double* mats, double* gaps) {
     return NWG_OK;
 }
 
-static int weighted_dev(nwg_batch* b, const double* w_dev, double* out_dev) {
-    nwg_ctx* ctx = b->ctx;
-    const int gs = b->na * b->na + 4;
-    const int64_t nblocks = (b->N + kSumBlock - 1) / kSumBlock;
-    if (b->partial.n < (size_t)(nblocks * gs)) NWG_HIP(ctx, b->partial.alloc((size_t)(nblocks * gs)));
-    {
-        const int threads = 64;
-        const int64_t total = nblocks * gs;
-        const int grid = (int)((total + threads - 1) / threads);
-        hipLaunchKernelGGL(nwg::block_sum_kernel, dim3(std::max(grid, 1)), dim3(threads), 0, ctx->stream,
-                           b->grads.p, w_dev, b->N, gs, kSumBlock, b->partial.p);
-        NWG_HIP(ctx, hipGetLastError());
-    }
-    hipLaunchKernelGGL(nwg::block_total_kernel, dim3((gs + 63) / 64), dim3(64), 0, ctx->stream, b->partial.p,
-                       nblocks, gs, out_dev);
-    NWG_HIP(ctx, hipGetLastError());
-    return NWG_OK;
-}
-
 int nwg_batch_weighted_grad(nwg_batch* b, const double* w, int64_t n, double* out) {
     if (!b || !out) return NWG_EINVAL;
     nwg_ctx* ctx = b->ctx;
     if (b->N == 0) return fail(ctx, NWG_EINVAL, "nwgrad: SeqPairBatch is empty");
     if (w && n != b->N)
-        return fail(ctx, NWG_EINVAL, "nwgrad: weighted_grad() needs one weight per pair (got %lld weights for %lld pairs)",
+        return fail(ctx, NWG_EINVAL,
+                    "nwgrad: weighted_grad() needs one weight per pair (got %lld weights for %lld pairs)",
                     (long long)n, (long long)b->N);
-    if (b->prob.grad_mode == 0)
-        return fail(ctx, NWG_ESTATE, "nwgrad: grad_mode \"none\" computes no gradient");
+    if (b->prob.grad_mode == 0) return fail(ctx, NWG_ESTATE, "nwgrad: grad_mode \"none\" computes no gradient");
     if (!b->results_valid) return fail(ctx, NWG_ESTATE, "nwgrad: gradient not valid; compute an alignment first");
     NWG_HIP(ctx, hipSetDevice(ctx->device));
     const int gs = b->na * b->na + 4;
     const double* wd = nullptr;
     if (w) {
-        if (b->wdev.n < (size_t)b->N) NWG_HIP(ctx, b->wdev.alloc((size_t)b->N));
+        NWG_HIP(ctx, b->wdev.reserve((size_t)b->N));
         NWG_HIP(ctx, hipMemcpyAsync(b->wdev.p, w, sizeof(double) * (size_t)b->N, hipMemcpyHostToDevice, ctx->stream));
         wd = b->wdev.p;
     }
-    if (b->outdev.n < (size_t)gs) NWG_HIP(ctx, b->outdev.alloc((size_t)gs));
+    NWG_HIP(ctx, b->outdev.reserve((size_t)gs));
     int rc = weighted_dev(b, wd, b->outdev.p);
     if (rc) return rc;
     NWG_HIP(ctx, hipMemcpyAsync(out, b->outdev.p, sizeof(double) * (size_t)gs, hipMemcpyDeviceToHost, ctx->stream));
@@ -673,13 +725,14 @@ int nwg_batch_weighted_grad(nwg_batch* b, const double* w, int64_t n, double* ou
 int nwg_batch_weighted_grad_dev(nwg_batch* b, const double* w_dev, double* out_dev) {
     if (!b || !out_dev) return NWG_EINVAL;
     if (b->N == 0) return fail(b->ctx, NWG_EINVAL, "nwgrad: SeqPairBatch is empty");
+    if (b->prob.grad_mode == 0) return fail(b->ctx, NWG_ESTATE, "nwgrad: grad_mode \"none\" computes no gradient");
     if (!b->results_valid) return fail(b->ctx, NWG_ESTATE, "nwgrad: gradient not valid; compute an alignment first");
     return weighted_dev(b, w_dev, out_dev);
 }
 
 int nwg_batch_plan_info(nwg_batch* b, int64_t* n_tiles, int64_t* n_cells) {
     if (!b) return NWG_EINVAL;
-    if (n_tiles) *n_tiles = (int64_t)b->tiles.size();
+    if (n_tiles) *n_tiles = (int64_t)b->plan.tiles.size();
     if (n_cells) *n_cells = b->cells;
     return NWG_OK;
 }
@@ -687,6 +740,12 @@ int nwg_batch_plan_info(nwg_batch* b, int64_t* n_tiles, int64_t* n_cells) {
 int nwg_batch_last_kernel_ms(nwg_batch* b, double* ms) {
     if (!b || !ms) return NWG_EINVAL;
     *ms = b->last_ms;
+    return NWG_OK;
+}
+
+int nwg_batch_soft_reruns(nwg_batch* b, int64_t* n) {
+    if (!b || !n) return NWG_EINVAL;
+    *n = b->soft_reruns;
     return NWG_OK;
 }
 

@@ -53,6 +53,11 @@ struct SoftArgs {
     uint8_t* ok;             // per pair: 1 in range, 0 the pass lost range (overflow / non-finite)
     SF* ftab;                // forward row segments, per wave slot
     int32_t* etab;           // their exponents
+    SF* mtab;                // their maxima (stored units, after renormalisation)
+    SF* ltab;                // 2^max(0, -k) of their renormalisation by 2^-k: a value lost
+                             // below MIN before a scale-up is below MIN * ltab stored
+    double tol;              // lost-mass tolerance (2^-20 float32, 2^-45 double)
+    double vmin;             // the working format's smallest normal value
     int64_t slot_rows;       // rows of one slot (max over tiles of max_m + G)
 };
 
@@ -179,6 +184,8 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
     const int coff_next = (j0 + S < n) ? (int)bp[j0 + S] : 0;
     SF* __restrict__ ft = A.ftab + slot * A.slot_rows * P * kWave;
     int* __restrict__ et = A.etab + slot * A.slot_rows * kWave;
+    SF* __restrict__ mt = A.mtab + slot * A.slot_rows * kWave;
+    SF* __restrict__ lt = A.ltab + slot * A.slot_rows * kWave;
     bool bad = false;
 
     // ── forward ──────────────────────────────────────────────────────────────
@@ -220,9 +227,21 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
         }
         store_seg<SF, S>(ft + (int64_t)g * P * kWave, lane, M, X, Y, cM, cX, cY);
         et[(int64_t)g * kWave + lane] = E;
+        SF mx0 = g == 0 ? cM + cX + cY : SF(0);
+#pragma unroll
+        for (int k = 0; k < S; ++k) if (k < ncols) mx0 = dmax(mx0, M[k] + X[k] + Y[k]);
+        mt[(int64_t)g * kWave + lane] = mx0;
+        lt[(int64_t)g * kWave + lane] = SF(1);
     }
     SF oM = M[S - 1], oX = X[S - 1], oY = Y[S - 1];
     int oE = E;
+    bool fzero = false;   // the lane's last row held no mass
+    {
+        SF z = SF(0);
+#pragma unroll
+        for (int k = 0; k < S; ++k) z = dmax(z, M[k] + X[k] + Y[k]);
+        fzero = !L && !(z > SF(0)) && !(cM + cX + cY > SF(0));
+    }
     const int fsteps = td.max_m + G - 1;
     for (int t = 0; t < fsteps; ++t) {
         const int i = t - g + 1;
@@ -235,7 +254,9 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
                 if constexpr (L) { nM = sldexp(SF(1), -Ec); nX = SF(0); nY = SF(0); }
                 else { nM = SF(0); nY = SF(0); nX = sfma(cX, eb, (cM + cY) * ob); }
             } else {
-                if (lE - R::headroom > Ec) {
+                if (fzero) {
+                    Ec = lE;   // the segment holds no mass yet: take the neighbour's exponent
+                } else if (lE - R::headroom > Ec) {
                     const int nE = lE - R::headroom;
 #pragma unroll
                     for (int k = 0; k < S; ++k) {
@@ -265,6 +286,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
             }
             cM = nM; cX = nX; cY = nY;
             if (!(mx <= R::fmax)) bad = true;
+            SF lfac = SF(1);
             if (mx > R::hi || (mx < R::lo && mx > SF(0))) {
                 const int k2 = silogb(mx);
 #pragma unroll
@@ -272,9 +294,14 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
                     M[k] = sldexp(M[k], -k2); X[k] = sldexp(X[k], -k2); Y[k] = sldexp(Y[k], -k2);
                 }
                 cM = sldexp(cM, -k2); cX = sldexp(cX, -k2); cY = sldexp(cY, -k2);
+                mx = sldexp(mx, -k2);
+                if (k2 < 0) lfac = sldexp(SF(1), -k2);
                 Ec += k2;
             }
+            lt[(int64_t)(i + g) * kWave + lane] = lfac;
             E = Ec;
+            mt[(int64_t)(i + g) * kWave + lane] = mx;
+            fzero = !L && !(mx > SF(0)) && !(cM + cX + cY > SF(0));
             if (L || (GL && i == m)) {
                 SF s = g == 0 ? cM + cX + cY : SF(0);
 #pragma unroll
@@ -331,11 +358,14 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
     SF c0M = SF(0), c0X = SF(0);   // lane 0: column 0 of the row below
     SF rdM = SF(0);                // BM(i+1, j0+S+1) in the segment's exponent
     int EB = 0;
-    double g_oa = 0.0, g_ea = 0.0, g_ob = 0.0, g_eb = 0.0;
+    bool bzero = true;             // the lane's last backward row held no mass
+    double g_oa = 0.0, g_ea = 0.0, g_ob = 0.0, g_eb = 0.0, loss = 0.0;
     const int kmin = (L || GL) ? 1 : 0;   // first row whose Y opens count
     const bool col0_opens = !L;           // column-0 X opens count (kmin0 == 0)
+    const bool owns_end = (n == 0) ? (g == 0) : (j0 < n && n <= j0 + S);
     SF fc[P], fp[P];
     int efc = 0, efp = 0;
+    SF mfc = SF(0), mfp = SF(0), lfc = SF(1), lfp = SF(1);
     bool loaded = false;
     SF pubM = SF(0), pubY = SF(0);
     int pubE = 0;
@@ -348,20 +378,32 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
         if (!loaded) {
             load_seg<SF, S>(ft + (int64_t)(i + g) * P * kWave, lane, fc);
             efc = et[(int64_t)(i + g) * kWave + lane];
+            mfc = mt[(int64_t)(i + g) * kWave + lane];
+            lfc = lt[(int64_t)(i + g) * kWave + lane];
             loaded = true;
         } else {
 #pragma unroll
             for (int k = 0; k < P; ++k) fc[k] = fp[k];
             efc = efp;
+            mfc = mfp;
+            lfc = lfp;
         }
         if (i >= 1) {
             load_seg<SF, S>(ft + (int64_t)(i - 1 + g) * P * kWave, lane, fp);
             efp = et[(int64_t)(i - 1 + g) * kWave + lane];
+            mfp = mt[(int64_t)(i - 1 + g) * kWave + lane];
+            lfp = lt[(int64_t)(i - 1 + g) * kWave + lane];
         }
-        // the right boundary: BY(i, j0+S+1) and BM(i, j0+S+1) of the lane to the right
+        // The right boundary: BY(i, j0+S+1) and BM(i, j0+S+1) of the lane to the right.  A
+        // segment without mass of its own (Global's row m left of column n, or rows whose
+        // mass has not arrived) takes the neighbour's exponent; otherwise the neighbour's
+        // values are converted, the segment's own moved down when they would not fit.
         const bool has_right = j0 + S < n;
+        const bool own_zero = (i == m) ? (!L && !GL && !owns_end) : (!L && bzero);
         int Ec = (i == m) ? 0 : EB;
-        if (has_right && rE - R::headroom > Ec) {
+        if (has_right && own_zero) {
+            Ec = rE;
+        } else if (has_right && rE - R::headroom > Ec) {
             const int nE = rE - R::headroom;
 #pragma unroll
             for (int k = 0; k < S; ++k) {
@@ -375,11 +417,11 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
         const SF init = L ? sldexp(SF(1), -Ec) : SF(0);
         const SF* __restrict__ pn = (i < m) ? spsub + (int)ap[i] * na : spsub;      // row i+1's profile
         const SF* __restrict__ pr = (i >= 1) ? spsub + (int)ap[i - 1] * na : spsub; // row i's profile
-        // row m end weights (soft_kernel_impl.inl:815-818)
         SF byr = rYc;          // BY(i, j+1), right to left
         SF dn = rdM;           // BM(i+1, j+1)
         SF mx = SF(0);
-        SF sx = 0, sy = 0, so = 0, sob = 0;
+        double sx = 0, sy = 0, so = 0, sob = 0;
+        const double gh0 = zbad ? 0.0 : __builtin_amdgcn_ldexp(izr, efc + Ec - ze);
         const double gv0 = (i >= 1 && !zbad) ? __builtin_amdgcn_ldexp(izr, efp + Ec - ze) : 0.0;
         double* __restrict__ h = hist + q * nn + (i >= 1 ? (int)ap[i - 1] * na : 0);
 #pragma unroll
@@ -394,7 +436,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
                 b1 = sfma(v, ob, d);
                 b2 = sfma(v, eb, d);
                 y = b1;
-            } else {
+            } else {   // row m end weights (soft_kernel_impl.inl:815-818)
                 const SF e = L ? init : (GL ? SF(1) : (j == n ? SF(1) : SF(0)));
                 b1 = e; b2 = e; y = e;
             }
@@ -407,22 +449,27 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
             BM[k] = bmn; BX[k] = bxn; BY[k] = byv;
             byr = byv;
             if (k < ncols) mx = dmax(mx, bmn + bxn + byv);
-            // fused gradient sums (soft_kernel_impl.inl:871-903)
-            const SF fM = fc[k], fX = fc[S + 1 + k], fY = fc[2 * (S + 1) + k];
-            const SF fMl = (k == 0) ? fc[S] : fc[k - 1];
-            const SF fXl = (k == 0) ? fc[2 * S + 1] : fc[S + 1 + k - 1];
-            sy = sfma(fY, byv, sy);
-            so = sfma(fMl + fXl, byv, so);
+            // fused gradient sums (soft_kernel_impl.inl:871-903); products in double so that
+            // a product of two small stored values never flushes
+            const double fX = (double)fc[S + 1 + k], fY = (double)fc[2 * (S + 1) + k];
+            const double fMl = (k == 0) ? (double)fc[S] : (double)fc[k - 1];
+            const double fXl = (k == 0) ? (double)fc[2 * S + 1] : (double)fc[S + 1 + k - 1];
+            sy = __builtin_fma(fY, (double)byv, sy);
+            so = __builtin_fma(fMl + fXl, (double)byv, so);
             if (i >= 1) {
-                const SF qM = fp[k], qY = fp[2 * (S + 1) + k];
-                const SF qMl = (k == 0) ? fp[S] : fp[k - 1];
-                const SF qXl = (k == 0) ? fp[2 * S + 1] : fp[S + 1 + k - 1];
-                const SF qYl = (k == 0) ? fp[3 * S + 2] : fp[2 * (S + 1) + k - 1];
-                sx = sfma(fX, bxn, sx);
-                sob = sfma(qM + qY, bxn, sob);
+                const double qM = (double)fp[k], qY = (double)fp[2 * (S + 1) + k];
+                const double qMl = (k == 0) ? (double)fp[S] : (double)fp[k - 1];
+                const double qXl = (k == 0) ? (double)fp[2 * S + 1] : (double)fp[S + 1 + k - 1];
+                const double qYl = (k == 0) ? (double)fp[3 * S + 2] : (double)fp[2 * (S + 1) + k - 1];
+                sx = __builtin_fma(fX, (double)bxn, sx);
+                sob = __builtin_fma(qM + qY, (double)bxn, sob);
                 if (k < ncols) {
-                    const SF tmatch = (qMl + qXl + qYl) * pr[coff[k]] * bmn;
-                    if (tmatch > SF(0)) atomicAdd(h + coff[k], (double)tmatch * gv0);
+                    const double tmatch = (qMl + qXl + qYl) * (double)pr[coff[k]] * (double)bmn;
+                    if (tmatch > 0.0) {
+                        const double c = tmatch * gv0;
+                        if (!(c <= 1.0e300)) bad = true;
+                        else atomicAdd(h + coff[k], c);
+                    }
                 }
             }
         }
@@ -442,16 +489,30 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
             c0X = b2 + hh;
             mx = dmax(mx, c0M + c0X);
             if (i >= 1) {
-                sx = sfma(fc[2 * S + 1], c0X, sx);
-                if (col0_opens) sob = sfma(fp[S] + fp[3 * S + 2], c0X, sob);
+                sx = __builtin_fma((double)fc[2 * S + 1], (double)c0X, sx);
+                if (col0_opens) sob = __builtin_fma((double)fp[S] + (double)fp[3 * S + 2], (double)c0X, sob);
             }
         }
+        // Lost mass (soft_kernel_impl.inl:141-178): any value flushed in this row's forward or
+        // backward segment was below the format's MIN in stored units; its effect on any
+        // posterior is at most MIN times the other direction's segment maximum times the
+        // row's posterior factor, charged for every operation of the cell.
+        if (!zbad && (mx > SF(0) || mfc > SF(0))) {
+            const double lr = 32.0 * (double)(ncols + 1) * A.vmin * ((double)lfc * (double)mx + (double)mfc) * gh0;
+            loss += (lr <= 1.0e300) ? lr : 1.0e300;
+        }
         if (!zbad) {
-            const double gh0 = __builtin_amdgcn_ldexp(izr, efc + Ec - ze);
-            g_eb += (double)sx * gh0;
-            g_ea += (double)sy * gh0;
-            if (i >= kmin) g_oa += (double)so * (double)oa * gh0;
-            if (i >= 1) g_ob += (double)sob * (double)ob * gv0;
+            auto acc = [&](double& dst, double v, double f) {
+                if (v != 0.0) {
+                    const double c = v * f;
+                    if (!(c <= 1.0e300 && c >= -1.0e300)) bad = true;
+                    else dst += c;
+                }
+            };
+            acc(g_eb, sx, gh0);
+            acc(g_ea, sy, gh0);
+            if (i >= kmin) acc(g_oa, so * (double)oa, gh0);
+            if (i >= 1) acc(g_ob, sob * (double)ob, gv0);
         }
         if (!(mx <= R::fmax)) bad = true;
         rdM = rMc;
@@ -465,6 +526,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
             Ec += k2;
         }
         EB = Ec;
+        bzero = !(mx > SF(0));
         pubM = BM[0]; pubY = BY[0]; pubE = EB;
     }
 
@@ -474,6 +536,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
         g_ea += __shfl_xor(g_ea, off, G);
         g_ob += __shfl_xor(g_ob, off, G);
         g_eb += __shfl_xor(g_eb, off, G);
+        loss += __shfl_xor(loss, off, G);
         bad = bad || __shfl_xor((int)bad, off, G);
     }
     wave_sync();
@@ -485,7 +548,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
         hist[qq * nn + e] = 0.0;
     }
     if (g == 0 && p >= 0) {
-        const bool okp = !bad && !zbad;
+        const bool okp = !bad && !zbad && loss <= A.tol;
         A.ok[p] = okp ? 1 : 0;
         const double logz = zbad ? 0.0 : ::log(zr) + (double)ze * 0.69314718055994530942;
         A.scores[p] = logz * A.temperature;

@@ -100,24 +100,28 @@ def mutate(r: np.random.Generator, a: np.ndarray, n_alpha: int, sub_p: float, in
            indel_mean: float) -> np.ndarray:
     """Point substitutions (uniform replacement by a different letter) and indel events (50/50
     insertion / deletion, geometric length with the given mean) per residue."""
-    out = []
-    k = 0
     L = len(a)
-    geo_p = 1.0 / indel_mean
-    while k < L:
-        if r.random() < indel_p:
-            ln = int(r.geometric(geo_p))
-            if r.random() < 0.5:
-                out.append(r.integers(0, n_alpha, size=ln).astype(np.uint8))
-            else:
-                k += ln
-                continue
-        c = a[k]
-        if r.random() < sub_p:
-            c = (c + r.integers(1, n_alpha)) % n_alpha
-        out.append(np.array([c], np.uint8))
-        k += 1
-    return np.concatenate(out) if out else np.zeros(0, np.uint8)
+    b = a.astype(np.int64)
+    subs = r.random(L) < sub_p
+    b[subs] = (b[subs] + r.integers(1, n_alpha, size=int(subs.sum()))) % n_alpha
+    ev = np.nonzero(r.random(L) < indel_p)[0]
+    if len(ev) == 0:
+        return b.astype(np.uint8)
+    lens = r.geometric(1.0 / indel_mean, size=len(ev))
+    ins = r.random(len(ev)) < 0.5
+    keep = np.ones(L, bool)
+    for pos, ln, is_ins in zip(ev, lens, ins):
+        if not is_ins:
+            keep[pos:pos + ln] = False
+    pieces = []
+    prev = 0
+    for pos, ln, is_ins in zip(ev, lens, ins):
+        if is_ins:
+            pieces.append(b[prev:pos][keep[prev:pos]])
+            pieces.append(r.integers(0, n_alpha, size=int(ln)))
+            prev = pos
+    pieces.append(b[prev:][keep[prev:]])
+    return np.concatenate(pieces).astype(np.uint8)
 
 
 def protein_global_hard(n_pairs: int = 4096) -> Workload:

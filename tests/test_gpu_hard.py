@@ -62,7 +62,8 @@ def test_dna_local_short_config(gpu):
     score-only, persistent and per-bucket launches, every pair against the oracle."""
     w = wl.dna_local_short()
     say("case dna_local_short", w.n_pairs, "pairs", w.cells, "cells")
-    s_ref, g_ref = oracle.hard(w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b)
+    s_ref, g_ref = oracle.hard(w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, gap_model=w.gap_model,
+                               mode=w.mode)
     for fill in ("interpair", "striped"):
         b = from_workload(w, fill=fill)
         b.score_and_grad()
