@@ -38,6 +38,7 @@ constexpr int kSoftHistBytes = 13312;  // LDS histogram budget per wave (soft, d
 constexpr int kBlocksPerCU = 8;        // persistent grid of the hard / score fills: waves per CU
 constexpr int kSoftBlocksPerCU = 4;    // the soft pass holds 256 VGPRs: one wave per SIMD
 constexpr int64_t kSumBlock = 4096;    // the reference's WEIGHTED_GRAD_BLOCK
+constexpr size_t kScratchBudget = size_t(48) << 30;   // per-slot HBM scratch of one launch, at most
 
 int fail(nwg_ctx* c, int code, const char* fmt, ...) {
     char buf[512];
@@ -111,6 +112,7 @@ struct Plan {
     std::vector<Bucket> buckets;
     std::vector<int32_t> order;
     int64_t max_steps = 1;   // max over tiles of max_m + G - 1
+    int panels = 1;          // max over tiles of the column panels of its pair
     DevPtr<int32_t> d_order;
     DevPtr<TileDesc> d_tiles, d_ptiles;
 };
@@ -127,17 +129,18 @@ struct nwg_batch {
     DevPtr<uint8_t> ca, cb;
     DevPtr<int64_t> oa, ob;
     DevPtr<double> scores, grads;
-    // plans: every pair, and the pairs a float32 soft pass refused
+    // plans: the pairs one lane group holds, the wider ones (column panels), and the pairs
+    // a float32 soft pass refused
     int gmin = 1, hist_slots = 64;
-    Plan plan, rerun;
+    Plan plan, lplan, rerun, lrerun;
     int64_t cells = 0;
     DevPtr<int32_t> queue;
     // hard: pointer codes per wave slot
     DevPtr<uint32_t> table;
     // soft: forward row segments, exponents, maxima, scale-up factors (per precision)
-    DevPtr<float> ft32, mt32, lt32, pf32;
-    DevPtr<double> ft64, mt64, lt64, pf64;
-    DevPtr<int32_t> etab;
+    DevPtr<float> ft32, mt32, lt32, pf32, pb32;
+    DevPtr<double> ft64, mt64, lt64, pf64, pb64;
+    DevPtr<int32_t> etab, pexp;
     DevPtr<uint8_t> soft_ok;
     std::vector<uint8_t> host_ok;
     int64_t soft_reruns = 0;
@@ -172,9 +175,13 @@ int build_plan(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& pl) {
     pl.tiles.clear();
     pl.buckets.clear();
     pl.max_steps = 1;
+    pl.panels = 1;
     const int64_t N = (int64_t)pairs.size();
     std::vector<int32_t> gsel((size_t)b->N > 0 ? (size_t)b->N : 1);
-    for (int32_t i : pairs) gsel[(size_t)i] = std::max(next_pow2((b->nlen[(size_t)i] + kS - 1) / kS), b->gmin);
+    for (int32_t i : pairs) {
+        gsel[(size_t)i] = std::min(std::max(next_pow2((b->nlen[(size_t)i] + kS - 1) / kS), b->gmin), kMaxG);
+        pl.panels = std::max(pl.panels, (b->nlen[(size_t)i] + kMaxN - 1) / kMaxN);
+    }
     pl.order = pairs;
     std::stable_sort(pl.order.begin(), pl.order.end(), [&](int32_t x, int32_t y) {
         if (gsel[(size_t)x] != gsel[(size_t)y]) return gsel[(size_t)x] < gsel[(size_t)y];
@@ -282,10 +289,10 @@ int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t
     for (int64_t i = 0; i < N; ++i) {
         const int64_t m = off_a[i + 1] - off_a[i], n = off_b[i + 1] - off_b[i];
         if (m < 0 || n < 0) return fail(ctx, NWG_EINVAL, "nwgrad_hip: offsets must be non-decreasing");
-        if (n > kMaxN)
+        if (n > kMaxN && prob->grad_mode != 2)
             return fail(ctx, NWG_EINVAL,
-                        "nwgrad_hip: pair %lld has len(B) = %lld; this build's lane-group kernels hold at most %d "
-                        "columns",
+                        "nwgrad_hip: pair %lld has len(B) = %lld; this build's hard and score-only fills hold at most "
+                        "%d columns",
                         (long long)i, (long long)n, kMaxN);
         if (m > (1 << 24)) return fail(ctx, NWG_EINVAL, "nwgrad_hip: pair %lld is too long", (long long)i);
         b->mlen[(size_t)i] = (int32_t)m;
@@ -318,9 +325,11 @@ int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t
     if (lb) NWG_HIP(ctx, hipMemcpy(b->cb.p, codes_b, (size_t)lb, hipMemcpyHostToDevice));
     NWG_HIP(ctx, hipMemcpy(b->oa.p, off_a, sizeof(int64_t) * (size_t)(N + 1), hipMemcpyHostToDevice));
     NWG_HIP(ctx, hipMemcpy(b->ob.p, off_b, sizeof(int64_t) * (size_t)(N + 1), hipMemcpyHostToDevice));
-    std::vector<int32_t> all((size_t)N);
-    std::iota(all.begin(), all.end(), 0);
-    int rc = build_plan(b.get(), all, b->plan);
+    std::vector<int32_t> shortp, longp;
+    for (int64_t i = 0; i < N; ++i) (b->nlen[(size_t)i] > kMaxN ? longp : shortp).push_back((int32_t)i);
+    int rc = build_plan(b.get(), shortp, b->plan);
+    if (rc) return rc;
+    rc = build_plan(b.get(), longp, b->lplan);
     if (rc) return rc;
     NWG_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&b->host_scores),
                                sizeof(double) * (size_t)std::max<int64_t>(N, 1), hipHostMallocDefault));
@@ -455,12 +464,14 @@ template <> struct SoftScratch<float> {
     static DevPtr<float>& mt(nwg_batch* b) { return b->mt32; }
     static DevPtr<float>& lt(nwg_batch* b) { return b->lt32; }
     static DevPtr<float>& pf(nwg_batch* b) { return b->pf32; }
+    static DevPtr<float>& pb(nwg_batch* b) { return b->pb32; }
 };
 template <> struct SoftScratch<double> {
     static DevPtr<double>& ft(nwg_batch* b) { return b->ft64; }
     static DevPtr<double>& mt(nwg_batch* b) { return b->mt64; }
     static DevPtr<double>& lt(nwg_batch* b) { return b->lt64; }
     static DevPtr<double>& pf(nwg_batch* b) { return b->pf64; }
+    static DevPtr<double>& pb(nwg_batch* b) { return b->pb64; }
 };
 
 template <typename SF>
@@ -491,17 +502,26 @@ int launch_soft(nwg_batch* b, const Plan& pl) {
                                       : b->prob.mode == 2 ? nwg::soft_kernel<SF, nwg::kGlocal, kS>
                                                           : nwg::soft_kernel<SF, nwg::kGlobal, kS>;
     const size_t lds = (size_t)b->hist_slots * nn * sizeof(double) + (size_t)nn * sizeof(SF);
-    const int64_t slots = std::max<int64_t>(
-        std::min<int64_t>((int64_t)pl.tiles.size(), resident_blocks(ctx, kern, lds, kSoftBlocksPerCU)), 1);
     constexpr int P = nwg::SegLayout<kS>::padded;
     const int64_t rows = pl.max_steps + 1;
-    const size_t seg = (size_t)(slots * rows * 64);
-    if (SS::ft(b).n < seg * P || b->etab.n < seg || SS::mt(b).n < seg || SS::lt(b).n < seg) {
+    const size_t slot_bytes = (size_t)pl.panels * (size_t)rows *
+                              ((size_t)P * 64 * sizeof(SF) + 64 * (sizeof(int32_t) + 2 * sizeof(SF)) + 6 * sizeof(SF) +
+                               2 * sizeof(int32_t));
+    const int64_t slots = std::max<int64_t>(
+        std::min<int64_t>(std::min<int64_t>((int64_t)pl.tiles.size(), resident_blocks(ctx, kern, lds, kSoftBlocksPerCU)),
+                          (int64_t)(kScratchBudget / std::max<size_t>(slot_bytes, 1))),
+        1);
+    const size_t seg = (size_t)(slots * pl.panels * rows * 64);
+    const size_t bnd = (size_t)(slots * pl.panels * rows);
+    if (SS::ft(b).n < seg * P || b->etab.n < seg || SS::mt(b).n < seg || SS::lt(b).n < seg ||
+        SS::pb(b).n < bnd * 6 || b->pexp.n < bnd * 2) {
         NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
         NWG_HIP(ctx, SS::ft(b).reserve(seg * P));
         NWG_HIP(ctx, b->etab.reserve(seg));
         NWG_HIP(ctx, SS::mt(b).reserve(seg));
         NWG_HIP(ctx, SS::lt(b).reserve(seg));
+        NWG_HIP(ctx, SS::pb(b).reserve(bnd * 6));
+        NWG_HIP(ctx, b->pexp.reserve(bnd * 2));
     }
     NWG_HIP(ctx, b->soft_ok.reserve((size_t)std::max<int64_t>(b->N, 1)));
     nwg::SoftArgs<SF> A{};
@@ -529,6 +549,9 @@ int launch_soft(nwg_batch* b, const Plan& pl) {
     A.tol = std::is_same<SF, float>::value ? 0x1p-20 : 0x1p-45;   // the reference's SoftRange<F>::tol
     A.vmin = (double)std::numeric_limits<SF>::min();
     A.slot_rows = rows;
+    A.pbnd = SS::pb(b).p;
+    A.pexp = b->pexp.p;
+    A.panels = pl.panels;
     if (pl.tiles.empty()) return NWG_OK;
     if (b->fill == 0) {
         NWG_HIP(ctx, hipMemsetAsync(b->queue.p, 0, sizeof(int32_t), ctx->stream));
@@ -572,9 +595,15 @@ int soft_finish(nwg_batch* b) {
     const bool ran_f32 = b->prob.precision == 0 && b->soft_precision == 0;
     if (ran_f32) {
         b->soft_reruns = (int64_t)refused.size();
-        int rc = build_plan(b, refused, b->rerun);
+        std::vector<int32_t> rs, rl;
+        for (int32_t i : refused) (b->nlen[(size_t)i] > kMaxN ? rl : rs).push_back(i);
+        int rc = build_plan(b, rs, b->rerun);
+        if (rc) return rc;
+        rc = build_plan(b, rl, b->lrerun);
         if (rc) return rc;
         rc = launch_soft<double>(b, b->rerun);
+        if (rc) return rc;
+        rc = launch_soft<double>(b, b->lrerun);
         if (rc) return rc;
         NWG_HIP(ctx, hipMemcpyAsync(b->host_ok.data(), b->soft_ok.p, (size_t)b->N, hipMemcpyDeviceToHost, ctx->stream));
         NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -596,13 +625,16 @@ int run_async(nwg_batch* b) {
     if (!b->have_params) return fail(ctx, NWG_ESTATE, "nwgrad_hip: set_params() has not been called");
     NWG_HIP(ctx, hipSetDevice(ctx->device));
     NWG_HIP(ctx, hipEventRecord(b->ev0, ctx->stream));
-    int rc;
-    if (b->prob.grad_mode == 2)
-        rc = (b->prob.precision == 0 && b->soft_precision == 0) ? launch_soft<float>(b, b->plan)
-                                                                : launch_soft<double>(b, b->plan);
-    else
-        rc = b->prob.precision == 0 ? launch_hard<float>(b, b->plan) : launch_hard<double>(b, b->plan);
-    if (rc) return rc;
+    int rc = NWG_OK;
+    for (Plan* pl : {&b->plan, &b->lplan}) {
+        if (pl->tiles.empty()) continue;
+        if (b->prob.grad_mode == 2)
+            rc = (b->prob.precision == 0 && b->soft_precision == 0) ? launch_soft<float>(b, *pl)
+                                                                    : launch_soft<double>(b, *pl);
+        else
+            rc = b->prob.precision == 0 ? launch_hard<float>(b, *pl) : launch_hard<double>(b, *pl);
+        if (rc) return rc;
+    }
     NWG_HIP(ctx, hipEventRecord(b->ev1, ctx->stream));
     b->results_valid = true;
     return NWG_OK;
@@ -732,7 +764,7 @@ int nwg_batch_weighted_grad_dev(nwg_batch* b, const double* w_dev, double* out_d
 
 int nwg_batch_plan_info(nwg_batch* b, int64_t* n_tiles, int64_t* n_cells) {
     if (!b) return NWG_EINVAL;
-    if (n_tiles) *n_tiles = (int64_t)b->plan.tiles.size();
+    if (n_tiles) *n_tiles = (int64_t)(b->plan.tiles.size() + b->lplan.tiles.size());
     if (n_cells) *n_cells = b->cells;
     return NWG_OK;
 }

@@ -92,9 +92,6 @@ def test_protein_soft_homologs_sample(gpu):
     w = wl.protein_soft_homologs(256, 256)
     idx = np.concatenate([np.arange(0, 256, 4), np.arange(256, 512, 4)])
     w = w.subset(idx)
-    lb = np.diff(w.off_b)
-    if lb.max() > 1024:   # pairs past one lane group's columns go through the panel path
-        w = w.subset(np.nonzero(lb <= 1024)[0])
     say("case protein_soft_homologs sample", w.n_pairs, "pairs", w.cells, "cells")
     s_ref, g_ref = oracle.soft_log(w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b)
     lens = (np.diff(w.off_a), np.diff(w.off_b))
@@ -118,3 +115,25 @@ def test_weighted_grad_soft(gpu):
     assert np.max(np.abs(flat - want)) <= 1e-5 * np.max(np.abs(want))
     # the device reduction equals the reference's blocked sum of the device rows, bit for bit
     np.testing.assert_array_equal(flat, oracle.weighted_sum(grads_rows(b), wts))
+
+
+@pytest.mark.parametrize("mode", ["global", "local", "glocal"])
+def test_column_panels(gpu, mode):
+    """Pairs wider than one wavefront's 1,024 columns run panel by panel (one pair per tile),
+    beside ordinary pairs in the same batch."""
+    r = wl.rng(f"panels_{mode}", "lens")
+    nb = np.array([1025, 1100, 2048, 2049, 2500, 3100, 40, 900, 1024, 1500])
+    na_ = np.array([3, 150, 80, 1, 60, 40, 200, 120, 70, 0])
+    A = [r.integers(0, 20, size=int(x)).astype(np.uint8) for x in na_]
+    B = [r.integers(0, 20, size=int(x)).astype(np.uint8) for x in nb]
+    B[1][:len(A[1])] = wl.mutate(r, A[1], 20, 0.1, 0.0, 1.0)[:len(A[1])]
+    ca, oa = wl._pack(A)
+    cb, ob = wl._pack(B)
+    mat = wl.random_matrix(r)
+    gaps = np.array([11.0, 1.0, 11.0, 1.0])
+    say("case soft panels", mode, len(nb), "pairs")
+    s_ref, g_ref = oracle.soft_log(mat, gaps, ca, oa, cb, ob, mode=mode)
+    for prec, tol in (("float32", 1e-5), ("float64", 1e-10)):
+        for fill in ("interpair", "striped"):
+            b = batch(wl.AA, mat, gaps, ca, oa, cb, ob, precision=prec, mode=mode, grad_mode="soft", fill=fill)
+            check(b, s_ref, g_ref, tol, (np.diff(oa), np.diff(ob)))
