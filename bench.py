@@ -128,7 +128,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default=os.environ.get("NWG_BENCH_WORKLOAD", "dna_local_short"),
+    ap.add_argument("--workload", default=os.environ.get("NWG_BENCH_WORKLOAD", "protein_soft_homologs"),
                     choices=WORKLOADS)
     ap.add_argument("--fill", default="interpair", choices=("interpair", "striped"))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
