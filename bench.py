@@ -85,7 +85,7 @@ def cpu_baseline(w, grad_mode, budget_s=8.0):
             rb = ref.RefBatch(sa, sb, sub.alphabet, sub.matrix, sub.gaps, gap_model=sub.gap_model, mode=sub.mode,
                               grad_mode=grad_mode, n_threads=cores)
             if grad_mode == "soft":
-                rb.set_soft(precision="float32")
+                rb.set_soft(precision="float32", impl="scaled_or_log")
             rb.score_and_grad()  # warm-up
             best = 1e30
             for _ in range(2):
@@ -147,11 +147,16 @@ def main():
           flush=True)
     dt, kms = time_steps(b, args.steps, args.warmup, sync)
     gcups = cells * args.steps / dt / 1e9
+    reruns = b.soft_reruns if grad_mode == "soft" else 0
+    print(f"bench: {args.fill}: {dt / args.steps * 1e3:.3f} ms/step, {gcups:.2f} GCUPS, kernels {kms:.3f} ms"
+          + (f", {reruns} pairs rerun in double" if grad_mode == "soft" else ""), file=sys.stderr, flush=True)
     # the other launch mode, same batch shape
     other = "striped" if args.fill == "interpair" else "interpair"
     b.fill = other
     dt2, kms2 = time_steps(b, max(2, args.steps // 2), 1, sync)
     gcups_other = cells * max(2, args.steps // 2) / dt2 / 1e9
+    print(f"bench: {other}: {dt2 / max(2, args.steps // 2) * 1e3:.3f} ms/step, {gcups_other:.2f} GCUPS",
+          file=sys.stderr, flush=True)
     b.fill = args.fill
     ops = OPS_PER_CELL[(w.gap_model, grad_mode)] + (1 if (w.mode == "local" and w.gap_model == "linear") else 0)
     achieved = ops * cells / (kms * 1e-3) / 1e12
@@ -159,6 +164,8 @@ def main():
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
             "valu_issue_peak": VALU_ISSUE_PEAK_TOPS, "valu_issue_frac": round(achieved / VALU_ISSUE_PEAK_TOPS, 4),
             "ops_per_cell": ops, "kernel_ms": round(kms, 4)}
+    if not args.no_cpu:
+        print("bench: CPU baseline (bounded sample)", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(w, grad_mode)
     dtype = "f32"
     line = {"metric": "GCUPS soft-grad, batched vs plain GPU fill @ bit-exact/1e-5; p99 latency",
@@ -172,6 +179,8 @@ def main():
                                  "kernel_gcups": round(cells / (kms2 * 1e-3) / 1e9, 3),
                                  "batched_over_plain": round(gcups / gcups_other, 3)},
             "roofline": roof, "cpu_baseline": cpu}
+    if grad_mode == "soft":
+        line["soft_pairs_rerun_in_double"] = int(reruns)
     print(json.dumps(line), flush=True)
     return 0
 
