@@ -148,8 +148,10 @@ def main():
     dt, kms = time_steps(b, args.steps, args.warmup, sync)
     gcups = cells * args.steps / dt / 1e9
     reruns = b.soft_reruns if grad_mode == "soft" else 0
+    launch = b.launch_info if grad_mode == "soft" else None
     print(f"bench: {args.fill}: {dt / args.steps * 1e3:.3f} ms/step, {gcups:.2f} GCUPS, kernels {kms:.3f} ms"
-          + (f", {reruns} pairs rerun in double" if grad_mode == "soft" else ""), file=sys.stderr, flush=True)
+          + (f", {reruns} pairs rerun in double, launch (grid, wide blocks, per CU) {launch}" if grad_mode == "soft"
+             else ""), file=sys.stderr, flush=True)
     # the other launch mode, same batch shape
     other = "striped" if args.fill == "interpair" else "interpair"
     b.fill = other
@@ -181,6 +183,7 @@ def main():
             "roofline": roof, "cpu_baseline": cpu}
     if grad_mode == "soft":
         line["soft_pairs_rerun_in_double"] = int(reruns)
+        line["launch"] = {"grid": int(launch[0]), "wide_queue_blocks": int(launch[1]), "resident_per_cu": int(launch[2])}
     print(json.dumps(line), flush=True)
     return 0
 

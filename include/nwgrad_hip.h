@@ -115,6 +115,10 @@ int nwg_batch_last_kernel_ms(nwg_batch *b, double *ms);
  * 2^-20) and that were rerun in double -- the reference's kF32Refused fallback
  * (aligner.hpp:6554-6572, batch_engine.hpp:1285-1306). */
 int nwg_batch_soft_reruns(nwg_batch *b, int64_t *n);
+/* Shape of the last soft launch (diagnostics; no counterpart in the reference, whose worker
+ * pool has no grid): blocks of the persistent launch (0 for plain launches), blocks that
+ * drain the wide-tile queue first, and resident blocks per CU from the occupancy query. */
+int nwg_batch_launch_info(nwg_batch *b, int64_t *grid, int64_t *wide_slots, int64_t *per_cu);
 
 #ifdef __cplusplus
 }

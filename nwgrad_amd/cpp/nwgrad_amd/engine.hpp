@@ -12,6 +12,7 @@
 // on the GPU through the C ABI; this layer only validates, encodes and converts.
 #pragma once
 
+#include <array>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -345,6 +346,12 @@ public:
         int64_t t = 0, c = 0;
         check_(nwg_batch_plan_info(b_, &t, &c));
         return c;
+    }
+    // grid, wide-queue blocks, resident blocks per CU of the last soft launch
+    std::array<int64_t, 3> launch_info() const {
+        std::array<int64_t, 3> r{0, 0, 0};
+        check_(nwg_batch_launch_info(b_, &r[0], &r[1], &r[2]));
+        return r;
     }
     int64_t soft_reruns() const {
         int64_t n = 0;

@@ -30,15 +30,16 @@ struct nwg_ctx {
 
 namespace {
 
-constexpr int kS = 16;                 // columns per lane of the lane-group kernels
+constexpr int kS = 16;                 // columns per lane: hard and score-only fills
+constexpr int kSS = 8;                 // columns per lane: soft pass (float32 and double)
 constexpr int kMaxG = 64;              // lanes per pair at most: one wavefront
-constexpr int kMaxN = kS * kMaxG;      // longest B a lane group holds
+constexpr int kMaxN = kS * kMaxG;      // longest B a lane group holds (hard / score-only)
 constexpr int kHistBytes = 16384;      // LDS histogram budget per wave (hard walk, uint16 counts)
-constexpr int kSoftHistBytes = 13312;  // LDS histogram budget per wave (soft, double expected counts)
+constexpr int kSoftHistBytes = 10240;  // LDS histogram budget per wave (soft, double expected counts)
 constexpr int kBlocksPerCU = 8;        // persistent grid of the hard / score fills: waves per CU
-constexpr int kSoftBlocksPerCU = 4;    // the soft pass holds 256 VGPRs: one wave per SIMD
+constexpr int kSoftBlocksPerCU = 8;    // fallback persistent grid of the soft pass
 constexpr int64_t kSumBlock = 4096;    // the reference's WEIGHTED_GRAD_BLOCK
-constexpr size_t kScratchBudget = size_t(48) << 30;   // per-slot HBM scratch of one launch, at most
+constexpr size_t kScratchBudget = size_t(96) << 30;   // soft row-segment scratch of one launch, at most
 
 int fail(nwg_ctx* c, int code, const char* fmt, ...) {
     char buf[512];
@@ -108,13 +109,27 @@ struct Bucket {
 // tile's rows are even; the plain order (bucket by bucket) and the persistent order
 // (longest tile first) of the same tiles.
 struct Plan {
+    int S = kS;              // columns per lane the tiles were cut for
     std::vector<TileDesc> tiles;
     std::vector<Bucket> buckets;
     std::vector<int32_t> order;
     int64_t max_steps = 1;   // max over tiles of max_m + G - 1
     int panels = 1;          // max over tiles of the column panels of its pair
+    bool built = false;
     DevPtr<int32_t> d_order;
     DevPtr<TileDesc> d_tiles, d_ptiles;
+};
+
+// The soft planner's output: tiles of one or more pairs side by side (nwg::SoftTile).
+struct SoftPlan {
+    std::vector<nwg::SoftTile> tiles;   // plan order (plain fill: bucket by bucket)
+    std::vector<Bucket> buckets;
+    std::vector<int32_t> order;
+    int64_t rows = 1;        // max over tiles of fsteps + 1
+    int panels = 1;          // max over tiles of the column panels
+    bool built = false;
+    DevPtr<int32_t> d_order;
+    DevPtr<nwg::SoftTile> d_tiles, d_ptiles;
 };
 
 }  // namespace
@@ -132,15 +147,20 @@ struct nwg_batch {
     // plans: the pairs one lane group holds, the wider ones (column panels), and the pairs
     // a float32 soft pass refused
     int gmin = 1, hist_slots = 64;
-    Plan plan, lplan, rerun, lrerun;
+    Plan plan, lplan;        // hard / score-only: len(B) within one wavefront's columns, and wider
+    // soft: packed tiles (persistent fill) and per-bucket tiles (plain fill), each split into
+    // one-panel and wide (column-panel) tiles; the pairs a float32 pass refused
+    SoftPlan sp_main, sp_wide, sq_main, sq_wide, rr_main, rr_wide;
     int64_t cells = 0;
     DevPtr<int32_t> queue;
     // hard: pointer codes per wave slot
     DevPtr<uint32_t> table;
-    // soft: forward row segments, exponents, maxima, scale-up factors (per precision)
-    DevPtr<float> ft32, mt32, lt32, pf32, pb32;
-    DevPtr<double> ft64, mt64, lt64, pf64, pb64;
-    DevPtr<int32_t> etab, pexp;
+    // soft: row segments of the main and the long-pair pools, panel columns, weights
+    DevPtr<float> seg32, lseg32, pb32, pf32;
+    DevPtr<double> seg64, lseg64, pb64, pf64;
+    DevPtr<int32_t> pexp;
+    DevPtr<int32_t> lqueue;
+    int64_t soft_grid = 0, soft_lslots = 0, soft_occ = 0;   // the last soft launch's shape
     DevPtr<uint8_t> soft_ok;
     std::vector<uint8_t> host_ok;
     int64_t soft_reruns = 0;
@@ -170,8 +190,17 @@ struct nwg_batch {
 
 namespace {
 
-int build_plan(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& pl) {
+// Lane-group width of a pair: enough lanes of S columns for len(B) (at least gmin, at most
+// one wavefront); the hard walk's cross-lane reductions need a power of two.
+int group_width(int n, int S, int gmin, bool pow2) {
+    int g = (n + S - 1) / S;
+    if (pow2) g = next_pow2(g);
+    return std::min(std::max(g, gmin), kMaxG);
+}
+
+int build_plan(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& pl, int S, bool pow2) {
     nwg_ctx* ctx = b->ctx;
+    pl.S = S;
     pl.tiles.clear();
     pl.buckets.clear();
     pl.max_steps = 1;
@@ -179,8 +208,8 @@ int build_plan(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& pl) {
     const int64_t N = (int64_t)pairs.size();
     std::vector<int32_t> gsel((size_t)b->N > 0 ? (size_t)b->N : 1);
     for (int32_t i : pairs) {
-        gsel[(size_t)i] = std::min(std::max(next_pow2((b->nlen[(size_t)i] + kS - 1) / kS), b->gmin), kMaxG);
-        pl.panels = std::max(pl.panels, (b->nlen[(size_t)i] + kMaxN - 1) / kMaxN);
+        gsel[(size_t)i] = group_width(b->nlen[(size_t)i], S, b->gmin, pow2);
+        pl.panels = std::max(pl.panels, (b->nlen[(size_t)i] + S * kMaxG - 1) / (S * kMaxG));
     }
     pl.order = pairs;
     std::stable_sort(pl.order.begin(), pl.order.end(), [&](int32_t x, int32_t y) {
@@ -212,10 +241,15 @@ int build_plan(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& pl) {
         bk.tile_end = (int)pl.tiles.size();
         pl.buckets.push_back(bk);
     }
+    // persistent order: most row steps first (a wave's time is its steps times its panels)
     std::vector<TileDesc> ptiles(pl.tiles);
-    std::stable_sort(ptiles.begin(), ptiles.end(), [](const TileDesc& a, const TileDesc& c) {
-        return (int64_t)(a.max_m + a.G - 1) * a.G > (int64_t)(c.max_m + c.G - 1) * c.G;
-    });
+    auto work = [S](const TileDesc& t) {
+        const int pw = t.G * S;
+        const int64_t np = t.max_n > pw ? (t.max_n + pw - 1) / pw : 1;
+        return ((int64_t)t.max_m + t.G - 1) * np;
+    };
+    std::stable_sort(ptiles.begin(), ptiles.end(),
+                     [&](const TileDesc& a, const TileDesc& c) { return work(a) > work(c); });
     NWG_HIP(ctx, pl.d_order.reserve(std::max<size_t>(pl.order.size(), 1)));
     NWG_HIP(ctx, pl.d_tiles.reserve(std::max<size_t>(pl.tiles.size(), 1)));
     NWG_HIP(ctx, pl.d_ptiles.reserve(std::max<size_t>(pl.tiles.size(), 1)));
@@ -229,7 +263,191 @@ int build_plan(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& pl) {
                                     hipMemcpyHostToDevice, ctx->stream));
     }
     NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    pl.built = true;
     return NWG_OK;
+}
+
+// The pairs of `pairs` cut for S columns per lane: those within one wavefront's columns
+// into `main`, the wider ones (column panels) into `wide`.
+int build_plans(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& main, Plan& wide, int S, bool pow2) {
+    std::vector<int32_t> sp, lp;
+    for (int32_t i : pairs) (b->nlen[(size_t)i] > S * kMaxG ? lp : sp).push_back(i);
+    int rc = build_plan(b, sp, main, S, pow2);
+    if (rc) return rc;
+    return build_plan(b, lp, wide, S, pow2);
+}
+
+// Soft tiles.  A pair needs ceil(n / (64 S)) column panels at least; in a tile of NP panels
+// it takes ceil(n / (S NP)) lanes.  Packed (persistent fill): pairs in order of len(A),
+// longest first, go into the first open tile with enough free lanes, a free histogram slot
+// and a len(A) within 1/16 of the tile's first (so the tile's rows stay even); a pair that
+// fits one panel may also join an open wide tile.  Plain fill: per-bucket tiles of one
+// power-of-two lane-group width (64 / G pairs each), and one wide pair per tile.
+struct SoftBin {
+    nwg::SoftTile t;
+    std::vector<int32_t> pairs;
+    int used = 0;
+};
+
+int upload_soft_plan(nwg_batch* b, SoftPlan& pl, std::vector<SoftBin>& bins, std::vector<Bucket> buckets) {
+    nwg_ctx* ctx = b->ctx;
+    pl.tiles.clear();
+    pl.order.clear();
+    pl.rows = 1;
+    pl.panels = 1;
+    for (SoftBin& sb : bins) {
+        nwg::SoftTile t = sb.t;
+        t.first = (int32_t)pl.order.size();
+        t.npairs = (int16_t)sb.pairs.size();
+        for (int32_t i : sb.pairs) pl.order.push_back(i);
+        pl.rows = std::max<int64_t>(pl.rows, (int64_t)t.fsteps + 1);
+        pl.panels = std::max<int>(pl.panels, t.panels);
+        pl.tiles.push_back(t);
+    }
+    pl.buckets = std::move(buckets);
+    std::vector<nwg::SoftTile> ptiles(pl.tiles);
+    std::stable_sort(ptiles.begin(), ptiles.end(), [](const nwg::SoftTile& a, const nwg::SoftTile& c) {
+        return (int64_t)a.fsteps * a.panels > (int64_t)c.fsteps * c.panels;
+    });
+    const size_t N = pl.order.size();
+    NWG_HIP(ctx, pl.d_order.reserve(std::max<size_t>(N, 1)));
+    NWG_HIP(ctx, pl.d_tiles.reserve(std::max<size_t>(pl.tiles.size(), 1)));
+    NWG_HIP(ctx, pl.d_ptiles.reserve(std::max<size_t>(pl.tiles.size(), 1)));
+    if (N)
+        NWG_HIP(ctx, hipMemcpyAsync(pl.d_order.p, pl.order.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice,
+                                    ctx->stream));
+    if (!pl.tiles.empty()) {
+        NWG_HIP(ctx, hipMemcpyAsync(pl.d_tiles.p, pl.tiles.data(), sizeof(nwg::SoftTile) * pl.tiles.size(),
+                                    hipMemcpyHostToDevice, ctx->stream));
+        NWG_HIP(ctx, hipMemcpyAsync(pl.d_ptiles.p, ptiles.data(), sizeof(nwg::SoftTile) * ptiles.size(),
+                                    hipMemcpyHostToDevice, ctx->stream));
+    }
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    pl.built = true;
+    return NWG_OK;
+}
+
+void bin_add(nwg_batch* b, SoftBin& sb, int32_t i, int G) {
+    nwg::SoftTile& t = sb.t;
+    const int k = (int)sb.pairs.size();
+    t.G[k] = (uint8_t)G;
+    t.lo[k] = (uint8_t)sb.used;
+    sb.used += G;
+    sb.pairs.push_back(i);
+    t.npairs = (int16_t)sb.pairs.size();
+    t.max_m = std::max(t.max_m, b->mlen[(size_t)i]);
+    t.max_n = std::max(t.max_n, b->nlen[(size_t)i]);
+    t.fsteps = std::max(t.fsteps, b->mlen[(size_t)i] + G - 1);
+}
+
+SoftBin new_bin(int panels) {
+    SoftBin sb;
+    std::memset(&sb.t, 0, sizeof sb.t);
+    sb.t.panels = (int16_t)panels;
+    sb.t.fsteps = 1;
+    return sb;
+}
+
+int build_soft_plans(nwg_batch* b, const std::vector<int32_t>& pairs, SoftPlan& main, SoftPlan& wide, bool packed) {
+    const int S = kSS;
+    const int slots = std::min(b->hist_slots, nwg::kTilePairs);
+    auto min_panels = [&](int32_t i) { return std::max(1, (b->nlen[(size_t)i] + S * kMaxG - 1) / (S * kMaxG)); };
+    auto lanes = [&](int32_t i, int np) { return std::max(1, (b->nlen[(size_t)i] + S * np - 1) / (S * np)); };
+    std::vector<SoftBin> mbins, wbins;
+    std::vector<Bucket> mb, wb;
+    if (packed) {
+        std::vector<int32_t> ord(pairs);
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](int32_t x, int32_t y) { return b->mlen[(size_t)x] > b->mlen[(size_t)y]; });
+        constexpr size_t kWindow = 32;
+        std::vector<size_t> mopen, wopen;   // indices of open bins
+        auto close_full = [&](std::vector<size_t>& open, std::vector<SoftBin>& bins) {
+            open.erase(std::remove_if(open.begin(), open.end(),
+                                      [&](size_t k) {
+                                          return bins[k].used >= kMaxG || (int)bins[k].pairs.size() >= slots;
+                                      }),
+                       open.end());
+            while (open.size() > kWindow) open.erase(open.begin());
+        };
+        auto near = [&](const SoftBin& sb, int32_t i) {
+            const int m0 = sb.t.max_m, m = b->mlen[(size_t)i];
+            return m0 - m <= std::max(4, m0 >> 4);
+        };
+        auto try_open = [&](std::vector<size_t>& open, std::vector<SoftBin>& bins, int32_t i, int need_np) {
+            for (size_t k : open) {
+                SoftBin& sb = bins[k];
+                if (sb.t.panels < need_np || !near(sb, i)) continue;
+                const int G = lanes(i, sb.t.panels);
+                if (sb.used + G <= kMaxG && (int)sb.pairs.size() < slots) {
+                    bin_add(b, sb, i, G);
+                    return true;
+                }
+            }
+            return false;
+        };
+        for (int32_t i : ord) {
+            const int np = min_panels(i);
+            if (np > 1) {
+                if (!try_open(wopen, wbins, i, np)) {
+                    wbins.push_back(new_bin(np));
+                    bin_add(b, wbins.back(), i, lanes(i, np));
+                    wopen.push_back(wbins.size() - 1);
+                }
+                close_full(wopen, wbins);
+            } else {
+                if (!try_open(wopen, wbins, i, 1) && !try_open(mopen, mbins, i, 1)) {
+                    mbins.push_back(new_bin(1));
+                    bin_add(b, mbins.back(), i, lanes(i, 1));
+                    mopen.push_back(mbins.size() - 1);
+                }
+                close_full(mopen, mbins);
+                close_full(wopen, wbins);
+            }
+        }
+        mb.push_back(Bucket{0, 0, (int)mbins.size()});
+        wb.push_back(Bucket{0, 0, (int)wbins.size()});
+    } else {
+        // plain: power-of-two lane groups, 64 / G pairs per tile, no more than the slots
+        int gmin = 1;
+        while (64 / gmin > slots) gmin <<= 1;
+        std::vector<int32_t> ms, ws;
+        for (int32_t i : pairs) (min_panels(i) > 1 ? ws : ms).push_back(i);
+        auto gsel = [&](int32_t i) { return std::min(std::max(next_pow2(lanes(i, 1)), gmin), kMaxG); };
+        std::stable_sort(ms.begin(), ms.end(), [&](int32_t x, int32_t y) {
+            if (gsel(x) != gsel(y)) return gsel(x) < gsel(y);
+            return b->mlen[(size_t)x] > b->mlen[(size_t)y];
+        });
+        size_t pos = 0;
+        while (pos < ms.size()) {
+            const int G = gsel(ms[pos]);
+            Bucket bk{G, (int)mbins.size(), 0};
+            while (pos < ms.size() && gsel(ms[pos]) == G) {
+                mbins.push_back(new_bin(1));
+                for (int k = 0; k < 64 / G && pos < ms.size() && gsel(ms[pos]) == G; ++k) bin_add(b, mbins.back(), ms[pos++], G);
+            }
+            bk.tile_end = (int)mbins.size();
+            mb.push_back(bk);
+        }
+        std::stable_sort(ws.begin(), ws.end(), [&](int32_t x, int32_t y) {
+            if (min_panels(x) != min_panels(y)) return min_panels(x) < min_panels(y);
+            return b->mlen[(size_t)x] > b->mlen[(size_t)y];
+        });
+        pos = 0;
+        while (pos < ws.size()) {
+            const int np = min_panels(ws[pos]);
+            Bucket bk{np, (int)wbins.size(), 0};
+            while (pos < ws.size() && min_panels(ws[pos]) == np) {
+                wbins.push_back(new_bin(np));
+                bin_add(b, wbins.back(), ws[pos], lanes(ws[pos], np));
+                ++pos;
+            }
+            bk.tile_end = (int)wbins.size();
+            wb.push_back(bk);
+        }
+    }
+    int rc = upload_soft_plan(b, main, mbins, mb);
+    if (rc) return rc;
+    return upload_soft_plan(b, wide, wbins, wb);
 }
 
 }  // namespace
@@ -307,12 +525,20 @@ int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t
     // The hard walk keeps uint16 match counts per pair slot in LDS, the soft pass double
     // expected counts: the slot count, hence the narrowest lane group, follows the budget.
     const int nn = n_alpha * n_alpha;
-    const int hist_entry = prob->grad_mode == 1 ? 2 : prob->grad_mode == 2 ? 8 : 0;
-    const int hist_budget = prob->grad_mode == 2 ? kSoftHistBytes : kHistBytes;
-    int gmin = 1;
-    while (gmin < kMaxG && (64 / gmin) * nn * hist_entry > hist_budget) gmin <<= 1;
-    b->gmin = gmin;
-    b->hist_slots = 64 / gmin;
+    if (prob->grad_mode == 2) {
+        // soft: any width, floor(64 / G) pair slots of nn doubles
+        const int slots = std::max(1, std::min(nwg::kTilePairs, kSoftHistBytes / (nn * 8)));
+        int gmin = 1;
+        while (64 / gmin > slots) ++gmin;
+        b->gmin = gmin;
+        b->hist_slots = slots;
+    } else {
+        const int hist_entry = prob->grad_mode == 1 ? 2 : 0;
+        int gmin = 1;
+        while (gmin < kMaxG && (64 / gmin) * nn * hist_entry > kHistBytes) gmin <<= 1;
+        b->gmin = gmin;
+        b->hist_slots = 64 / gmin;
+    }
 
     NWG_HIP(ctx, b->ca.alloc((size_t)std::max<int64_t>(la, 1)));
     NWG_HIP(ctx, b->cb.alloc((size_t)std::max<int64_t>(lb, 1)));
@@ -325,12 +551,12 @@ int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t
     if (lb) NWG_HIP(ctx, hipMemcpy(b->cb.p, codes_b, (size_t)lb, hipMemcpyHostToDevice));
     NWG_HIP(ctx, hipMemcpy(b->oa.p, off_a, sizeof(int64_t) * (size_t)(N + 1), hipMemcpyHostToDevice));
     NWG_HIP(ctx, hipMemcpy(b->ob.p, off_b, sizeof(int64_t) * (size_t)(N + 1), hipMemcpyHostToDevice));
-    std::vector<int32_t> shortp, longp;
-    for (int64_t i = 0; i < N; ++i) (b->nlen[(size_t)i] > kMaxN ? longp : shortp).push_back((int32_t)i);
-    int rc = build_plan(b.get(), shortp, b->plan);
-    if (rc) return rc;
-    rc = build_plan(b.get(), longp, b->lplan);
-    if (rc) return rc;
+    if (prob->grad_mode != 2) {
+        std::vector<int32_t> all((size_t)N);
+        std::iota(all.begin(), all.end(), 0);
+        const int rc = build_plans(b.get(), all, b->plan, b->lplan, kS, true);
+        if (rc) return rc;
+    }
     NWG_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&b->host_scores),
                                sizeof(double) * (size_t)std::max<int64_t>(N, 1), hipHostMallocDefault));
     NWG_HIP(ctx, hipEventCreate(&b->ev0));
@@ -460,23 +686,36 @@ int launch_hard(nwg_batch* b, const Plan& pl) {
 // ── soft pass ──────────────────────────────────────────────────────────────
 template <typename SF> struct SoftScratch;
 template <> struct SoftScratch<float> {
-    static DevPtr<float>& ft(nwg_batch* b) { return b->ft32; }
-    static DevPtr<float>& mt(nwg_batch* b) { return b->mt32; }
-    static DevPtr<float>& lt(nwg_batch* b) { return b->lt32; }
+    static DevPtr<float>& seg(nwg_batch* b) { return b->seg32; }
+    static DevPtr<float>& lseg(nwg_batch* b) { return b->lseg32; }
     static DevPtr<float>& pf(nwg_batch* b) { return b->pf32; }
     static DevPtr<float>& pb(nwg_batch* b) { return b->pb32; }
 };
 template <> struct SoftScratch<double> {
-    static DevPtr<double>& ft(nwg_batch* b) { return b->ft64; }
-    static DevPtr<double>& mt(nwg_batch* b) { return b->mt64; }
-    static DevPtr<double>& lt(nwg_batch* b) { return b->lt64; }
+    static DevPtr<double>& seg(nwg_batch* b) { return b->seg64; }
+    static DevPtr<double>& lseg(nwg_batch* b) { return b->lseg64; }
     static DevPtr<double>& pf(nwg_batch* b) { return b->pf64; }
     static DevPtr<double>& pb(nwg_batch* b) { return b->pb64; }
 };
 
+using nwg::SoftTile;
+
 template <typename SF>
-int launch_soft(nwg_batch* b, const Plan& pl) {
+using SoftKernel = void (*)(nwg::SoftArgs<SF>);
+
+template <typename SF>
+SoftKernel<SF> soft_kernel_for(int mode) {
+    return mode == 1 ? nwg::soft_kernel<SF, nwg::kLocal, kSS>
+                     : mode == 2 ? nwg::soft_kernel<SF, nwg::kGlocal, kSS> : nwg::soft_kernel<SF, nwg::kGlobal, kSS>;
+}
+
+// One soft pass over the tiles of `pl` (one panel) and `wl` (column panels).  Persistent:
+// one launch whose blocks below n_lslots first drain the wide tiles' queue, then blocks below
+// n_mslots the main queue.  Plain: one launch per bucket, main plan then wide plan.
+template <typename SF>
+int launch_soft(nwg_batch* b, const SoftPlan& pl, const SoftPlan& wl) {
     using SS = SoftScratch<SF>;
+    using LY = nwg::Seg<SF, kSS>;
     nwg_ctx* ctx = b->ctx;
     const int nn = b->na * b->na;
     const bool lin = b->prob.gap_model == 0;
@@ -495,41 +734,47 @@ int launch_soft(nwg_batch* b, const Plan& pl) {
     const double ob = lin ? eb : std::exp(-(b->gaps[2] + b->gaps[3]) * it);
     for (double v : {ea, eb, oa, ob})
         if (!(v <= big)) return fail(ctx, NWG_ERANGE, "nwgrad: soft path: a gap weight overflows");
+    if (pl.tiles.empty() && wl.tiles.empty()) return NWG_OK;
     DevPtr<SF>& dps = SS::pf(b);
     NWG_HIP(ctx, dps.reserve((size_t)nn));
     NWG_HIP(ctx, hipMemcpyAsync(dps.p, ps.data(), sizeof(SF) * (size_t)nn, hipMemcpyHostToDevice, ctx->stream));
-    void (*kern)(nwg::SoftArgs<SF>) = b->prob.mode == 1   ? nwg::soft_kernel<SF, nwg::kLocal, kS>
-                                      : b->prob.mode == 2 ? nwg::soft_kernel<SF, nwg::kGlocal, kS>
-                                                          : nwg::soft_kernel<SF, nwg::kGlobal, kS>;
-    const size_t lds = (size_t)b->hist_slots * nn * sizeof(double) + (size_t)nn * sizeof(SF);
-    constexpr int P = nwg::SegLayout<kS>::padded;
-    const int64_t rows = pl.max_steps + 1;
-    const size_t slot_bytes = (size_t)pl.panels * (size_t)rows *
-                              ((size_t)P * 64 * sizeof(SF) + 64 * (sizeof(int32_t) + 2 * sizeof(SF)) + 6 * sizeof(SF) +
-                               2 * sizeof(int32_t));
-    const int64_t slots = std::max<int64_t>(
-        std::min<int64_t>(std::min<int64_t>((int64_t)pl.tiles.size(), resident_blocks(ctx, kern, lds, kSoftBlocksPerCU)),
-                          (int64_t)(kScratchBudget / std::max<size_t>(slot_bytes, 1))),
-        1);
-    const size_t seg = (size_t)(slots * pl.panels * rows * 64);
-    const size_t bnd = (size_t)(slots * pl.panels * rows);
-    if (SS::ft(b).n < seg * P || b->etab.n < seg || SS::mt(b).n < seg || SS::lt(b).n < seg ||
-        SS::pb(b).n < bnd * 6 || b->pexp.n < bnd * 2) {
+    SoftKernel<SF> kern = soft_kernel_for<SF>(b->prob.mode);
+    const size_t lds = (size_t)b->hist_slots * nn * sizeof(double) + (size_t)(nn + b->na) * sizeof(SF);
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64, lds) != hipSuccess || occ <= 0)
+        occ = kSoftBlocksPerCU;
+    b->soft_occ = occ;
+    const int64_t resident = (int64_t)occ * ctx->n_cu;
+    const bool persistent = b->fill == 0;
+    // Slots: the main pool holds one panel region of pl.rows rows per slot; the wide pool
+    // wl.panels regions of wl.rows rows.  Plain launches use one slot per tile of a launch.
+    const size_t row_bytes = (size_t)LY::row * sizeof(SF);
+    const size_t mslot = (size_t)pl.rows * row_bytes, lslot = (size_t)wl.panels * (size_t)wl.rows * row_bytes;
+    int64_t mslots = pl.tiles.empty() ? 0 : std::min<int64_t>((int64_t)pl.tiles.size(), resident);
+    int64_t lslots = wl.tiles.empty() ? 0 : std::min<int64_t>((int64_t)wl.tiles.size(), resident);
+    // within the scratch budget: the wide pool first takes what it needs of half of it
+    if (lslots > 0) lslots = std::max<int64_t>(1, std::min<int64_t>(lslots, (int64_t)(kScratchBudget / 2 / lslot)));
+    if (mslots > 0)
+        mslots = std::max<int64_t>(
+            1, std::min<int64_t>(mslots, (int64_t)((kScratchBudget - (size_t)lslots * lslot) / std::max<size_t>(mslot, 1))));
+    const size_t mseg = (size_t)mslots * pl.rows * LY::row, lseg = (size_t)lslots * wl.panels * wl.rows * LY::row;
+    const size_t bnd = (size_t)lslots * wl.panels * wl.rows * nwg::kTilePairs;
+    if (SS::seg(b).n < mseg || SS::lseg(b).n < lseg || SS::pb(b).n < bnd * 6 || b->pexp.n < bnd * 2) {
         NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        NWG_HIP(ctx, SS::ft(b).reserve(seg * P));
-        NWG_HIP(ctx, b->etab.reserve(seg));
-        NWG_HIP(ctx, SS::mt(b).reserve(seg));
-        NWG_HIP(ctx, SS::lt(b).reserve(seg));
-        NWG_HIP(ctx, SS::pb(b).reserve(bnd * 6));
-        NWG_HIP(ctx, b->pexp.reserve(bnd * 2));
+        NWG_HIP(ctx, SS::seg(b).reserve(std::max<size_t>(mseg, 1)));
+        NWG_HIP(ctx, SS::lseg(b).reserve(std::max<size_t>(lseg, 1)));
+        NWG_HIP(ctx, SS::pb(b).reserve(std::max<size_t>(bnd * 6, 1)));
+        NWG_HIP(ctx, b->pexp.reserve(std::max<size_t>(bnd * 2, 1)));
     }
     NWG_HIP(ctx, b->soft_ok.reserve((size_t)std::max<int64_t>(b->N, 1)));
+    NWG_HIP(ctx, b->lqueue.reserve(1));
     nwg::SoftArgs<SF> A{};
     A.codes_a = b->ca.p;
     A.codes_b = b->cb.p;
     A.off_a = b->oa.p;
     A.off_b = b->ob.p;
-    A.order = pl.d_order.p;
+    A.view = nwg::SoftView<SF>{SS::seg(b).p, SS::pb(b).p, b->pexp.p, pl.rows, 1};
+    A.lview = nwg::SoftView<SF>{SS::lseg(b).p, SS::pb(b).p, b->pexp.p, wl.rows, wl.panels};
     A.psub = dps.p;
     A.na = b->na;
     A.hist_slots = b->hist_slots;
@@ -542,40 +787,69 @@ int launch_soft(nwg_batch* b, const Plan& pl) {
     A.scores = b->scores.p;
     A.grads = b->grads.p;
     A.ok = b->soft_ok.p;
-    A.ftab = SS::ft(b).p;
-    A.etab = b->etab.p;
-    A.mtab = SS::mt(b).p;
-    A.ltab = SS::lt(b).p;
     A.tol = std::is_same<SF, float>::value ? 0x1p-20 : 0x1p-45;   // the reference's SoftRange<F>::tol
     A.vmin = (double)std::numeric_limits<SF>::min();
-    A.slot_rows = rows;
-    A.pbnd = SS::pb(b).p;
-    A.pexp = b->pexp.p;
-    A.panels = pl.panels;
-    if (pl.tiles.empty()) return NWG_OK;
-    if (b->fill == 0) {
+    A.vtiny = (double)std::numeric_limits<SF>::denorm_min();
+    if (persistent) {
+        const int64_t grid = std::max<int64_t>(mslots, lslots);
         NWG_HIP(ctx, hipMemsetAsync(b->queue.p, 0, sizeof(int32_t), ctx->stream));
+        NWG_HIP(ctx, hipMemsetAsync(b->lqueue.p, 0, sizeof(int32_t), ctx->stream));
+        A.order = pl.d_order.p;
         A.tiles = pl.d_ptiles.p;
         A.n_tiles = (int32_t)pl.tiles.size();
         A.queue = b->queue.p;
-        A.tile_base = 0;
-        hipLaunchKernelGGL(kern, dim3((unsigned)slots), dim3(64), lds, ctx->stream, A);
+        A.n_mslots = (int32_t)mslots;
+        A.lorder = wl.d_order.p;
+        A.ltiles = wl.d_ptiles.p;
+        A.n_ltiles = (int32_t)wl.tiles.size();
+        A.n_lslots = (int32_t)lslots;
+        A.lqueue = b->lqueue.p;
+        b->soft_grid = grid;
+        b->soft_lslots = lslots;
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64), lds, ctx->stream, A);
         NWG_HIP(ctx, hipGetLastError());
-    } else {
-        // plain: one launch per lane-group bucket, at most `slots` tiles per launch
-        A.tiles = pl.d_tiles.p;
-        A.queue = nullptr;
-        for (const Bucket& bk : pl.buckets) {
+        return NWG_OK;
+    }
+    // plain: one launch per bucket, at most `slots` tiles per launch
+    A.queue = nullptr;
+    A.lqueue = nullptr;
+    b->soft_grid = 0;
+    b->soft_lslots = 0;
+    for (int w = 0; w < 2; ++w) {
+        const SoftPlan& P = w == 0 ? pl : wl;
+        const int64_t slots = w == 0 ? mslots : lslots;
+        if (P.tiles.empty()) continue;
+        nwg::SoftArgs<SF> B = A;
+        B.order = P.d_order.p;
+        B.tiles = P.d_tiles.p;
+        if (w == 1) B.view = A.lview;
+        for (const Bucket& bk : P.buckets) {
             for (int t0 = bk.tile_begin; t0 < bk.tile_end; t0 += (int)slots) {
                 const int t1 = std::min<int>(bk.tile_end, t0 + (int)slots);
-                A.tile_base = t0;
-                A.n_tiles = t1;
-                hipLaunchKernelGGL(kern, dim3(t1 - t0), dim3(64), lds, ctx->stream, A);
+                B.tile_base = t0;
+                B.n_tiles = t1;
+                hipLaunchKernelGGL(kern, dim3(t1 - t0), dim3(64), lds, ctx->stream, B);
                 NWG_HIP(ctx, hipGetLastError());
             }
         }
     }
     return NWG_OK;
+}
+
+// A batch's own soft pass: float32 or double, packed tiles (persistent) or per-bucket tiles
+// (plain), planned on first use.
+int launch_soft_batch(nwg_batch* b) {
+    const bool packed = b->fill == 0;
+    SoftPlan& mp = packed ? b->sp_main : b->sq_main;
+    SoftPlan& wp = packed ? b->sp_wide : b->sq_wide;
+    if (!mp.built) {
+        std::vector<int32_t> all((size_t)b->N);
+        std::iota(all.begin(), all.end(), 0);
+        const int rc = build_soft_plans(b, all, mp, wp, packed);
+        if (rc) return rc;
+    }
+    if (b->prob.precision == 0 && b->soft_precision == 0) return launch_soft<float>(b, mp, wp);
+    return launch_soft<double>(b, mp, wp);
 }
 
 // After a soft pass: a pair whose float32 pass lost range or mass beyond 2^-20 reruns in
@@ -595,15 +869,9 @@ int soft_finish(nwg_batch* b) {
     const bool ran_f32 = b->prob.precision == 0 && b->soft_precision == 0;
     if (ran_f32) {
         b->soft_reruns = (int64_t)refused.size();
-        std::vector<int32_t> rs, rl;
-        for (int32_t i : refused) (b->nlen[(size_t)i] > kMaxN ? rl : rs).push_back(i);
-        int rc = build_plan(b, rs, b->rerun);
+        int rc = build_soft_plans(b, refused, b->rr_main, b->rr_wide, b->fill == 0);
         if (rc) return rc;
-        rc = build_plan(b, rl, b->lrerun);
-        if (rc) return rc;
-        rc = launch_soft<double>(b, b->rerun);
-        if (rc) return rc;
-        rc = launch_soft<double>(b, b->lrerun);
+        rc = launch_soft<double>(b, b->rr_main, b->rr_wide);
         if (rc) return rc;
         NWG_HIP(ctx, hipMemcpyAsync(b->host_ok.data(), b->soft_ok.p, (size_t)b->N, hipMemcpyDeviceToHost, ctx->stream));
         NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -626,14 +894,15 @@ int run_async(nwg_batch* b) {
     NWG_HIP(ctx, hipSetDevice(ctx->device));
     NWG_HIP(ctx, hipEventRecord(b->ev0, ctx->stream));
     int rc = NWG_OK;
-    for (Plan* pl : {&b->plan, &b->lplan}) {
-        if (pl->tiles.empty()) continue;
-        if (b->prob.grad_mode == 2)
-            rc = (b->prob.precision == 0 && b->soft_precision == 0) ? launch_soft<float>(b, *pl)
-                                                                    : launch_soft<double>(b, *pl);
-        else
-            rc = b->prob.precision == 0 ? launch_hard<float>(b, *pl) : launch_hard<double>(b, *pl);
+    if (b->prob.grad_mode == 2) {
+        rc = launch_soft_batch(b);
         if (rc) return rc;
+    } else {
+        for (Plan* pl : {&b->plan, &b->lplan}) {
+            if (pl->tiles.empty()) continue;
+            rc = b->prob.precision == 0 ? launch_hard<float>(b, *pl) : launch_hard<double>(b, *pl);
+            if (rc) return rc;
+        }
     }
     NWG_HIP(ctx, hipEventRecord(b->ev1, ctx->stream));
     b->results_valid = true;
@@ -764,7 +1033,21 @@ int nwg_batch_weighted_grad_dev(nwg_batch* b, const double* w_dev, double* out_d
 
 int nwg_batch_plan_info(nwg_batch* b, int64_t* n_tiles, int64_t* n_cells) {
     if (!b) return NWG_EINVAL;
-    if (n_tiles) *n_tiles = (int64_t)(b->plan.tiles.size() + b->lplan.tiles.size());
+    if (n_tiles) {
+        if (b->prob.grad_mode == 2) {
+            SoftPlan& mp = b->fill == 0 ? b->sp_main : b->sq_main;
+            SoftPlan& wp = b->fill == 0 ? b->sp_wide : b->sq_wide;
+            if (!mp.built) {
+                std::vector<int32_t> all((size_t)b->N);
+                std::iota(all.begin(), all.end(), 0);
+                const int rc = build_soft_plans(b, all, mp, wp, b->fill == 0);
+                if (rc) return rc;
+            }
+            *n_tiles = (int64_t)(mp.tiles.size() + wp.tiles.size());
+        } else {
+            *n_tiles = (int64_t)(b->plan.tiles.size() + b->lplan.tiles.size());
+        }
+    }
     if (n_cells) *n_cells = b->cells;
     return NWG_OK;
 }
@@ -772,6 +1055,14 @@ int nwg_batch_plan_info(nwg_batch* b, int64_t* n_tiles, int64_t* n_cells) {
 int nwg_batch_last_kernel_ms(nwg_batch* b, double* ms) {
     if (!b || !ms) return NWG_EINVAL;
     *ms = b->last_ms;
+    return NWG_OK;
+}
+
+int nwg_batch_launch_info(nwg_batch* b, int64_t* grid, int64_t* wide_slots, int64_t* per_cu) {
+    if (!b) return NWG_EINVAL;
+    if (grid) *grid = b->soft_grid;
+    if (wide_slots) *wide_slots = b->soft_lslots;
+    if (per_cu) *per_cu = b->soft_occ;
     return NWG_OK;
 }
 

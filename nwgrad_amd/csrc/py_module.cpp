@@ -247,6 +247,8 @@ void bind_batch(py::module_& m, const char* name) {
         .def_property_readonly("cells", [](const PB& s) { return s.b->cells(); },
                                "Sum of len(A)*len(B) over the pairs (the GCUPS numerator).")
         .def_property_readonly("tiles", [](const PB& s) { return s.b->tiles(); })
+        .def_property_readonly("launch_info", [](const PB& s) { auto r = s.b->launch_info(); return py::make_tuple(r[0], r[1], r[2]); },
+                               "(grid, wide-queue blocks, resident blocks per CU) of the last soft launch")
         .def_property_readonly("soft_reruns", [](const PB& s) { return s.b->soft_reruns(); },
                                "Pairs of the last soft pass rerun in double after the float32 pass refused them.")
         .def_property_readonly("last_kernel_ms", [](const PB& s) { return s.b->last_kernel_ms(); },

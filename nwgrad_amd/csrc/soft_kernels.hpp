@@ -5,31 +5,71 @@
 //   backward + fused sums   soft_kernel_impl.inl:790-977
 //   Z and the end rules     soft_kernel_impl.inl:294-321 (sp_log_z)
 //   gap-target rows         soft_kernel_impl.inl:736-737 (kmin, kmin0)
+//   lost-mass bound         soft_kernel_impl.inl:141-178 (sp_loss), 933-948 (flushed products)
 //   score = T log Z, gap fields negated  aligner.hpp:6554-6626, soft_kernel_impl.inl:987
 // Linear gaps run as the affine model with open = 0: with oa = ea and ob = eb the affine
 // total M + X + Y obeys H(i,j) = H(i-1,j-1) P + H(i-1,j) eb + H(i,j-1) ea, which is the linear
 // recurrence (aligner.hpp:6231-6260), and the expected X / Y moves are its gap counts.
 //
-// Mapping: the lane-group layout of the hard fill (hard_kernels.hpp).  The reference scales
-// each DP row by a power of two chosen from the row maximum; lanes of a slot run skewed and
-// never hold a whole row at once, so here every (lane, row) segment carries its own
-// power-of-two exponent E (true value = stored value * 2^E).  A lane computes its row in its
-// own exponent, converts the boundary values it receives from its neighbour (an exact power-
-// of-two product), and renormalises the segment when its maximum leaves [2^-w, 2^w].  Values
-// more than the format's range below a segment's maximum flush, as in the reference's row
-// scaling; a segment is never coarser than the reference's row.
+// Mapping.  A tile is one wavefront holding up to kTilePairs pairs side by side: pair q owns
+// lanes lo[q] .. lo[q]+G[q]-1, and lane g of it owns S columns of each of the tile's column
+// panels (panel pp: columns pp*G*S + g*S + 1 .. + S).  Lanes run skewed (row t-g+1 at step t)
+// and receive the boundary column from the lane below by a DPP wave shift; a panel's last
+// column passes to the next panel through HBM.  The backward runs the mirrored skew.
 //
-// The forward stores every row segment of the lane (M, X, Y of its S columns plus the
-// boundary column it received, converted to its exponent) and the exponent to HBM scratch;
-// the backward runs the mirrored skew (lane G-1 first), carries F(i) and F(i-1) in registers,
-// and accumulates the gradient sums per row in the working precision, scaled into double
-// by the row's posterior factor 2^(E_F + E_B - ze) / zr, with match posteriors added into a
-// per-pair double LDS histogram (ds_add_f64).
+// Scaling.  The reference scales each DP row by a power of two chosen from the row maximum;
+// lanes of a slot run skewed and never hold a whole row at once, so every (lane, row)
+// segment carries its own power-of-two exponent E (true value = stored value * 2^E).  A row
+// is computed and stored in one exponent; when its maximum leaves [2^-w, 2^w] the NEXT row is
+// computed in new units, by folding the factor 2^-k into that row's weights (the profile
+// values and the two vertical gap weights) instead of rescaling the registers.  Every value
+// is therefore computed once, in the units it is stored in, and a value that flushes was
+// below the format's MIN in its own stored units -- the premise of the lost-mass bound.
+// Boundary values from a neighbour are converted by an exact power-of-two product; a segment
+// without mass of its own takes units in which the neighbour's boundary values are of order
+// one, and one whose neighbour's values would not fit below 2^headroom of its own units moves
+// to the neighbour's units minus the headroom (its old values then flush towards zero, as in
+// the reference's row scaling).
+//
+// Storage.  The forward stores every row segment of every lane -- M, X, Y of its S columns,
+// the boundary column it received, the exponent and the segment maximum -- as 16-byte
+// vectors laid out so that each vector of the 64 lanes is contiguous.  The backward needs ONE
+// forward row per step: the sums of row i+1 that read forward row i (the X-open counts and
+// the match posteriors) are formed one step late, from the row-(i+1) backward values still in
+// registers.  The next forward row is loaded one step ahead.  Row sums are formed in the
+// working precision, as in the reference, then scaled into double by the row's posterior
+// factor 2^(E_F + E_B - ze) / zr; match posteriors go into a per-pair double LDS histogram
+// (ds_add_f64).
 #pragma once
 
 #include "common.hpp"
 
 namespace nwg {
+
+constexpr int kTilePairs = 16;  // pairs per soft tile at most
+
+// One soft tile: pairs order[first .. first + npairs), pair q on lanes lo[q] .. lo[q]+G[q]-1.
+struct SoftTile {
+    int32_t first;
+    int32_t max_m;
+    int32_t max_n;
+    int32_t fsteps;    // max over pairs of m + G - 1: row steps of one forward panel
+    int16_t npairs;
+    int16_t panels;    // column panels (every pair of the tile runs the same count)
+    uint8_t G[kTilePairs];
+    uint8_t lo[kTilePairs];
+};
+static_assert(sizeof(SoftTile) == 52, "SoftTile is 52 bytes");
+
+// One scratch pool: row segments and panel-boundary columns per wave slot.
+template <typename SF>
+struct SoftView {
+    SF* seg;          // row segments: per slot, panels x rows x (NV x 64) vectors
+    SF* pbnd;         // panel boundary columns: per slot, panel, row, pair: M, X, Y, BM, BY
+    int32_t* pexp;    // their exponents: forward, backward
+    int64_t rows;     // rows of one panel region (at least fsteps + 1 over the pool's tiles)
+    int32_t panels;   // panel regions per slot
+};
 
 template <typename SF>
 struct SoftArgs {
@@ -37,12 +77,22 @@ struct SoftArgs {
     const uint8_t* codes_b;
     const int64_t* off_a;
     const int64_t* off_b;
+    // main queue (or, without a queue, the plain launch's tiles from tile_base)
     const int32_t* order;
-    const TileDesc* tiles;
+    const SoftTile* tiles;
     int32_t n_tiles;
     int32_t tile_base;
     int32_t* queue;
-    const SF* psub;          // exp(matrix / T), n_alpha * n_alpha
+    int32_t n_mslots;        // blocks below this own a main-pool slot and drain the main queue
+    SoftView<SF> view;
+    // wide queue: tiles in column panels; blocks below n_lslots drain it first
+    const int32_t* lorder;
+    const SoftTile* ltiles;
+    int32_t n_ltiles;
+    int32_t n_lslots;
+    int32_t* lqueue;
+    SoftView<SF> lview;
+    const SF* psub;          // exp(matrix / T), n_alpha * n_alpha (LDS adds a row of ones)
     int32_t na;
     int32_t hist_slots;
     SF oa, ea, ob, eb;       // exp(-(go+ge)/T), exp(-ge/T) per sequence
@@ -50,34 +100,26 @@ struct SoftArgs {
     double temperature;
     double* scores;
     double* grads;
-    uint8_t* ok;             // per pair: 1 in range, 0 the pass lost range (overflow / non-finite)
-    SF* ftab;                // forward row segments, per wave slot
-    int32_t* etab;           // their exponents
-    SF* mtab;                // their maxima (stored units, after renormalisation)
-    SF* ltab;                // 2^max(0, -k) of their renormalisation by 2^-k: a value lost
-                             // below MIN before a scale-up is below MIN * ltab stored
+    uint8_t* ok;             // per pair: 1 accepted, 0 refused (range, non-finite or lost mass)
     double tol;              // lost-mass tolerance (2^-20 float32, 2^-45 double)
     double vmin;             // the working format's smallest normal value
-    SF* pbnd;                // panel boundary columns: per row M, X, Y (forward), BM, BY (backward)
-    int32_t* pexp;           // their exponents: forward, backward
-    int32_t panels;          // panel regions per slot
-    int64_t slot_rows;       // rows of one panel region (max over tiles of max_m + G)
+    double vtiny;            // its smallest subnormal value
 };
 
 template <typename SF>
 struct SoftRange;
 template <>
 struct SoftRange<float> {
-    static constexpr int w = 16;         // renormalise outside [2^-16, 2^16]
-    static constexpr int headroom = 64;  // boundary values at most 2^64 above a segment's scale
-    static constexpr float hi = 65536.0f, lo = 1.0f / 65536.0f;
+    static constexpr int headroom = 64;  // boundary values at most 2^64 above a segment's units
+    static constexpr int kmax = 120;     // largest renormalisation step
+    static constexpr float hi = 65536.0f, lo = 1.0f / 65536.0f;   // renormalise outside [2^-16, 2^16]
     static constexpr float fmax = 3.4028234663852886e38f;
 };
 template <>
 struct SoftRange<double> {
-    static constexpr int w = 256;
     static constexpr int headroom = 512;
-    static constexpr double hi = 1.157920892373162e77, lo = 8.636168555094445e-78;
+    static constexpr int kmax = 1000;
+    static constexpr double hi = 1.157920892373162e77, lo = 8.636168555094445e-78;   // 2^+-256
     static constexpr double fmax = 1.7976931348623157e308;
 };
 
@@ -87,6 +129,38 @@ __device__ __forceinline__ int silogb(float x) { return __builtin_amdgcn_frexp_e
 __device__ __forceinline__ int silogb(double x) { return __builtin_amdgcn_frexp_exp(x) - 1; }
 __device__ __forceinline__ float sfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double sfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float e2sf(int e, float) { return __int_as_float(e); }
+__device__ __forceinline__ double e2sf(int e, double) { return __longlong_as_double((long long)e); }
+__device__ __forceinline__ int sf2e(float v) { return __float_as_int(v); }
+__device__ __forceinline__ int sf2e(double v) { return (int)__double_as_longlong(v); }
+
+// One-lane wave shifts on the DPP path (a VALU move, no LDS round trip): up = from the lane
+// below (wave_shr:1), down = from the lane above (wave_shl:1).  Lane 0 (up) and lane 63
+// (down) receive 0; every caller replaces or ignores that value.
+__device__ __forceinline__ int dpp_up(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_down(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ float wshift_up(float v) { return __int_as_float(dpp_up(__float_as_int(v))); }
+__device__ __forceinline__ float wshift_down(float v) { return __int_as_float(dpp_down(__float_as_int(v))); }
+__device__ __forceinline__ int wshift_up(int v) { return dpp_up(v); }
+__device__ __forceinline__ int wshift_down(int v) { return dpp_down(v); }
+__device__ __forceinline__ double wshift_up(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_up((int)(b & 0xffffffff)), hi = dpp_up((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double wshift_down(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_down((int)(b & 0xffffffff)), hi = dpp_down((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// The value, hidden from loop-invariant code motion: a per-column comparison against it is
+// evaluated where it is used (one compare into VCC) instead of being hoisted out of the row
+// loop as S live 64-bit lane masks.
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
 
 // Mass with a separate binary exponent: value = acc * 2^E.
 struct ExpAcc {
@@ -103,52 +177,52 @@ __device__ __forceinline__ void expacc_add(ExpAcc& a, double v, int e) {
     a.E += k;
 }
 
-// Row-segment storage: 3 states x (S + 1) columns (column S: the boundary), padded to
-// a multiple of 4 and laid out so each 16-byte vector of the 64 lanes is contiguous.
-template <int S>
-struct SegLayout {
-    static constexpr int cols = S + 1;
-    static constexpr int vals = 3 * cols;
-    static constexpr int padded = (vals + 3) & ~3;
+// Row segment of one lane: M[S], X[S], Y[S], then {cM, cX, cY, E} (the boundary column and
+// the exponent's bits), then {max, 0, 0, 0}; NV 16-byte vectors, vector v of lane l at
+// (v * 64 + l) * VW values from the row's base.
+template <typename SF, int S>
+struct Seg {
+    static constexpr int VW = 16 / (int)sizeof(SF);
+    static constexpr int vals = 3 * S + 8;
+    static constexpr int NV = vals / VW;
+    static constexpr int row = NV * kWave * VW;   // values per stored row of a wave slot
+    static constexpr int iX = S, iY = 2 * S, iC = 3 * S, iE = 3 * S + 3, iMX = 3 * S + 4;
+    static_assert(vals % VW == 0, "segment is a whole number of vectors");
 };
 
 template <typename SF, int S>
 __device__ __forceinline__ void store_seg(SF* __restrict__ base, int lane, const SF (&M)[S], const SF (&X)[S],
-                                          const SF (&Y)[S], SF cM, SF cX, SF cY) {
-    constexpr int P = SegLayout<S>::padded;
-    SF v[P];
+                                          const SF (&Y)[S], SF cM, SF cX, SF cY, int E, SF mx) {
+    using L = Seg<SF, S>;
+    SF v[L::vals];
 #pragma unroll
-    for (int k = 0; k < S; ++k) { v[k] = M[k]; v[S + 1 + k] = X[k]; v[2 * (S + 1) + k] = Y[k]; }
-    v[S] = cM; v[2 * S + 1] = cX; v[3 * S + 2] = cY;
+    for (int k = 0; k < S; ++k) { v[k] = M[k]; v[L::iX + k] = X[k]; v[L::iY + k] = Y[k]; }
+    v[L::iC] = cM; v[L::iC + 1] = cX; v[L::iC + 2] = cY; v[L::iE] = e2sf(E, SF(0)); v[L::iMX] = mx;
 #pragma unroll
-    for (int k = SegLayout<S>::vals; k < P; ++k) v[k] = SF(0);
+    for (int k = L::iMX + 1; k < L::vals; ++k) v[k] = SF(0);
     if constexpr (sizeof(SF) == 4) {
 #pragma unroll
-        for (int q = 0; q < P / 4; ++q) {
-            float4 f = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
-            reinterpret_cast<float4*>(base)[q * kWave + lane] = f;
-        }
+        for (int q = 0; q < L::NV; ++q)
+            reinterpret_cast<float4*>(base)[q * kWave + lane] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
     } else {
 #pragma unroll
-        for (int q = 0; q < P / 2; ++q) {
-            double2 f = make_double2(v[2 * q], v[2 * q + 1]);
-            reinterpret_cast<double2*>(base)[q * kWave + lane] = f;
-        }
+        for (int q = 0; q < L::NV; ++q)
+            reinterpret_cast<double2*>(base)[q * kWave + lane] = make_double2(v[2 * q], v[2 * q + 1]);
     }
 }
 
 template <typename SF, int S>
-__device__ __forceinline__ void load_seg(const SF* __restrict__ base, int lane, SF (&v)[SegLayout<S>::padded]) {
-    constexpr int P = SegLayout<S>::padded;
+__device__ __forceinline__ void load_seg(const SF* __restrict__ base, int lane, SF (&v)[Seg<SF, S>::vals]) {
+    using L = Seg<SF, S>;
     if constexpr (sizeof(SF) == 4) {
 #pragma unroll
-        for (int q = 0; q < P / 4; ++q) {
+        for (int q = 0; q < L::NV; ++q) {
             const float4 f = reinterpret_cast<const float4*>(base)[q * kWave + lane];
             v[4 * q] = f.x; v[4 * q + 1] = f.y; v[4 * q + 2] = f.z; v[4 * q + 3] = f.w;
         }
     } else {
 #pragma unroll
-        for (int q = 0; q < P / 2; ++q) {
+        for (int q = 0; q < L::NV; ++q) {
             const double2 f = reinterpret_cast<const double2*>(base)[q * kWave + lane];
             v[2 * q] = f.x; v[2 * q + 1] = f.y;
         }
@@ -156,16 +230,23 @@ __device__ __forceinline__ void load_seg(const SF* __restrict__ base, int lane, 
 }
 
 template <typename SF, int AM, int S>
-__device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, double* __restrict__ hist, int tile,
-                          int64_t slot, int lane) {
+__device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int32_t* __restrict__ order,
+                          const SoftTile* __restrict__ tdp, int64_t slot, const SF* __restrict__ spsub,
+                          double* __restrict__ hist, int lane) {
     using R = SoftRange<SF>;
-    constexpr int P = SegLayout<S>::padded;
+    using LY = Seg<SF, S>;
     constexpr bool L = AM == kLocal, GL = AM == kGlocal;
-    const TileDesc td = A.tiles[tile];
-    const int G = td.G;
-    const int lg = __builtin_ctz(G);
-    const int q = lane >> lg, g = lane & (G - 1);
-    const int p = q < td.npairs ? A.order[td.first + q] : -1;
+    // this lane's pair slot
+    struct {
+        int32_t first, fsteps, npairs, panels;
+    } td{tdp->first, tdp->fsteps, tdp->npairs, tdp->panels};
+    int q = -1, G = 1, lo = lane;
+    for (int k = 0; k < td.npairs; ++k) {
+        const int lk = tdp->lo[k], gk = tdp->G[k];
+        if (lane >= lk && lane < lk + gk) { q = k; G = gk; lo = lk; }
+    }
+    const int g = lane - lo;
+    const int p = q >= 0 ? order[td.first + q] : -1;
     const int na = A.na, nn = na * na, gs = nn + 4;
     int m = 0, n = 0;
     const uint8_t* ap = A.codes_a;
@@ -177,20 +258,23 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
         ap += a0;
         bp += b0;
     }
-    // Columns per panel: a pair wider than one wavefront's lanes runs panel by panel (one
-    // pair per tile then), passing each row's boundary column through HBM.
-    const int PW = G * S;
-    const int NP = (G == kWave && n > PW) ? (n + PW - 1) / PW : 1;
+    const int NP = td.panels;
+    const int PW = G * S;   // columns of one panel of this pair
     const SF oa = A.oa, ea = A.ea, ob = A.ob, eb = A.eb;
-    const int64_t rows = A.slot_rows;
-    const int64_t pbase = slot * (int64_t)A.panels;   // this slot's first panel region
+    const int64_t rows = V.rows;
+    const int64_t pbase = slot * (int64_t)V.panels;   // this slot's first panel region
+    // panel boundary column of row i of region r, this pair: 6 values, 2 exponents
+    auto bnd = [&](int64_t r, int i) { return V.pbnd + ((r * rows + i) * kTilePairs + (q < 0 ? 0 : q)) * 6; };
+    auto bex = [&](int64_t r, int i) { return V.pexp + ((r * rows + i) * kTilePairs + (q < 0 ? 0 : q)) * 2; };
     bool bad = false;
 
     // ── forward ──────────────────────────────────────────────────────────────
-    SF M[S], X[S], Y[S];
-    SF cM = SF(0), cX = SF(0), cY = SF(0);   // the segment's boundary column j0 (column 0 at j0 = 0)
-    int E = 0;
+    SF M[S], X[S], Y[S], T[S];               // row i-1 of the lane's columns, T = M + X + Y
+    SF cM = SF(0), cX = SF(0), cY = SF(0);   // its boundary column j0 (column 0 at j0 = 0)
+    int E = 0;                               // the units of M, X, Y, T, c*
     ExpAcc zacc{0.0, 0};
+    SF corner = SF(0);                       // Global: the (m, n) cell, held by the lane of column n
+    int cornerE = 0;
     for (int pp = 0; pp < NP; ++pp) {
         const int j0 = pp * PW + g * S;
         const int ncols = n - j0 < 0 ? 0 : (n - j0 > S ? S : n - j0);
@@ -198,14 +282,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
         int coff[S];
 #pragma unroll
         for (int k = 0; k < S; ++k) coff[k] = (k < ncols) ? (int)bp[j0 + k] : 0;
-        SF* __restrict__ ft = A.ftab + (pbase + pp) * rows * P * kWave;
-        int* __restrict__ et = A.etab + (pbase + pp) * rows * kWave;
-        SF* __restrict__ mt = A.mtab + (pbase + pp) * rows * kWave;
-        SF* __restrict__ lt = A.ltab + (pbase + pp) * rows * kWave;
-        SF* __restrict__ bout = A.pbnd + (pbase + pp) * rows * 6;         // this panel's last column
-        const SF* __restrict__ bin = A.pbnd + (pbase + pp - 1) * rows * 6;  // the previous panel's
-        int* __restrict__ eout = A.pexp + (pbase + pp) * rows * 2;
-        const int* __restrict__ ein = A.pexp + (pbase + pp - 1) * rows * 2;
+        SF* __restrict__ seg = V.seg + (pbase + pp) * rows * LY::row;
         E = 0;
         if (L || GL) {
 #pragma unroll
@@ -213,125 +290,181 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
             cM = SF(1); cX = SF(0); cY = SF(0);
         } else {
             // Row 0 of Global: M(0,0) = 1, Y(0,j) = oa * ea^(j-1) (soft_kernel_impl.inl:740-744),
-            // evaluated per segment through log2 so that far columns keep their own exponent.
-            const double l2oa = ::log2((double)oa), l2ea = ::log2((double)ea);
+            // formed in double with a separate exponent (repeated squaring from the segment's
+            // first column) so that far columns keep their own range.
             if (j0 == 0) {
                 cM = SF(1); cX = SF(0); cY = SF(0);
-            } else {
-                double top = l2oa + (double)(j0 - 1) * l2ea;   // column j0 (the boundary)
-                const double last = l2oa + (double)(j0 + (ncols > 0 ? ncols : 1) - 1) * l2ea;
-                if (last > top) top = last;
-                E = (int)::floor(top);
+            }
+            // v = oa * ea^(j0 - 1) as vm * 2^ve (the boundary column; column 1 when j0 == 0)
+            int ve = 0, be = 0;
+            double vm = (double)oa, bm = (double)ea;
+            if (j0 > 0) {
+                for (int r = j0 - 1; r > 0; r >>= 1) {
+                    if (r & 1) {
+                        vm *= bm;
+                        const int k1 = __builtin_amdgcn_frexp_exp(vm);
+                        vm = __builtin_amdgcn_ldexp(vm, -k1);
+                        ve += be + k1;
+                    }
+                    bm *= bm;
+                    const int k2 = __builtin_amdgcn_frexp_exp(bm);
+                    bm = __builtin_amdgcn_ldexp(bm, -k2);
+                    be = 2 * be + k2;
+                }
+            }
+            // Y of the S columns: the boundary's value times ea^(k+1) (column 1 at j0 == 0: oa)
+            double ym[S];
+            int yex[S];
+            {
+                double cm = vm;
+                int ce = ve;
+                const double em = (double)ea;
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    if (j0 > 0 || k > 0) {
+                        cm *= em;
+                        const int k1 = __builtin_amdgcn_frexp_exp(cm);
+                        cm = __builtin_amdgcn_ldexp(cm, -k1);
+                        ce += k1;
+                    }
+                    ym[k] = cm;
+                    yex[k] = ce;
+                }
+            }
+            if (j0 > 0) {
+                // the segment's exponent: that of its largest value (the boundary or the last column)
+                const int last = ncols > 0 ? ncols - 1 : 0;
+                const int kb = __builtin_amdgcn_frexp_exp(vm), kl = __builtin_amdgcn_frexp_exp(ym[last]);
+                E = (ve + kb > yex[last] + kl ? ve + kb : yex[last] + kl) - 1;
                 cM = SF(0); cX = SF(0);
-                cY = (SF)::exp2(l2oa + (double)(j0 - 1) * l2ea - (double)E);
+                cY = (SF)__builtin_amdgcn_ldexp(vm, ve - E);
             }
 #pragma unroll
             for (int k = 0; k < S; ++k) {
                 M[k] = SF(0); X[k] = SF(0);
-                Y[k] = (k < ncols) ? (SF)::exp2(l2oa + (double)(j0 + k) * l2ea - (double)E) : SF(0);
+                Y[k] = (k < ncols) ? (SF)__builtin_amdgcn_ldexp(ym[k], yex[k] - E) : SF(0);
             }
         }
+        SF z = SF(0);
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            T[k] = M[k] + X[k] + Y[k];
+            if (k < ncols) z = dmax(z, T[k]);
+        }
+        if (j0 == 0) z = dmax(z, cM + cX + cY);
         if (live) {
             if (L || (GL && m == 0)) {
                 SF s = j0 == 0 ? cM + cX + cY : SF(0);
 #pragma unroll
-                for (int k = 0; k < S; ++k) s += (k < ncols) ? M[k] + X[k] + Y[k] : SF(0);
+                for (int k = 0; k < S; ++k) s += (k < ncols) ? T[k] : SF(0);
                 expacc_add(zacc, (double)s, E);
             }
-            store_seg<SF, S>(ft + (int64_t)g * P * kWave, lane, M, X, Y, cM, cX, cY);
-            et[(int64_t)g * kWave + lane] = E;
-            SF mx0 = j0 == 0 ? cM + cX + cY : SF(0);
-#pragma unroll
-            for (int k = 0; k < S; ++k) if (k < ncols) mx0 = dmax(mx0, M[k] + X[k] + Y[k]);
-            mt[(int64_t)g * kWave + lane] = mx0;
-            lt[(int64_t)g * kWave + lane] = SF(1);
+            store_seg<SF, S>(seg + (int64_t)g * LY::row, lane, M, X, Y, cM, cX, cY, E, z);
         }
         SF oM = M[S - 1], oX = X[S - 1], oY = Y[S - 1];
         int oE = E;
-        bool fzero;   // the lane's last row held no mass
-        {
-            SF z = SF(0);
-#pragma unroll
-            for (int k = 0; k < S; ++k) z = dmax(z, M[k] + X[k] + Y[k]);
-            fzero = !L && !(z > SF(0)) && !(cM + cX + cY > SF(0));
+        int kpend = 0;                                                // next row's units: E + kpend
+        bool fzero = !L && !(z > SF(0)) && !(cM + cX + cY > SF(0));   // the lane's last row held no mass
+        const bool pin = g == 0 && j0 > 0;                            // reads the previous panel's column
+        const bool pout = g == G - 1 && pp + 1 < NP;                  // writes this panel's last column
+        int acode = (live && m >= 1) ? (int)ap[0] : 0;                // code of the lane's next row
+        SF qM = SF(0), qX = SF(0), qY = SF(0);
+        int qE = 0;
+        if (pin && live && m >= 1) {
+            const SF* bq = bnd(pbase + pp - 1, 1);
+            qM = bq[0]; qX = bq[1]; qY = bq[2]; qE = bex(pbase + pp - 1, 1)[0];
         }
-        const int fsteps = td.max_m + G - 1;
+        const int fsteps = td.fsteps;
         for (int t = 0; t < fsteps; ++t) {
             const int i = t - g + 1;
-            SF lM = shift_up(oM), lX = shift_up(oX), lY = shift_up(oY);
-            int lE = shift_up(oE);
+            SF lM = wshift_up(oM), lX = wshift_up(oX), lY = wshift_up(oY);
+            int lE = wshift_up(oE);
             if (live && i >= 1 && i <= m) {
-                if (g == 0 && j0 > 0) {   // the previous panel's last column of row i
-                    lM = bin[(int64_t)i * 6 + 0]; lX = bin[(int64_t)i * 6 + 1]; lY = bin[(int64_t)i * 6 + 2];
-                    lE = ein[(int64_t)i * 2];
+                const int ac = acode;
+                if (i < m) acode = (int)ap[i];   // one row ahead
+                if (pin) {
+                    lM = qM; lX = qX; lY = qY; lE = qE;
+                    if (i < m) {
+                        const SF* bq = bnd(pbase + pp - 1, i + 1);
+                        qM = bq[0]; qX = bq[1]; qY = bq[2]; qE = bex(pbase + pp - 1, i + 1)[0];
+                    }
                 }
-                int Ec = E;
+                int Ec = E + kpend;                 // this row's units
+                SF s = sldexp(SF(1), -kpend);       // the previous row's values into them
                 SF nM, nX, nY;
-                if (j0 == 0) {   // column 0 of row i (soft_kernel_impl.inl:669-674)
-                    if constexpr (L) { nM = sldexp(SF(1), -Ec); nX = SF(0); nY = SF(0); }
-                    else { nM = SF(0); nY = SF(0); nX = sfma(cX, eb, (cM + cY) * ob); }
-                } else {
+                if (j0 != 0) {
                     if (fzero) {
-                        Ec = lE;   // the segment holds no mass yet: take the neighbour's exponent
+                        // no mass yet: take the neighbour's units, normalised to its boundary values
+                        const SF ml = dmax(lM, dmax(lX, lY));
+                        Ec = ml > SF(0) ? lE + silogb(ml) : lE;
+                        s = SF(1);
                     } else if (lE - R::headroom > Ec) {
-                        const int nE = lE - R::headroom;
-#pragma unroll
-                        for (int k = 0; k < S; ++k) {
-                            M[k] = sldexp(M[k], Ec - nE); X[k] = sldexp(X[k], Ec - nE); Y[k] = sldexp(Y[k], Ec - nE);
-                        }
-                        cM = sldexp(cM, Ec - nE); cX = sldexp(cX, Ec - nE); cY = sldexp(cY, Ec - nE);
-                        Ec = nE;
+                        Ec = lE - R::headroom;
+                        s = sldexp(SF(1), E - Ec);
                     }
                     nM = sldexp(lM, lE - Ec); nX = sldexp(lX, lE - Ec); nY = sldexp(lY, lE - Ec);
                 }
+                const SF ebs = eb * s, obs = ob * s;
+                if (j0 == 0) {   // column 0 of row i (soft_kernel_impl.inl:669-674)
+                    if constexpr (L) { nM = sldexp(SF(1), -Ec); nX = SF(0); nY = SF(0); }
+                    else { nM = SF(0); nY = SF(0); nX = sfma(cX, ebs, (cM + cY) * obs); }
+                }
                 const SF fr = L ? sldexp(SF(1), -Ec) : SF(0);
-                const SF* __restrict__ prow = spsub + (int)ap[i - 1] * na;
-                SF dT = cM + cX + cY;
+                const SF* __restrict__ prow = spsub + ac * na;
+                SF pw[S];
+#pragma unroll
+                for (int k = 0; k < S; ++k) pw[k] = prow[coff[k]];
+                SF dT = cM + cX + cY;   // the diagonal of column j0+1: row i-1, column j0
                 SF lm = nM, lx = nX, ly = nY;
                 SF mx = (j0 == 0) ? nM + nX + nY : SF(0);
+                const int nc = opaque(ncols);
 #pragma unroll
                 for (int k = 0; k < S; ++k) {
-                    const SF pw = prow[coff[k]];
                     const SF uM = M[k], uX = X[k], uY = Y[k];
-                    const SF mv = sfma(dT, pw, fr);
-                    const SF xv = sfma(uX, eb, (uM + uY) * ob);
+                    const SF mv = sfma(dT, pw[k] * s, fr);
+                    const SF xv = sfma(uX, ebs, (uM + uY) * obs);
                     const SF yv = sfma(ly, ea, (lm + lx) * oa);
-                    dT = uM + uX + uY;
-                    M[k] = mv; X[k] = xv; Y[k] = yv;
+                    dT = T[k];
+                    const SF tv = mv + xv + yv;
+                    M[k] = mv; X[k] = xv; Y[k] = yv; T[k] = tv;
                     lm = mv; lx = xv; ly = yv;
-                    if (k < ncols) mx = dmax(mx, mv + xv + yv);
+                    mx = dmax(mx, k < nc ? tv : SF(0));
                 }
                 cM = nM; cX = nX; cY = nY;
                 if (!(mx <= R::fmax)) bad = true;
-                SF lfac = SF(1);
+                E = Ec;
+                kpend = 0;
                 if (mx > R::hi || (mx < R::lo && mx > SF(0))) {
                     const int k2 = silogb(mx);
-#pragma unroll
-                    for (int k = 0; k < S; ++k) {
-                        M[k] = sldexp(M[k], -k2); X[k] = sldexp(X[k], -k2); Y[k] = sldexp(Y[k], -k2);
-                    }
-                    cM = sldexp(cM, -k2); cX = sldexp(cX, -k2); cY = sldexp(cY, -k2);
-                    mx = sldexp(mx, -k2);
-                    if (k2 < 0) lfac = sldexp(SF(1), -k2);
-                    Ec += k2;
+                    kpend = k2 > R::kmax ? R::kmax : (k2 < -R::kmax ? -R::kmax : k2);
                 }
-                lt[(int64_t)(i + g) * kWave + lane] = lfac;
-                E = Ec;
-                mt[(int64_t)(i + g) * kWave + lane] = mx;
                 fzero = !L && !(mx > SF(0)) && !(cM + cX + cY > SF(0));
                 if (L || (GL && i == m)) {
-                    SF s = j0 == 0 ? cM + cX + cY : SF(0);
+                    SF sr = j0 == 0 ? cM + cX + cY : SF(0);
 #pragma unroll
-                    for (int k = 0; k < S; ++k) s += (k < ncols) ? M[k] + X[k] + Y[k] : SF(0);
-                    expacc_add(zacc, (double)s, E);
+                    for (int k = 0; k < S; ++k) sr += (k < nc) ? T[k] : SF(0);
+                    expacc_add(zacc, (double)sr, E);
                 }
-                store_seg<SF, S>(ft + (int64_t)(i + g) * P * kWave, lane, M, X, Y, cM, cX, cY);
-                et[(int64_t)(i + g) * kWave + lane] = E;
+                store_seg<SF, S>(seg + (int64_t)(i + g) * LY::row, lane, M, X, Y, cM, cX, cY, E, mx);
                 oM = M[S - 1]; oX = X[S - 1]; oY = Y[S - 1]; oE = E;
-                if (g == G - 1 && pp + 1 < NP) {
-                    bout[(int64_t)i * 6 + 0] = oM; bout[(int64_t)i * 6 + 1] = oX; bout[(int64_t)i * 6 + 2] = oY;
-                    eout[(int64_t)i * 2] = oE;
+                if (pout) {
+                    SF* bo = bnd(pbase + pp, i);
+                    bo[0] = oM; bo[1] = oX; bo[2] = oY;
+                    bex(pbase + pp, i)[0] = oE;
                 }
+            }
+        }
+        // Global's end cell: row m of column n, in the panel that holds column n (a narrow
+        // pair in a wide tile ends before the last panel)
+        if constexpr (!L && !GL) {
+            if (p >= 0 && ((n == 0) ? (j0 == 0) : (j0 < n && n <= j0 + S))) {
+                if (n == 0) corner = cM + cX + cY;
+                else {
+#pragma unroll
+                    for (int k = 0; k < S; ++k)
+                        if (k == n - 1 - j0) corner = T[k];
+                }
+                cornerE = E;
             }
         }
         if (NP > 1) __threadfence();   // the panel's boundary column is read by the next panel
@@ -339,30 +472,17 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
 
     // ── Z ───────────────────────────────────────────────────────────────────
     if constexpr (!L && !GL) {
-        // Global: the (m, n) cell, held by the lane of column n in the last panel (lane 0
-        // when n == 0).
-        const int jl = (NP - 1) * PW;
-        const int gn = n == 0 ? 0 : (n - 1 - jl) / S;
-        SF corner = SF(0);
-        if (p >= 0 && g == gn) {
-            if (n == 0) corner = cM + cX + cY;
-            else {
-#pragma unroll
-                for (int k = 0; k < S; ++k)
-                    if (k == n - 1 - jl - g * S) corner = M[k] + X[k] + Y[k];
-            }
-        }
-        zacc.acc = (double)corner;
-        zacc.E = E;
-        const int src = (lane & ~(G - 1)) + gn;
-        zacc.acc = __shfl(zacc.acc, src, kWave);
-        zacc.E = __shfl(zacc.E, src, kWave);
+        const int gn = n == 0 ? 0 : ((n - 1) % PW) / S;   // the lane of column n
+        zacc.acc = __shfl((double)corner, lo + gn, kWave);
+        zacc.E = __shfl(cornerE, lo + gn, kWave);
     } else {
-        for (int off = 1; off < G; off <<= 1) {
-            const double oacc = __shfl_xor(zacc.acc, off, G);
-            const int oe = __shfl_xor(zacc.E, off, G);
-            expacc_add(zacc, oacc, oe);
+        for (int off = 1; off < kWave; off <<= 1) {
+            const double oacc = __shfl_down(zacc.acc, off, kWave);
+            const int oe = __shfl_down(zacc.E, off, kWave);
+            if (g + off < G) expacc_add(zacc, oacc, oe);
         }
+        zacc.acc = __shfl(zacc.acc, lo, kWave);
+        zacc.E = __shfl(zacc.E, lo, kWave);
     }
     const bool zbad = !(zacc.acc > 0.0) || !(zacc.acc <= 1.7976931348623157e308);
     // normalise: Z = zr * 2^ze, zr in [1, 2)
@@ -379,223 +499,232 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
     double g_oa = 0.0, g_ea = 0.0, g_ob = 0.0, g_eb = 0.0, loss = 0.0;
     const int kmin = (L || GL) ? 1 : 0;   // first row whose Y opens count
     const bool col0_opens = !L;           // column-0 X opens count (kmin0 == 0)
+    double* __restrict__ hq = hist + (q < 0 ? 0 : q) * nn;
     for (int pp = NP - 1; pp >= 0; --pp) {
         const int j0 = pp * PW + g * S;
         const int ncols = n - j0 < 0 ? 0 : (n - j0 > S ? S : n - j0);
         const bool live = p >= 0 && (ncols > 0 || j0 == 0);
-        int coff[S];
+        int coff[S + 1];   // the lane's column letters and the next lane's first
 #pragma unroll
         for (int k = 0; k < S; ++k) coff[k] = (k < ncols) ? (int)bp[j0 + k] : 0;
-        const int coff_next = (j0 + S < n) ? (int)bp[j0 + S] : 0;
-        const SF* __restrict__ ft = A.ftab + (pbase + pp) * rows * P * kWave;
-        const int* __restrict__ et = A.etab + (pbase + pp) * rows * kWave;
-        const SF* __restrict__ mt = A.mtab + (pbase + pp) * rows * kWave;
-        const SF* __restrict__ lt = A.ltab + (pbase + pp) * rows * kWave;
-        SF* __restrict__ bout = A.pbnd + (pbase + pp) * rows * 6;          // this panel's first column
-        const SF* __restrict__ bin = A.pbnd + (pbase + pp + 1) * rows * 6;  // the next panel's
-        int* __restrict__ eout = A.pexp + (pbase + pp) * rows * 2;
-        const int* __restrict__ ein = A.pexp + (pbase + pp + 1) * rows * 2;
-        SF BM[S], BX[S], BY[S];
+        coff[S] = (j0 + S < n) ? (int)bp[j0 + S] : 0;
+        const SF* __restrict__ seg = V.seg + (pbase + pp) * rows * LY::row;
+        // Row i+1 of the lane's columns.  Row m starts from a virtual row m+1 whose profile
+        // is 1 (letter index na) and whose BM holds the end weights one column to the right,
+        // so that row m's d = P BM(m+1, j+1) is its end weight: Global 1 at column n only,
+        // Glocal 1 at every column 0..n (soft_kernel_impl.inl:815-818); Local's end weight is
+        // the init term of every row.
+        SF BM[S], BX[S];
+        SF rdM = SF(0);                // BM(i+1, j0+S+1)
 #pragma unroll
-        for (int k = 0; k < S; ++k) { BM[k] = SF(0); BX[k] = SF(0); BY[k] = SF(0); }
-        SF c0M = SF(0), c0X = SF(0);   // column 0 of the row below (the lane of column 0)
-        SF rdM = SF(0);                // BM(i+1, j0+S+1) in the segment's exponent
-        int EB = 0;
+        for (int k = 0; k < S; ++k) {
+            BX[k] = SF(0);
+            BM[k] = SF(0);
+            if constexpr (GL) BM[k] = (j0 + k <= n) ? SF(1) : SF(0);
+            else if constexpr (!L) BM[k] = (j0 + k == n) ? SF(1) : SF(0);
+        }
+        if constexpr (GL) rdM = (j0 + S <= n) ? SF(1) : SF(0);
+        else if constexpr (!L) rdM = (j0 + S == n) ? SF(1) : SF(0);
+        SF c0M = SF(0), c0X = SF(0);   // column 0 of row i+1 (the lane of column 0)
+        int EB = 0, kpend = 0;         // units of BM, BX, c0*, rdM; next row's: EB + kpend
         bool bzero = true;             // the lane's last backward row held no mass
         const bool owns_end = (n == 0) ? (j0 == 0) : (j0 < n && n <= j0 + S);
-        SF fc[P], fp[P];
-        int efc = 0, efp = 0;
-        SF mfc = SF(0), mfp = SF(0), lfc = SF(1), lfp = SF(1);
-        bool loaded = false;
+        const bool has_right = j0 + S < n;
+        const bool pin = g == G - 1 && pp + 1 < NP;   // reads the next panel's column
+        const bool pout = g == 0 && pp > 0;           // writes this panel's first column
+        SF Fa[LY::vals], Fb[LY::vals];
+        if (live) {   // row m, into the buffer of the lane's first active step u = G-1-g
+            if (((G - 1 - g) & 1) == 0) load_seg<SF, S>(seg + (int64_t)(m + g) * LY::row, lane, Fa);
+            else load_seg<SF, S>(seg + (int64_t)(m + g) * LY::row, lane, Fb);
+        }
+        SF qM = SF(0), qY = SF(0);
+        int qE = 0;
+        if (pin && live) {
+            const SF* bq = bnd(pbase + pp + 1, m);
+            qM = bq[3]; qY = bq[4]; qE = bex(pbase + pp + 1, m)[1];
+        }
+        int acode = na;                // code of row i+1 (row m: the virtual row)
         SF pubM = SF(0), pubY = SF(0);
         int pubE = 0;
-        const int bsteps = td.max_m + G;
-        for (int u = 0; u < bsteps; ++u) {
-            const int i = m + G - 1 - g - u;
-            SF rM = shift_down(pubM), rY = shift_down(pubY);
-            int rE = shift_down(pubE);
-            if (!(live && i >= 0 && i <= m)) continue;
-            if (g == G - 1 && pp + 1 < NP) {   // the next panel's first column of row i
-                rM = bin[(int64_t)i * 6 + 3]; rY = bin[(int64_t)i * 6 + 4];
-                rE = ein[(int64_t)i * 2 + 1];
-            }
-            if (!loaded) {
-                load_seg<SF, S>(ft + (int64_t)(i + g) * P * kWave, lane, fc);
-                efc = et[(int64_t)(i + g) * kWave + lane];
-                mfc = mt[(int64_t)(i + g) * kWave + lane];
-                lfc = lt[(int64_t)(i + g) * kWave + lane];
-                loaded = true;
-            } else {
-#pragma unroll
-                for (int k = 0; k < P; ++k) fc[k] = fp[k];
-                efc = efp;
-                mfc = mfp;
-                lfc = lfp;
-            }
-            if (i >= 1) {
-                load_seg<SF, S>(ft + (int64_t)(i - 1 + g) * P * kWave, lane, fp);
-                efp = et[(int64_t)(i - 1 + g) * kWave + lane];
-                mfp = mt[(int64_t)(i - 1 + g) * kWave + lane];
-                lfp = lt[(int64_t)(i - 1 + g) * kWave + lane];
-            }
-            // The right boundary: BY(i, j0+S+1) and BM(i, j0+S+1).  A segment without mass
-            // of its own (Global's row m left of column n, or rows whose mass has not
-            // arrived) takes the neighbour's exponent; otherwise the neighbour's values are
-            // converted, the segment's own moved down when they would not fit.
-            const bool has_right = j0 + S < n;
-            const bool own_zero = (i == m) ? (!L && !GL && !owns_end) : (!L && bzero);
-            int Ec = (i == m) ? 0 : EB;
-            if (has_right && own_zero) {
-                Ec = rE;
-            } else if (has_right && rE - R::headroom > Ec) {
-                const int nE = rE - R::headroom;
-#pragma unroll
-                for (int k = 0; k < S; ++k) {
-                    BM[k] = sldexp(BM[k], Ec - nE); BX[k] = sldexp(BX[k], Ec - nE); BY[k] = sldexp(BY[k], Ec - nE);
+
+        // One backward step on row i: Fc holds forward row i, Fn receives row i-1.
+        auto step = [&](int i, SF rM, SF rY, int rE, SF(&Fc)[LY::vals], SF(&Fn)[LY::vals]) {
+            const int ac = acode;
+            if (i >= 1) acode = (int)ap[i - 1];   // one row ahead
+            if (pin) {
+                rM = qM; rY = qY; rE = qE;
+                if (i >= 1) {
+                    const SF* bq = bnd(pbase + pp + 1, i - 1);
+                    qM = bq[3]; qY = bq[4]; qE = bex(pbase + pp + 1, i - 1)[1];
                 }
-                c0M = sldexp(c0M, Ec - nE); c0X = sldexp(c0X, Ec - nE); rdM = sldexp(rdM, Ec - nE);
-                Ec = nE;
             }
-            const SF rYc = has_right ? sldexp(rY, rE - Ec) : SF(0);
-            const SF rMc = has_right ? sldexp(rM, rE - Ec) : SF(0);
+            if (i >= 1) load_seg<SF, S>(seg + (int64_t)(i - 1 + g) * LY::row, lane, Fn);
+            const SF* __restrict__ pn = spsub + ac * na;   // row i+1's profile
+            SF pw[S + 1];
+#pragma unroll
+            for (int k = 0; k <= S; ++k) pw[k] = pn[coff[k]];
+            const int efc = sf2e(Fc[LY::iE]);
+            const SF mfc = Fc[LY::iMX];
+            // This row's units.  A segment without mass of its own (Global's row m left of
+            // column n, or rows whose mass has not arrived) takes the neighbour's exponent; one
+            // whose neighbour's values would not fit moves up to them minus the headroom.
+            const bool own_zero = (i == m) ? (!L && !GL && !owns_end) : (!L && bzero);
+            int Ec = (i == m) ? 0 : EB + kpend;
+            SF s = (i == m) ? SF(1) : sldexp(SF(1), -kpend);   // row i+1's values into them
+            if (has_right && own_zero) {
+                // no mass yet: take the neighbour's units, normalised to its boundary values
+                const SF mr = dmax(rM, rY);
+                Ec = mr > SF(0) ? rE + silogb(mr) : rE;
+                s = SF(1);
+            } else if (has_right && rE - R::headroom > Ec) {
+                Ec = rE - R::headroom;
+                s = sldexp(SF(1), EB - Ec);
+            }
+            const SF rYc = has_right ? sldexp(rY, rE - Ec) : SF(0);   // BY(i, j0+S+1)
+            const SF rMc = has_right ? sldexp(rM, rE - Ec) : SF(0);   // BM(i, j0+S+1)
             const SF init = L ? sldexp(SF(1), -Ec) : SF(0);
-            const SF* __restrict__ pn = (i < m) ? spsub + (int)ap[i] * na : spsub;      // row i+1's profile
-            const SF* __restrict__ pr = (i >= 1) ? spsub + (int)ap[i - 1] * na : spsub; // row i's profile
-            SF byr = rYc;          // BY(i, j+1), right to left
-            SF dn = rdM;           // BM(i+1, j+1)
-            SF mx = SF(0);
-            double sx = 0, sy = 0, so = 0, sob = 0;
-            const double gh0 = zbad ? 0.0 : __builtin_amdgcn_ldexp(izr, efc + Ec - ze);
-            const double gv0 = (i >= 1 && !zbad) ? __builtin_amdgcn_ldexp(izr, efp + Ec - ze) : 0.0;
-            double* __restrict__ h = hist + q * nn + (i >= 1 ? (int)ap[i - 1] * na : 0);
+            const SF obs = ob * s, ebs = eb * s;
+            // posterior factors: of row i's units, and of row i+1's (the late sums); none of
+            // the late sums exist at row m
+            const double fnew = zbad ? 0.0 : __builtin_amdgcn_ldexp(izr, efc + Ec - ze);
+            const double fold = (zbad || i == m) ? 0.0 : __builtin_amdgcn_ldexp(izr, efc + EB - ze);
+            const double fmatch = i == m ? 0.0 : fnew;
+            const int nc = opaque(ncols);
+            SF byr = rYc;   // BY(i, j+1), right to left
+            SF dn = rdM;    // BM(i+1, j+1), row i+1's units
+            SF mx = SF(0), tmax = SF(0);
+            SF sx = SF(0), sy = SF(0), so = SF(0), sob = SF(0);
+            SF tmk[S + 1];  // match posteriors of (i+1, j0+k+1), stored units
 #pragma unroll
             for (int kk = 0; kk < S; ++kk) {
                 const int k = S - 1 - kk;
-                const int j = j0 + k + 1;
-                SF b1, b2, y;
-                if (i < m) {
-                    const SF pw = pn[k == S - 1 ? coff_next : coff[k + 1]];
-                    const SF d = sfma(pw, dn, init);
-                    const SF v = BX[k];
-                    b1 = sfma(v, ob, d);
-                    b2 = sfma(v, eb, d);
-                    y = b1;
-                } else {   // row m end weights (soft_kernel_impl.inl:815-818)
-                    const SF e = L ? init : (GL ? SF(1) : (j == n ? SF(1) : SF(0)));
-                    b1 = e; b2 = e; y = e;
-                }
-                const SF byn = sfma(ea, byr, y);
+                const SF pd = (pw[k + 1] * s) * dn;   // P(i+1, j+1) BM(i+1, j+1), row i's units
+                const SF d = L ? pd + init : pd;
+                const SF v = BX[k];                    // BX(i+1, j), row i+1's units
+                const SF b1 = sfma(v, obs, d);
+                const SF b2 = sfma(v, ebs, d);
+                // row i+1's sums on forward row i (soft_kernel_impl.inl:883-884): X opens into
+                // (i+1, j), and the match posterior of (i+1, j+1)
+                sob = sfma(Fc[k] + Fc[LY::iY + k], v, sob);
+                const SF tm = dmax((Fc[k] + Fc[LY::iX + k] + Fc[LY::iY + k]) * pd, SF(0));
+                tmk[k + 1] = tm;
+                tmax = dmax(tmax, tm);
+                const SF byn = sfma(ea, byr, b1);
                 const SF hh = byr * oa;
-                const SF bmn = (k < ncols) ? b1 + hh : SF(0);
-                const SF bxn = (k < ncols) ? b2 + hh : SF(0);
-                const SF byv = (k < ncols) ? byn : SF(0);
+                const bool in = k < nc;
+                const SF bmn = in ? b1 + hh : SF(0);
+                const SF bxn = in ? b2 + hh : SF(0);
+                const SF byv = in ? byn : SF(0);
                 dn = BM[k];
-                BM[k] = bmn; BX[k] = bxn; BY[k] = byv;
+                BM[k] = bmn; BX[k] = bxn;
+                if (k == 0) pubY = byv;
                 byr = byv;
-                if (k < ncols) mx = dmax(mx, bmn + bxn + byv);
-                // fused gradient sums (soft_kernel_impl.inl:871-903); products in double so
-                // that a product of two small stored values never flushes
-                const double fX = (double)fc[S + 1 + k], fY = (double)fc[2 * (S + 1) + k];
-                const double fMl = (k == 0) ? (double)fc[S] : (double)fc[k - 1];
-                const double fXl = (k == 0) ? (double)fc[2 * S + 1] : (double)fc[S + 1 + k - 1];
-                sy = __builtin_fma(fY, (double)byv, sy);
-                so = __builtin_fma(fMl + fXl, (double)byv, so);
-                if (i >= 1) {
-                    const double qM = (double)fp[k], qY = (double)fp[2 * (S + 1) + k];
-                    const double qMl = (k == 0) ? (double)fp[S] : (double)fp[k - 1];
-                    const double qXl = (k == 0) ? (double)fp[2 * S + 1] : (double)fp[S + 1 + k - 1];
-                    const double qYl = (k == 0) ? (double)fp[3 * S + 2] : (double)fp[2 * (S + 1) + k - 1];
-                    sx = __builtin_fma(fX, (double)bxn, sx);
-                    sob = __builtin_fma(qM + qY, (double)bxn, sob);
-                    if (k < ncols) {
-                        const double tmatch = (qMl + qXl + qYl) * (double)pr[coff[k]] * (double)bmn;
-                        if (tmatch > 0.0) {
-                            const double c = tmatch * gv0;
-                            if (!(c <= 1.0e300)) bad = true;
-                            else atomicAdd(h + coff[k], c);
-                        }
-                    }
-                }
+                mx = dmax(mx, dmax(bmn, dmax(bxn, byv)));
+                // row i's sums (soft_kernel_impl.inl:879-882)
+                sy = sfma(Fc[LY::iY + k], byv, sy);
+                const SF fml = (k == 0) ? Fc[LY::iC] + Fc[LY::iC + 1] : Fc[k - 1] + Fc[LY::iX + k - 1];
+                so = sfma(fml, byv, so);
+                sx = sfma(Fc[LY::iX + k], bxn, sx);
             }
+            tmk[0] = SF(0);
             if (j0 == 0) {   // column 0 (soft_kernel_impl.inl:849-857)
-                SF b1, b2;
-                if (i < m) {
-                    const SF pw = pn[coff[0]];
-                    const SF d = sfma(pw, dn, init);
-                    b1 = sfma(c0X, ob, d);
-                    b2 = sfma(c0X, eb, d);
-                } else {
-                    const SF e = L ? init : (GL ? SF(1) : (n == 0 ? SF(1) : SF(0)));
-                    b1 = e; b2 = e;
-                }
+                const SF pd = (pw[0] * s) * dn;   // P(i+1, 1) BM(i+1, 1)
+                const SF d = L ? pd + init : pd;
+                const SF v = c0X;
+                const SF b1 = sfma(v, obs, d);
+                const SF b2 = sfma(v, ebs, d);
+                if (col0_opens) sob = sfma(Fc[LY::iC] + Fc[LY::iC + 2], v, sob);
+                const SF tm = dmax((Fc[LY::iC] + Fc[LY::iC + 1] + Fc[LY::iC + 2]) * pd, SF(0));
+                tmk[0] = tm;
+                tmax = dmax(tmax, tm);
                 const SF hh = byr * oa;
                 c0M = b1 + hh;
                 c0X = b2 + hh;
-                mx = dmax(mx, c0M + c0X);
-                if (i >= 1) {
-                    sx = __builtin_fma((double)fc[2 * S + 1], (double)c0X, sx);
-                    if (col0_opens) sob = __builtin_fma((double)fp[S] + (double)fp[3 * S + 2], (double)c0X, sob);
-                }
+                mx = dmax(mx, dmax(c0M, c0X));
+                sx = sfma(Fc[LY::iC + 1], c0X, sx);
             }
-            // Lost mass (soft_kernel_impl.inl:141-178): any value flushed in this row's
-            // forward or backward segment was below the format's MIN in its stored units
-            // (times the forward's scale-up factor); its effect on any posterior is at most
-            // that times the other direction's segment maximum times the row's posterior
-            // factor, charged for every operation of the cell.
+            // match posteriors into the pair's histogram row a_{i+1} (after this step's reads)
+            if (ac < na) {
+                double* __restrict__ h = hq + ac * na;
+#pragma unroll
+                for (int k = 0; k <= S; ++k) atomicAdd(h + coff[k], (double)tmk[k] * fmatch);
+            }
+            // Lost mass (soft_kernel_impl.inl:141-178): any value flushed in this row's forward
+            // or backward segment was below MIN in its own units; its effect on any posterior
+            // is at most MIN times the other direction's segment maximum (3 mx bounds a cell's
+            // M + X + Y) times the row's posterior factor, charged for every operation of the
+            // cell.  A product of two stored values that leaves the normal range loses at most
+            // the smallest subnormal (soft_kernel_impl.inl:933-948).
             if (!zbad && (mx > SF(0) || mfc > SF(0))) {
-                const double lr =
-                    32.0 * (double)(ncols + 1) * A.vmin * ((double)lfc * (double)mx + (double)mfc) * gh0;
+                const double lr = 32.0 * (double)(ncols + 1) * A.vmin * (3.0 * (double)mx + (double)mfc) * fnew +
+                                  8.0 * (double)(ncols + 1) * A.vtiny * (fnew + fold);
                 loss += (lr <= 1.0e300) ? lr : 1.0e300;
             }
             if (!zbad) {
-                auto acc = [&](double& dst, double v, double f) {
-                    if (v != 0.0) {
-                        const double c = v * f;
-                        if (!(c <= 1.0e300 && c >= -1.0e300)) bad = true;
-                        else dst += c;
-                    }
-                };
-                acc(g_eb, sx, gh0);
-                acc(g_ea, sy, gh0);
-                if (i >= kmin) acc(g_oa, so * (double)oa, gh0);
-                if (i >= 1) acc(g_ob, sob * (double)ob, gv0);
+                // every term is a sum of non-negative products: one range check covers them
+                const double c1 = i >= 1 ? (double)sx * fnew : 0.0;   // no X move ends in row 0
+                const double c2 = (double)sy * fnew;
+                const double c3 = i >= kmin ? (double)so * (double)oa * fnew : 0.0;
+                const double c4 = (double)sob * (double)ob * fold;
+                const double cm = (double)tmax * fmatch;
+                if (!(c1 + c2 + c3 + c4 + cm <= 1.0e300)) bad = true;
+                else { g_eb += c1; g_ea += c2; g_oa += c3; g_ob += c4; }
             }
             if (!(mx <= R::fmax)) bad = true;
             rdM = rMc;
+            EB = Ec;
+            kpend = 0;
             if (mx > R::hi || (mx < R::lo && mx > SF(0))) {
                 const int k2 = silogb(mx);
-#pragma unroll
-                for (int k = 0; k < S; ++k) {
-                    BM[k] = sldexp(BM[k], -k2); BX[k] = sldexp(BX[k], -k2); BY[k] = sldexp(BY[k], -k2);
-                }
-                c0M = sldexp(c0M, -k2); c0X = sldexp(c0X, -k2); rdM = sldexp(rdM, -k2);
-                Ec += k2;
+                kpend = k2 > R::kmax ? R::kmax : (k2 < -R::kmax ? -R::kmax : k2);
             }
-            EB = Ec;
             bzero = !(mx > SF(0));
-            pubM = BM[0]; pubY = BY[0]; pubE = EB;
-            if (g == 0 && pp > 0) {
-                bout[(int64_t)i * 6 + 3] = pubM; bout[(int64_t)i * 6 + 4] = pubY;
-                eout[(int64_t)i * 2 + 1] = pubE;
+            pubM = BM[0]; pubE = EB;
+            if (pout) {
+                SF* bo = bnd(pbase + pp, i);
+                bo[3] = pubM; bo[4] = pubY;
+                bex(pbase + pp, i)[1] = pubE;
+            }
+        };
+
+        // Rows run m .. 0 at steps u = G-1-g .. G-1-g+m.  The two register buffers swap roles
+        // with the parity of u, the same for every lane: at an even step Fa holds the row and
+        // Fb receives the next one.  The step count is rounded up to even (an extra step in
+        // which no lane is active) so that the loop body holds exactly two steps.
+        const int bsteps = (td.fsteps + 2) & ~1;
+        for (int u = 0; u < bsteps; u += 2) {
+            {
+                const int i = m + G - 1 - g - u;
+                const SF rM = wshift_down(pubM), rY = wshift_down(pubY);
+                const int rE = wshift_down(pubE);
+                if (live && i >= 0 && i <= m) step(i, rM, rY, rE, Fa, Fb);
+            }
+            {
+                const int i = m + G - 1 - g - (u + 1);
+                const SF rM = wshift_down(pubM), rY = wshift_down(pubY);
+                const int rE = wshift_down(pubE);
+                if (live && i >= 0 && i <= m) step(i, rM, rY, rE, Fb, Fa);
             }
         }
         if (NP > 1) __threadfence();   // the panel's boundary column is read by the previous panel
     }
 
     // ── per-pair results ────────────────────────────────────────────────────
-    for (int off = 1; off < G; off <<= 1) {
-        g_oa += __shfl_xor(g_oa, off, G);
-        g_ea += __shfl_xor(g_ea, off, G);
-        g_ob += __shfl_xor(g_ob, off, G);
-        g_eb += __shfl_xor(g_eb, off, G);
-        loss += __shfl_xor(loss, off, G);
-        bad = bad || __shfl_xor((int)bad, off, G);
+    for (int off = 1; off < kWave; off <<= 1) {
+        const double o1 = __shfl_down(g_oa, off, kWave), o2 = __shfl_down(g_ea, off, kWave);
+        const double o3 = __shfl_down(g_ob, off, kWave), o4 = __shfl_down(g_eb, off, kWave);
+        const double o5 = __shfl_down(loss, off, kWave);
+        const int o6 = __shfl_down((int)bad, off, kWave);
+        if (g + off < G) {
+            g_oa += o1; g_ea += o2; g_ob += o3; g_eb += o4; loss += o5;
+            bad = bad || o6 != 0;
+        }
     }
     wave_sync();
     const int np = td.npairs;
     for (int idx = lane; idx < np * nn; idx += kWave) {
         const int qq = idx / nn, e = idx - qq * nn;
-        const int pq = A.order[td.first + qq];
+        const int pq = order[td.first + qq];
         A.grads[(int64_t)pq * gs + e] = hist[qq * nn + e];
         hist[qq * nn + e] = 0.0;
     }
@@ -613,27 +742,51 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SF* __restrict__ spsub, d
     wave_sync();
 }
 
+// Persistent: blocks below n_lslots first take wide tiles (column panels) from the wide
+// queue, then blocks below n_mslots take tiles from the main queue; each block works in
+// its own scratch slot.  Plain (no queue): block b runs tile tile_base + b.  One call site of
+// the tile body, so the kernel holds one copy of it.
+// Occupancy: the float32 pass is held to two waves per SIMD (256 VGPRs); the double pass, twice
+// the register bytes, to one.
 template <typename SF, int AM, int S>
-__global__ __launch_bounds__(64) void soft_kernel(SoftArgs<SF> A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(SF) == 4 ? 2 : 1)))
+void soft_kernel(SoftArgs<SF> A) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int na = A.na, nn = na * na;
     double* hist = reinterpret_cast<double*>(lds);
     SF* spsub = reinterpret_cast<SF*>(lds + (size_t)A.hist_slots * nn * sizeof(double));
     const int lane = threadIdx.x;
     for (int k = lane; k < nn; k += kWave) spsub[k] = A.psub[k];
+    for (int k = nn + lane; k < nn + na; k += kWave) spsub[k] = SF(1);   // the backward's virtual row
     for (int k = lane; k < A.hist_slots * nn; k += kWave) hist[k] = 0.0;
     __syncthreads();
-    if (A.queue == nullptr) {
-        const int tile = A.tile_base + (int)blockIdx.x;
-        if (tile < A.n_tiles) soft_tile<SF, AM, S>(A, spsub, hist, tile, (int64_t)blockIdx.x, lane);
-        return;
-    }
+    const int blk = (int)blockIdx.x;
+    int phase = A.queue == nullptr ? 2 : ((A.lqueue != nullptr && blk < A.n_lslots) ? 0 : 1);
     for (;;) {
         int tile = 0;
-        if (lane == 0) tile = atomicAdd(A.queue, 1);
-        tile = __shfl(tile, 0, kWave);
-        if (tile >= A.n_tiles) break;
-        soft_tile<SF, AM, S>(A, spsub, hist, tile, (int64_t)blockIdx.x, lane);
+        bool wide = false;
+        if (phase == 2) {
+            tile = A.tile_base + blk;
+            if (tile >= A.n_tiles) break;
+            phase = 3;   // one tile only
+        } else if (phase == 3) {
+            break;
+        } else {
+            if (phase == 0) {
+                if (lane == 0) tile = atomicAdd(A.lqueue, 1);
+                tile = __shfl(tile, 0, kWave);
+                if (tile < A.n_ltiles) wide = true;
+                else phase = 1;
+            }
+            if (phase == 1) {
+                if (blk >= A.n_mslots) break;
+                if (lane == 0) tile = atomicAdd(A.queue, 1);
+                tile = __shfl(tile, 0, kWave);
+                if (tile >= A.n_tiles) break;
+            }
+        }
+        soft_tile<SF, AM, S>(A, wide ? A.lview : A.view, wide ? A.lorder : A.order,
+                             (wide ? A.ltiles : A.tiles) + tile, (int64_t)blk, spsub, hist, lane);
     }
 }
 

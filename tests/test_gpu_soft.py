@@ -65,7 +65,8 @@ def test_golden(gpu, path):
 @pytest.mark.parametrize("gm", ["affine", "linear"])
 @pytest.mark.parametrize("mode", ["global", "local", "glocal"])
 def test_random_vs_oracle(gpu, gm, mode):
-    """Lane-group widths 1..64 (len(B) 0..1024), ragged len(A), related and unrelated pairs."""
+    """len(B) 0..1024 (one to three column panels of 512), ragged len(A), related and unrelated
+    pairs, several pairs of different lane-group widths in one wavefront."""
     r = wl.rng(f"soft_{gm}_{mode}", "lens")
     nb = np.concatenate([np.arange(0, 24), r.integers(0, 1025, size=24), [1024, 16, 17, 256, 257]])
     na_ = np.concatenate([r.integers(0, 200, size=len(nb) - 4), [0, 1, 2, 700]])
@@ -119,8 +120,8 @@ def test_weighted_grad_soft(gpu):
 
 @pytest.mark.parametrize("mode", ["global", "local", "glocal"])
 def test_column_panels(gpu, mode):
-    """Pairs wider than one wavefront's 1,024 columns run panel by panel (one pair per tile),
-    beside ordinary pairs in the same batch."""
+    """Pairs wider than one wavefront's 512 columns run panel by panel, up to seven panels,
+    sharing tiles with narrower pairs."""
     r = wl.rng(f"panels_{mode}", "lens")
     nb = np.array([1025, 1100, 2048, 2049, 2500, 3100, 40, 900, 1024, 1500])
     na_ = np.array([3, 150, 80, 1, 60, 40, 200, 120, 70, 0])
