@@ -29,8 +29,24 @@ OPS_PER_CELL = {("affine", "hard"): 13, ("affine", "none"): 13, ("linear", "hard
                 ("affine", "soft"): 32, ("linear", "soft"): 14}
 FP32_PEAK_TFLOPS = 157.3        # MI355X published FP32 vector peak (packed FMA = 4 flops / lane / clock)
 VALU_ISSUE_PEAK_TOPS = 39.3     # 256 CUs x 64 lanes x 2.4 GHz non-packed VALU lane-ops
+HBM_PEAK_GBS = 8000.0           # MI355X HBM3E
+# Algorithmic HBM bytes per cell of the soft pass: 3 states x 4 B written by the forward and
+# read back by the backward (24 B/cell without checkpointing).
+SOFT_BYTES_PER_CELL = 24
 
 WORKLOADS = ("protein_soft_homologs", "dna_local_short", "dna_local_short_score", "protein_global_hard")
+
+
+def measured_traffic(workload, fill):
+    """HBM bytes per launch of the workload's dominant kernel, from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the counter reads for a
+    1 GiB device copy), or None when no summary exists for this workload and fill."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{fill}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return int((t["fetch_kib"] * t["fetch_factor"] + t["write_kib"] * t["write_factor"]) * 1024)
 
 
 def make_workload(name):
@@ -132,6 +148,9 @@ def main():
                     choices=WORKLOADS)
     ap.add_argument("--fill", default="interpair", choices=("interpair", "striped"))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-launch comparison leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only time a 1 GiB device-to-device copy (the known-size reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -139,6 +158,20 @@ def main():
         return 2
     import torch
     sync = torch.cuda.synchronize
+    if args.copy_calibration:
+        nbytes = 1 << 30
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            dst.copy_(src)
+        sync()
+        dt = (time.perf_counter() - t0) / 5
+        print(json.dumps({"copy_bytes": nbytes, "ms": round(dt * 1e3, 4), "GBps_read_plus_write": round(2 * nbytes / dt / 1e9, 1)}),
+              flush=True)
+        return 0
     print(f"bench: workload {args.workload}", file=sys.stderr, flush=True)
     w, grad_mode = make_workload(args.workload)
     b = make_batch(w, grad_mode, args.fill)
@@ -154,18 +187,30 @@ def main():
              else ""), file=sys.stderr, flush=True)
     # the other launch mode, same batch shape
     other = "striped" if args.fill == "interpair" else "interpair"
-    b.fill = other
-    dt2, kms2 = time_steps(b, max(2, args.steps // 2), 1, sync)
-    gcups_other = cells * max(2, args.steps // 2) / dt2 / 1e9
-    print(f"bench: {other}: {dt2 / max(2, args.steps // 2) * 1e3:.3f} ms/step, {gcups_other:.2f} GCUPS",
-          file=sys.stderr, flush=True)
+    gcups_other = kms2 = None
+    if not args.no_plain:
+        b.fill = other
+        dt2, kms2 = time_steps(b, max(2, args.steps // 2), 1, sync)
+        gcups_other = cells * max(2, args.steps // 2) / dt2 / 1e9
+        print(f"bench: {other}: {dt2 / max(2, args.steps // 2) * 1e3:.3f} ms/step, {gcups_other:.2f} GCUPS",
+              file=sys.stderr, flush=True)
     b.fill = args.fill
     ops = OPS_PER_CELL[(w.gap_model, grad_mode)] + (1 if (w.mode == "local" and w.gap_model == "linear") else 0)
     achieved = ops * cells / (kms * 1e-3) / 1e12
-    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
-            "valu_issue_peak": VALU_ISSUE_PEAK_TOPS, "valu_issue_frac": round(achieved / VALU_ISSUE_PEAK_TOPS, 4),
-            "ops_per_cell": ops, "kernel_ms": round(kms, 4)}
+    valu = {"achieved": round(achieved, 3), "unit": "T lane-ops/s", "ops_per_cell": ops,
+            "issue_peak": VALU_ISSUE_PEAK_TOPS, "issue_frac": round(achieved / VALU_ISSUE_PEAK_TOPS, 4),
+            "fp32_peak_tflops": FP32_PEAK_TFLOPS, "fp32_frac": round(achieved / FP32_PEAK_TFLOPS, 4)}
+    if grad_mode == "soft":
+        # the soft pass stores its forward rows and reads them back: HBM is its roof
+        gbs = SOFT_BYTES_PER_CELL * cells / (kms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload, args.fill),
+                "bytes_per_cell": SOFT_BYTES_PER_CELL, "kernel_ms": round(kms, 4), "valu": valu}
+    else:
+        # max-plus DP with 1 B/cell of pointer codes: VALU instruction issue is the roof
+        roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_ISSUE_PEAK_TOPS, "unit": "T lane-ops/s",
+                "frac": round(achieved / VALU_ISSUE_PEAK_TOPS, 4), "traffic": measured_traffic(args.workload, args.fill),
+                "kernel_ms": round(kms, 4), "valu": valu}
     if not args.no_cpu:
         print("bench: CPU baseline (bounded sample)", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(w, grad_mode)
@@ -177,9 +222,9 @@ def main():
             "config": {"workload": args.workload, "pairs": w.n_pairs, "cells": cells, "grad_mode": grad_mode,
                        "gap_model": w.gap_model, "mode": w.mode, "fill": args.fill},
             "kernel_gcups": round(cells / (kms * 1e-3) / 1e9, 3),
-            "plain_vs_batched": {"fill": other, "gcups": round(gcups_other, 3),
-                                 "kernel_gcups": round(cells / (kms2 * 1e-3) / 1e9, 3),
-                                 "batched_over_plain": round(gcups / gcups_other, 3)},
+            "plain_vs_batched": None if gcups_other is None else
+                {"fill": other, "gcups": round(gcups_other, 3), "kernel_gcups": round(cells / (kms2 * 1e-3) / 1e9, 3),
+                 "batched_over_plain": round(gcups / gcups_other, 3)},
             "roofline": roof, "cpu_baseline": cpu}
     if grad_mode == "soft":
         line["soft_pairs_rerun_in_double"] = int(reruns)

@@ -229,6 +229,32 @@ __device__ __forceinline__ void load_seg(const SF* __restrict__ base, int lane, 
     }
 }
 
+// The boundary record of a row that is not stored in full: {cM, cX, cY, E}, the values the
+// full row holds at Seg::iC .. iE (one 16-byte vector for float, two for double).
+template <typename SF, int S>
+__device__ __forceinline__ void store_rec(SF* __restrict__ base, int lane, SF cM, SF cX, SF cY, int E) {
+    using L = Seg<SF, S>;
+    if constexpr (sizeof(SF) == 4) {
+        reinterpret_cast<float4*>(base)[(L::iC / 4) * kWave + lane] = make_float4(cM, cX, cY, e2sf(E, SF(0)));
+    } else {
+        reinterpret_cast<double2*>(base)[(L::iC / 2) * kWave + lane] = make_double2(cM, cX);
+        reinterpret_cast<double2*>(base)[(L::iC / 2 + 1) * kWave + lane] = make_double2(cY, e2sf(E, SF(0)));
+    }
+}
+
+template <typename SF, int S>
+__device__ __forceinline__ void load_rec(const SF* __restrict__ base, int lane, SF (&r)[4]) {
+    using L = Seg<SF, S>;
+    if constexpr (sizeof(SF) == 4) {
+        const float4 f = reinterpret_cast<const float4*>(base)[(L::iC / 4) * kWave + lane];
+        r[0] = f.x; r[1] = f.y; r[2] = f.z; r[3] = f.w;
+    } else {
+        const double2 a = reinterpret_cast<const double2*>(base)[(L::iC / 2) * kWave + lane];
+        const double2 b = reinterpret_cast<const double2*>(base)[(L::iC / 2 + 1) * kWave + lane];
+        r[0] = a.x; r[1] = a.y; r[2] = b.x; r[3] = b.y;
+    }
+}
+
 template <typename SF, int AM, int S>
 __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int32_t* __restrict__ order,
                           const SoftTile* __restrict__ tdp, int64_t slot, const SF* __restrict__ spsub,
@@ -445,7 +471,10 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                     for (int k = 0; k < S; ++k) sr += (k < nc) ? T[k] : SF(0);
                     expacc_add(zacc, (double)sr, E);
                 }
-                store_seg<SF, S>(seg + (int64_t)(i + g) * LY::row, lane, M, X, Y, cM, cX, cY, E, mx);
+                // Rows the backward reaches on an odd step are stored in full, the others as a
+                // boundary record from which the backward recomputes them (row 0 always in full).
+                if (((m + G - 1 - g - i) & 1) != 0) store_seg<SF, S>(seg + (int64_t)(i + g) * LY::row, lane, M, X, Y, cM, cX, cY, E, mx);
+                else store_rec<SF, S>(seg + (int64_t)(i + g) * LY::row, lane, cM, cX, cY, E);
                 oM = M[S - 1]; oX = X[S - 1]; oY = Y[S - 1]; oE = E;
                 if (pout) {
                     SF* bo = bnd(pbase + pp, i);
@@ -532,10 +561,20 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         const bool has_right = j0 + S < n;
         const bool pin = g == G - 1 && pp + 1 < NP;   // reads the next panel's column
         const bool pout = g == 0 && pp > 0;           // writes this panel's first column
+        // Forward rows.  Step u handles row i = m+G-1-g-u.  Rows met on odd steps (and row 0)
+        // are stored in full; a row met on an even step is recomputed at that step from the row
+        // above it -- the full row of the next step -- and its boundary record.  Two buffers
+        // cycle with u mod 4 (the same for every lane): u%4 = 0 recomputes Fb -> Fa, 1 uses Fb
+        // and loads the next full row into Fa, 2 recomputes Fa -> Fb, 3 uses Fa and loads into Fb.
         SF Fa[LY::vals], Fb[LY::vals];
-        if (live) {   // row m, into the buffer of the lane's first active step u = G-1-g
-            if (((G - 1 - g) & 1) == 0) load_seg<SF, S>(seg + (int64_t)(m + g) * LY::row, lane, Fa);
-            else load_seg<SF, S>(seg + (int64_t)(m + g) * LY::row, lane, Fb);
+        SF rec[4] = {SF(0), SF(0), SF(0), SF(0)};   // the record of the next recomputed row
+        const int u0 = G - 1 - g;                    // the lane's first step: row m
+        int rcode = (live && m >= 1) ? (int)ap[m - 1] : 0;   // code of the next recomputed row
+        if (live) {
+            const int src = (u0 & 1) ? m : (m >= 1 ? m - 1 : 0);   // full row needed first
+            if ((u0 & 3) < 2) load_seg<SF, S>(seg + (int64_t)(src + g) * LY::row, lane, Fb);
+            else load_seg<SF, S>(seg + (int64_t)(src + g) * LY::row, lane, Fa);
+            if (!(u0 & 1) && m >= 1) load_rec<SF, S>(seg + (int64_t)(m + g) * LY::row, lane, rec);
         }
         SF qM = SF(0), qY = SF(0);
         int qE = 0;
@@ -547,8 +586,57 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         SF pubM = SF(0), pubY = SF(0);
         int pubE = 0;
 
-        // One backward step on row i: Fc holds forward row i, Fn receives row i-1.
-        auto step = [&](int i, SF rM, SF rY, int rE, SF(&Fc)[LY::vals], SF(&Fn)[LY::vals]) {
+        // Recompute forward row i (i >= 1) into Fd from row i-1 in Fs and row i's record: the
+        // forward's row step (same operations, same order) in the record's units.
+        auto recompute = [&](const SF(&Fs)[LY::vals], SF(&Fd)[LY::vals]) {
+            const int Ec = sf2e(rec[3]);
+            const SF s = sldexp(SF(1), sf2e(Fs[LY::iE]) - Ec);   // row i-1's values into row i's units
+            const SF ebs = eb * s, obs = ob * s;
+            const SF fr = L ? sldexp(SF(1), -Ec) : SF(0);
+            const SF* __restrict__ prow = spsub + rcode * na;
+            SF pw[S];
+#pragma unroll
+            for (int k = 0; k < S; ++k) pw[k] = prow[coff[k]];
+            SF dT = Fs[LY::iC] + Fs[LY::iC + 1] + Fs[LY::iC + 2];
+            SF lm = rec[0], lx = rec[1], ly = rec[2];
+            SF mx = (j0 == 0) ? rec[0] + rec[1] + rec[2] : SF(0);
+            const int nc = opaque(ncols);
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                const SF uM = Fs[k], uX = Fs[LY::iX + k], uY = Fs[LY::iY + k];
+                const SF mv = sfma(dT, pw[k] * s, fr);
+                const SF xv = sfma(uX, ebs, (uM + uY) * obs);
+                const SF yv = sfma(ly, ea, (lm + lx) * oa);
+                dT = uM + uX + uY;
+                const SF tv = mv + xv + yv;
+                Fd[k] = mv; Fd[LY::iX + k] = xv; Fd[LY::iY + k] = yv;
+                lm = mv; lx = xv; ly = yv;
+                mx = dmax(mx, k < nc ? tv : SF(0));
+            }
+            Fd[LY::iC] = rec[0]; Fd[LY::iC + 1] = rec[1]; Fd[LY::iC + 2] = rec[2];
+            Fd[LY::iE] = rec[3];
+            Fd[LY::iMX] = mx;
+        };
+        auto copy_row = [&](const SF(&Fs)[LY::vals], SF(&Fd)[LY::vals]) {
+#pragma unroll
+            for (int k = 0; k < LY::vals; ++k) Fd[k] = Fs[k];
+        };
+        // Prefetch on an odd step at row i: the full row that the next even step recomputes
+        // from (row i-2, or row 0 itself when the next row is row 0), the record and code of
+        // the next row i-1.
+        auto prefetch = [&](int i, SF(&Fn)[LY::vals]) {
+            if (i >= 1) {
+                const int nf = i >= 2 ? i - 2 : 0;
+                load_seg<SF, S>(seg + (int64_t)(nf + g) * LY::row, lane, Fn);
+                if (i >= 2) {
+                    load_rec<SF, S>(seg + (int64_t)(i - 1 + g) * LY::row, lane, rec);
+                    rcode = (int)ap[i - 2];
+                }
+            }
+        };
+
+        // One backward step on row i: Fc holds forward row i.
+        auto step = [&](int i, SF rM, SF rY, int rE, SF(&Fc)[LY::vals]) {
             const int ac = acode;
             if (i >= 1) acode = (int)ap[i - 1];   // one row ahead
             if (pin) {
@@ -558,7 +646,6 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                     qM = bq[3]; qY = bq[4]; qE = bex(pbase + pp + 1, i - 1)[1];
                 }
             }
-            if (i >= 1) load_seg<SF, S>(seg + (int64_t)(i - 1 + g) * LY::row, lane, Fn);
             const SF* __restrict__ pn = spsub + ac * na;   // row i+1's profile
             SF pw[S + 1];
 #pragma unroll
@@ -687,23 +774,48 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             }
         };
 
-        // Rows run m .. 0 at steps u = G-1-g .. G-1-g+m.  The two register buffers swap roles
-        // with the parity of u, the same for every lane: at an even step Fa holds the row and
-        // Fb receives the next one.  The step count is rounded up to even (an extra step in
-        // which no lane is active) so that the loop body holds exactly two steps.
-        const int bsteps = (td.fsteps + 2) & ~1;
-        for (int u = 0; u < bsteps; u += 2) {
-            {
+        // Rows run m .. 0 at steps u = G-1-g .. G-1-g+m; the step count is rounded up to a
+        // multiple of 4 (steps in which no lane is active) so that the loop body holds the
+        // buffers' full cycle.
+        const int bsteps = (td.fsteps + 4) & ~3;
+        for (int u = 0; u < bsteps; u += 4) {
+            {   // u % 4 == 0: recompute Fb -> Fa
                 const int i = m + G - 1 - g - u;
                 const SF rM = wshift_down(pubM), rY = wshift_down(pubY);
                 const int rE = wshift_down(pubE);
-                if (live && i >= 0 && i <= m) step(i, rM, rY, rE, Fa, Fb);
+                if (live && i >= 0 && i <= m) {
+                    if (i >= 1) recompute(Fb, Fa);
+                    else copy_row(Fb, Fa);
+                    step(i, rM, rY, rE, Fa);
+                }
             }
-            {
+            {   // u % 4 == 1: Fb holds the row; load the next full row into Fa
                 const int i = m + G - 1 - g - (u + 1);
                 const SF rM = wshift_down(pubM), rY = wshift_down(pubY);
                 const int rE = wshift_down(pubE);
-                if (live && i >= 0 && i <= m) step(i, rM, rY, rE, Fb, Fa);
+                if (live && i >= 0 && i <= m) {
+                    prefetch(i, Fa);
+                    step(i, rM, rY, rE, Fb);
+                }
+            }
+            {   // u % 4 == 2: recompute Fa -> Fb
+                const int i = m + G - 1 - g - (u + 2);
+                const SF rM = wshift_down(pubM), rY = wshift_down(pubY);
+                const int rE = wshift_down(pubE);
+                if (live && i >= 0 && i <= m) {
+                    if (i >= 1) recompute(Fa, Fb);
+                    else copy_row(Fa, Fb);
+                    step(i, rM, rY, rE, Fb);
+                }
+            }
+            {   // u % 4 == 3: Fa holds the row; load the next full row into Fb
+                const int i = m + G - 1 - g - (u + 3);
+                const SF rM = wshift_down(pubM), rY = wshift_down(pubY);
+                const int rE = wshift_down(pubE);
+                if (live && i >= 0 && i <= m) {
+                    prefetch(i, Fb);
+                    step(i, rM, rY, rE, Fa);
+                }
             }
         }
         if (NP > 1) __threadfence();   // the panel's boundary column is read by the previous panel
