@@ -74,6 +74,11 @@ void nwg_batch_free(nwg_batch *b);
  * batch's precision on upload as static_cast<T>(double).  Invalidates cached results.
  * Replaces BatchEngine::set_params (batch_engine.hpp:597-606). */
 int nwg_batch_set_params(nwg_batch *b, const double *matrix, const double *gaps4);
+/* The same from a device buffer of n_alpha*n_alpha + 4 doubles (matrix, then go_a, ge_a,
+ * go_b, ge_b), copied on the context's stream without synchronising: the parameter update of
+ * a training step stays on the device (and inside a captured graph).  Soft batches only; no
+ * counterpart in the reference, whose parameters live on the host. */
+int nwg_batch_set_params_dev(nwg_batch *b, const double *theta_dev);
 
 /* fill 0: one persistent launch pulling tiles from a device work queue (default);
  * fill 1: plain per-bucket launches.  Results do not depend on it
@@ -91,6 +96,10 @@ int nwg_batch_set_soft(nwg_batch *b, double temperature, int precision);
 int nwg_batch_run(nwg_batch *b, double *total_out);
 /* Enqueue the pass on the context's stream and return (no total, no synchronisation). */
 int nwg_batch_run_async(nwg_batch *b);
+/* After nwg_batch_run_async of a soft batch: synchronise, rerun in double the pairs the
+ * float32 pass refused, and fail as nwg_batch_run would if any remain (the reference's
+ * float32 -> double -> error policy, aligner.hpp:6554-6586). */
+int nwg_batch_soft_finish(nwg_batch *b);
 
 /* Copies of the cached results (BatchEngine::scores / grads_into,
  * batch_engine.hpp:689-695, 722-746).  mats: N*n_alpha*n_alpha, gaps: N*4. */
@@ -105,6 +114,9 @@ int nwg_batch_weighted_grad(nwg_batch *b, const double *w, int64_t n, double *ou
 int nwg_batch_weighted_grad_dev(nwg_batch *b, const double *w_dev, double *out_dev);
 /* Device pointer to the per-pair scores (N doubles), valid until the batch is freed. */
 int nwg_batch_scores_dev(nwg_batch *b, const double **out_dev);
+/* Device-to-device copy of the per-pair scores into dst_dev (N doubles), on the context's
+ * stream: a caller-owned copy that the next pass does not overwrite. */
+int nwg_batch_scores_copy_dev(nwg_batch *b, double *dst_dev);
 
 /* Introspection for tests and the bench: number of tiles and the lane-group width used. */
 int nwg_batch_plan_info(nwg_batch *b, int64_t *n_tiles, int64_t *n_cells);

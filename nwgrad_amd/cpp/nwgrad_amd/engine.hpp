@@ -337,6 +337,16 @@ public:
         if (gd_ == GradMode::None) throw std::logic_error("nwgrad: grad_mode \"none\" computes no gradient");
         check_(nwg_batch_weighted_grad_dev(b_, w_dev, out_dev));
     }
+    // Params from a device buffer (matrix, go_a, ge_a, go_b, ge_b as doubles), enqueued on the
+    // device's stream: soft batches only.
+    void set_params_dev(const double* theta_dev) { check_(nwg_batch_set_params_dev(b_, theta_dev)); }
+    // Device-to-device copy of the scores into a caller buffer of size() doubles.
+    void scores_copy_dev(double* dst_dev) const { check_(nwg_batch_scores_copy_dev(b_, dst_dev)); }
+    // After score_and_grad_async: rerun refused float32 pairs in double; throws as
+    // score_and_grad would when a pair fits neither.
+    void soft_finish() { check_(nwg_batch_soft_finish(b_)); }
+    // Work of this device's context goes to `stream` (a hipStream_t, e.g. torch's current one).
+    void set_stream(void* stream) { check_(nwg_set_stream(ctx_, stream)); }
     double last_kernel_ms() const {
         double ms = 0.0;
         check_(nwg_batch_last_kernel_ms(b_, &ms));

@@ -156,11 +156,13 @@ struct nwg_batch {
     // hard: pointer codes per wave slot
     DevPtr<uint32_t> table;
     // soft: row segments of the main and the long-pair pools, panel columns, weights
-    DevPtr<float> seg32, lseg32, pb32, pf32;
-    DevPtr<double> seg64, lseg64, pb64, pf64;
+    DevPtr<float> seg32, lseg32, pb32, pf32, gw32;
+    DevPtr<double> seg64, lseg64, pb64, pf64, gw64;
+    DevPtr<double> theta;    // params on the device: matrix, go_a, ge_a, go_b, ge_b
     DevPtr<int32_t> pexp;
     DevPtr<int32_t> lqueue;
     int64_t soft_grid = 0, soft_lslots = 0, soft_occ = 0;   // the last soft launch's shape
+    int soft_occ_cache[2] = {0, 0};                          // resident blocks per CU: float, double
     DevPtr<uint8_t> soft_ok;
     std::vector<uint8_t> host_ok;
     int64_t soft_reruns = 0;
@@ -527,7 +529,7 @@ int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t
     const int nn = n_alpha * n_alpha;
     if (prob->grad_mode == 2) {
         // soft: any width, floor(64 / G) pair slots of nn doubles
-        const int slots = std::max(1, std::min(nwg::kTilePairs, kSoftHistBytes / (nn * 8)));
+        const int slots = std::max(1, std::min(nwg::kTilePairs, kSoftHistBytes / (n_alpha * (n_alpha + 1) * 8)));
         int gmin = 1;
         while (64 / gmin > slots) ++gmin;
         b->gmin = gmin;
@@ -583,6 +585,25 @@ int nwg_batch_set_params(nwg_batch* b, const double* matrix, const double* gaps4
         NWG_HIP(ctx, b->sub_f64.reserve((size_t)nn));
         NWG_HIP(ctx, hipMemcpy(b->sub_f64.p, matrix, sizeof(double) * (size_t)nn, hipMemcpyHostToDevice));
     }
+    NWG_HIP(ctx, b->theta.reserve((size_t)nn + 4));
+    std::vector<double> th(b->matrix);
+    th.insert(th.end(), gaps4, gaps4 + 4);
+    NWG_HIP(ctx, hipMemcpy(b->theta.p, th.data(), sizeof(double) * th.size(), hipMemcpyHostToDevice));
+    b->have_params = true;
+    b->results_valid = false;
+    return NWG_OK;
+}
+
+int nwg_batch_set_params_dev(nwg_batch* b, const double* theta_dev) {
+    if (!b || !theta_dev) return NWG_EINVAL;
+    nwg_ctx* ctx = b->ctx;
+    if (b->prob.grad_mode != 2)
+        return fail(ctx, NWG_EINVAL, "nwgrad_hip: set_params_dev serves soft batches (grad_mode=\"soft\") only");
+    const int nn = b->na * b->na;
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    if (b->theta.n < (size_t)nn + 4) NWG_HIP(ctx, b->theta.reserve((size_t)nn + 4));
+    NWG_HIP(ctx, hipMemcpyAsync(b->theta.p, theta_dev, sizeof(double) * ((size_t)nn + 4), hipMemcpyDeviceToDevice,
+                                ctx->stream));
     b->have_params = true;
     b->results_valid = false;
     return NWG_OK;
@@ -686,12 +707,14 @@ int launch_hard(nwg_batch* b, const Plan& pl) {
 // ── soft pass ──────────────────────────────────────────────────────────────
 template <typename SF> struct SoftScratch;
 template <> struct SoftScratch<float> {
+    static DevPtr<float>& gw(nwg_batch* b) { return b->gw32; }
     static DevPtr<float>& seg(nwg_batch* b) { return b->seg32; }
     static DevPtr<float>& lseg(nwg_batch* b) { return b->lseg32; }
     static DevPtr<float>& pf(nwg_batch* b) { return b->pf32; }
     static DevPtr<float>& pb(nwg_batch* b) { return b->pb32; }
 };
 template <> struct SoftScratch<double> {
+    static DevPtr<double>& gw(nwg_batch* b) { return b->gw64; }
     static DevPtr<double>& seg(nwg_batch* b) { return b->seg64; }
     static DevPtr<double>& lseg(nwg_batch* b) { return b->lseg64; }
     static DevPtr<double>& pf(nwg_batch* b) { return b->pf64; }
@@ -719,29 +742,16 @@ int launch_soft(nwg_batch* b, const SoftPlan& pl, const SoftPlan& wl) {
     nwg_ctx* ctx = b->ctx;
     const int nn = b->na * b->na;
     const bool lin = b->prob.gap_model == 0;
-    const double it = 1.0 / b->temperature;
-    // Weights in double, cast once (aligner.hpp:6870-6885): the profile exp(s / T) and the
-    // gap weights exp(-(go + ge) / T), exp(-ge / T); linear gaps have no open.
-    std::vector<SF> ps((size_t)nn);
-    const double big = (double)std::numeric_limits<SF>::max();
-    for (int k = 0; k < nn; ++k) {
-        const double v = std::exp(b->matrix[(size_t)k] * it);
-        if (!(v <= big)) return fail(ctx, NWG_ERANGE, "nwgrad: soft path: exp(score / T) overflows");
-        ps[(size_t)k] = static_cast<SF>(v);
-    }
-    const double ea = std::exp(-b->gaps[1] * it), eb = std::exp(-b->gaps[3] * it);
-    const double oa = lin ? ea : std::exp(-(b->gaps[0] + b->gaps[1]) * it);
-    const double ob = lin ? eb : std::exp(-(b->gaps[2] + b->gaps[3]) * it);
-    for (double v : {ea, eb, oa, ob})
-        if (!(v <= big)) return fail(ctx, NWG_ERANGE, "nwgrad: soft path: a gap weight overflows");
     if (pl.tiles.empty() && wl.tiles.empty()) return NWG_OK;
     DevPtr<SF>& dps = SS::pf(b);
     NWG_HIP(ctx, dps.reserve((size_t)nn));
-    NWG_HIP(ctx, hipMemcpyAsync(dps.p, ps.data(), sizeof(SF) * (size_t)nn, hipMemcpyHostToDevice, ctx->stream));
+    NWG_HIP(ctx, SS::gw(b).reserve(4));
     SoftKernel<SF> kern = soft_kernel_for<SF>(b->prob.mode);
-    const size_t lds = (size_t)b->hist_slots * nn * sizeof(double) + (size_t)(nn + b->na) * sizeof(SF);
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64, lds) != hipSuccess || occ <= 0)
+    const size_t lds = (size_t)b->hist_slots * b->na * (b->na + 1) * sizeof(double) +
+                       (size_t)(b->na + 1) * (b->na + 1) * sizeof(SF);
+    // resident blocks per CU: queried once per kernel and LDS size (no query inside a captured step)
+    int& occ = b->soft_occ_cache[std::is_same<SF, float>::value ? 0 : 1];
+    if (occ <= 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64, lds) != hipSuccess || occ <= 0))
         occ = kSoftBlocksPerCU;
     b->soft_occ = occ;
     const int64_t resident = (int64_t)occ * ctx->n_cu;
@@ -778,10 +788,7 @@ int launch_soft(nwg_batch* b, const SoftPlan& pl, const SoftPlan& wl) {
     A.psub = dps.p;
     A.na = b->na;
     A.hist_slots = b->hist_slots;
-    A.oa = static_cast<SF>(oa);
-    A.ea = static_cast<SF>(ea);
-    A.ob = static_cast<SF>(ob);
-    A.eb = static_cast<SF>(eb);
+    A.gapw = SS::gw(b).p;
     A.linear = lin ? 1 : 0;
     A.temperature = b->temperature;
     A.scores = b->scores.p;
@@ -790,10 +797,13 @@ int launch_soft(nwg_batch* b, const SoftPlan& pl, const SoftPlan& wl) {
     A.tol = std::is_same<SF, float>::value ? 0x1p-20 : 0x1p-45;   // the reference's SoftRange<F>::tol
     A.vmin = (double)std::numeric_limits<SF>::min();
     A.vtiny = (double)std::numeric_limits<SF>::denorm_min();
+    // the weights (and, persistent, the queue counters) from the device params
+    hipLaunchKernelGGL(nwg::soft_weights_kernel<SF>, dim3(1), dim3(256), 0, ctx->stream, b->theta.p, nn,
+                       1.0 / b->temperature, lin ? 1 : 0, dps.p, SS::gw(b).p, persistent ? b->queue.p : nullptr,
+                       persistent ? b->lqueue.p : nullptr);
+    NWG_HIP(ctx, hipGetLastError());
     if (persistent) {
         const int64_t grid = std::max<int64_t>(mslots, lslots);
-        NWG_HIP(ctx, hipMemsetAsync(b->queue.p, 0, sizeof(int32_t), ctx->stream));
-        NWG_HIP(ctx, hipMemsetAsync(b->lqueue.p, 0, sizeof(int32_t), ctx->stream));
         A.order = pl.d_order.p;
         A.tiles = pl.d_ptiles.p;
         A.n_tiles = (int32_t)pl.tiles.size();
@@ -892,7 +902,10 @@ int run_async(nwg_batch* b) {
     nwg_ctx* ctx = b->ctx;
     if (!b->have_params) return fail(ctx, NWG_ESTATE, "nwgrad_hip: set_params() has not been called");
     NWG_HIP(ctx, hipSetDevice(ctx->device));
-    NWG_HIP(ctx, hipEventRecord(b->ev0, ctx->stream));
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    NWG_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cap));
+    const bool timed = cap == hipStreamCaptureStatusNone;   // no event nodes in a captured graph
+    if (timed) NWG_HIP(ctx, hipEventRecord(b->ev0, ctx->stream));
     int rc = NWG_OK;
     if (b->prob.grad_mode == 2) {
         rc = launch_soft_batch(b);
@@ -904,7 +917,7 @@ int run_async(nwg_batch* b) {
             if (rc) return rc;
         }
     }
-    NWG_HIP(ctx, hipEventRecord(b->ev1, ctx->stream));
+    if (timed) NWG_HIP(ctx, hipEventRecord(b->ev1, ctx->stream));
     b->results_valid = true;
     return NWG_OK;
 }
@@ -933,6 +946,14 @@ extern "C" {
 int nwg_batch_run_async(nwg_batch* b) {
     if (!b) return NWG_EINVAL;
     return run_async(b);
+}
+
+int nwg_batch_soft_finish(nwg_batch* b) {
+    if (!b) return NWG_EINVAL;
+    if (b->prob.grad_mode != 2) return NWG_OK;
+    if (!b->results_valid) return fail(b->ctx, NWG_ESTATE, "nwgrad: no soft pass to finish");
+    NWG_HIP(b->ctx, hipSetDevice(b->ctx->device));
+    return soft_finish(b);
 }
 
 int nwg_batch_run(nwg_batch* b, double* total_out) {
@@ -975,6 +996,16 @@ int nwg_batch_scores(nwg_batch* b, double* out) {
 int nwg_batch_scores_dev(nwg_batch* b, const double** out_dev) {
     if (!b || !out_dev) return NWG_EINVAL;
     *out_dev = b->scores.p;
+    return NWG_OK;
+}
+
+int nwg_batch_scores_copy_dev(nwg_batch* b, double* dst_dev) {
+    if (!b || !dst_dev) return NWG_EINVAL;
+    if (!b->results_valid) return fail(b->ctx, NWG_ESTATE, "nwgrad: score not valid; compute an alignment first");
+    if (b->N == 0) return NWG_OK;
+    NWG_HIP(b->ctx, hipSetDevice(b->ctx->device));
+    NWG_HIP(b->ctx, hipMemcpyAsync(dst_dev, b->scores.p, sizeof(double) * (size_t)b->N, hipMemcpyDeviceToDevice,
+                                   b->ctx->stream));
     return NWG_OK;
 }
 

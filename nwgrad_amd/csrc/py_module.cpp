@@ -255,6 +255,14 @@ void bind_batch(py::module_& m, const char* name) {
                                "Device time of the last pass's kernels (HIP events), milliseconds.")
         .def("_handle", [](const PB& s) { return reinterpret_cast<uintptr_t>(s.b->handle()); })
         .def("_scores_dev_ptr", [](const PB& s) { return reinterpret_cast<uintptr_t>(s.b->scores_dev()); })
+        .def("_set_params_dev", [](PB& s, uintptr_t theta) { s.b->set_params_dev(reinterpret_cast<const double*>(theta)); },
+             py::arg("theta_dev"))
+        .def("_run_async", [](PB& s) { s.b->score_and_grad_async(); })
+        .def("_scores_copy_dev", [](const PB& s, uintptr_t dst) { s.b->scores_copy_dev(reinterpret_cast<double*>(dst)); },
+             py::arg("dst_dev"))
+        .def("_soft_finish", [](PB& s) { s.b->soft_finish(); return s.b->soft_reruns(); })
+        .def("_set_stream", [](PB& s, uintptr_t stream) { s.b->set_stream(reinterpret_cast<void*>(stream)); },
+             py::arg("stream"))
         .def(
             "_weighted_grad_dev",
             [](const PB& s, uintptr_t w_dev, uintptr_t out_dev) {

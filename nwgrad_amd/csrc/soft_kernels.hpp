@@ -95,7 +95,7 @@ struct SoftArgs {
     const SF* psub;          // exp(matrix / T), n_alpha * n_alpha (LDS adds a row of ones)
     int32_t na;
     int32_t hist_slots;
-    SF oa, ea, ob, eb;       // exp(-(go+ge)/T), exp(-ge/T) per sequence
+    const SF* gapw;          // oa, ea, ob, eb: exp(-(go+ge)/T), exp(-ge/T) per sequence
     int32_t linear;          // 1: report the open fields as 0
     double temperature;
     double* scores;
@@ -129,10 +129,33 @@ __device__ __forceinline__ int silogb(float x) { return __builtin_amdgcn_frexp_e
 __device__ __forceinline__ int silogb(double x) { return __builtin_amdgcn_frexp_exp(x) - 1; }
 __device__ __forceinline__ float sfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double sfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ __forceinline__ float e2sf(int e, float) { return __int_as_float(e); }
-__device__ __forceinline__ double e2sf(int e, double) { return __longlong_as_double((long long)e); }
-__device__ __forceinline__ int sf2e(float v) { return __float_as_int(v); }
-__device__ __forceinline__ int sf2e(double v) { return (int)__double_as_longlong(v); }
+// A row segment's exponent E and a power-of-two bound on its maximum, 2^x >= max (x = -128:
+// no mass), packed into the bits of one stored value: float (E << 8 | x) with |E| < 2^23,
+// double (E << 16 | x).
+constexpr int kPackEMax = 1 << 23;
+__device__ __forceinline__ int mx_exp(float mx) {
+    if (!(mx > 0.0f)) return -128;
+    const int e = __builtin_amdgcn_frexp_expf(mx);   // mx < 2^e
+    return e > 127 ? 127 : (e < -127 ? -127 : e);
+}
+__device__ __forceinline__ int mx_exp(double mx) {
+    if (!(mx > 0.0)) return -32768;
+    return __builtin_amdgcn_frexp_exp(mx);
+}
+__device__ __forceinline__ float pack_em(int E, float mx) { return __int_as_float((int)((unsigned)E << 8) | (mx_exp(mx) & 0xff)); }
+__device__ __forceinline__ double pack_em(int E, double mx) {
+    return __longlong_as_double((long long)((unsigned long long)(long long)E << 16) | (mx_exp(mx) & 0xffff));
+}
+__device__ __forceinline__ int em_E(float v) { return __float_as_int(v) >> 8; }
+__device__ __forceinline__ int em_E(double v) { return (int)(__double_as_longlong(v) >> 16); }
+__device__ __forceinline__ float em_max(float v) {
+    const int x = (int)(signed char)(__float_as_int(v) & 0xff);
+    return x == -128 ? 0.0f : __builtin_amdgcn_ldexpf(1.0f, x);
+}
+__device__ __forceinline__ double em_max(double v) {
+    const int x = (int)(short)(__double_as_longlong(v) & 0xffff);
+    return x == -32768 ? 0.0 : __builtin_amdgcn_ldexp(1.0, x);
+}
 
 // One-lane wave shifts on the DPP path (a VALU move, no LDS round trip): up = from the lane
 // below (wave_shr:1), down = from the lane above (wave_shl:1).  Lane 0 (up) and lane 63
@@ -177,16 +200,16 @@ __device__ __forceinline__ void expacc_add(ExpAcc& a, double v, int e) {
     a.E += k;
 }
 
-// Row segment of one lane: M[S], X[S], Y[S], then {cM, cX, cY, E} (the boundary column and
-// the exponent's bits), then {max, 0, 0, 0}; NV 16-byte vectors, vector v of lane l at
+// Row segment of one lane: M[S], X[S], Y[S], then {cM, cX, cY, E|max} (the boundary column
+// and the packed exponent and maximum bound); NV 16-byte vectors, vector v of lane l at
 // (v * 64 + l) * VW values from the row's base.
 template <typename SF, int S>
 struct Seg {
     static constexpr int VW = 16 / (int)sizeof(SF);
-    static constexpr int vals = 3 * S + 8;
+    static constexpr int vals = 3 * S + 4;
     static constexpr int NV = vals / VW;
     static constexpr int row = NV * kWave * VW;   // values per stored row of a wave slot
-    static constexpr int iX = S, iY = 2 * S, iC = 3 * S, iE = 3 * S + 3, iMX = 3 * S + 4;
+    static constexpr int iX = S, iY = 2 * S, iC = 3 * S, iE = 3 * S + 3;
     static_assert(vals % VW == 0, "segment is a whole number of vectors");
 };
 
@@ -197,9 +220,7 @@ __device__ __forceinline__ void store_seg(SF* __restrict__ base, int lane, const
     SF v[L::vals];
 #pragma unroll
     for (int k = 0; k < S; ++k) { v[k] = M[k]; v[L::iX + k] = X[k]; v[L::iY + k] = Y[k]; }
-    v[L::iC] = cM; v[L::iC + 1] = cX; v[L::iC + 2] = cY; v[L::iE] = e2sf(E, SF(0)); v[L::iMX] = mx;
-#pragma unroll
-    for (int k = L::iMX + 1; k < L::vals; ++k) v[k] = SF(0);
+    v[L::iC] = cM; v[L::iC + 1] = cX; v[L::iC + 2] = cY; v[L::iE] = pack_em(E, mx);
     if constexpr (sizeof(SF) == 4) {
 #pragma unroll
         for (int q = 0; q < L::NV; ++q)
@@ -235,10 +256,10 @@ template <typename SF, int S>
 __device__ __forceinline__ void store_rec(SF* __restrict__ base, int lane, SF cM, SF cX, SF cY, int E) {
     using L = Seg<SF, S>;
     if constexpr (sizeof(SF) == 4) {
-        reinterpret_cast<float4*>(base)[(L::iC / 4) * kWave + lane] = make_float4(cM, cX, cY, e2sf(E, SF(0)));
+        reinterpret_cast<float4*>(base)[(L::iC / 4) * kWave + lane] = make_float4(cM, cX, cY, pack_em(E, SF(0)));
     } else {
         reinterpret_cast<double2*>(base)[(L::iC / 2) * kWave + lane] = make_double2(cM, cX);
-        reinterpret_cast<double2*>(base)[(L::iC / 2 + 1) * kWave + lane] = make_double2(cY, e2sf(E, SF(0)));
+        reinterpret_cast<double2*>(base)[(L::iC / 2 + 1) * kWave + lane] = make_double2(cY, pack_em(E, SF(0)));
     }
 }
 
@@ -286,7 +307,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
     }
     const int NP = td.panels;
     const int PW = G * S;   // columns of one panel of this pair
-    const SF oa = A.oa, ea = A.ea, ob = A.ob, eb = A.eb;
+    const SF oa = A.gapw[0], ea = A.gapw[1], ob = A.gapw[2], eb = A.gapw[3];
     const int64_t rows = V.rows;
     const int64_t pbase = slot * (int64_t)V.panels;   // this slot's first panel region
     // panel boundary column of row i of region r, this pair: 6 values, 2 exponents
@@ -307,7 +328,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         const bool live = p >= 0 && (ncols > 0 || j0 == 0);   // the lane of column 0 always computes
         int coff[S];
 #pragma unroll
-        for (int k = 0; k < S; ++k) coff[k] = (k < ncols) ? (int)bp[j0 + k] : 0;
+        for (int k = 0; k < S; ++k) coff[k] = (k < ncols) ? (int)bp[j0 + k] : na;   // na: no column
         SF* __restrict__ seg = V.seg + (pbase + pp) * rows * LY::row;
         E = 0;
         if (L || GL) {
@@ -436,7 +457,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                     else { nM = SF(0); nY = SF(0); nX = sfma(cX, ebs, (cM + cY) * obs); }
                 }
                 const SF fr = L ? sldexp(SF(1), -Ec) : SF(0);
-                const SF* __restrict__ prow = spsub + ac * na;
+                const SF* __restrict__ prow = spsub + ac * (na + 1);
                 SF pw[S];
 #pragma unroll
                 for (int k = 0; k < S; ++k) pw[k] = prow[coff[k]];
@@ -454,10 +475,11 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                     const SF tv = mv + xv + yv;
                     M[k] = mv; X[k] = xv; Y[k] = yv; T[k] = tv;
                     lm = mv; lx = xv; ly = yv;
-                    mx = dmax(mx, k < nc ? tv : SF(0));
+                    mx = dmax(mx, tv);
                 }
                 cM = nM; cX = nX; cY = nY;
                 if (!(mx <= R::fmax)) bad = true;
+                if (sizeof(SF) == 4 && (Ec >= kPackEMax || Ec <= -kPackEMax)) bad = true;   // no room to pack E
                 E = Ec;
                 kpend = 0;
                 if (mx > R::hi || (mx < R::lo && mx > SF(0))) {
@@ -528,15 +550,16 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
     double g_oa = 0.0, g_ea = 0.0, g_ob = 0.0, g_eb = 0.0, loss = 0.0;
     const int kmin = (L || GL) ? 1 : 0;   // first row whose Y opens count
     const bool col0_opens = !L;           // column-0 X opens count (kmin0 == 0)
-    double* __restrict__ hq = hist + (q < 0 ? 0 : q) * nn;
+    const int hrow = na + 1;   // histogram rows carry a spill column for the no-column letter
+    double* __restrict__ hq = hist + (q < 0 ? 0 : q) * na * hrow;
     for (int pp = NP - 1; pp >= 0; --pp) {
         const int j0 = pp * PW + g * S;
         const int ncols = n - j0 < 0 ? 0 : (n - j0 > S ? S : n - j0);
         const bool live = p >= 0 && (ncols > 0 || j0 == 0);
         int coff[S + 1];   // the lane's column letters and the next lane's first
 #pragma unroll
-        for (int k = 0; k < S; ++k) coff[k] = (k < ncols) ? (int)bp[j0 + k] : 0;
-        coff[S] = (j0 + S < n) ? (int)bp[j0 + S] : 0;
+        for (int k = 0; k < S; ++k) coff[k] = (k < ncols) ? (int)bp[j0 + k] : na;   // na: no column
+        coff[S] = (j0 + S < n) ? (int)bp[j0 + S] : na;
         const SF* __restrict__ seg = V.seg + (pbase + pp) * rows * LY::row;
         // Row i+1 of the lane's columns.  Row m starts from a virtual row m+1 whose profile
         // is 1 (letter index na) and whose BM holds the end weights one column to the right,
@@ -589,11 +612,11 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         // Recompute forward row i (i >= 1) into Fd from row i-1 in Fs and row i's record: the
         // forward's row step (same operations, same order) in the record's units.
         auto recompute = [&](const SF(&Fs)[LY::vals], SF(&Fd)[LY::vals]) {
-            const int Ec = sf2e(rec[3]);
-            const SF s = sldexp(SF(1), sf2e(Fs[LY::iE]) - Ec);   // row i-1's values into row i's units
+            const int Ec = em_E(rec[3]);
+            const SF s = sldexp(SF(1), em_E(Fs[LY::iE]) - Ec);   // row i-1's values into row i's units
             const SF ebs = eb * s, obs = ob * s;
             const SF fr = L ? sldexp(SF(1), -Ec) : SF(0);
-            const SF* __restrict__ prow = spsub + rcode * na;
+            const SF* __restrict__ prow = spsub + rcode * (na + 1);
             SF pw[S];
 #pragma unroll
             for (int k = 0; k < S; ++k) pw[k] = prow[coff[k]];
@@ -611,11 +634,10 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 const SF tv = mv + xv + yv;
                 Fd[k] = mv; Fd[LY::iX + k] = xv; Fd[LY::iY + k] = yv;
                 lm = mv; lx = xv; ly = yv;
-                mx = dmax(mx, k < nc ? tv : SF(0));
+                mx = dmax(mx, tv);
             }
             Fd[LY::iC] = rec[0]; Fd[LY::iC + 1] = rec[1]; Fd[LY::iC + 2] = rec[2];
-            Fd[LY::iE] = rec[3];
-            Fd[LY::iMX] = mx;
+            Fd[LY::iE] = pack_em(Ec, mx);
         };
         auto copy_row = [&](const SF(&Fs)[LY::vals], SF(&Fd)[LY::vals]) {
 #pragma unroll
@@ -646,12 +668,12 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                     qM = bq[3]; qY = bq[4]; qE = bex(pbase + pp + 1, i - 1)[1];
                 }
             }
-            const SF* __restrict__ pn = spsub + ac * na;   // row i+1's profile
+            const SF* __restrict__ pn = spsub + ac * (na + 1);   // row i+1's profile
             SF pw[S + 1];
 #pragma unroll
             for (int k = 0; k <= S; ++k) pw[k] = pn[coff[k]];
-            const int efc = sf2e(Fc[LY::iE]);
-            const SF mfc = Fc[LY::iMX];
+            const int efc = em_E(Fc[LY::iE]);
+            const SF mfc = em_max(Fc[LY::iE]);   // a power of two at least the segment's maximum
             // This row's units.  A segment without mass of its own (Global's row m left of
             // column n, or rows whose mass has not arrived) takes the neighbour's exponent; one
             // whose neighbour's values would not fit moves up to them minus the headroom.
@@ -679,7 +701,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             const int nc = opaque(ncols);
             SF byr = rYc;   // BY(i, j+1), right to left
             SF dn = rdM;    // BM(i+1, j+1), row i+1's units
-            SF mx = SF(0), tmax = SF(0);
+            SF mx = SF(0);
             SF sx = SF(0), sy = SF(0), so = SF(0), sob = SF(0);
             SF tmk[S + 1];  // match posteriors of (i+1, j0+k+1), stored units
 #pragma unroll
@@ -693,12 +715,13 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 // row i+1's sums on forward row i (soft_kernel_impl.inl:883-884): X opens into
                 // (i+1, j), and the match posterior of (i+1, j+1)
                 sob = sfma(Fc[k] + Fc[LY::iY + k], v, sob);
-                const SF tm = dmax((Fc[k] + Fc[LY::iX + k] + Fc[LY::iY + k]) * pd, SF(0));
-                tmk[k + 1] = tm;
-                tmax = dmax(tmax, tm);
+                tmk[k + 1] = (Fc[k] + Fc[LY::iX + k] + Fc[LY::iY + k]) * pd;
                 const SF byn = sfma(ea, byr, b1);
                 const SF hh = byr * oa;
-                const bool in = k < nc;
+                // Columns past len(B) hold exact zeros in Global and Glocal: their profile entry is
+                // 0 and no end weight lies there.  Local's start term reaches every cell, so its
+                // columns past len(B) are cleared.
+                const bool in = !L || k < nc;
                 const SF bmn = in ? b1 + hh : SF(0);
                 const SF bxn = in ? b2 + hh : SF(0);
                 const SF byv = in ? byn : SF(0);
@@ -721,9 +744,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 const SF b1 = sfma(v, obs, d);
                 const SF b2 = sfma(v, ebs, d);
                 if (col0_opens) sob = sfma(Fc[LY::iC] + Fc[LY::iC + 2], v, sob);
-                const SF tm = dmax((Fc[LY::iC] + Fc[LY::iC + 1] + Fc[LY::iC + 2]) * pd, SF(0));
-                tmk[0] = tm;
-                tmax = dmax(tmax, tm);
+                tmk[0] = (Fc[LY::iC] + Fc[LY::iC + 1] + Fc[LY::iC + 2]) * pd;
                 const SF hh = byr * oa;
                 c0M = b1 + hh;
                 c0X = b2 + hh;
@@ -732,7 +753,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             }
             // match posteriors into the pair's histogram row a_{i+1} (after this step's reads)
             if (ac < na) {
-                double* __restrict__ h = hq + ac * na;
+                double* __restrict__ h = hq + ac * hrow;
 #pragma unroll
                 for (int k = 0; k <= S; ++k) atomicAdd(h + coff[k], (double)tmk[k] * fmatch);
             }
@@ -753,8 +774,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 const double c2 = (double)sy * fnew;
                 const double c3 = i >= kmin ? (double)so * (double)oa * fnew : 0.0;
                 const double c4 = (double)sob * (double)ob * fold;
-                const double cm = (double)tmax * fmatch;
-                if (!(c1 + c2 + c3 + c4 + cm <= 1.0e300)) bad = true;
+                if (!(c1 + c2 + c3 + c4 <= 1.0e300)) bad = true;
                 else { g_eb += c1; g_ea += c2; g_oa += c3; g_ob += c4; }
             }
             if (!(mx <= R::fmax)) bad = true;
@@ -833,15 +853,24 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         }
     }
     wave_sync();
-    const int np = td.npairs;
-    for (int idx = lane; idx < np * nn; idx += kWave) {
-        const int qq = idx / nn, e = idx - qq * nn;
-        const int pq = order[td.first + qq];
-        A.grads[(int64_t)pq * gs + e] = hist[qq * nn + e];
-        hist[qq * nn + e] = 0.0;
+    // Expected match counts out of the histogram (and the spill column cleared); a count that
+    // left the double range marks its pair.
+    const int np = td.npairs, hsz = na * hrow;
+    unsigned nonfinite = 0u;
+    for (int idx = lane; idx < np * hsz; idx += kWave) {
+        const int qq = idx / hsz, r = idx - qq * hsz;
+        const int a = r / hrow, c = r - a * hrow;
+        const double v = hist[idx];
+        if (c < na) {
+            const int pq = order[td.first + qq];
+            A.grads[(int64_t)pq * gs + a * na + c] = v;
+            if (!(v <= 1.0e300)) nonfinite |= 1u << qq;
+        }
+        hist[idx] = 0.0;
     }
+    for (int off = 1; off < kWave; off <<= 1) nonfinite |= (unsigned)__shfl_xor((int)nonfinite, off, kWave);
     if (g == 0 && p >= 0) {
-        const bool okp = !bad && !zbad && loss <= A.tol;
+        const bool okp = !bad && !zbad && loss <= A.tol && !((nonfinite >> q) & 1u);
         A.ok[p] = okp ? 1 : 0;
         const double logz = zbad ? 0.0 : ::log(zr) + (double)ze * 0.69314718055994530942;
         A.scores[p] = logz * A.temperature;
@@ -858,6 +887,28 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
 // queue, then blocks below n_mslots take tiles from the main queue; each block works in
 // its own scratch slot.  Plain (no queue): block b runs tile tile_base + b.  One call site of
 // the tile body, so the kernel holds one copy of it.
+// The soft weights from the parameter vector on the device, as the reference forms them
+// (aligner.hpp:6870-6885): exp(s / T) per matrix entry and exp(-(go+ge)/T), exp(-ge/T) per
+// sequence (linear gaps: open = extend), each in double and cast once.  Also zeroes the
+// persistent launch's queue counters (a kernel, not a memset, so that a captured step holds
+// no memset node).
+template <typename SF>
+__global__ __launch_bounds__(256) void soft_weights_kernel(const double* __restrict__ theta, int nn, double it,
+                                                           int linear, SF* __restrict__ psub, SF* __restrict__ gapw,
+                                                           int32_t* q0, int32_t* q1) {
+    for (int k = threadIdx.x; k < nn; k += blockDim.x) psub[k] = (SF)::exp(theta[k] * it);
+    if (threadIdx.x == 0) {
+        const double go_a = theta[nn], ge_a = theta[nn + 1], go_b = theta[nn + 2], ge_b = theta[nn + 3];
+        const double ea = ::exp(-ge_a * it), eb = ::exp(-ge_b * it);
+        gapw[0] = (SF)(linear ? ea : ::exp(-(go_a + ge_a) * it));
+        gapw[1] = (SF)ea;
+        gapw[2] = (SF)(linear ? eb : ::exp(-(go_b + ge_b) * it));
+        gapw[3] = (SF)eb;
+        if (q0) *q0 = 0;
+        if (q1) *q1 = 0;
+    }
+}
+
 // Occupancy: the float32 pass is held to two waves per SIMD (256 VGPRs); the double pass, twice
 // the register bytes, to one.
 template <typename SF, int AM, int S>
@@ -866,11 +917,16 @@ void soft_kernel(SoftArgs<SF> A) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int na = A.na, nn = na * na;
     double* hist = reinterpret_cast<double*>(lds);
-    SF* spsub = reinterpret_cast<SF*>(lds + (size_t)A.hist_slots * nn * sizeof(double));
+    const int hsz = na * (na + 1);
+    SF* spsub = reinterpret_cast<SF*>(lds + (size_t)A.hist_slots * hsz * sizeof(double));
     const int lane = threadIdx.x;
-    for (int k = lane; k < nn; k += kWave) spsub[k] = A.psub[k];
-    for (int k = nn + lane; k < nn + na; k += kWave) spsub[k] = SF(1);   // the backward's virtual row
-    for (int k = lane; k < A.hist_slots * nn; k += kWave) hist[k] = 0.0;
+    // The profile with a row and a column added: column na (a column past len(B)) weighs 0 and
+    // row na (the backward's virtual row m+1) weighs 1 everywhere.
+    for (int k = lane; k < (na + 1) * (na + 1); k += kWave) {
+        const int a = k / (na + 1), c = k - a * (na + 1);
+        spsub[k] = a == na ? SF(1) : (c == na ? SF(0) : A.psub[a * na + c]);
+    }
+    for (int k = lane; k < A.hist_slots * hsz; k += kWave) hist[k] = 0.0;
     __syncthreads();
     const int blk = (int)blockIdx.x;
     int phase = A.queue == nullptr ? 2 : ((A.lqueue != nullptr && blk < A.n_lslots) ? 0 : 1);
