@@ -17,6 +17,30 @@ import os as _os
 
 _here = _os.path.dirname(_os.path.abspath(__file__))
 
+
+def _share_torch_hip_runtime():
+    """PyTorch-ROCm ships its own HIP runtime under the same soname as the system one.  Two
+    copies in one process cannot both open the GPU, and the library would otherwise bind the
+    system copy first; so when torch is installed, its copy is loaded (only that library, torch
+    itself is not imported) before the native module, which then binds to it."""
+    import ctypes
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return
+    if spec is None or not spec.submodule_search_locations:
+        return
+    lib = _os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if _os.path.exists(lib):
+        try:
+            ctypes.CDLL(lib, mode=ctypes.RTLD_GLOBAL)
+        except OSError:
+            pass
+
+
+_share_torch_hip_runtime()
+
 try:
     from ._nwgrad_amd import AlignParams, SeqPairBatch, SeqPairBatchDouble, SubstMatrix, device_available
 except ImportError as _e:  # the native module is the product: fail loudly

@@ -747,7 +747,7 @@ int launch_soft(nwg_batch* b, const SoftPlan& pl, const SoftPlan& wl) {
     NWG_HIP(ctx, dps.reserve((size_t)nn));
     NWG_HIP(ctx, SS::gw(b).reserve(4));
     SoftKernel<SF> kern = soft_kernel_for<SF>(b->prob.mode);
-    const size_t lds = (size_t)b->hist_slots * b->na * (b->na + 1) * sizeof(double) +
+    const size_t lds = (size_t)b->hist_slots * b->na * (b->na + 1) * sizeof(double) + 5 * 64 * sizeof(double) +
                        (size_t)(b->na + 1) * (b->na + 1) * sizeof(SF);
     // resident blocks per CU: queried once per kernel and LDS size (no query inside a captured step)
     int& occ = b->soft_occ_cache[std::is_same<SF, float>::value ? 0 : 1];

@@ -47,6 +47,7 @@
 namespace nwg {
 
 constexpr int kTilePairs = 16;  // pairs per soft tile at most
+constexpr int kCK = 4;          // backward bands: one forward row in kCK stored in full
 
 // One soft tile: pairs order[first .. first + npairs), pair q on lanes lo[q] .. lo[q]+G[q]-1.
 struct SoftTile {
@@ -279,7 +280,7 @@ __device__ __forceinline__ void load_rec(const SF* __restrict__ base, int lane, 
 template <typename SF, int AM, int S>
 __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int32_t* __restrict__ order,
                           const SoftTile* __restrict__ tdp, int64_t slot, const SF* __restrict__ spsub,
-                          double* __restrict__ hist, int lane) {
+                          double* __restrict__ hist, double* __restrict__ lacc, int lane) {
     using R = SoftRange<SF>;
     using LY = Seg<SF, S>;
     constexpr bool L = AM == kLocal, GL = AM == kGlocal;
@@ -493,9 +494,10 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                     for (int k = 0; k < S; ++k) sr += (k < nc) ? T[k] : SF(0);
                     expacc_add(zacc, (double)sr, E);
                 }
-                // Rows the backward reaches on an odd step are stored in full, the others as a
-                // boundary record from which the backward recomputes them (row 0 always in full).
-                if (((m + G - 1 - g - i) & 1) != 0) store_seg<SF, S>(seg + (int64_t)(i + g) * LY::row, lane, M, X, Y, cM, cX, cY, E, mx);
+                // Rows the backward reaches at the last step of a band of kCK steps are stored in
+                // full, the others as a boundary record from which the backward recomputes them
+                // (row 0 always in full).
+                if ((m + G - 1 - g - i) % kCK == kCK - 1) store_seg<SF, S>(seg + (int64_t)(i + g) * LY::row, lane, M, X, Y, cM, cX, cY, E, mx);
                 else store_rec<SF, S>(seg + (int64_t)(i + g) * LY::row, lane, cM, cX, cY, E);
                 oM = M[S - 1]; oX = X[S - 1]; oY = Y[S - 1]; oE = E;
                 if (pout) {
@@ -547,7 +549,11 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
     const double izr = zbad ? 0.0 : 1.0 / zr;
 
     // ── backward ────────────────────────────────────────────────────────────
-    double g_oa = 0.0, g_ea = 0.0, g_ob = 0.0, g_eb = 0.0, loss = 0.0;
+    // The lane's running sums live in LDS (five doubles per lane, added without return): the
+    // gap counts oa, ea, ob, eb and the lost-mass bound.
+    double* __restrict__ acc = lacc + lane;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) acc[c * kWave] = 0.0;
     const int kmin = (L || GL) ? 1 : 0;   // first row whose Y opens count
     const bool col0_opens = !L;           // column-0 X opens count (kmin0 == 0)
     const int hrow = na + 1;   // histogram rows carry a spill column for the no-column letter
@@ -584,20 +590,36 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         const bool has_right = j0 + S < n;
         const bool pin = g == G - 1 && pp + 1 < NP;   // reads the next panel's column
         const bool pout = g == 0 && pp > 0;           // writes this panel's first column
-        // Forward rows.  Step u handles row i = m+G-1-g-u.  Rows met on odd steps (and row 0)
-        // are stored in full; a row met on an even step is recomputed at that step from the row
-        // above it -- the full row of the next step -- and its boundary record.  Two buffers
-        // cycle with u mod 4 (the same for every lane): u%4 = 0 recomputes Fb -> Fa, 1 uses Fb
-        // and loads the next full row into Fa, 2 recomputes Fa -> Fb, 3 uses Fa and loads into Fb.
+        // Forward rows.  Step u handles row i = m+G-1-g-u; steps form bands of kCK (u / kCK, the
+        // same for every lane).  The row met at a band's last step (and row 0) is stored in
+        // full; the band's other rows are recomputed from it, forward, through their boundary
+        // records.  Two buffers alternate with the band's parity: S holds the band's full row
+        // (loaded during the previous band), W receives the recomputed rows.
         SF Fa[LY::vals], Fb[LY::vals];
-        SF rec[4] = {SF(0), SF(0), SF(0), SF(0)};   // the record of the next recomputed row
-        const int u0 = G - 1 - g;                    // the lane's first step: row m
-        int rcode = (live && m >= 1) ? (int)ap[m - 1] : 0;   // code of the next recomputed row
-        if (live) {
-            const int src = (u0 & 1) ? m : (m >= 1 ? m - 1 : 0);   // full row needed first
-            if ((u0 & 3) < 2) load_seg<SF, S>(seg + (int64_t)(src + g) * LY::row, lane, Fb);
-            else load_seg<SF, S>(seg + (int64_t)(src + g) * LY::row, lane, Fa);
-            if (!(u0 & 1) && m >= 1) load_rec<SF, S>(seg + (int64_t)(m + g) * LY::row, lane, rec);
+        SF rec[kCK - 1][4];     // records of the rows above the band's full row, in order
+        int rcode[kCK - 1];     // their codes
+#pragma unroll
+        for (int j = 0; j < kCK - 1; ++j) {
+            rec[j][0] = rec[j][1] = rec[j][2] = rec[j][3] = SF(0);
+            rcode[j] = 0;
+        }
+        auto load_recs = [&](int src, int last) {   // the records of rows src+1 .. last
+#pragma unroll
+            for (int j = 0; j < kCK - 1; ++j) {
+                const int r = src + 1 + j;
+                if (r <= last) {
+                    load_rec<SF, S>(seg + (int64_t)(r + g) * LY::row, lane, rec[j]);
+                    rcode[j] = (int)ap[r - 1];
+                }
+            }
+        };
+        if (live) {   // the first band: its full row (row 0 when the band reaches it) and records
+            const int u0 = G - 1 - g;
+            const int c0 = m + u0 % kCK - (kCK - 1);
+            const int src = c0 > 0 ? c0 : 0;
+            if (((u0 / kCK) & 1) == 0) load_seg<SF, S>(seg + (int64_t)(src + g) * LY::row, lane, Fa);
+            else load_seg<SF, S>(seg + (int64_t)(src + g) * LY::row, lane, Fb);
+            load_recs(src, m);
         }
         SF qM = SF(0), qY = SF(0);
         int qE = 0;
@@ -611,18 +633,18 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
 
         // Recompute forward row i (i >= 1) into Fd from row i-1 in Fs and row i's record: the
         // forward's row step (same operations, same order) in the record's units.
-        auto recompute = [&](const SF(&Fs)[LY::vals], SF(&Fd)[LY::vals]) {
-            const int Ec = em_E(rec[3]);
+        auto recompute = [&](const SF(&Fs)[LY::vals], SF(&Fd)[LY::vals], const SF(&rc)[4], int code) {
+            const int Ec = em_E(rc[3]);
             const SF s = sldexp(SF(1), em_E(Fs[LY::iE]) - Ec);   // row i-1's values into row i's units
             const SF ebs = eb * s, obs = ob * s;
             const SF fr = L ? sldexp(SF(1), -Ec) : SF(0);
-            const SF* __restrict__ prow = spsub + rcode * (na + 1);
+            const SF* __restrict__ prow = spsub + code * (na + 1);
             SF pw[S];
 #pragma unroll
             for (int k = 0; k < S; ++k) pw[k] = prow[coff[k]];
             SF dT = Fs[LY::iC] + Fs[LY::iC + 1] + Fs[LY::iC + 2];
-            SF lm = rec[0], lx = rec[1], ly = rec[2];
-            SF mx = (j0 == 0) ? rec[0] + rec[1] + rec[2] : SF(0);
+            SF lm = rc[0], lx = rc[1], ly = rc[2];
+            SF mx = (j0 == 0) ? rc[0] + rc[1] + rc[2] : SF(0);
             const int nc = opaque(ncols);
 #pragma unroll
             for (int k = 0; k < S; ++k) {
@@ -636,27 +658,13 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 lm = mv; lx = xv; ly = yv;
                 mx = dmax(mx, tv);
             }
-            Fd[LY::iC] = rec[0]; Fd[LY::iC + 1] = rec[1]; Fd[LY::iC + 2] = rec[2];
+            Fd[LY::iC] = rc[0]; Fd[LY::iC + 1] = rc[1]; Fd[LY::iC + 2] = rc[2];
             Fd[LY::iE] = pack_em(Ec, mx);
         };
         auto copy_row = [&](const SF(&Fs)[LY::vals], SF(&Fd)[LY::vals]) {
 #pragma unroll
             for (int k = 0; k < LY::vals; ++k) Fd[k] = Fs[k];
         };
-        // Prefetch on an odd step at row i: the full row that the next even step recomputes
-        // from (row i-2, or row 0 itself when the next row is row 0), the record and code of
-        // the next row i-1.
-        auto prefetch = [&](int i, SF(&Fn)[LY::vals]) {
-            if (i >= 1) {
-                const int nf = i >= 2 ? i - 2 : 0;
-                load_seg<SF, S>(seg + (int64_t)(nf + g) * LY::row, lane, Fn);
-                if (i >= 2) {
-                    load_rec<SF, S>(seg + (int64_t)(i - 1 + g) * LY::row, lane, rec);
-                    rcode = (int)ap[i - 2];
-                }
-            }
-        };
-
         // One backward step on row i: Fc holds forward row i.
         auto step = [&](int i, SF rM, SF rY, int rE, SF(&Fc)[LY::vals]) {
             const int ac = acode;
@@ -703,7 +711,8 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             SF dn = rdM;    // BM(i+1, j+1), row i+1's units
             SF mx = SF(0);
             SF sx = SF(0), sy = SF(0), so = SF(0), sob = SF(0);
-            SF tmk[S + 1];  // match posteriors of (i+1, j0+k+1), stored units
+            // match posteriors go into the pair's histogram row a_{i+1} (row m: none, added as 0)
+            double* __restrict__ h = hq + (ac < na ? ac : 0) * hrow;
 #pragma unroll
             for (int kk = 0; kk < S; ++kk) {
                 const int k = S - 1 - kk;
@@ -715,7 +724,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 // row i+1's sums on forward row i (soft_kernel_impl.inl:883-884): X opens into
                 // (i+1, j), and the match posterior of (i+1, j+1)
                 sob = sfma(Fc[k] + Fc[LY::iY + k], v, sob);
-                tmk[k + 1] = (Fc[k] + Fc[LY::iX + k] + Fc[LY::iY + k]) * pd;
+                atomicAdd(h + coff[k + 1], (double)((Fc[k] + Fc[LY::iX + k] + Fc[LY::iY + k]) * pd) * fmatch);
                 const SF byn = sfma(ea, byr, b1);
                 const SF hh = byr * oa;
                 // Columns past len(B) hold exact zeros in Global and Glocal: their profile entry is
@@ -736,7 +745,6 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 so = sfma(fml, byv, so);
                 sx = sfma(Fc[LY::iX + k], bxn, sx);
             }
-            tmk[0] = SF(0);
             if (j0 == 0) {   // column 0 (soft_kernel_impl.inl:849-857)
                 const SF pd = (pw[0] * s) * dn;   // P(i+1, 1) BM(i+1, 1)
                 const SF d = L ? pd + init : pd;
@@ -744,18 +752,12 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 const SF b1 = sfma(v, obs, d);
                 const SF b2 = sfma(v, ebs, d);
                 if (col0_opens) sob = sfma(Fc[LY::iC] + Fc[LY::iC + 2], v, sob);
-                tmk[0] = (Fc[LY::iC] + Fc[LY::iC + 1] + Fc[LY::iC + 2]) * pd;
+                atomicAdd(h + coff[0], (double)((Fc[LY::iC] + Fc[LY::iC + 1] + Fc[LY::iC + 2]) * pd) * fmatch);
                 const SF hh = byr * oa;
                 c0M = b1 + hh;
                 c0X = b2 + hh;
                 mx = dmax(mx, dmax(c0M, c0X));
                 sx = sfma(Fc[LY::iC + 1], c0X, sx);
-            }
-            // match posteriors into the pair's histogram row a_{i+1} (after this step's reads)
-            if (ac < na) {
-                double* __restrict__ h = hq + ac * hrow;
-#pragma unroll
-                for (int k = 0; k <= S; ++k) atomicAdd(h + coff[k], (double)tmk[k] * fmatch);
             }
             // Lost mass (soft_kernel_impl.inl:141-178): any value flushed in this row's forward
             // or backward segment was below MIN in its own units; its effect on any posterior
@@ -766,7 +768,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             if (!zbad && (mx > SF(0) || mfc > SF(0))) {
                 const double lr = 32.0 * (double)(ncols + 1) * A.vmin * (3.0 * (double)mx + (double)mfc) * fnew +
                                   8.0 * (double)(ncols + 1) * A.vtiny * (fnew + fold);
-                loss += (lr <= 1.0e300) ? lr : 1.0e300;
+                atomicAdd(acc + 4 * kWave, (lr <= 1.0e300) ? lr : 1.0e300);
             }
             if (!zbad) {
                 // every term is a sum of non-negative products: one range check covers them
@@ -775,7 +777,12 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 const double c3 = i >= kmin ? (double)so * (double)oa * fnew : 0.0;
                 const double c4 = (double)sob * (double)ob * fold;
                 if (!(c1 + c2 + c3 + c4 <= 1.0e300)) bad = true;
-                else { g_eb += c1; g_ea += c2; g_oa += c3; g_ob += c4; }
+                else {
+                    atomicAdd(acc, c3);
+                    atomicAdd(acc + kWave, c2);
+                    atomicAdd(acc + 2 * kWave, c4);
+                    atomicAdd(acc + 3 * kWave, c1);
+                }
             }
             if (!(mx <= R::fmax)) bad = true;
             rdM = rMc;
@@ -794,54 +801,47 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             }
         };
 
-        // Rows run m .. 0 at steps u = G-1-g .. G-1-g+m; the step count is rounded up to a
-        // multiple of 4 (steps in which no lane is active) so that the loop body holds the
-        // buffers' full cycle.
-        const int bsteps = (td.fsteps + 4) & ~3;
-        for (int u = 0; u < bsteps; u += 4) {
-            {   // u % 4 == 0: recompute Fb -> Fa
-                const int i = m + G - 1 - g - u;
-                const SF rM = wshift_down(pubM), rY = wshift_down(pubY);
-                const int rE = wshift_down(pubE);
-                if (live && i >= 0 && i <= m) {
-                    if (i >= 1) recompute(Fb, Fa);
-                    else copy_row(Fb, Fa);
-                    step(i, rM, rY, rE, Fa);
+        // One step at phase p of a band, S / W the band's buffers.  Phases before the last
+        // recompute the row from S through the records (row 0 is copied: it is stored in full);
+        // the last phase uses S and loads the next band's full row into W.
+        auto phase = [&](int u, int p, SF(&Sb)[LY::vals], SF(&Wb)[LY::vals]) {
+            const int i = m + G - 1 - g - u;
+            const SF rM = wshift_down(pubM), rY = wshift_down(pubY);
+            const int rE = wshift_down(pubE);
+            if (!(live && i >= 0 && i <= m)) return;
+            if (p < kCK - 1) {
+                const int c = i - (kCK - 1 - p);
+                const int nr = i - (c > 0 ? c : 0);   // rows to recompute, ending at row i
+                if (nr >= 1) recompute(Sb, Wb, rec[0], rcode[0]);
+#pragma unroll
+                for (int j = 1; j < kCK - 1 - p; ++j)
+                    if (nr > j) recompute(Wb, Wb, rec[j], rcode[j]);
+                if (nr == 0) copy_row(Sb, Wb);
+                step(i, rM, rY, rE, Wb);
+            } else {
+                if (i >= 1) {
+                    const int src = i - kCK > 0 ? i - kCK : 0;
+                    load_seg<SF, S>(seg + (int64_t)(src + g) * LY::row, lane, Wb);
+                    load_recs(src, i - 1);
                 }
+                step(i, rM, rY, rE, Sb);
             }
-            {   // u % 4 == 1: Fb holds the row; load the next full row into Fa
-                const int i = m + G - 1 - g - (u + 1);
-                const SF rM = wshift_down(pubM), rY = wshift_down(pubY);
-                const int rE = wshift_down(pubE);
-                if (live && i >= 0 && i <= m) {
-                    prefetch(i, Fa);
-                    step(i, rM, rY, rE, Fb);
-                }
-            }
-            {   // u % 4 == 2: recompute Fa -> Fb
-                const int i = m + G - 1 - g - (u + 2);
-                const SF rM = wshift_down(pubM), rY = wshift_down(pubY);
-                const int rE = wshift_down(pubE);
-                if (live && i >= 0 && i <= m) {
-                    if (i >= 1) recompute(Fa, Fb);
-                    else copy_row(Fa, Fb);
-                    step(i, rM, rY, rE, Fb);
-                }
-            }
-            {   // u % 4 == 3: Fa holds the row; load the next full row into Fb
-                const int i = m + G - 1 - g - (u + 3);
-                const SF rM = wshift_down(pubM), rY = wshift_down(pubY);
-                const int rE = wshift_down(pubE);
-                if (live && i >= 0 && i <= m) {
-                    prefetch(i, Fb);
-                    step(i, rM, rY, rE, Fa);
-                }
-            }
+        };
+        // Rows run m .. 0 at steps u = G-1-g .. G-1-g+m; the step count is rounded up to two
+        // bands (steps in which no lane is active) so that the loop body holds both parities.
+        const int bsteps = (td.fsteps + 2 * kCK) / (2 * kCK) * (2 * kCK);
+        for (int u = 0; u < bsteps; u += 2 * kCK) {
+#pragma unroll
+            for (int p = 0; p < kCK; ++p) phase(u + p, p, Fa, Fb);
+#pragma unroll
+            for (int p = 0; p < kCK; ++p) phase(u + kCK + p, p, Fb, Fa);
         }
         if (NP > 1) __threadfence();   // the panel's boundary column is read by the previous panel
     }
 
     // ── per-pair results ────────────────────────────────────────────────────
+    wave_sync();
+    double g_oa = acc[0], g_ea = acc[kWave], g_ob = acc[2 * kWave], g_eb = acc[3 * kWave], loss = acc[4 * kWave];
     for (int off = 1; off < kWave; off <<= 1) {
         const double o1 = __shfl_down(g_oa, off, kWave), o2 = __shfl_down(g_ea, off, kWave);
         const double o3 = __shfl_down(g_ob, off, kWave), o4 = __shfl_down(g_eb, off, kWave);
@@ -918,7 +918,8 @@ void soft_kernel(SoftArgs<SF> A) {
     const int na = A.na, nn = na * na;
     double* hist = reinterpret_cast<double*>(lds);
     const int hsz = na * (na + 1);
-    SF* spsub = reinterpret_cast<SF*>(lds + (size_t)A.hist_slots * hsz * sizeof(double));
+    double* lacc = hist + (size_t)A.hist_slots * hsz;   // 5 x 64 per-lane sums
+    SF* spsub = reinterpret_cast<SF*>(lacc + 5 * kWave);
     const int lane = threadIdx.x;
     // The profile with a row and a column added: column na (a column past len(B)) weighs 0 and
     // row na (the backward's virtual row m+1) weighs 1 everywhere.
@@ -954,7 +955,7 @@ void soft_kernel(SoftArgs<SF> A) {
             }
         }
         soft_tile<SF, AM, S>(A, wide ? A.lview : A.view, wide ? A.lorder : A.order,
-                             (wide ? A.ltiles : A.tiles) + tile, (int64_t)blk, spsub, hist, lane);
+                             (wide ? A.ltiles : A.tiles) + tile, (int64_t)blk, spsub, hist, lacc, lane);
     }
 }
 
