@@ -34,7 +34,8 @@ HBM_PEAK_GBS = 8000.0           # MI355X HBM3E
 # read back by the backward (24 B/cell without checkpointing).
 SOFT_BYTES_PER_CELL = 24
 
-WORKLOADS = ("protein_soft_homologs", "dna_local_short", "dna_local_short_score", "protein_global_hard")
+WORKLOADS = ("protein_soft_homologs", "dna_local_short", "dna_local_short_score", "protein_global_hard",
+             "long_hard_36000", "long_hard_36000_double", "long_hard_4096", "long_soft_8000", "serving")
 
 
 def measured_traffic(workload, fill):
@@ -59,14 +60,22 @@ def make_workload(name):
         return wl.dna_local_short(), "none"
     if name == "protein_soft_homologs":
         return wl.protein_soft_homologs(), "soft"
+    # long pairs: B a mutated copy of A (substitution 0.1, indels 0.01, mean length 5), global affine 11/1
+    if name in ("long_hard_36000", "long_hard_36000_double"):
+        return wl.long_pairs(16, 36000), "hard"
+    if name == "long_hard_4096":
+        return wl.long_pairs(256, 4096), "hard"
+    if name == "long_soft_8000":
+        return wl.long_pairs(4, 8000, grad_mode="soft"), "soft"
     raise SystemExit(f"unknown workload {name}")
 
 
-def make_batch(w, grad_mode, fill):
+def make_batch(w, grad_mode, fill, precision="float32"):
     import nwgrad_amd as nw
     p = nw.AlignParams(w.matrix, alphabet=w.alphabet, gap_open_a=w.gaps[0], gap_extend_a=w.gaps[1],
                        gap_open_b=w.gaps[2], gap_extend_b=w.gaps[3])
-    b = nw.SeqPairBatch.from_codes(w.codes_a, w.off_a, w.codes_b, w.off_b, p, gap_model=w.gap_model, mode=w.mode,
+    cls = nw.SeqPairBatch if precision == "float32" else nw.SeqPairBatchDouble
+    b = cls.from_codes(w.codes_a, w.off_a, w.codes_b, w.off_b, p, gap_model=w.gap_model, mode=w.mode,
                                    grad_mode=grad_mode)
     b.fill = fill
     return b
@@ -86,7 +95,7 @@ def time_steps(b, steps, warmup, sync):
     return dt, float(np.mean(kms))
 
 
-def cpu_baseline(w, grad_mode, budget_s=8.0):
+def cpu_baseline(w, grad_mode, precision="float32", budget_s=8.0):
     """The reference engine (oracle/_ref) on all host cores, else the C oracle on one: a
     bounded sample of the workload's pairs, best of 2 timed passes after a warm-up."""
     import oracle
@@ -99,7 +108,7 @@ def cpu_baseline(w, grad_mode, budget_s=8.0):
         def run(sub):
             sa, sb = sub.strings()
             rb = ref.RefBatch(sa, sb, sub.alphabet, sub.matrix, sub.gaps, gap_model=sub.gap_model, mode=sub.mode,
-                              grad_mode=grad_mode, n_threads=cores)
+                              grad_mode=grad_mode, n_threads=cores, double=precision == "float64")
             if grad_mode == "soft":
                 rb.set_soft(precision="float32", impl="scaled_or_log")
             rb.score_and_grad()  # warm-up
@@ -120,10 +129,11 @@ def cpu_baseline(w, grad_mode, budget_s=8.0):
                                 gap_model=sub.gap_model, mode=sub.mode)
             else:
                 oracle.hard(sub.matrix, sub.gaps, sub.codes_a, sub.off_a, sub.codes_b, sub.off_b,
-                            gap_model=sub.gap_model, mode=sub.mode, with_grad=grad_mode == "hard")
+                            gap_model=sub.gap_model, mode=sub.mode, with_grad=grad_mode == "hard",
+                            precision=precision)
             return time.perf_counter() - t0
     n = w.n_pairs
-    k = min(n, 256)
+    k = max(1, min(n, 256, int(2e8 * n / max(w.cells, 1))))   # a first sample of about 2e8 cells
     idx = np.sort(rng.choice(n, size=k, replace=False))
     sub = w.subset(idx)
     t = run(sub)
@@ -139,6 +149,42 @@ def cpu_baseline(w, grad_mode, budget_s=8.0):
                       f"best of 2 passes after a warm-up"}
 
 
+def serving_leg(duration, rate=200.0, warm=3.0):
+    """The serving workload (Poisson arrivals at `rate`/s; pairs per request log-U[1, 4096];
+    lengths log-U[30, 5000]; 70 / 20 / 10 % score / hard / soft; global affine 11/1, float32;
+    HBM budget 200 GB) replayed against nwgrad_amd.serving.Server in real time.  The first
+    `warm` seconds of arrivals run first, unmeasured (slot creation, buffer growth); requests
+    are generated before the replay, so the client's own work is not in the latencies."""
+    import nwgrad_amd as nw
+    from nwgrad_amd import workloads as wl
+    from nwgrad_amd.serving import Server, replay
+    load = wl.ServingLoad(rate=rate, duration=duration + warm)
+    reqs = [load.request(k) for k in range(len(load))]
+    for q in reqs:
+        q.t -= warm
+    warm_reqs = [q for q in reqs if q.t < 0]
+    for q in warm_reqs:
+        q.t += warm
+    timed = [q for q in reqs if q.t >= 0]
+    p = nw.AlignParams(load.matrix, alphabet=wl.AA, gap_open_a=11.0, gap_extend_a=1.0, gap_open_b=11.0,
+                       gap_extend_b=1.0)
+    srv = Server(p, hbm_budget_bytes=200e9)
+    print(f"bench: serving warm-up, {len(warm_reqs)} requests", file=sys.stderr, flush=True)
+    replay(srv, warm_reqs)
+    print(f"bench: serving, {len(timed)} requests over {duration:.0f} s", file=sys.stderr, flush=True)
+    out = replay(srv, timed)
+    offered = sum(q.cells for q in timed) / duration / 1e9
+    res = {"workload": "serving", "rate_per_s": rate, "duration_s": duration, "requests": out["jobs"],
+           "failed": out["failed"], "p50_ms": round(out["p50_ms"], 3), "p99_ms": round(out["p99_ms"], 3),
+           "max_ms": round(out["max_ms"], 3), "gcups_delivered": round(out["gcups"], 3),
+           "gcups_offered": round(offered, 3), "launches": out["launches"],
+           "mix": "70/20/10 score/hard/soft", "pairs_per_request": "log-U[1,4096]", "lengths": "log-U[30,5000]",
+           "warmup_s": warm}
+    print(f"bench: serving p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms, {res['gcups_delivered']} GCUPS "
+          f"delivered of {res['gcups_offered']} offered", file=sys.stderr, flush=True)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +195,9 @@ def main():
     ap.add_argument("--fill", default="interpair", choices=("interpair", "striped"))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-launch comparison leg")
+    ap.add_argument("--serve-seconds", type=float, default=None,
+                    help="seconds of serving arrivals (default: 10 beside the main workload, 60 for "
+                         "--workload serving; 0: no serving leg)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only time a 1 GiB device-to-device copy (the known-size reference for HBM counters)")
     args = ap.parse_args()
@@ -173,8 +222,17 @@ def main():
               flush=True)
         return 0
     print(f"bench: workload {args.workload}", file=sys.stderr, flush=True)
+    if args.workload == "serving":
+        res = serving_leg(60.0 if args.serve_seconds is None else args.serve_seconds)
+        line = {"metric": "GCUPS delivered under Poisson arrivals; p50 / p99 latency", "value": res["gcups_delivered"],
+                "unit": "GCUPS", "n_gpus": 1, "steps": res["requests"], "warmup": 0, "ms_per_step": res["p50_ms"],
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                "data": "synthetic", "config": {"workload": "serving"}, "serving": res}
+        print(json.dumps(line), flush=True)
+        return 0
     w, grad_mode = make_workload(args.workload)
-    b = make_batch(w, grad_mode, args.fill)
+    precision = "float64" if args.workload.endswith("_double") else "float32"
+    b = make_batch(w, grad_mode, args.fill, precision)
     cells = b.cells
     print(f"bench: {w.n_pairs} pairs, {cells} cells, {b.tiles} tiles; timing {args.steps} steps", file=sys.stderr,
           flush=True)
@@ -213,8 +271,10 @@ def main():
                 "kernel_ms": round(kms, 4), "valu": valu}
     if not args.no_cpu:
         print("bench: CPU baseline (bounded sample)", file=sys.stderr, flush=True)
-    cpu = None if args.no_cpu else cpu_baseline(w, grad_mode)
-    dtype = "f32"
+    cpu = None if args.no_cpu else cpu_baseline(w, grad_mode, precision)
+    serve_s = 10.0 if args.serve_seconds is None else args.serve_seconds
+    serving = serving_leg(serve_s) if serve_s > 0 else None
+    dtype = "f32" if precision == "float32" else "f64"
     line = {"metric": "GCUPS soft-grad, batched vs plain GPU fill @ bit-exact/1e-5; p99 latency",
             "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
@@ -225,7 +285,7 @@ def main():
             "plain_vs_batched": None if gcups_other is None else
                 {"fill": other, "gcups": round(gcups_other, 3), "kernel_gcups": round(cells / (kms2 * 1e-3) / 1e9, 3),
                  "batched_over_plain": round(gcups / gcups_other, 3)},
-            "roofline": roof, "cpu_baseline": cpu}
+            "roofline": roof, "cpu_baseline": cpu, "serving": serving}
     if grad_mode == "soft":
         line["soft_pairs_rerun_in_double"] = int(reruns)
         line["launch"] = {"grid": int(launch[0]), "wide_queue_blocks": int(launch[1]), "resident_per_cu": int(launch[2])}

@@ -61,6 +61,10 @@ void nwg_destroy(nwg_ctx *ctx);
 const char *nwg_last_error(const nwg_ctx *ctx);
 /* Work of this context goes to `hip_stream` (a hipStream_t; NULL: the null stream). */
 int nwg_set_stream(nwg_ctx *ctx, void *hip_stream);
+/* Give the context a stream of its own (non-blocking, default priority; destroyed with the
+ * context), so that batches of different contexts run concurrently.  No counterpart in the
+ * reference, whose calls are synchronous; the serving loop runs one context per slot. */
+int nwg_ctx_own_stream(nwg_ctx *ctx);
 
 /* Upload N pairs (codes in [0, n_alpha), offsets of N+1 entries) and plan them into
  * tiles.  Replaces BatchEngine's constructor and planner (batch_engine.hpp:284-350,
@@ -69,6 +73,17 @@ int nwg_batch_create(nwg_ctx *ctx, const nwg_problem *prob, int n_alpha, int64_t
                      const uint8_t *codes_a, const int64_t *off_a, const uint8_t *codes_b,
                      const int64_t *off_b, nwg_batch **out);
 void nwg_batch_free(nwg_batch *b);
+/* Replace the batch's pairs with N new ones of the same problem type and alphabet, keeping
+ * its device buffers (grown when too small, never shrunk) and its params.  Uploads go to the
+ * context's stream; the call returns once the new plan is on the device, and leaves
+ * results invalid.  The batch must be idle (nwg_batch_query).  No counterpart in the
+ * reference (a batch there is built once); the serving loop reloads its slots so that a
+ * launch allocates and frees no device memory once the slots have grown. */
+int nwg_batch_reload(nwg_batch *b, int64_t n_pairs, const uint8_t *codes_a, const int64_t *off_a,
+                     const uint8_t *codes_b, const int64_t *off_b);
+/* *done = 1 when all work enqueued for the batch on its context's stream has finished, else 0
+ * (hipStreamQuery; never blocks).  No counterpart in the reference (synchronous calls). */
+int nwg_batch_query(nwg_batch *b, int *done);
 
 /* Params: matrix n_alpha*n_alpha row-major, gaps4 = go_a, ge_a, go_b, ge_b.  Cast to the
  * batch's precision on upload as static_cast<T>(double).  Invalidates cached results.

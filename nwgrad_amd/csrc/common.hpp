@@ -45,6 +45,23 @@ __device__ __forceinline__ float shift_down(float v) { return __shfl_down(v, 1, 
 __device__ __forceinline__ double shift_down(double v) { return __shfl_down(v, 1, kWave); }
 __device__ __forceinline__ int shift_down(int v) { return __shfl_down(v, 1, kWave); }
 
+// The value of the wave's first active lane as a wave-uniform (scalar) value: branches on it
+// are uniform to the compiler.  Persistent loops fetch their next tile with it, from an
+// atomicAdd that every lane issues (lane 0 adding 1, the others 0) rather than one issued
+// under `if (lane == 0)`: with a lane-dependent branch in the loop the compiler has split the
+// loop into an outer loop for lane 0 (the fetch) and an inner one that the other lanes never
+// leave, each lane then running different tiles and the wave never reconverging.
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float wave_uniform(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double wave_uniform(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 // Wavefront-local memory ordering for LDS handed between lanes of one wave.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");

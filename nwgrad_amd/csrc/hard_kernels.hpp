@@ -86,6 +86,141 @@ __device__ __forceinline__ Cand<T> reduce_slot(Cand<T> c, int G) {
     return c;
 }
 
+// One row of a lane's strip: cells j0+1 .. j0+S of row i, updated in place.  (lM, lX, lY) is
+// the left neighbour (column j0 of row i), (dM, dX, dY) the diagonal (column j0 of row i-1).
+// cw receives the predecessor codes, four cells to a dword; rowmax the row's maximum over
+// the lane's real columns (Local).  Affine: aligner.hpp:4020-4047; linear: aligner.hpp:3828-3849.
+template <typename T, int GM, int AM, int S, bool HARD>
+__device__ __forceinline__ void strip_row(T (&M)[S], T (&X)[S], T (&Y)[S], T lM, T lX, T lY, T dM, T dX, T dY,
+                                          const T* __restrict__ srow, const int (&coff)[S], int ncols, T go_a,
+                                          T ge_a, T go_b, T ge_b, uint32_t (&cw)[S / 4], T& rowmax) {
+#pragma unroll
+    for (int k = 0; k < S / 4; ++k) cw[k] = 0u;
+    rowmax = T(0);
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const T s = srow[coff[k]];
+        if constexpr (GM == kAffine) {
+            const T uM = M[k], uX = X[k], uY = Y[k];
+            const T tXY = dmax(dX, dY);
+            T mv = dmax(dM, tXY) + s;
+            uint32_t km = 0;
+            if constexpr (HARD) km = (dM >= tXY) ? 0u : ((dX >= dY) ? 1u : 2u);
+            const T ux = (uM - go_b) - ge_b, vx = uX - ge_b, wx = (uY - go_b) - ge_b;
+            const T tx = dmax(vx, wx);
+            const T xv = dmax(ux, tx);
+            const T uy = (lM - go_a) - ge_a, vy = (lX - go_a) - ge_a, wy = lY - ge_a;
+            const T ty = dmax(vy, wy);
+            const T yv = dmax(uy, ty);
+            if constexpr (AM == kLocal) {
+                mv = dmax(mv, T(0));
+                if constexpr (HARD) km = (mv <= T(0)) ? 3u : km;
+                if (k < ncols) rowmax = dmax(rowmax, dmax(mv, dmax(xv, yv)));
+            }
+            if constexpr (HARD) {
+                const uint32_t kx = (ux >= tx) ? 0u : ((vx >= wx) ? 1u : 2u);
+                const uint32_t ky = (uy >= ty) ? 0u : ((vy >= wy) ? 1u : 2u);
+                cw[k >> 2] |= (km | (kx << 2) | (ky << 4)) << (8 * (k & 3));
+            }
+            M[k] = mv; X[k] = xv; Y[k] = yv;
+            dM = uM; dX = uX; dY = uY;
+            lM = mv; lX = xv; lY = yv;
+        } else {
+            const T uH = M[k];
+            const T d0 = dM + s;
+            const T u = uH - ge_b;
+            const T tt = dmax(d0, u);
+            const T l = lM - ge_a;
+            T v = dmax(tt, l);
+            uint32_t c = 0;
+            if constexpr (HARD) c = (tt < l) ? 2u : ((d0 < u) ? 1u : 0u);
+            if constexpr (AM == kLocal) {
+                v = dmax(v, T(0));
+                if constexpr (HARD) c = (v <= T(0)) ? 3u : c;
+                if (k < ncols) rowmax = dmax(rowmax, v);
+            }
+            if constexpr (HARD) cw[k >> 2] |= c << (8 * (k & 3));
+            M[k] = v;
+            dM = uH;
+            lM = v;
+        }
+    }
+}
+
+// local_row_best_ (aligner.hpp:5975-5984): the first column holding the row maximum, when
+// the row beats the lane's best so far.
+template <typename T, int GM, int S>
+__device__ __forceinline__ void local_update(Cand<T>& best, const T (&M)[S], const T (&X)[S], const T (&Y)[S],
+                                             T rowmax, int i, int j0, int ncols) {
+    if (rowmax > best.v) {
+        bool found = false;
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            T v3;
+            if constexpr (GM == kAffine) v3 = dmax(dmax(M[k], X[k]), Y[k]);
+            else v3 = M[k];
+            if (!found && k < ncols && v3 == rowmax) {
+                found = true;
+                best.v = rowmax; best.i = i; best.j = j0 + k + 1;
+                if constexpr (GM == kAffine) best.st = state_of(M[k], X[k], Y[k]);
+                else best.st = 0;
+            }
+        }
+    }
+}
+
+// The lane's end-cell candidate once row m is done (aligner.hpp:4054-4066, 3851-3860; Glocal
+// 6121-6134).  col0: this lane also offers column 0 (the first lane of the pair's first panel).
+template <typename T, int GM, int AM, int S>
+__device__ __forceinline__ Cand<T> end_candidate(const T (&M)[S], const T (&X)[S], const T (&Y)[S],
+                                                 const Cand<T>& best, int m, int n, int j0, int ncols, bool col0,
+                                                 T go_b, T ge_b) {
+    Cand<T> c;
+    c.v = T(0); c.i = -1; c.j = 0; c.st = 0;
+    if constexpr (AM == kLocal) {
+        c = best;
+    } else if constexpr (AM == kGlobal) {
+        if (n == 0) {
+            if (col0) {
+                c.i = m; c.j = 0;
+                if (m == 0) { c.v = T(0); c.st = 0; }
+                else if constexpr (GM == kAffine) { c.v = -(go_b + (T)m * ge_b); c.st = 1; }
+                else { c.v = (-(T)m) * ge_b; c.st = 0; }
+            }
+        } else if (n - 1 >= j0 && n - 1 < j0 + S) {
+            const int kn = n - 1 - j0;
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                if (k == kn) {
+                    c.i = m; c.j = n;
+                    if constexpr (GM == kAffine) {
+                        c.v = dmax(dmax(M[k], X[k]), Y[k]);
+                        c.st = state_of(M[k], X[k], Y[k]);
+                    } else { c.v = M[k]; c.st = 0; }
+                }
+            }
+        }
+    } else {   // Glocal: row m, the first column with the strictly largest value
+        if (col0) {   // column 0 first
+            c.i = m; c.j = 0;
+            if (m == 0) { c.v = T(0); c.st = 0; }
+            else if constexpr (GM == kAffine) { c.v = -(go_b + (T)m * ge_b); c.st = 1; }
+            else { c.v = (-(T)m) * ge_b; c.st = 0; }
+        }
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            if (k < ncols) {
+                T v3;
+                int st3 = 0;
+                if constexpr (GM == kAffine) { v3 = dmax(dmax(M[k], X[k]), Y[k]); st3 = state_of(M[k], X[k], Y[k]); }
+                else v3 = M[k];
+                if (c.i < 0 || v3 > c.v) { c.v = v3; c.i = m; c.j = j0 + k + 1; c.st = st3; }
+            }
+        }
+    }
+    return c;
+}
+
 template <typename T, int GM, int AM, int S, bool HARD>
 __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint16_t* __restrict__ hist,
                           int tile, int64_t slot, int lane) {
@@ -167,60 +302,10 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
         if (active) {
             const int rowoff = (int)ap[i - 1] * na;
             const T* __restrict__ srow = ssub + rowoff;
-            T lM = iM, lX = iX, lY = iY;
-            T dM = pM, dX = pX, dY = pY;
             uint32_t cw[S / 4];
-#pragma unroll
-            for (int k = 0; k < S / 4; ++k) cw[k] = 0u;
-            T rowmax = T(0);
-#pragma unroll
-            for (int k = 0; k < S; ++k) {
-                const T s = srow[coff[k]];
-                if constexpr (GM == kAffine) {
-                    const T uM = M[k], uX = X[k], uY = Y[k];
-                    const T tXY = dmax(dX, dY);
-                    T mv = dmax(dM, tXY) + s;
-                    uint32_t km = 0;
-                    if constexpr (HARD) km = (dM >= tXY) ? 0u : ((dX >= dY) ? 1u : 2u);
-                    const T ux = (uM - go_b) - ge_b, vx = uX - ge_b, wx = (uY - go_b) - ge_b;
-                    const T tx = dmax(vx, wx);
-                    const T xv = dmax(ux, tx);
-                    const T uy = (lM - go_a) - ge_a, vy = (lX - go_a) - ge_a, wy = lY - ge_a;
-                    const T ty = dmax(vy, wy);
-                    const T yv = dmax(uy, ty);
-                    if constexpr (AM == kLocal) {
-                        mv = dmax(mv, T(0));
-                        if constexpr (HARD) km = (mv <= T(0)) ? 3u : km;
-                        if (k < ncols) rowmax = dmax(rowmax, dmax(mv, dmax(xv, yv)));
-                    }
-                    if constexpr (HARD) {
-                        const uint32_t kx = (ux >= tx) ? 0u : ((vx >= wx) ? 1u : 2u);
-                        const uint32_t ky = (uy >= ty) ? 0u : ((vy >= wy) ? 1u : 2u);
-                        cw[k >> 2] |= (km | (kx << 2) | (ky << 4)) << (8 * (k & 3));
-                    }
-                    M[k] = mv; X[k] = xv; Y[k] = yv;
-                    dM = uM; dX = uX; dY = uY;
-                    lM = mv; lX = xv; lY = yv;
-                } else {
-                    const T uH = M[k];
-                    const T d0 = dM + s;
-                    const T u = uH - ge_b;
-                    const T tt = dmax(d0, u);
-                    const T l = lM - ge_a;
-                    T v = dmax(tt, l);
-                    uint32_t c = 0;
-                    if constexpr (HARD) c = (tt < l) ? 2u : ((d0 < u) ? 1u : 0u);
-                    if constexpr (AM == kLocal) {
-                        v = dmax(v, T(0));
-                        if constexpr (HARD) c = (v <= T(0)) ? 3u : c;
-                        if (k < ncols) rowmax = dmax(rowmax, v);
-                    }
-                    if constexpr (HARD) cw[k >> 2] |= c << (8 * (k & 3));
-                    M[k] = v;
-                    dM = uH;
-                    lM = v;
-                }
-            }
+            T rowmax;
+            strip_row<T, GM, AM, S, HARD>(M, X, Y, iM, iX, iY, pM, pX, pY, srow, coff, ncols, go_a, ge_a, go_b, ge_b,
+                                          cw, rowmax);
             oM = M[S - 1];
             if constexpr (GM == kAffine) { oX = X[S - 1]; oY = Y[S - 1]; }
             if constexpr (HARD) {
@@ -228,75 +313,14 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
                 for (int k = 0; k < S / 4; ++k)
                     tb[((int64_t)t * (S / 4) + k) * kWave + lane] = cw[k];
             }
-            if constexpr (AM == kLocal) {
-                // local_row_best_ (aligner.hpp:5975-5984): the first column holding the
-                // row maximum, when the row beats the best so far.
-                if (rowmax > best.v) {
-                    bool found = false;
-#pragma unroll
-                    for (int k = 0; k < S; ++k) {
-                        T v3;
-                        if constexpr (GM == kAffine) v3 = dmax(dmax(M[k], X[k]), Y[k]);
-                        else v3 = M[k];
-                        if (!found && k < ncols && v3 == rowmax) {
-                            found = true;
-                            best.v = rowmax; best.i = i; best.j = j0 + k + 1;
-                            if constexpr (GM == kAffine) best.st = state_of(M[k], X[k], Y[k]);
-                            else best.st = 0;
-                        }
-                    }
-                }
-            }
+            if constexpr (AM == kLocal) local_update<T, GM, S>(best, M, X, Y, rowmax, i, j0, ncols);
         }
         pM = iM; pX = iX; pY = iY;
     }
 
-    // End cells (aligner.hpp:4054-4066, 3851-3860).
     Cand<T> c;
     c.v = T(0); c.i = -1; c.j = 0; c.st = 0;
-    if (p >= 0) {
-        if constexpr (AM == kLocal) {
-            c = best;
-        } else if constexpr (AM == kGlobal) {
-            if (n == 0) {
-                if (g == 0) {
-                    c.i = m; c.j = 0;
-                    if (m == 0) { c.v = T(0); c.st = 0; }
-                    else if constexpr (GM == kAffine) { c.v = -(go_b + (T)m * ge_b); c.st = 1; }
-                    else { c.v = (-(T)m) * ge_b; c.st = 0; }
-                }
-            } else if ((n - 1) / S == g) {
-                const int kn = n - 1 - j0;
-#pragma unroll
-                for (int k = 0; k < S; ++k) {
-                    if (k == kn) {
-                        c.i = m; c.j = n;
-                        if constexpr (GM == kAffine) {
-                            c.v = dmax(dmax(M[k], X[k]), Y[k]);
-                            c.st = state_of(M[k], X[k], Y[k]);
-                        } else { c.v = M[k]; c.st = 0; }
-                    }
-                }
-            }
-        } else {   // Glocal: row m, the first column with the strictly largest value
-            if (g == 0) {   // column 0 first
-                c.i = m; c.j = 0;
-                if (m == 0) { c.v = T(0); c.st = 0; }
-                else if constexpr (GM == kAffine) { c.v = -(go_b + (T)m * ge_b); c.st = 1; }
-                else { c.v = (-(T)m) * ge_b; c.st = 0; }
-            }
-#pragma unroll
-            for (int k = 0; k < S; ++k) {
-                if (k < ncols) {
-                    T v3;
-                    int st3 = 0;
-                    if constexpr (GM == kAffine) { v3 = dmax(dmax(M[k], X[k]), Y[k]); st3 = state_of(M[k], X[k], Y[k]); }
-                    else v3 = M[k];
-                    if (c.i < 0 || v3 > c.v) { c.v = v3; c.i = m; c.j = j0 + k + 1; c.st = st3; }
-                }
-            }
-        }
-    }
+    if (p >= 0) c = end_candidate<T, GM, AM, S>(M, X, Y, best, m, n, j0, ncols, g == 0, go_b, ge_b);
     c = reduce_slot(c, G);
 
     int c_goa = 0, c_gea = 0, c_gob = 0, c_geb = 0;
@@ -395,9 +419,7 @@ __global__ __launch_bounds__(64) void hard_fill_kernel(HardArgs<T> A) {
         return;
     }
     for (;;) {
-        int tile = 0;
-        if (lane == 0) tile = atomicAdd(A.queue, 1);
-        tile = __shfl(tile, 0, kWave);
+        const int tile = wave_uniform(atomicAdd(A.queue, lane == 0 ? 1 : 0));
         if (tile >= A.n_tiles) break;
         hard_tile<T, GM, AM, S, HARD>(A, ssub, hist, tile, (int64_t)blockIdx.x, lane);
     }

@@ -1,0 +1,378 @@
+// Hard and score-only fills of pairs wider than one wavefront's columns (len(B) > 64 S):
+// column panels pipelined across waves, then a traceback walk per pair.
+//
+// Reference behaviour: the same cells, codes and end cells as hard_kernels.hpp (relative to
+// the reference's src/nwgrad/cpp/nwgrad/: aligner.hpp:3955-4071, 3756-3865, 5975-5984,
+// 6121-6134), over the full pointer table the reference's own long-pair default keeps
+// (traceback="auto": Grid, aligner.hpp:89-100 and 5267-5742, returns the Pointers fill's
+// exact path); the walk is aligner.hpp:2986-3056 / 2826-2855.
+//
+// Fill.  A pair of len(B) = n is cut into ceil(n / (64 S)) panels of 64 S columns.  Each
+// (pair, panel) is one item of a device queue, filled by one wave exactly as hard_tile fills
+// a one-pair tile, except that lane 0 takes its left neighbour from the previous panel's
+// right boundary column instead of the matrix border.  The panel's last lane writes its
+// right column, row by row, to HBM and publishes every RB rows (and row m) through the
+// item's counter with a release store; the next panel's wave waits for a block's rows with
+// acquire loads before it starts the block.  A pair's items are queued in panel order, so
+// every awaited block belongs to an item a running wave took earlier: the waits end.  Each
+// wait is still bounded in real time; past it the wave flags an error (the host raises)
+// and goes on, and no wave waits once the flag is up.  Pointer codes go to a
+// per-item table in the hard fill's layout.
+//
+// Walk.  One block per pair combines the panels' end-cell candidates (larger value, ties to
+// the smaller (i, j): the reference's row-major first rule), then walks the codes back.
+// The walk only ever moves to the same, the previous or the one-before-previous step of a
+// panel's table, so it reads the table through an LDS window of kWalkSteps steps that all
+// threads of the block refill together.
+#pragma once
+
+#include "hard_kernels.hpp"
+
+namespace nwg {
+
+constexpr int kWalkSteps = 48;     // table steps held in the walk's LDS window (48 KB)
+constexpr int kWalkThreads = 256;  // threads of a walk block (they refill the window together)
+
+struct PanelItem {
+    int32_t pair;
+    int32_t panel;
+    int64_t table_off;   // dwords: this panel's pointer codes (hard)
+    int64_t bnd_in;      // T elements: the previous panel's right column (3 per row); -1 for the first panel
+    int64_t bnd_out;     // T elements: this panel's right column; -1 for the last panel
+};
+static_assert(sizeof(PanelItem) == 32, "PanelItem is 32 bytes");
+
+struct LongPair {
+    int32_t pair;
+    int32_t item;     // its first item (panel 0)
+    int32_t panels;
+    int32_t pad;
+};
+
+template <typename T>
+struct CandRec {
+    T v;
+    int32_t i, j, st;
+};
+
+template <typename T>
+struct LongArgs {
+    const uint8_t* codes_a;
+    const uint8_t* codes_b;
+    const int64_t* off_a;
+    const int64_t* off_b;
+    const PanelItem* items;
+    int32_t n_items;
+    int32_t* queue;
+    const LongPair* lpairs;
+    const T* sub;
+    int32_t na;
+    T go_a, ge_a, go_b, ge_b;
+    int32_t* progress;    // per item: rows of its right column published
+    T* bnd;               // right boundary columns
+    CandRec<T>* cand;     // per item: the panel's end-cell candidate
+    uint32_t* table;      // pointer codes (hard)
+    double* scores;
+    double* grads;
+    int32_t* err;         // bit 0: a wait passed its bound (or the walk met an invalid end cell)
+    uint64_t wait_ticks;  // bound of one wait, in s_memrealtime ticks
+};
+
+template <typename T, int GM, int AM, int S, bool HARD, int RB>
+__device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int it, int lane) {
+    static_assert(RB >= 1 && RB <= kWave, "a row block fits the lanes of a wave");
+    const PanelItem item = A.items[it];
+    const int p = item.pair, na = A.na;
+    const int64_t a0 = A.off_a[p], b0 = A.off_b[p];
+    const int m = (int)(A.off_a[p + 1] - a0), n = (int)(A.off_b[p + 1] - b0);
+    const uint8_t* ap = A.codes_a + a0;
+    const uint8_t* bp = A.codes_b + b0;
+    const int g = lane;
+    const int j0 = item.panel * (kWave * S) + g * S;
+    const int ncols = n - j0 < 0 ? 0 : (n - j0 > S ? S : n - j0);
+    const T go_a = A.go_a, ge_a = A.ge_a, go_b = A.go_b, ge_b = A.ge_b;
+    const T NI = ninf<T>();
+    const bool first = item.bnd_in < 0, last = item.bnd_out < 0;
+
+    int coff[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) coff[k] = (k < ncols) ? (int)bp[j0 + k] : 0;
+
+    // Row 0 of the strip and the diagonal of row 1's first cell, as in hard_tile.
+    T M[S], X[S], Y[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const T jt = (T)(j0 + k + 1);
+        if constexpr (GM == kAffine) {
+            if (AM == kGlobal) { M[k] = NI; X[k] = NI; Y[k] = -(go_a + jt * ge_a); }
+            else { M[k] = T(0); X[k] = NI; Y[k] = NI; }
+        } else {
+            M[k] = (AM == kGlobal) ? (-jt) * ge_a : T(0);
+            X[k] = NI; Y[k] = NI;
+        }
+    }
+    T pM, pX, pY;
+    {
+        const T jt = (T)j0;
+        if constexpr (GM == kAffine) {
+            if (AM == kGlobal) {
+                pM = (j0 == 0) ? T(0) : NI; pX = NI; pY = (j0 == 0) ? NI : -(go_a + jt * ge_a);
+            } else { pM = T(0); pX = NI; pY = NI; }
+        } else {
+            pM = (AM == kGlobal) ? (-jt) * ge_a : T(0); pX = NI; pY = NI;
+        }
+    }
+    T oM = M[S - 1], oX = X[S - 1], oY = Y[S - 1];
+
+    Cand<T> best;
+    best.v = T(0); best.i = (AM == kLocal) ? 0 : -1; best.j = 0; best.st = 0;
+
+    const T* __restrict__ bin = A.bnd + (first ? 0 : item.bnd_in);
+    T* __restrict__ bout = A.bnd + (last ? 0 : item.bnd_out);
+    int32_t* prog_in = A.progress + (first ? it : it - 1);
+    int32_t* prog_out = A.progress + it;
+    uint32_t* tb = A.table + item.table_off;
+    T bM = NI, bX = NI, bY = NI;   // lane r: the left boundary of row t0 + 1 + r of the current block
+
+    const int steps = m + kWave - 1;
+    for (int t = 0; t < steps; ++t) {
+        const int i = t - g + 1;
+        if (!first && t < m && t % RB == 0) {
+            // Rows t+1 .. min(t+RB, m) of the previous panel's right column.
+            const int need = t + RB < m ? t + RB : m;
+            // Poll with relaxed loads (an acquire per poll would invalidate the L2 each time),
+            // then one acquire fence.  The wait is bounded in real time (s_memrealtime ticks);
+            // past the bound, or once any wave has flagged, nothing waits and the call fails.
+            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            for (;;) {
+                const int have = __builtin_amdgcn_readfirstlane(
+                    __hip_atomic_load(prog_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                if (have >= need) break;
+                const int flagged = __builtin_amdgcn_readfirstlane(
+                    __hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                if (flagged != 0 || __builtin_amdgcn_s_memrealtime() - t0 > A.wait_ticks) {
+                    if (lane == 0) __hip_atomic_fetch_or(A.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(8);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const int r = t + 1 + lane;
+            if (lane < RB && r <= m) {
+                bM = bin[3 * (int64_t)r];
+                if constexpr (GM == kAffine) { bX = bin[3 * (int64_t)r + 1]; bY = bin[3 * (int64_t)r + 2]; }
+            }
+        }
+        T iM = shift_up(oM);
+        T iX, iY;
+        if constexpr (GM == kAffine) { iX = shift_up(oX); iY = shift_up(oY); }
+        else { iX = NI; iY = NI; }
+        if (!first) {
+            const int sl = t % RB;
+            const T lM = __shfl(bM, sl, kWave);
+            T lX = NI, lY = NI;
+            if constexpr (GM == kAffine) { lX = __shfl(bX, sl, kWave); lY = __shfl(bY, sl, kWave); }
+            if (g == 0) { iM = lM; iX = lX; iY = lY; }
+        } else if (g == 0) {   // column 0 of row i (aligner.hpp:4009-4018, 3823-3826)
+            const T itv = (T)i;
+            if constexpr (GM == kAffine) {
+                if (AM == kLocal) { iM = T(0); iX = NI; }
+                else { iM = NI; iX = -(go_b + itv * ge_b); }
+                iY = NI;
+            } else {
+                iM = (AM == kLocal) ? T(0) : (-itv) * ge_b;
+            }
+        }
+        const bool active = i >= 1 && i <= m && ncols > 0;
+        if (active) {
+            const T* __restrict__ srow = ssub + (int)ap[i - 1] * na;
+            uint32_t cw[S / 4];
+            T rowmax;
+            strip_row<T, GM, AM, S, HARD>(M, X, Y, iM, iX, iY, pM, pX, pY, srow, coff, ncols, go_a, ge_a, go_b, ge_b,
+                                          cw, rowmax);
+            oM = M[S - 1];
+            if constexpr (GM == kAffine) { oX = X[S - 1]; oY = Y[S - 1]; }
+            if constexpr (HARD) {
+#pragma unroll
+                for (int k = 0; k < S / 4; ++k)
+                    tb[((int64_t)t * (S / 4) + k) * kWave + lane] = cw[k];
+            }
+            if constexpr (AM == kLocal) local_update<T, GM, S>(best, M, X, Y, rowmax, i, j0, ncols);
+            if (!last && g == kWave - 1) {
+                bout[3 * (int64_t)i] = M[S - 1];
+                if constexpr (GM == kAffine) { bout[3 * (int64_t)i + 1] = X[S - 1]; bout[3 * (int64_t)i + 2] = Y[S - 1]; }
+                if (i % RB == 0 || i == m)
+                    __hip_atomic_store(prog_out, i, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        pM = iM; pX = iX; pY = iY;
+    }
+
+    Cand<T> c = end_candidate<T, GM, AM, S>(M, X, Y, best, m, n, j0, ncols, item.panel == 0 && g == 0, go_b, ge_b);
+    c = reduce_slot(c, kWave);
+    if (lane == 0) {
+        CandRec<T> r;
+        r.v = c.v; r.i = c.i; r.j = c.j; r.st = c.st;
+        A.cand[it] = r;
+    }
+}
+
+// One wavefront per block, pulling (pair, panel) items until the queue is drained; every
+// wave reaches the exit test once the counter passes n_items.
+template <typename T, int GM, int AM, int S, bool HARD, int RB>
+__global__ __launch_bounds__(64) void long_fill_kernel(LongArgs<T> A) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int nn = A.na * A.na;
+    T* ssub = reinterpret_cast<T*>(lds);
+    const int lane = threadIdx.x;
+    for (int k = lane; k < nn; k += kWave) ssub[k] = A.sub[k];
+    __syncthreads();
+    for (;;) {
+        const int it = wave_uniform(atomicAdd(A.queue, lane == 0 ? 1 : 0));
+        if (it >= A.n_items) break;
+        long_panel<T, GM, AM, S, HARD, RB>(A, ssub, it, lane);
+    }
+}
+
+// Block b: pair lpairs[b].  Combines its panels' candidates into the end cell, writes the
+// score, and (hard) walks the codes back, counting matches in an LDS histogram and gap moves
+// in registers (every thread runs the walk with the same values; thread 0 writes).
+template <typename T, int GM, int AM, int S, bool HARD>
+__global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1);
+    const int na = A.na, nn = na * na, gs = nn + 4;
+    const LongPair lp = A.lpairs[blockIdx.x];
+    const int p = lp.pair;
+
+    Cand<T> c;
+    c.v = T(0); c.i = -1; c.j = 0; c.st = 0;
+    for (int base = 0; base < lp.panels; base += kWave) {
+        Cand<T> o;
+        o.v = T(0); o.i = -1; o.j = 0; o.st = 0;
+        if (base + lane < lp.panels) {
+            const CandRec<T> r = A.cand[lp.item + base + lane];
+            o.v = r.v; o.i = r.i; o.j = r.j; o.st = r.st;
+        }
+        o = reduce_slot(o, kWave);
+        if (cand_better(o, c)) c = o;
+    }
+    c.v = wave_uniform(c.v);   // lane 0 holds the combined candidate
+    c.i = wave_uniform(c.i);
+    c.j = wave_uniform(c.j);
+    c.st = wave_uniform(c.st);
+    if (tid == 0) A.scores[p] = (double)c.v;
+    if constexpr (HARD) {
+        uint32_t* hist = reinterpret_cast<uint32_t*>(lds);
+        uint32_t* win = reinterpret_cast<uint32_t*>(lds + ((nn * 4 + 15) & ~15));
+        for (int k = tid; k < nn; k += kWalkThreads) hist[k] = 0u;
+        __syncthreads();
+        const int64_t a0 = A.off_a[p], b0 = A.off_b[p];
+        const int m = (int)(A.off_a[p + 1] - a0), n = (int)(A.off_b[p + 1] - b0);
+        const uint8_t* ap = A.codes_a + a0;
+        const uint8_t* bp = A.codes_b + b0;
+        constexpr int PW = kWave * S;
+        constexpr int SD = (S / 4) * kWave;   // dwords per table step
+        int c_goa = 0, c_gea = 0, c_gob = 0, c_geb = 0;
+        int i = c.i, j = c.j, st = c.st;
+        if (i < 0 || i > m || j < 0 || j > n || st < 0 || st > 2 ||
+            __hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            // a failed fill (the call raises): no walk
+            if (tid == 0) __hip_atomic_fetch_or(A.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            i = 0; j = 0;
+        }
+        int wpanel = -1;
+        int wlo = 1, whi = 0;   // the window holds steps [wlo, whi] of panel wpanel
+        for (;;) {
+            if (i <= 0 && j <= 0) break;
+            if (i == 0 || j == 0) {   // borders: runs (Global / Glocal column 0) or a stop
+                if (AM == kLocal) break;
+                if (i == 0) {
+                    if (AM == kGlocal) break;
+                    c_gea += j;
+                    if (GM == kAffine) c_goa += 1;
+                    break;
+                }
+                c_geb += i;
+                if (GM == kAffine) c_gob += 1;
+                break;
+            }
+            const int jj = j - 1;
+            const int P = jj / PW, jr = jj - P * PW;
+            const int go = jr / S, k = jr - go * S;
+            const int t = i - 1 + go;
+            if (P != wpanel || t < wlo || t > whi) {
+                __syncthreads();   // every read of the old window is done
+                wpanel = P;
+                whi = t;
+                wlo = t - (kWalkSteps - 1) > 0 ? t - (kWalkSteps - 1) : 0;
+                const uint4* src = reinterpret_cast<const uint4*>(A.table + A.items[lp.item + P].table_off +
+                                                                  (int64_t)wlo * SD);
+                uint4* dst = reinterpret_cast<uint4*>(win);
+                const int n4 = (whi - wlo + 1) * (SD / 4);
+                constexpr int per = kWalkSteps * (SD / 4) / kWalkThreads;
+                uint4 buf[per];
+#pragma unroll
+                for (int r = 0; r < per; ++r) {   // every load in flight at once (n4 >= 64)
+                    const int q = tid + r * kWalkThreads;
+                    buf[r] = src[q < n4 ? q : n4 - 1];
+                }
+#pragma unroll
+                for (int r = 0; r < per; ++r) {
+                    const int q = tid + r * kWalkThreads;
+                    if (q < n4) dst[q] = buf[r];
+                }
+                __syncthreads();
+            }
+            // every thread reads the same word: a wave-uniform value keeps the walk's control
+            // flow scalar
+            const uint32_t w = (uint32_t)wave_uniform((int)win[(t - wlo) * SD + (k >> 2) * kWave + go]);
+            const uint32_t cc = (w >> (8 * (k & 3))) & 0xffu;
+            if constexpr (GM == kAffine) {
+                if (st == 0) {
+                    if (AM == kLocal && (cc & 3u) == 3u) break;
+                    const uint32_t kk = cc & 3u;
+                    if (tid == 0)
+                        __hip_atomic_fetch_add(&hist[(int)ap[i - 1] * na + (int)bp[j - 1]], 1u, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    st = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
+                    --i; --j;
+                } else if (st == 1) {
+                    const uint32_t kk = (cc >> 2) & 3u;
+                    const int pv = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
+                    c_geb += 1;
+                    c_gob += (pv != 1);
+                    st = pv;
+                    --i;
+                } else {
+                    const uint32_t kk = cc >> 4;
+                    const int pv = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
+                    c_gea += 1;
+                    c_goa += (pv != 2);
+                    st = pv;
+                    --j;
+                }
+            } else {
+                if (cc == 3u) break;
+                if (cc == 0u) {
+                    if (tid == 0)
+                        __hip_atomic_fetch_add(&hist[(int)ap[i - 1] * na + (int)bp[j - 1]], 1u, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    --i; --j;
+                } else if (cc == 1u) { c_geb += 1; --i; }
+                else { c_gea += 1; --j; }
+            }
+        }
+        __syncthreads();
+        double* gr = A.grads + (int64_t)p * gs;
+        for (int k = tid; k < nn; k += kWalkThreads) gr[k] = (double)hist[k];
+        if (tid == 0) {
+            gr[nn + 0] = 0.0 - (double)c_goa;
+            gr[nn + 1] = 0.0 - (double)c_gea;
+            gr[nn + 2] = 0.0 - (double)c_gob;
+            gr[nn + 3] = 0.0 - (double)c_geb;
+        }
+    }
+}
+
+}  // namespace nwg

@@ -16,6 +16,7 @@
 
 #include "../../include/nwgrad_hip.h"
 #include "hard_kernels.hpp"
+#include "long_kernels.hpp"
 #include "reduce_kernels.hpp"
 #include "soft_kernels.hpp"
 
@@ -24,8 +25,12 @@ using nwg::TileDesc;
 struct nwg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t own = nullptr;   // a stream the context created (nwg_ctx_own_stream)
     int n_cu = 256;
     std::string err;
+    ~nwg_ctx() {
+        if (own) (void)hipStreamDestroy(own);
+    }
 };
 
 namespace {
@@ -33,13 +38,16 @@ namespace {
 constexpr int kS = 16;                 // columns per lane: hard and score-only fills
 constexpr int kSS = 8;                 // columns per lane: soft pass (float32 and double)
 constexpr int kMaxG = 64;              // lanes per pair at most: one wavefront
-constexpr int kMaxN = kS * kMaxG;      // longest B a lane group holds (hard / score-only)
+constexpr int kMaxN = kS * kMaxG;      // longest B a lane group holds (hard / score-only); wider: column panels
 constexpr int kHistBytes = 16384;      // LDS histogram budget per wave (hard walk, uint16 counts)
 constexpr int kSoftHistBytes = 10240;  // LDS histogram budget per wave (soft, double expected counts)
 constexpr int kBlocksPerCU = 8;        // persistent grid of the hard / score fills: waves per CU
 constexpr int kSoftBlocksPerCU = 8;    // fallback persistent grid of the soft pass
 constexpr int64_t kSumBlock = 4096;    // the reference's WEIGHTED_GRAD_BLOCK
 constexpr size_t kScratchBudget = size_t(96) << 30;   // soft row-segment scratch of one launch, at most
+constexpr size_t kTableBudget = size_t(64) << 30;     // long-pair pointer codes of one launch group, at most
+constexpr int kRowBlock = 64;          // long pairs: rows a panel publishes at a time
+constexpr uint64_t kWaitTicks = 200000000;   // long pairs: a row-block wait, at most (2 s of the 100 MHz s_memrealtime)
 
 int fail(nwg_ctx* c, int code, const char* fmt, ...) {
     char buf[512];
@@ -81,6 +89,8 @@ struct DevPtr {
     }
     // grow to at least `count` elements (contents are not kept)
     hipError_t reserve(size_t count) { return n >= count && p ? hipSuccess : alloc(count); }
+    // the same with headroom, for buffers a reloaded batch regrows
+    hipError_t grow(size_t count) { return n >= count && p ? hipSuccess : alloc(count + count / 2); }
 };
 
 // Persistent grid: as many one-wave blocks as the kernel keeps resident per CU (registers
@@ -132,6 +142,21 @@ struct SoftPlan {
     DevPtr<nwg::SoftTile> d_tiles, d_ptiles;
 };
 
+// Long pairs (len(B) > kMaxN): one item per (pair, column panel), in launch groups whose
+// pointer tables fit kTableBudget.  Offsets in the items are relative to their group.
+struct LongGroup {
+    int pair_begin, pair_end, item_begin, item_end;
+    int64_t table_dwords, bnd_elems;
+};
+struct LongPlan {
+    std::vector<nwg::PanelItem> items;
+    std::vector<nwg::LongPair> lpairs;
+    std::vector<LongGroup> groups;
+    int64_t max_items = 0, max_table = 0, max_bnd = 0;   // over groups
+    DevPtr<nwg::PanelItem> d_items;
+    DevPtr<nwg::LongPair> d_lpairs;
+};
+
 }  // namespace
 
 struct nwg_batch {
@@ -147,7 +172,8 @@ struct nwg_batch {
     // plans: the pairs one lane group holds, the wider ones (column panels), and the pairs
     // a float32 soft pass refused
     int gmin = 1, hist_slots = 64;
-    Plan plan, lplan;        // hard / score-only: len(B) within one wavefront's columns, and wider
+    Plan plan;               // hard / score-only: len(B) within one wavefront's columns
+    LongPlan lpl;            // hard / score-only: wider pairs (column panels)
     // soft: packed tiles (persistent fill) and per-bucket tiles (plain fill), each split into
     // one-panel and wide (column-panel) tiles; the pairs a float32 pass refused
     SoftPlan sp_main, sp_wide, sq_main, sq_wide, rr_main, rr_wide;
@@ -155,6 +181,12 @@ struct nwg_batch {
     DevPtr<int32_t> queue;
     // hard: pointer codes per wave slot
     DevPtr<uint32_t> table;
+    // long pairs: pointer codes, boundary columns, per-item counters and candidates, error word
+    DevPtr<uint32_t> ltable;
+    DevPtr<float> lbnd32;
+    DevPtr<double> lbnd64;
+    DevPtr<int32_t> lprog, lerr;
+    DevPtr<unsigned char> lcand;
     // soft: row segments of the main and the long-pair pools, panel columns, weights
     DevPtr<float> seg32, lseg32, pb32, pf32, gw32;
     DevPtr<double> seg64, lseg64, pb64, pf64, gw64;
@@ -175,6 +207,7 @@ struct nwg_batch {
     // reduction scratch
     DevPtr<double> partial, wdev, outdev;
     double* host_scores = nullptr;   // pinned
+    int64_t host_cap = 0;
     // settings
     int fill = 0;
     double temperature = 1.0;
@@ -269,14 +302,82 @@ int build_plan(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& pl, int S,
     return NWG_OK;
 }
 
-// The pairs of `pairs` cut for S columns per lane: those within one wavefront's columns
-// into `main`, the wider ones (column panels) into `wide`.
-int build_plans(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& main, Plan& wide, int S, bool pow2) {
+// Long pairs: items in order of cells (largest first), a pair's panels consecutive and in
+// panel order -- the order the fill's waits rely on.  Groups close when the next pair's
+// pointer table would pass kTableBudget.
+int build_long_plan(nwg_batch* b, std::vector<int32_t> pairs, LongPlan& L) {
+    nwg_ctx* ctx = b->ctx;
+    const bool hard = b->prob.grad_mode == 1;
+    constexpr int PW = kS * kMaxG;
+    constexpr int64_t SD = (kS / 4) * kMaxG;   // table dwords per step
+    L.items.clear();
+    L.lpairs.clear();
+    L.groups.clear();
+    L.max_items = L.max_table = L.max_bnd = 0;
+    std::stable_sort(pairs.begin(), pairs.end(), [&](int32_t x, int32_t y) {
+        return (int64_t)b->mlen[(size_t)x] * b->nlen[(size_t)x] > (int64_t)b->mlen[(size_t)y] * b->nlen[(size_t)y];
+    });
+    LongGroup gr{0, 0, 0, 0, 0, 0};
+    auto close = [&]() {
+        gr.pair_end = (int)L.lpairs.size();
+        gr.item_end = (int)L.items.size();
+        if (gr.pair_end > gr.pair_begin) {
+            L.groups.push_back(gr);
+            L.max_items = std::max<int64_t>(L.max_items, gr.item_end - gr.item_begin);
+            L.max_table = std::max(L.max_table, gr.table_dwords);
+            L.max_bnd = std::max(L.max_bnd, gr.bnd_elems);
+        }
+        gr = LongGroup{gr.pair_end, 0, gr.item_end, 0, 0, 0};
+    };
+    for (int32_t p : pairs) {
+        const int m = b->mlen[(size_t)p], n = b->nlen[(size_t)p];
+        const int P = (n + PW - 1) / PW;
+        const int64_t per_panel = hard ? ((int64_t)m + kMaxG - 1) * SD : 0;
+        const int64_t tbl = per_panel * P;
+        if ((size_t)tbl * 4 > kTableBudget)
+            return fail(ctx, NWG_EINVAL,
+                        "nwgrad_hip: pair %d (%d x %d) needs a %.1f GiB pointer table; this build keeps at most %.0f GiB "
+                        "of codes per launch",
+                        (int)p, m, n, (double)tbl * 4 / (1 << 30), (double)(kTableBudget >> 30));
+        if (gr.table_dwords + tbl > (int64_t)(kTableBudget / 4)) close();
+        nwg::LongPair lp{p, (int32_t)(L.items.size() - gr.item_begin), P, 0};
+        L.lpairs.push_back(lp);
+        for (int k = 0; k < P; ++k) {
+            nwg::PanelItem it{};
+            it.pair = p;
+            it.panel = k;
+            it.table_off = gr.table_dwords + per_panel * k;
+            it.bnd_in = k == 0 ? -1 : gr.bnd_elems - 3 * ((int64_t)m + 1);
+            it.bnd_out = -1;
+            if (k + 1 < P) {
+                it.bnd_out = gr.bnd_elems;
+                gr.bnd_elems += 3 * ((int64_t)m + 1);
+            }
+            L.items.push_back(it);
+        }
+        gr.table_dwords += tbl;
+    }
+    close();
+    NWG_HIP(ctx, L.d_items.reserve(std::max<size_t>(L.items.size(), 1)));
+    NWG_HIP(ctx, L.d_lpairs.reserve(std::max<size_t>(L.lpairs.size(), 1)));
+    if (!L.items.empty()) {
+        NWG_HIP(ctx, hipMemcpyAsync(L.d_items.p, L.items.data(), sizeof(nwg::PanelItem) * L.items.size(),
+                                    hipMemcpyHostToDevice, ctx->stream));
+        NWG_HIP(ctx, hipMemcpyAsync(L.d_lpairs.p, L.lpairs.data(), sizeof(nwg::LongPair) * L.lpairs.size(),
+                                    hipMemcpyHostToDevice, ctx->stream));
+        NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return NWG_OK;
+}
+
+// The pairs of `pairs`: those within one wavefront's columns into the tile plan, the wider
+// ones into the long-pair plan.
+int build_plans(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& main, LongPlan& wide) {
     std::vector<int32_t> sp, lp;
-    for (int32_t i : pairs) (b->nlen[(size_t)i] > S * kMaxG ? lp : sp).push_back(i);
-    int rc = build_plan(b, sp, main, S, pow2);
+    for (int32_t i : pairs) (b->nlen[(size_t)i] > kMaxN ? lp : sp).push_back(i);
+    int rc = build_plan(b, sp, main, kS, true);
     if (rc) return rc;
-    return build_plan(b, lp, wide, S, pow2);
+    return build_long_plan(b, lp, wide);
 }
 
 // Soft tiles.  A pair needs ceil(n / (64 S)) column panels at least; in a tile of NP panels
@@ -452,6 +553,72 @@ int build_soft_plans(nwg_batch* b, const std::vector<int32_t>& pairs, SoftPlan& 
     return upload_soft_plan(b, wide, wbins, wb);
 }
 
+// Validate and upload N pairs into the batch, (re)build the hard / score-only plan and mark
+// the soft plans stale.  Nothing of the batch changes when validation fails.  `keep`: grow
+// the device buffers with headroom and never shrink them (a reloaded batch).
+int load_pairs(nwg_batch* b, int64_t N, const uint8_t* codes_a, const int64_t* off_a, const uint8_t* codes_b,
+               const int64_t* off_b, bool keep) {
+    nwg_ctx* ctx = b->ctx;
+    const int n_alpha = b->na, nn = n_alpha * n_alpha;
+    if (N < 0 || N > INT32_MAX) return fail(ctx, NWG_EINVAL, "nwgrad_hip: invalid pair count");
+    if (N > 0 && (!off_a || !off_b)) return fail(ctx, NWG_EINVAL, "nwgrad_hip: missing offsets");
+    std::vector<int32_t> mlen((size_t)N), nlen((size_t)N);
+    int64_t cells = 0;
+    const int64_t la = N ? off_a[N] : 0, lb = N ? off_b[N] : 0;
+    if (N && (off_a[0] < 0 || off_b[0] < 0)) return fail(ctx, NWG_EINVAL, "nwgrad_hip: negative offset");
+    for (int64_t i = 0; i < N; ++i) {
+        const int64_t m = off_a[i + 1] - off_a[i], n = off_b[i + 1] - off_b[i];
+        if (m < 0 || n < 0) return fail(ctx, NWG_EINVAL, "nwgrad_hip: offsets must be non-decreasing");
+        if (m > (1 << 24) || n > (1 << 24)) return fail(ctx, NWG_EINVAL, "nwgrad_hip: pair %lld is too long", (long long)i);
+        mlen[(size_t)i] = (int32_t)m;
+        nlen[(size_t)i] = (int32_t)n;
+        cells += m * n;
+    }
+    for (int64_t k = 0; k < la; ++k)
+        if (codes_a[k] >= n_alpha) return fail(ctx, NWG_EINVAL, "nwgrad_hip: code out of range in A");
+    for (int64_t k = 0; k < lb; ++k)
+        if (codes_b[k] >= n_alpha) return fail(ctx, NWG_EINVAL, "nwgrad_hip: code out of range in B");
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    auto size = [&](auto& d, size_t count) { return keep ? d.grow(count) : d.alloc(count); };
+    NWG_HIP(ctx, size(b->ca, (size_t)std::max<int64_t>(la, 1)));
+    NWG_HIP(ctx, size(b->cb, (size_t)std::max<int64_t>(lb, 1)));
+    NWG_HIP(ctx, size(b->oa, (size_t)N + 1));
+    NWG_HIP(ctx, size(b->ob, (size_t)N + 1));
+    NWG_HIP(ctx, size(b->scores, (size_t)std::max<int64_t>(N, 1)));
+    if (b->prob.grad_mode != 0) NWG_HIP(ctx, size(b->grads, (size_t)std::max<int64_t>(N, 1) * (nn + 4)));
+    if (la) NWG_HIP(ctx, hipMemcpyAsync(b->ca.p, codes_a, (size_t)la, hipMemcpyHostToDevice, ctx->stream));
+    if (lb) NWG_HIP(ctx, hipMemcpyAsync(b->cb.p, codes_b, (size_t)lb, hipMemcpyHostToDevice, ctx->stream));
+    const int64_t zero[1] = {0};
+    NWG_HIP(ctx, hipMemcpyAsync(b->oa.p, N ? off_a : zero, sizeof(int64_t) * (size_t)(N + 1), hipMemcpyHostToDevice,
+                                ctx->stream));
+    NWG_HIP(ctx, hipMemcpyAsync(b->ob.p, N ? off_b : zero, sizeof(int64_t) * (size_t)(N + 1), hipMemcpyHostToDevice,
+                                ctx->stream));
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (b->host_cap < std::max<int64_t>(N, 1)) {
+        if (b->host_scores) NWG_HIP(ctx, hipHostFree(b->host_scores));
+        b->host_scores = nullptr;
+        b->host_cap = 0;
+        const int64_t cap = keep ? std::max<int64_t>(N, 1) + N / 2 : std::max<int64_t>(N, 1);
+        NWG_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&b->host_scores), sizeof(double) * (size_t)cap,
+                                   hipHostMallocDefault));
+        b->host_cap = cap;
+    }
+    b->N = N;
+    b->mlen.swap(mlen);
+    b->nlen.swap(nlen);
+    b->cells = cells;
+    b->results_valid = false;
+    b->soft_reruns = 0;
+    for (SoftPlan* sp : {&b->sp_main, &b->sp_wide, &b->sq_main, &b->sq_wide}) sp->built = false;
+    if (b->prob.grad_mode != 2) {
+        std::vector<int32_t> all((size_t)N);
+        std::iota(all.begin(), all.end(), 0);
+        const int rc = build_plans(b, all, b->plan, b->lpl);
+        if (rc) return rc;
+    }
+    return NWG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -486,6 +653,18 @@ int nwg_set_stream(nwg_ctx* ctx, void* s) {
     return NWG_OK;
 }
 
+int nwg_ctx_own_stream(nwg_ctx* ctx) {
+    if (!ctx) return NWG_EINVAL;
+    if (ctx->own) {
+        ctx->stream = ctx->own;
+        return NWG_OK;
+    }
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    NWG_HIP(ctx, hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking));
+    ctx->stream = ctx->own;
+    return NWG_OK;
+}
+
 int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t N, const uint8_t* codes_a,
                      const int64_t* off_a, const uint8_t* codes_b, const int64_t* off_b, nwg_batch** out) {
     if (!ctx || !prob || !out) return NWG_EINVAL;
@@ -503,30 +682,9 @@ int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t
     b->prob = *prob;
     b->na = n_alpha;
     b->N = N;
-    b->mlen.resize((size_t)N);
-    b->nlen.resize((size_t)N);
-    const int64_t la = N ? off_a[N] : 0, lb = N ? off_b[N] : 0;
-    for (int64_t i = 0; i < N; ++i) {
-        const int64_t m = off_a[i + 1] - off_a[i], n = off_b[i + 1] - off_b[i];
-        if (m < 0 || n < 0) return fail(ctx, NWG_EINVAL, "nwgrad_hip: offsets must be non-decreasing");
-        if (n > kMaxN && prob->grad_mode != 2)
-            return fail(ctx, NWG_EINVAL,
-                        "nwgrad_hip: pair %lld has len(B) = %lld; this build's hard and score-only fills hold at most "
-                        "%d columns",
-                        (long long)i, (long long)n, kMaxN);
-        if (m > (1 << 24)) return fail(ctx, NWG_EINVAL, "nwgrad_hip: pair %lld is too long", (long long)i);
-        b->mlen[(size_t)i] = (int32_t)m;
-        b->nlen[(size_t)i] = (int32_t)n;
-        b->cells += m * n;
-    }
-    for (int64_t k = 0; k < la; ++k)
-        if (codes_a[k] >= n_alpha) return fail(ctx, NWG_EINVAL, "nwgrad_hip: code out of range in A");
-    for (int64_t k = 0; k < lb; ++k)
-        if (codes_b[k] >= n_alpha) return fail(ctx, NWG_EINVAL, "nwgrad_hip: code out of range in B");
-
+    const int nn = n_alpha * n_alpha;
     // The hard walk keeps uint16 match counts per pair slot in LDS, the soft pass double
     // expected counts: the slot count, hence the narrowest lane group, follows the budget.
-    const int nn = n_alpha * n_alpha;
     if (prob->grad_mode == 2) {
         // soft: any width, floor(64 / G) pair slots of nn doubles
         const int slots = std::max(1, std::min(nwg::kTilePairs, kSoftHistBytes / (n_alpha * (n_alpha + 1) * 8)));
@@ -541,26 +699,9 @@ int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t
         b->gmin = gmin;
         b->hist_slots = 64 / gmin;
     }
-
-    NWG_HIP(ctx, b->ca.alloc((size_t)std::max<int64_t>(la, 1)));
-    NWG_HIP(ctx, b->cb.alloc((size_t)std::max<int64_t>(lb, 1)));
-    NWG_HIP(ctx, b->oa.alloc((size_t)N + 1));
-    NWG_HIP(ctx, b->ob.alloc((size_t)N + 1));
     NWG_HIP(ctx, b->queue.alloc(4));
-    NWG_HIP(ctx, b->scores.alloc((size_t)std::max<int64_t>(N, 1)));
-    if (prob->grad_mode != 0) NWG_HIP(ctx, b->grads.alloc((size_t)std::max<int64_t>(N, 1) * (nn + 4)));
-    if (la) NWG_HIP(ctx, hipMemcpy(b->ca.p, codes_a, (size_t)la, hipMemcpyHostToDevice));
-    if (lb) NWG_HIP(ctx, hipMemcpy(b->cb.p, codes_b, (size_t)lb, hipMemcpyHostToDevice));
-    NWG_HIP(ctx, hipMemcpy(b->oa.p, off_a, sizeof(int64_t) * (size_t)(N + 1), hipMemcpyHostToDevice));
-    NWG_HIP(ctx, hipMemcpy(b->ob.p, off_b, sizeof(int64_t) * (size_t)(N + 1), hipMemcpyHostToDevice));
-    if (prob->grad_mode != 2) {
-        std::vector<int32_t> all((size_t)N);
-        std::iota(all.begin(), all.end(), 0);
-        const int rc = build_plans(b.get(), all, b->plan, b->lplan, kS, true);
-        if (rc) return rc;
-    }
-    NWG_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&b->host_scores),
-                               sizeof(double) * (size_t)std::max<int64_t>(N, 1), hipHostMallocDefault));
+    const int rc = load_pairs(b.get(), N, codes_a, off_a, codes_b, off_b, false);
+    if (rc) return rc;
     NWG_HIP(ctx, hipEventCreate(&b->ev0));
     NWG_HIP(ctx, hipEventCreate(&b->ev1));
     *out = b.release();
@@ -568,6 +709,28 @@ int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t
 }
 
 void nwg_batch_free(nwg_batch* b) { delete b; }
+
+int nwg_batch_reload(nwg_batch* b, int64_t N, const uint8_t* codes_a, const int64_t* off_a, const uint8_t* codes_b,
+                     const int64_t* off_b) {
+    if (!b) return NWG_EINVAL;
+    return load_pairs(b, N, codes_a, off_a, codes_b, off_b, true);
+}
+
+int nwg_batch_query(nwg_batch* b, int* done) {
+    if (!b || !done) return NWG_EINVAL;
+    nwg_ctx* ctx = b->ctx;
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    const hipError_t e = hipStreamQuery(ctx->stream);
+    if (e == hipSuccess) {
+        *done = 1;
+        return NWG_OK;
+    }
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return NWG_OK;
+    }
+    return fail(ctx, NWG_EDEVICE, "hipStreamQuery failed: %s", hipGetErrorString(e));
+}
 
 int nwg_batch_set_params(nwg_batch* b, const double* matrix, const double* gaps4) {
     if (!b || !matrix || !gaps4) return NWG_EINVAL;
@@ -702,6 +865,109 @@ int launch_hard(nwg_batch* b, const Plan& pl) {
         }
     }
     return NWG_OK;
+}
+
+// ── long pairs: column panels ───────────────────────────────────────────────
+template <typename T> struct LongBnd;
+template <> struct LongBnd<float> { static DevPtr<float>& get(nwg_batch* b) { return b->lbnd32; } };
+template <> struct LongBnd<double> { static DevPtr<double>& get(nwg_batch* b) { return b->lbnd64; } };
+
+template <typename T>
+int launch_long(nwg_batch* b) {
+    nwg_ctx* ctx = b->ctx;
+    LongPlan& L = b->lpl;
+    const bool hard = b->prob.grad_mode == 1;
+    const int nn = b->na * b->na;
+    const int gm = b->prob.gap_model, am = b->prob.mode;
+    using FillK = void (*)(nwg::LongArgs<T>);
+    FillK fill = nullptr, walk = nullptr;
+    auto pick = [&](auto gmc, auto amc, auto hardc) {
+        constexpr int G = decltype(gmc)::value, AM = decltype(amc)::value;
+        constexpr bool H = decltype(hardc)::value;
+        fill = nwg::long_fill_kernel<T, G, AM, kS, H, kRowBlock>;
+        walk = nwg::long_walk_kernel<T, G, AM, kS, H>;
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using BT = std::true_type;
+    using BF = std::false_type;
+#define NWG_PICK(GMV, AMV) \
+    if (gm == GMV::value && am == AMV::value) { if (hard) pick(GMV{}, AMV{}, BT{}); else pick(GMV{}, AMV{}, BF{}); }
+    NWG_PICK(I0, I0) NWG_PICK(I0, I1) NWG_PICK(I0, I2) NWG_PICK(I1, I0) NWG_PICK(I1, I1) NWG_PICK(I1, I2)
+#undef NWG_PICK
+    if (!fill) return fail(ctx, NWG_EINVAL, "nwgrad_hip: unsupported problem type");
+    const size_t lds_fill = (size_t)((nn * (int)sizeof(T) + 15) & ~15);
+    const size_t lds_walk =
+        hard ? (size_t)((nn * 4 + 15) & ~15) + (size_t)nwg::kWalkSteps * (kS / 4) * kMaxG * 4 : 16;
+    DevPtr<T>& bnd = LongBnd<T>::get(b);
+    const bool grow = b->lprog.n < (size_t)L.max_items || b->lcand.n < (size_t)L.max_items * sizeof(nwg::CandRec<T>) ||
+                      bnd.n < (size_t)std::max<int64_t>(L.max_bnd, 1) ||
+                      (hard && b->ltable.n < (size_t)std::max<int64_t>(L.max_table, 1)) || b->lerr.n < 1;
+    if (grow) {
+        NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        NWG_HIP(ctx, b->lprog.reserve((size_t)L.max_items));
+        NWG_HIP(ctx, b->lcand.reserve((size_t)L.max_items * sizeof(nwg::CandRec<T>)));
+        NWG_HIP(ctx, bnd.reserve((size_t)std::max<int64_t>(L.max_bnd, 1)));
+        if (hard) NWG_HIP(ctx, b->ltable.reserve((size_t)std::max<int64_t>(L.max_table, 1)));
+        if (b->lerr.n < 1) {
+            NWG_HIP(ctx, b->lerr.alloc(1));
+            NWG_HIP(ctx, hipMemset(b->lerr.p, 0, sizeof(int32_t)));
+        }
+    }
+    nwg::LongArgs<T> A{};
+    A.codes_a = b->ca.p;
+    A.codes_b = b->cb.p;
+    A.off_a = b->oa.p;
+    A.off_b = b->ob.p;
+    A.sub = std::is_same<T, float>::value ? reinterpret_cast<const T*>(b->sub_f32.p)
+                                           : reinterpret_cast<const T*>(b->sub_f64.p);
+    A.na = b->na;
+    A.go_a = static_cast<T>(b->gaps[0]);
+    A.ge_a = static_cast<T>(b->gaps[1]);
+    A.go_b = static_cast<T>(b->gaps[2]);
+    A.ge_b = static_cast<T>(b->gaps[3]);
+    A.progress = b->lprog.p;
+    A.bnd = bnd.p;
+    A.cand = reinterpret_cast<nwg::CandRec<T>*>(b->lcand.p);
+    A.table = hard ? b->ltable.p : nullptr;
+    A.scores = b->scores.p;
+    A.grads = b->grads.p;
+    A.err = b->lerr.p;
+    A.wait_ticks = kWaitTicks;
+    A.queue = b->queue.p + 1;
+    const int64_t resident = resident_blocks(ctx, fill, lds_fill, kBlocksPerCU);
+    for (const LongGroup& gr : L.groups) {
+        const int items = gr.item_end - gr.item_begin, pairs = gr.pair_end - gr.pair_begin;
+        A.items = L.d_items.p + gr.item_begin;
+        A.n_items = items;
+        A.lpairs = L.d_lpairs.p + gr.pair_begin;
+        NWG_HIP(ctx, hipMemsetAsync(b->lprog.p, 0, sizeof(int32_t) * (size_t)items, ctx->stream));
+        NWG_HIP(ctx, hipMemsetAsync(A.queue, 0, sizeof(int32_t), ctx->stream));
+        const int64_t grid = std::max<int64_t>(std::min<int64_t>(items, resident), 1);
+        hipLaunchKernelGGL(fill, dim3((unsigned)grid), dim3(64), lds_fill, ctx->stream, A);
+        NWG_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(walk, dim3((unsigned)pairs), dim3(nwg::kWalkThreads), lds_walk, ctx->stream, A);
+        NWG_HIP(ctx, hipGetLastError());
+    }
+    return NWG_OK;
+}
+
+// A failed long-pair fill leaves bits in the error word (1: a wait passed its bound, 2: a walk
+// found no valid end cell): report and clear them.
+int check_long_err(nwg_batch* b) {
+    if (b->lpl.items.empty() || b->lerr.n < 1) return NWG_OK;
+    nwg_ctx* ctx = b->ctx;
+    int32_t e = 0;
+    NWG_HIP(ctx, hipMemcpyAsync(&e, b->lerr.p, sizeof e, hipMemcpyDeviceToHost, ctx->stream));
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (e == 0) return NWG_OK;
+    NWG_HIP(ctx, hipMemsetAsync(b->lerr.p, 0, sizeof(int32_t), ctx->stream));
+    b->results_valid = false;
+    return fail(ctx, NWG_EDEVICE,
+                "nwgrad_hip: the long-pair fill failed (error word %d: 1 = a column panel waited past its bound for "
+                "its left neighbour, 2 = a pair had no valid end cell); the results of this call are not valid",
+                (int)e);
 }
 
 // ── soft pass ──────────────────────────────────────────────────────────────
@@ -911,9 +1177,12 @@ int run_async(nwg_batch* b) {
         rc = launch_soft_batch(b);
         if (rc) return rc;
     } else {
-        for (Plan* pl : {&b->plan, &b->lplan}) {
-            if (pl->tiles.empty()) continue;
-            rc = b->prob.precision == 0 ? launch_hard<float>(b, *pl) : launch_hard<double>(b, *pl);
+        if (!b->plan.tiles.empty()) {
+            rc = b->prob.precision == 0 ? launch_hard<float>(b, b->plan) : launch_hard<double>(b, b->plan);
+            if (rc) return rc;
+        }
+        if (!b->lpl.items.empty()) {
+            rc = b->prob.precision == 0 ? launch_long<float>(b) : launch_long<double>(b);
             if (rc) return rc;
         }
     }
@@ -964,6 +1233,8 @@ int nwg_batch_run(nwg_batch* b, double* total_out) {
     NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, b->ev0, b->ev1) == hipSuccess) b->last_ms = ms;
+    rc = check_long_err(b);
+    if (rc) return rc;
     if (b->prob.grad_mode == 2) {
         rc = soft_finish(b);
         if (rc) return rc;
@@ -1076,7 +1347,7 @@ int nwg_batch_plan_info(nwg_batch* b, int64_t* n_tiles, int64_t* n_cells) {
             }
             *n_tiles = (int64_t)(mp.tiles.size() + wp.tiles.size());
         } else {
-            *n_tiles = (int64_t)(b->plan.tiles.size() + b->lplan.tiles.size());
+            *n_tiles = (int64_t)(b->plan.tiles.size() + b->lpl.items.size());
         }
     }
     if (n_cells) *n_cells = b->cells;

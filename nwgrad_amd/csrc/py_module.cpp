@@ -10,11 +10,14 @@
 #include <vector>
 
 #include "../cpp/nwgrad_amd/engine.hpp"
+#include "../cpp/nwgrad_amd/serving.hpp"
 
 namespace py = pybind11;
 using namespace nwgrad_amd;
 
 using F64Arr = py::array_t<double, py::array::c_style | py::array::forcecast>;
+using U8Arr = py::array_t<uint8_t, py::array::c_style | py::array::forcecast>;
+using I64Arr = py::array_t<int64_t, py::array::c_style | py::array::forcecast>;
 
 namespace {
 
@@ -46,6 +49,76 @@ void check_traceback(const std::string& s) {
                               "\"pointers\" and \"scores\" (identical results) are");
     throw py::value_error("nwgrad: unknown traceback \"" + s + "\"");
 }
+JobKind parse_kind(const std::string& s) {
+    if (s == "score" || s == "none") return JobKind::Score;
+    if (s == "hard") return JobKind::Hard;
+    if (s == "soft") return JobKind::Soft;
+    throw py::value_error("nwgrad_amd: unknown job kind \"" + s + "\" (expected \"score\", \"hard\" or \"soft\")");
+}
+
+void bind_server(py::module_& m) {
+    py::class_<AlignServer>(m, "AlignServer")
+        .def(py::init([](const AlignParams& p, const std::string& gap_model, const std::string& mode, bool dbl,
+                         double hbm_budget_bytes, int slots_per_kind, int64_t max_cells, int64_t max_pairs,
+                         double soft_temperature, int device) {
+                 ServeOptions o;
+                 o.hbm_budget_bytes = hbm_budget_bytes;
+                 o.slots_per_kind = slots_per_kind;
+                 o.max_cells_per_launch = max_cells;
+                 o.max_pairs_per_launch = max_pairs;
+                 o.soft_temperature = soft_temperature;
+                 return std::make_unique<AlignServer>(p, parse_gap_model(gap_model), parse_mode(mode), dbl, o, device);
+             }),
+             py::arg("params"), py::kw_only(), py::arg("gap_model") = "affine", py::arg("mode") = "global",
+             py::arg("double") = false, py::arg("hbm_budget_bytes") = 200e9, py::arg("slots_per_kind") = 2,
+             py::arg("max_cells_per_launch") = int64_t(4000000000), py::arg("max_pairs_per_launch") = int64_t(1) << 20,
+             py::arg("soft_temperature") = 1.0, py::arg("device") = 0)
+        .def("submit",
+             [](AlignServer& s, const std::string& kind, U8Arr ca, I64Arr oa, U8Arr cb, I64Arr ob) {
+                 if (oa.ndim() != 1 || ob.ndim() != 1 || oa.size() < 1 || oa.size() != ob.size())
+                     throw py::value_error("nwgrad_amd: off_a and off_b need n + 1 entries each");
+                 const int64_t n = static_cast<int64_t>(oa.size()) - 1;
+                 if (oa.data()[n] > static_cast<int64_t>(ca.size()) || ob.data()[n] > static_cast<int64_t>(cb.size()))
+                     throw py::value_error("nwgrad_amd: offsets run past the codes");
+                 return s.submit(parse_kind(kind), ca.data(), oa.data(), cb.data(), ob.data(), n);
+             },
+             py::arg("kind"), py::arg("codes_a"), py::arg("off_a"), py::arg("codes_b"), py::arg("off_b"))
+        .def("poll", &AlignServer::poll)
+        .def("now", &AlignServer::now)
+        .def("state", [](const AlignServer& s, int64_t id) {
+            static const char* names[] = {"waiting", "running", "done", "failed"};
+            return std::string(names[s.job(id).state]);
+        })
+        .def("result", [](const AlignServer& s, int64_t id) {
+            const AlignServer::Job& j = s.job(id);
+            py::dict d;
+            d["state"] = j.state;
+            d["n"] = j.n;
+            d["cells"] = j.cells;
+            d["t_submit"] = j.t_submit;
+            d["t_launch"] = j.t_launch;
+            d["t_done"] = j.t_done;
+            d["error"] = j.error;
+            py::array_t<double> sc(static_cast<py::ssize_t>(j.scores.size()));
+            if (!j.scores.empty()) std::memcpy(sc.mutable_data(), j.scores.data(), sizeof(double) * j.scores.size());
+            d["scores"] = sc;
+            if (!j.grads.empty()) {
+                const py::ssize_t gs = s.n_alpha() * s.n_alpha() + 4;
+                py::array_t<double> g({static_cast<py::ssize_t>(j.n), gs});
+                std::memcpy(g.mutable_data(), j.grads.data(), sizeof(double) * j.grads.size());
+                d["grads"] = g;
+            } else {
+                d["grads"] = py::none();
+            }
+            return d;
+        })
+        .def("release", &AlignServer::release)
+        .def_property_readonly("waiting", &AlignServer::waiting)
+        .def_property_readonly("running_launches", &AlignServer::running_launches)
+        .def_property_readonly("launches", &AlignServer::launches)
+        .def_property_readonly("bytes_in_flight", &AlignServer::bytes_in_flight);
+}
+
 const char* gm_name(GapModel g) { return g == GapModel::Linear ? "linear" : "affine"; }
 const char* am_name(AlignMode a) {
     return a == AlignMode::Local ? "local" : a == AlignMode::Glocal ? "glocal" : "global";
@@ -342,6 +415,7 @@ PYBIND11_MODULE(_nwgrad_amd, m) {
 
     bind_batch<float>(m, "SeqPairBatch");
     bind_batch<double>(m, "SeqPairBatchDouble");
+    bind_server(m);
 
     m.def("device_available", []() {
         try {

@@ -942,15 +942,13 @@ void soft_kernel(SoftArgs<SF> A) {
             break;
         } else {
             if (phase == 0) {
-                if (lane == 0) tile = atomicAdd(A.lqueue, 1);
-                tile = __shfl(tile, 0, kWave);
+                tile = wave_uniform(atomicAdd(A.lqueue, lane == 0 ? 1 : 0));
                 if (tile < A.n_ltiles) wide = true;
                 else phase = 1;
             }
             if (phase == 1) {
                 if (blk >= A.n_mslots) break;
-                if (lane == 0) tile = atomicAdd(A.queue, 1);
-                tile = __shfl(tile, 0, kWave);
+                tile = wave_uniform(atomicAdd(A.queue, lane == 0 ? 1 : 0));
                 if (tile >= A.n_tiles) break;
             }
         }

@@ -228,3 +228,80 @@ def random_pairs(name: str, n_pairs: int, alphabet: str, len_a, len_b, *, matrix
     cb, ob = _pack(B)
     return Workload(name, alphabet, np.asarray(matrix, np.float64), np.asarray(gaps, np.float64), gap_model, mode,
                     grad_mode, ca, oa, cb, ob)
+
+
+@dataclasses.dataclass
+class Request:
+    """One alignment job of the serving workload: pairs of one kind, arriving at time t (s)."""
+    t: float
+    kind: str                   # "score", "hard" or "soft"
+    codes_a: np.ndarray
+    off_a: np.ndarray
+    codes_b: np.ndarray
+    off_b: np.ndarray
+
+    @property
+    def n_pairs(self) -> int:
+        return len(self.off_a) - 1
+
+    @property
+    def cells(self) -> int:
+        return int(np.sum(np.diff(self.off_a) * np.diff(self.off_b)))
+
+
+def _log_uniform_int(r: np.random.Generator, lo: int, hi: int, size) -> np.ndarray:
+    """Integers log-uniform on [lo, hi]: floor(exp(U(ln lo, ln(hi + 1))))."""
+    x = np.floor(np.exp(r.uniform(np.log(lo), np.log(hi + 1), size=size))).astype(np.int64)
+    return np.clip(x, lo, hi)
+
+
+class ServingLoad:
+    """The serving configuration: Poisson arrivals at ``rate`` requests/s for ``duration`` s;
+    pairs per request log-uniform on [1, max_pairs] (mean about 492 at 4,096); len(A) and len(B)
+    log-uniform on [lo, hi], independently (mean about 972 at [30, 5000]); kinds score / hard /
+    soft in the proportions ``mix``.  Residues are i.i.d. uniform protein letters: a request's
+    A (and B) codes are one contiguous slice of a seeded pool, cut into its pairs, so making a
+    request costs O(pairs); a soft request's B is its A with 30 % point substitutions (a
+    homolog: the scaled soft pass needs the dynamic range of related sequences, as the
+    reference's does).  Request k is a deterministic function of (name, k)."""
+
+    def __init__(self, rate: float = 200.0, duration: float = 60.0, max_pairs: int = 4096, lo: int = 30,
+                 hi: int = 5000, mix=(0.7, 0.2, 0.1), name: str = "serving"):
+        self.name, self.max_pairs, self.lo, self.hi = name, max_pairs, lo, hi
+        r = rng(name, "arrivals")
+        t, times = 0.0, []
+        while True:
+            t += r.exponential(1.0 / rate)
+            if t >= duration:
+                break
+            times.append(t)
+        self.times = np.array(times)
+        self.kinds = r.choice(np.array(["score", "hard", "soft"]), size=len(times), p=np.asarray(mix, float))
+        self.pool = rng(name, "pool").integers(0, 20, size=(1 << 25) + max_pairs * hi).astype(np.uint8)
+        self.matrix = random_matrix(rng(name, "matrix"))
+        self.gaps = np.array([11.0, 1.0, 11.0, 1.0])
+
+    def __len__(self) -> int:
+        return len(self.times)
+
+    def _slices(self, r, lens):
+        off = np.zeros(len(lens) + 1, np.int64)
+        np.cumsum(lens, out=off[1:])
+        s0 = int(r.integers(0, len(self.pool) - off[-1]))
+        return self.pool[s0:s0 + off[-1]], off
+
+    def request(self, k: int) -> Request:
+        r = rng(self.name, f"request{k}")
+        n = int(_log_uniform_int(r, 1, self.max_pairs, 1)[0])
+        la = _log_uniform_int(r, self.lo, self.hi, n)
+        kind = str(self.kinds[k])
+        ca, oa = self._slices(r, la)
+        if kind == "soft":
+            cb = ca.copy()
+            subs = r.random(len(cb)) < 0.3
+            cb[subs] = (cb[subs] + r.integers(1, 20, size=int(subs.sum()))) % 20
+            ob = oa.copy()
+        else:
+            lb = _log_uniform_int(r, self.lo, self.hi, n)
+            cb, ob = self._slices(r, lb)
+        return Request(float(self.times[k]), kind, ca, oa, cb.astype(np.uint8), ob)

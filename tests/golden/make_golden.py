@@ -1,7 +1,8 @@
 """Writes the golden fixtures of tests/golden/ from the reference engine built from its own
 sources (oracle/_ref/libnwgref.so; see oracle/Makefile.ref).  Run in the build container:
 
-    make -f oracle/Makefile.ref && python tests/golden/make_golden.py
+    make -f oracle/Makefile.ref && python tests/golden/make_golden.py        # all sets
+    python tests/golden/make_golden.py long                                  # the long-pair sets only
 
 Each fixture is an .npz (no pickles) holding the inputs -- alphabet, matrix, gaps, problem
 type, encoded pairs -- and the reference's outputs: per-pair scores and gradients, and for the
@@ -65,7 +66,27 @@ def edge_pairs(w):
     return w
 
 
+def long_sets():
+    """Pairs wider than one wavefront's columns (len(B) 1,025 to 2,600: two or three column
+    panels on the GPU), every problem type, hard and score-only, plus one soft set."""
+    for gm in ("affine", "linear"):
+        for mode in ("global", "local", "glocal"):
+            w = wl.random_pairs(f"golden_long_{gm}_{mode}", 5, wl.AA, (150, 400), (1025, 2600),
+                                gaps=(10.5, 0.75, 9.25, 1.5) if gm == "affine" else (0.0, 1.5, 0.0, 2.0),
+                                gap_model=gm, mode=mode, related=0.0)
+            for prec in ("float32", "float64"):
+                save(f"hard_long_{gm}_{mode}_{prec}", w, prec, "hard", run_ref(w, prec, "hard"))
+            save(f"score_long_{gm}_{mode}_float32", w, "float32", "none", run_ref(w, "float32", "none"))
+    w = wl.random_pairs("golden_long_soft", 4, wl.AA, (1100, 1400), (1100, 1400), gaps=(11.0, 1.0, 11.0, 1.0),
+                        gap_model="affine", mode="global", related=1.0)
+    save("soft_long_affine_global_float32", w, "float32", "soft",
+         run_ref(w, "float32", "soft", soft_precision="float32"))
+
+
 def main():
+    if sys.argv[1:] == ["long"]:
+        long_sets()
+        return
     for gm in ("affine", "linear"):
         for mode in ("global", "local", "glocal"):
             for prec in ("float32", "float64"):

@@ -177,3 +177,19 @@ def test_config_sample_matches_reference_engine():
         np.testing.assert_array_equal(s, rb.scores())
         mats, gaps = rb.grads()
         np.testing.assert_array_equal(g, np.concatenate([mats.reshape(64, -1), gaps], 1))
+
+
+@pytest.mark.skipif(not oracle.have_ref(), reason="oracle/_ref not built (needs the reference sources)")
+def test_long_pairs_match_reference_engine():
+    """Two 4,096-residue homolog pairs (the long-pair workload's shape) against the reference
+    engine at its default traceback: the oracle's full pointer table gives the same path."""
+    from oracle import ref
+    w = wl.long_pairs(2, 4096)
+    sa, sb = w.strings()
+    for prec in ("float32", "float64"):
+        rb = ref.RefBatch(sa, sb, w.alphabet, w.matrix, w.gaps, double=prec == "float64", n_threads=2)
+        rb.score_and_grad()
+        s, g = oracle.hard(w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, precision=prec)
+        np.testing.assert_array_equal(s, rb.scores())
+        mats, gaps = rb.grads()
+        np.testing.assert_array_equal(g, np.concatenate([mats.reshape(2, -1), gaps], 1))
