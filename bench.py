@@ -133,7 +133,9 @@ def cpu_baseline(w, grad_mode, precision="float32", budget_s=8.0):
                             precision=precision)
             return time.perf_counter() - t0
     n = w.n_pairs
-    k = max(1, min(n, 256, int(2e8 * n / max(w.cells, 1))))   # a first sample of about 2e8 cells
+    # a first sample of about 2e8 cells, and at least one pair per core (the reference
+    # parallelises over pairs: fewer pairs than cores would leave cores idle)
+    k = min(n, max(cores if kind == "reference" else 1, min(256, int(2e8 * n / max(w.cells, 1)))))
     idx = np.sort(rng.choice(n, size=k, replace=False))
     sub = w.subset(idx)
     t = run(sub)
