@@ -57,6 +57,11 @@ class _Params(ctypes.Structure):
                 ("go_b", ctypes.c_double), ("ge_b", ctypes.c_double)]
 
 
+class _TParams(ctypes.Structure):
+    _fields_ = [("gap_model", ctypes.c_int), ("mode", ctypes.c_int), ("n_alpha", ctypes.c_int), ("K", ctypes.c_int),
+                ("matrices", ctypes.POINTER(ctypes.c_double)), ("gaps", ctypes.POINTER(ctypes.c_double))]
+
+
 _lib = None
 
 
@@ -77,6 +82,10 @@ def lib():
         L.nwo_weighted_sum.argtypes = [ctypes.c_int64, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double),
                                        P(ctypes.c_double)]
         L.nwo_weighted_sum.restype = None
+        L.nwo_hard_tracked.argtypes = [P(_TParams), ctypes.c_int, ctypes.c_int64, P(ctypes.c_uint8),
+                                       P(ctypes.c_int64), P(ctypes.c_uint8), P(ctypes.c_int64), P(ctypes.c_uint16),
+                                       P(ctypes.c_uint16), ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)]
+        L.nwo_hard_tracked.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -153,3 +162,50 @@ def weighted_sum(grads, weights=None):
 
 def have_ref() -> bool:
     return os.path.exists(REF_PATH)
+
+
+def default_gap_tracks(tracks, off_a):
+    """The reference's default gap track of each pair (setup_tracks_, aligner.hpp:1963-1967):
+    row 0 takes the first residue's set (0 for an empty A), row i >= 1 the set of residue i."""
+    tracks = np.asarray(tracks, np.uint16)
+    oa = np.asarray(off_a, np.int64)
+    N = len(oa) - 1
+    out = np.zeros(int(oa[-1]) + N, np.uint16)
+    for q in range(N):
+        m = int(oa[q + 1] - oa[q])
+        t = tracks[oa[q]:oa[q + 1]]
+        g = out[oa[q] + q: oa[q + 1] + q + 1]
+        g[0] = t[0] if m else 0
+        g[1:] = t
+    return out
+
+
+def hard_tracked(matrices, gaps, codes_a, off_a, codes_b, off_b, tracks, gap_tracks=None, *, gap_model="affine",
+                 mode="global", precision="float32", with_grad=True):
+    """Tracked hard path (or score-only): K parameter sets, matrices (K, na, na) and gaps (K, 4);
+    tracks one set per residue of A, concatenated in pair order (pair q at off_a[q] ..
+    off_a[q+1]); gap_tracks one per DP row 0..m (pair q at off_a[q] + q, m + 1 entries), or
+    None for the default.  Scores (N,) and gradients (N, K*(na*na + 4)): K matrix blocks,
+    then K rows of go_a, ge_a, go_b, ge_b."""
+    mats = np.ascontiguousarray(matrices, np.float64)
+    K, na = mats.shape[0], mats.shape[1]
+    g4 = np.ascontiguousarray(gaps, np.float64).reshape(K, 4)
+    p = _TParams(GAP_MODELS[gap_model], MODES[mode], na, K, _ptr(mats, ctypes.c_double), _ptr(g4, ctypes.c_double))
+    ca, oa, cb, ob = _pairs(codes_a, off_a, codes_b, off_b)
+    N = len(oa) - 1
+    trk = np.ascontiguousarray(tracks, np.uint16)
+    gtrk = default_gap_tracks(trk, oa) if gap_tracks is None else np.ascontiguousarray(gap_tracks, np.uint16)
+    if trk.size == 0:
+        trk = np.zeros(1, np.uint16)
+    scores = np.zeros(N, np.float64)
+    gs = K * (na * na + 4)
+    grads = np.zeros((N, gs), np.float64) if with_grad else np.zeros((1, 1))
+    rc = lib().nwo_hard_tracked(ctypes.byref(p), 1 if precision in ("float64", "double") else 0, N,
+                                _ptr(ca, ctypes.c_uint8), _ptr(oa, ctypes.c_int64), _ptr(cb, ctypes.c_uint8),
+                                _ptr(ob, ctypes.c_int64), _ptr(trk, ctypes.c_uint16), _ptr(gtrk, ctypes.c_uint16),
+                                1 if with_grad else 0, _ptr(scores, ctypes.c_double), _ptr(grads, ctypes.c_double))
+    if rc == -2:
+        raise ValueError("oracle: a track names a set outside [0, K)")
+    if rc:
+        raise MemoryError("oracle: allocation failed")
+    return scores, (grads if with_grad else None)

@@ -527,3 +527,317 @@ void nwo_weighted_sum(int64_t N, int gs, const double *grads, const double *w, d
     }
     free(acc);
 }
+
+/* ── tracks: K parameter sets, one per DP row (aligner.hpp:1800-1826, 1903-2008) ──────
+   Row i (1..m) scores matches with matrix trk[i-1] and charges a gap in B entering it
+   set trk[i-1]'s go_b / ge_b; a gap in A within row i (0..m) charges set gtrk[i]'s go_a /
+   ge_a.  The Global column-0 border of row i is -(go_b(1) + C_i), C_i the sum of ge_b over
+   rows 1..i accumulated by runs of bitwise-equal extends (setup_tracks_, aligner.hpp:
+   1981-2007), so a track whose sets are all equal reproduces the untracked fill bit for bit.
+   Each gradient count lands in the set of its row (hard_grad_affine_impl, aligner.hpp:
+   3199-3230; hard_grad_linear, aligner.hpp:3896-3911).  grads: K blocks of na*na, then K
+   rows of go_a, ge_a, go_b, ge_b. */
+typedef struct {
+    int gap_model, mode, n_alpha, K;
+    const double *matrices; /* K * na * na */
+    const double *gaps;     /* K * 4 */
+} nwo_tparams;
+
+#define NWO_DEFINE_TRACKED(T, SUFFIX, NINF)                                                    \
+    /* per-row penalties of one pair, setup_tracks_ (aligner.hpp:1919-2008) */                 \
+    static void track_rows_##SUFFIX(const nwo_tparams *p, const uint16_t *trk, const uint16_t *gtrk, \
+                                    int m, T *goa, T *gea, T *gob, T *geb, T *bx0) {           \
+        for (int i = 0; i <= m; ++i) {                                                         \
+            goa[i] = (T)p->gaps[4 * gtrk[i] + 0];                                              \
+            gea[i] = (T)p->gaps[4 * gtrk[i] + 1];                                              \
+        }                                                                                      \
+        gob[0] = geb[0] = bx0[0] = (T)0;                                                       \
+        for (int i = 1; i <= m; ++i) {                                                         \
+            gob[i] = (T)p->gaps[4 * trk[i - 1] + 2];                                           \
+            geb[i] = (T)p->gaps[4 * trk[i - 1] + 3];                                           \
+        }                                                                                      \
+        T csum = (T)0;                                                                         \
+        int s = 1;                                                                             \
+        for (int i = 1; i <= m; ++i) {                                                         \
+            if (i > 1 && memcmp(&geb[i], &geb[i - 1], sizeof(T)) != 0) {                      \
+                csum = (s == 1) ? (T)(i - 1) * geb[i - 1] : csum + (T)(i - s) * geb[i - 1];  \
+                s = i;                                                                         \
+            }                                                                                  \
+            const T ge = geb[i];                                                               \
+            if (p->gap_model == 0) bx0[i] = (s == 1) ? -(T)i * ge : -(csum + (T)(i - s + 1) * ge); \
+            else {                                                                             \
+                const T go = gob[1];                                                           \
+                bx0[i] = (s == 1) ? -(go + (T)i * ge) : -(go + (csum + (T)(i - s + 1) * ge)); \
+            }                                                                                  \
+        }                                                                                      \
+    }                                                                                          \
+    static T affine_fill_tr_##SUFFIX(const nwo_tparams *p, const T *blk, const uint8_t *a,    \
+                                     const uint16_t *trk, const T *goa, const T *gea, const T *gob, \
+                                     const T *geb, const T *bx0, int m, const uint8_t *b, int n, \
+                                     unsigned char *D, T *rows, int *bi, int *bj, int *bst) {  \
+        const int AM = p->mode, na = p->n_alpha;                                               \
+        const size_t st = (size_t)n + 1;                                                       \
+        T *pM = rows, *pX = rows + st, *pY = rows + 2 * st;                                    \
+        T *cM = rows + 3 * st, *cX = rows + 4 * st, *cY = rows + 5 * st;                       \
+        const int free_row0 = (AM == 1 || AM == 2);                                            \
+        for (int j = 0; j <= n; ++j) {                                                         \
+            if (free_row0) { pM[j] = 0; pX[j] = NINF; pY[j] = NINF; }                          \
+            else {                                                                             \
+                pM[j] = (j == 0) ? (T)0 : NINF;                                                \
+                pX[j] = NINF;                                                                  \
+                pY[j] = (j == 0) ? NINF : -(goa[0] + (T)j * gea[0]);                           \
+            }                                                                                  \
+            D[j] = (j <= 1 || free_row0) ? 3 : (3 | 2 << 4);                                   \
+        }                                                                                      \
+        *bi = 0; *bj = 0; *bst = ST_M;                                                         \
+        T best_local = 0;                                                                      \
+        for (int i = 1; i <= m; ++i) {                                                         \
+            const size_t rb = (size_t)i * st;                                                  \
+            const T go_a = goa[i], ge_a = gea[i], go_b = gob[i], ge_b = geb[i];               \
+            const T *srow = blk + ((size_t)trk[i - 1] * na + a[i - 1]) * na;                   \
+            if (AM == 1) { cM[0] = 0; cX[0] = NINF; }                                          \
+            else { cM[0] = NINF; cX[0] = bx0[i]; }                                             \
+            cY[0] = NINF;                                                                      \
+            D[rb] = (i <= 1) ? 3 : (3 | 1 << 2);                                               \
+            T rowmax = 0;                                                                      \
+            for (int j = 1; j <= n; ++j) {                                                     \
+                const T dgM = pM[j - 1], dgX = pX[j - 1], dgY = pY[j - 1];                     \
+                T m_val = max3_##SUFFIX(dgM, dgX, dgY) + srow[b[j - 1]];                       \
+                unsigned char km = pick_##SUFFIX(dgM, dgX, dgY);                               \
+                const T ux = (pM[j] - go_b) - ge_b, vx = pX[j] - ge_b, wx = (pY[j] - go_b) - ge_b; \
+                const T x_val = max3_##SUFFIX(ux, vx, wx);                                     \
+                const unsigned char kx = pick_##SUFFIX(ux, vx, wx);                            \
+                const T uy = (cM[j - 1] - go_a) - ge_a, vy = (cX[j - 1] - go_a) - ge_a,        \
+                        wy = cY[j - 1] - ge_a;                                                 \
+                const T y_val = max3_##SUFFIX(uy, vy, wy);                                     \
+                const unsigned char ky = pick_##SUFFIX(uy, vy, wy);                            \
+                if (AM == 1) {                                                                 \
+                    m_val = max2_##SUFFIX(m_val, (T)0);                                        \
+                    rowmax = max2_##SUFFIX(rowmax, max3_##SUFFIX(m_val, x_val, y_val));        \
+                    if (m_val <= (T)0) km = 3;                                                 \
+                }                                                                              \
+                cM[j] = m_val; cX[j] = x_val; cY[j] = y_val;                                   \
+                D[rb + j] = (unsigned char)(km | kx << 2 | ky << 4);                          \
+            }                                                                                  \
+            if (AM == 1 && rowmax > best_local) {                                              \
+                int j = 1;                                                                     \
+                while (!(max3_##SUFFIX(cM[j], cX[j], cY[j]) == rowmax)) ++j;                   \
+                best_local = rowmax; *bi = i; *bj = j;                                         \
+                if (cM[j] >= cX[j] && cM[j] >= cY[j]) *bst = ST_M;                             \
+                else if (cX[j] >= cY[j]) *bst = ST_X;                                          \
+                else *bst = ST_Y;                                                              \
+            }                                                                                  \
+            T *t;                                                                              \
+            t = pM; pM = cM; cM = t; t = pX; pX = cX; cX = t; t = pY; pY = cY; cY = t;         \
+        }                                                                                      \
+        if (AM == 0) {                                                                         \
+            const T vm = pM[n], vx = pX[n], vy = pY[n];                                        \
+            *bi = m; *bj = n;                                                                  \
+            if (vm >= vx && vm >= vy) *bst = ST_M;                                             \
+            else if (vx >= vy) *bst = ST_X;                                                    \
+            else *bst = ST_Y;                                                                  \
+            return max3_##SUFFIX(vm, vx, vy);                                                  \
+        }                                                                                      \
+        if (AM == 2) {                                                                         \
+            int bj2 = 0;                                                                       \
+            T best = max3_##SUFFIX(pM[0], pX[0], pY[0]);                                       \
+            for (int j = 1; j <= n; ++j) {                                                     \
+                const T v = max3_##SUFFIX(pM[j], pX[j], pY[j]);                                \
+                if (v > best) { best = v; bj2 = j; }                                           \
+            }                                                                                  \
+            *bi = m; *bj = bj2;                                                                \
+            const T vm = pM[bj2], vx = pX[bj2], vy = pY[bj2];                                  \
+            if (vm >= vx && vm >= vy) *bst = ST_M;                                             \
+            else if (vx >= vy) *bst = ST_X;                                                    \
+            else *bst = ST_Y;                                                                  \
+            return best;                                                                       \
+        }                                                                                      \
+        return best_local;                                                                     \
+    }                                                                                          \
+    static T linear_fill_tr_##SUFFIX(const nwo_tparams *p, const T *blk, const uint8_t *a,    \
+                                     const uint16_t *trk, const T *gea, const T *geb, const T *bx0, \
+                                     int m, const uint8_t *b, int n, unsigned char *D, T *rows, \
+                                     int *bi, int *bj) {                                       \
+        const int AM = p->mode, na = p->n_alpha, local = (AM == 1), free0 = (AM == 1 || AM == 2); \
+        const size_t st = (size_t)n + 1;                                                       \
+        T *prev = rows, *cur = rows + st;                                                      \
+        prev[0] = 0; D[0] = 3;                                                                 \
+        for (int j = 1; j <= n; ++j) {                                                         \
+            prev[j] = free0 ? (T)0 : -(T)j * gea[0];                                           \
+            D[j] = free0 ? 3 : 2;                                                              \
+        }                                                                                      \
+        int bbi = 0, bbj = 0;                                                                  \
+        T best_local = 0;                                                                      \
+        for (int i = 1; i <= m; ++i) {                                                         \
+            unsigned char *d = D + (size_t)i * st;                                             \
+            const T *prow = blk + ((size_t)trk[i - 1] * na + a[i - 1]) * na;                   \
+            const T ge_a = gea[i], ge_b = geb[i];                                              \
+            cur[0] = local ? (T)0 : bx0[i];                                                    \
+            d[0] = local ? 3 : 1;                                                              \
+            T left = cur[0];                                                                   \
+            for (int j = 1; j <= n; ++j) {                                                     \
+                const T d0 = prev[j - 1] + prow[b[j - 1]];                                     \
+                const T u = prev[j] - ge_b;                                                    \
+                const T t = max2_##SUFFIX(d0, u);                                              \
+                unsigned char c = (unsigned char)(d0 < u);                                     \
+                const T l = left - ge_a;                                                       \
+                T v = max2_##SUFFIX(t, l);                                                     \
+                if (t < l) c = 2;                                                              \
+                if (local) {                                                                   \
+                    v = max2_##SUFFIX(v, (T)0);                                                \
+                    if (v <= (T)0) c = (unsigned char)(c | 3);                                 \
+                    if (v > best_local) { best_local = v; bbi = i; bbj = j; }                  \
+                }                                                                              \
+                cur[j] = v; d[j] = c;                                                          \
+                left = v;                                                                      \
+            }                                                                                  \
+            T *tt = prev; prev = cur; cur = tt;                                                \
+        }                                                                                      \
+        if (AM == 2) {                                                                         \
+            int bj2 = 0;                                                                       \
+            T best = prev[0];                                                                  \
+            for (int j = 1; j <= n; ++j) if (prev[j] > best) { best = prev[j]; bj2 = j; }      \
+            *bi = m; *bj = bj2;                                                                \
+            return best;                                                                       \
+        }                                                                                      \
+        if (local) { *bi = bbi; *bj = bbj; return best_local; }                               \
+        *bi = m; *bj = n;                                                                      \
+        return prev[n];                                                                        \
+    }
+
+NWO_DEFINE_TRACKED(float, f32, -INFINITY)
+NWO_DEFINE_TRACKED(double, f64, -INFINITY)
+
+/* Counts of a tracked walk: matrix block trk[i-1] for a match; a gap in B (X, into row i)
+   the gap fields of set trk[i-1]; a gap in A (Y, within row i) those of set gtrk[i]. */
+static void emit_tracked(double *g, int na, int K, const uint8_t *a, const uint8_t *b, const uint16_t *trk,
+                         const uint16_t *gtrk, int st, int i, int j, int open) {
+    const int nn = na * na;
+    double *gap = g + (size_t)K * nn;
+    if (st == ST_M) g[(size_t)trk[i - 1] * nn + a[i - 1] * na + b[j - 1]] += 1.0;
+    else if (st == ST_X) { gap[4 * trk[i - 1] + 3] -= 1.0; if (open) gap[4 * trk[i - 1] + 2] -= 1.0; }
+    else { gap[4 * gtrk[i] + 1] -= 1.0; if (open) gap[4 * gtrk[i] + 0] -= 1.0; }
+}
+
+/* walk_affine_impl<Ptr, Tr=true> (aligner.hpp:2986-3056): the untracked walk's moves,
+   each count in its row's set. */
+static void walk_affine_tr(int mode, const unsigned char *D, size_t st, const uint8_t *a, const uint8_t *b,
+                           int na, int K, const uint16_t *trk, const uint16_t *gtrk, int i, int j, int tbl,
+                           double *g) {
+    for (;;) {
+        if (i == 0 && j == 0) break;
+        unsigned char c = 0;
+        if (mode == 1) {
+            c = D[(size_t)i * st + j];
+            if (tbl == ST_M && (c & 3) == 3) break;
+        } else {
+            if (i == 0) {
+                if (mode == 2) break;
+                for (; j > 0; --j) emit_tracked(g, na, K, a, b, trk, gtrk, ST_Y, 0, j, j == 1);
+                break;
+            }
+            if (j == 0) {
+                for (; i > 0; --i) emit_tracked(g, na, K, a, b, trk, gtrk, ST_X, i, 0, i == 1);
+                break;
+            }
+            c = D[(size_t)i * st + j];
+        }
+        if (i == 0) tbl = ST_Y;
+        else if (j == 0) tbl = ST_X;
+        int prev;
+        if (tbl == ST_M) {
+            const int k = c & 3;
+            prev = k == 1 ? ST_X : (k == 2 ? ST_Y : ST_M);
+            emit_tracked(g, na, K, a, b, trk, gtrk, ST_M, i, j, 0);
+            --i; --j;
+        } else if (tbl == ST_X) {
+            const int k = (c >> 2) & 3;
+            prev = k == 1 ? ST_X : (k == 2 ? ST_Y : ST_M);
+            emit_tracked(g, na, K, a, b, trk, gtrk, ST_X, i, j, prev != ST_X);
+            --i;
+        } else {
+            const int k = c >> 4;
+            prev = k == 1 ? ST_X : (k == 2 ? ST_Y : ST_M);
+            emit_tracked(g, na, K, a, b, trk, gtrk, ST_Y, i, j, prev != ST_Y);
+            --j;
+        }
+        tbl = prev;
+    }
+}
+
+/* walk_linear with hard_grad_linear's tracked emit (aligner.hpp:2826-2855, 3896-3911). */
+static void walk_linear_tr(int mode, const unsigned char *D, size_t st, const uint8_t *a, const uint8_t *b,
+                           int na, int K, const uint16_t *trk, const uint16_t *gtrk, int i, int j, double *g) {
+    const int nn = na * na;
+    double *gap = g + (size_t)K * nn;
+    for (;;) {
+        if (i == 0 && j == 0) break;
+        if (mode == 2 && i == 0) break;
+        const int k = D[(size_t)i * st + j];
+        if (k == 3) break;
+        if (k == 0) { g[(size_t)trk[i - 1] * nn + a[i - 1] * na + b[j - 1]] += 1.0; --i; --j; }
+        else if (k == 1) { gap[4 * trk[i - 1] + 3] -= 1.0; --i; }
+        else { gap[4 * gtrk[i] + 1] -= 1.0; --j; }
+    }
+}
+
+/* Tracked batch entry.  Pair q's track is trk[oa[q] .. oa[q+1]) (one set per residue of
+   A) and its gap track gtrk[oa[q] + q .. oa[q+1] + q] (one per DP row 0..m).  grads: N
+   rows of K*(na*na + 4) doubles.  Returns 0, -1 on allocation failure, -2 on a set index
+   outside [0, K). */
+int nwo_hard_tracked(const nwo_tparams *p, int precision, int64_t N, const uint8_t *ca, const int64_t *oa,
+                     const uint8_t *cb, const int64_t *ob, const uint16_t *trk, const uint16_t *gtrk,
+                     int with_grad, double *scores, double *grads) {
+    const int na = p->n_alpha, K = p->K, nn = na * na;
+    const size_t gs = (size_t)K * (nn + 4);
+    size_t maxcells = 1, maxrow = 1, maxm = 1;
+    for (int64_t q = 0; q < N; ++q) {
+        const size_t m = (size_t)(oa[q + 1] - oa[q]), n = (size_t)(ob[q + 1] - ob[q]);
+        if ((m + 1) * (n + 1) > maxcells) maxcells = (m + 1) * (n + 1);
+        if (n + 1 > maxrow) maxrow = n + 1;
+        if (m + 1 > maxm) maxm = m + 1;
+        for (size_t i = 0; i < m; ++i) if (trk[oa[q] + i] >= K) return -2;
+        for (size_t i = 0; i <= m; ++i) if (gtrk[oa[q] + q + i] >= K) return -2;
+    }
+    unsigned char *D = (unsigned char *)malloc(maxcells);
+    double *rows = (double *)malloc(6 * maxrow * sizeof(double));
+    double *rowp = (double *)malloc(5 * maxm * sizeof(double));
+    float *blk32 = (float *)malloc((size_t)K * nn * sizeof(float));
+    double *blk64 = (double *)malloc((size_t)K * nn * sizeof(double));
+    if (!D || !rows || !rowp || !blk32 || !blk64) {
+        free(D); free(rows); free(rowp); free(blk32); free(blk64);
+        return -1;
+    }
+    for (size_t k = 0; k < (size_t)K * nn; ++k) { blk32[k] = (float)p->matrices[k]; blk64[k] = p->matrices[k]; }
+    for (int64_t q = 0; q < N; ++q) {
+        const uint8_t *a = ca + oa[q], *b = cb + ob[q];
+        const uint16_t *tq = trk + oa[q], *gq = gtrk + oa[q] + q;
+        const int m = (int)(oa[q + 1] - oa[q]), n = (int)(ob[q + 1] - ob[q]);
+        int bi, bj, bst = ST_M;
+        double sc;
+        if (precision) {
+            double *goa = rowp, *gea = rowp + maxm, *gob = rowp + 2 * maxm, *geb = rowp + 3 * maxm, *bx0 = rowp + 4 * maxm;
+            track_rows_f64(p, tq, gq, m, goa, gea, gob, geb, bx0);
+            sc = p->gap_model == 1
+                     ? affine_fill_tr_f64(p, blk64, a, tq, goa, gea, gob, geb, bx0, m, b, n, D, rows, &bi, &bj, &bst)
+                     : linear_fill_tr_f64(p, blk64, a, tq, gea, geb, bx0, m, b, n, D, rows, &bi, &bj);
+        } else {
+            float *r32 = (float *)rowp;
+            float *goa = r32, *gea = r32 + maxm, *gob = r32 + 2 * maxm, *geb = r32 + 3 * maxm, *bx0 = r32 + 4 * maxm;
+            track_rows_f32(p, tq, gq, m, goa, gea, gob, geb, bx0);
+            sc = p->gap_model == 1
+                     ? (double)affine_fill_tr_f32(p, blk32, a, tq, goa, gea, gob, geb, bx0, m, b, n, D, (float *)rows,
+                                                  &bi, &bj, &bst)
+                     : (double)linear_fill_tr_f32(p, blk32, a, tq, gea, geb, bx0, m, b, n, D, (float *)rows, &bi, &bj);
+        }
+        scores[q] = sc;
+        if (with_grad) {
+            double *g = grads + (size_t)q * gs;
+            memset(g, 0, gs * sizeof(double));
+            if (p->gap_model == 1) walk_affine_tr(p->mode, D, (size_t)n + 1, a, b, na, K, tq, gq, bi, bj, bst, g);
+            else walk_linear_tr(p->mode, D, (size_t)n + 1, a, b, na, K, tq, gq, bi, bj, g);
+        }
+    }
+    free(D); free(rows); free(rowp); free(blk32); free(blk64);
+    return 0;
+}

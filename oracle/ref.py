@@ -31,6 +31,11 @@ def lib():
                                     ctypes.c_long, ctypes.c_char_p, P(ctypes.c_long), ctypes.c_char_p,
                                     P(ctypes.c_long), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, P(vp)]
+        L.nwgref_create_tracked.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, P(ctypes.c_double),
+                                            P(ctypes.c_double), ctypes.c_long, ctypes.c_char_p, P(ctypes.c_long),
+                                            ctypes.c_char_p, P(ctypes.c_long), P(ctypes.c_uint16),
+                                            P(ctypes.c_uint16), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, P(vp)]
         L.nwgref_destroy.argtypes = [vp]
         L.nwgref_set_fill.argtypes = [vp, ctypes.c_int]
         L.nwgref_set_soft.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
@@ -58,11 +63,21 @@ def _d(a):
 
 class RefBatch:
     def __init__(self, seqs_a, seqs_b, alphabet, matrix, gaps, *, double=False, gap_model="affine",
-                 mode="global", grad_mode="hard", n_threads=0, traceback="auto"):
+                 mode="global", grad_mode="hard", n_threads=0, traceback="auto", tracks=None, gap_tracks=None):
+        """matrix (n, n) and gaps (4,), or K sets: matrix (K, n, n) and gaps (K, 4), with
+        `tracks` (one set per residue of A, concatenated in pair order) and `gap_tracks` (one
+        per DP row 0..m of each pair, concatenated; None: the reference's default)."""
         L = lib()
         self.alphabet = alphabet
         self.n_alpha = len(alphabet)
         self.N = len(seqs_a)
+        mat = np.ascontiguousarray(matrix, np.float64)
+        self.K = mat.shape[0] if mat.ndim == 3 else 1
+        if mat.ndim == 3 or tracks is not None:
+            self._create_tracked(seqs_a, seqs_b, mat.reshape(self.K, self.n_alpha, self.n_alpha),
+                                 np.ascontiguousarray(gaps, np.float64).reshape(self.K, 4), double, gap_model, mode,
+                                 grad_mode, n_threads, tracks, gap_tracks)
+            return
         sa = "".join(seqs_a).encode()
         sb = "".join(seqs_b).encode()
         oa = np.zeros(self.N + 1, np.int64)
@@ -77,6 +92,28 @@ class RefBatch:
                                oa.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), sb,
                                ob.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), _GM[gap_model], _MODE[mode],
                                _GRAD[grad_mode], n_threads, _TB[traceback], ctypes.byref(h)))
+        self._h = h
+
+    def _create_tracked(self, seqs_a, seqs_b, mats, gaps, double, gap_model, mode, grad_mode, n_threads, tracks,
+                        gap_tracks):
+        sa = "".join(seqs_a).encode()
+        sb = "".join(seqs_b).encode()
+        oa = np.zeros(self.N + 1, np.int64)
+        ob = np.zeros(self.N + 1, np.int64)
+        oa[1:] = np.cumsum([len(s) for s in seqs_a])
+        ob[1:] = np.cumsum([len(s) for s in seqs_b])
+        tk = None if tracks is None else np.ascontiguousarray(tracks, np.uint16)
+        gk = None if gap_tracks is None else np.ascontiguousarray(gap_tracks, np.uint16)
+        if tk is not None and tk.size == 0:
+            tk = np.zeros(1, np.uint16)
+        self._keep = (sa, sb, oa, ob, mats, gaps, tk, gk)
+        P16 = ctypes.POINTER(ctypes.c_uint16)
+        h = ctypes.c_void_p()
+        _check(lib().nwgref_create_tracked(
+            1 if double else 0, self.alphabet.encode(), self.K, _d(mats), _d(gaps), self.N, sa,
+            oa.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), sb, ob.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+            None if tk is None else tk.ctypes.data_as(P16), None if gk is None else gk.ctypes.data_as(P16),
+            _GM[gap_model], _MODE[mode], _GRAD[grad_mode], n_threads, ctypes.byref(h)))
         self._h = h
 
     def __del__(self):
@@ -107,19 +144,21 @@ class RefBatch:
         return out
 
     def grads(self):
-        na = self.n_alpha
-        mats = np.zeros((self.N, na, na), np.float64)
-        gaps = np.zeros((self.N, 4), np.float64)
+        """(N, n, n) and (N, 4); K sets: (N, K, n, n) and (N, K, 4)."""
+        na, K = self.n_alpha, getattr(self, "K", 1)
+        mats = np.zeros((self.N, K, na, na) if K > 1 else (self.N, na, na), np.float64)
+        gaps = np.zeros((self.N, K, 4) if K > 1 else (self.N, 4), np.float64)
         _check(lib().nwgref_grads(self._h, _d(mats), _d(gaps)))
         return mats, gaps
 
     def weighted_grad(self, w):
+        """K*(n*n + 4) doubles: the K matrices, then K rows of go_a, ge_a, go_b, ge_b."""
         w = np.ascontiguousarray(w, np.float64)
-        out = np.zeros(self.n_alpha ** 2 + 4, np.float64)
+        out = np.zeros(getattr(self, "K", 1) * (self.n_alpha ** 2 + 4), np.float64)
         _check(lib().nwgref_weighted_grad(self._h, _d(w), len(w), _d(out)))
         return out
 
     def compute_grad(self):
-        out = np.zeros(self.n_alpha ** 2 + 4, np.float64)
+        out = np.zeros(getattr(self, "K", 1) * (self.n_alpha ** 2 + 4), np.float64)
         _check(lib().nwgref_compute_grad(self._h, _d(out)))
         return out

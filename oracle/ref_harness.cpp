@@ -109,6 +109,46 @@ int nwgref_create(int dbl, const char* alphabet, const double* matrix, const dou
     });
 }
 
+// The same with K parameter sets (matrices K*n*n, gaps K*4) and tracks: `tracks` one set per
+// residue of A, pair i at off_a[i] .. off_a[i+1]; `gap_tracks` one per DP row, pair i at
+// off_a[i] + i, m + 1 entries (NULL: the reference's default).  The reference's flat
+// PairTracks form (batch_result.hpp:23-34).
+int nwgref_create_tracked(int dbl, const char* alphabet, int K, const double* matrices, const double* gaps,
+                          long n_pairs, const char* seq_a, const long* off_a, const char* seq_b, const long* off_b,
+                          const uint16_t* tracks, const uint16_t* gap_tracks, int gap_model, int mode,
+                          int grad_mode, int n_threads, void** out) {
+    return guarded([&] {
+        auto h = std::make_unique<Handle>();
+        h->dbl = dbl;
+        h->grad_mode = grad_mode;
+        h->chars_a.assign(seq_a, static_cast<size_t>(off_a[n_pairs]));
+        h->chars_b.assign(seq_b, static_cast<size_t>(off_b[n_pairs]));
+        std::vector<std::string_view> va, vb;
+        for (long i = 0; i < n_pairs; ++i) {
+            va.emplace_back(h->chars_a.data() + off_a[i], static_cast<size_t>(off_a[i + 1] - off_a[i]));
+            vb.emplace_back(h->chars_b.data() + off_b[i], static_cast<size_t>(off_b[i + 1] - off_b[i]));
+        }
+        std::vector<size_t> ot(static_cast<size_t>(n_pairs) + 1), og(static_cast<size_t>(n_pairs) + 1);
+        for (long i = 0; i <= n_pairs; ++i) {
+            ot[static_cast<size_t>(i)] = static_cast<size_t>(off_a[i]);
+            og[static_cast<size_t>(i)] = static_cast<size_t>(off_a[i] + i);
+        }
+        const AlignParams p(SubstMatrix(matrices, Alphabet::get(alphabet), K), gaps);
+        const GapModel gm = gap_model ? GapModel::Affine : GapModel::Linear;
+        const AlignMode am = mode == 1 ? AlignMode::Local : mode == 2 ? AlignMode::Glocal : AlignMode::Global;
+        const GradMode gd = grad_mode == 0 ? GradMode::None : grad_mode == 2 ? GradMode::Soft : GradMode::Hard;
+        const PairTracks tk = tracks ? PairTracks(tracks, ot.data(), static_cast<size_t>(n_pairs)) : PairTracks();
+        const PairTracks gk = gap_tracks ? PairTracks(gap_tracks, og.data(), static_cast<size_t>(n_pairs)) : PairTracks();
+        if (dbl)
+            h->d = std::make_unique<SeqPairBatchT<double>>(va, vb, p, gm, am, gd, n_threads, TracebackMode::Default,
+                                                           kBackendAuto, tk, gk);
+        else
+            h->f = std::make_unique<SeqPairBatchT<float>>(va, vb, p, gm, am, gd, n_threads, TracebackMode::Default,
+                                                          kBackendAuto, tk, gk);
+        *out = h.release();
+    });
+}
+
 void nwgref_destroy(void* hp) { delete static_cast<Handle*>(hp); }
 
 // fill 0 interpair / 1 striped / 2 rowwise
@@ -155,22 +195,22 @@ int nwgref_scores(void* hp, double* out) {
     });
 }
 
-// mats: N*n*n, gaps: N*4
+// mats: N*K*n*n, gaps: N*K*4
 int nwgref_grads(void* hp, double* mats, double* gaps) {
     Handle* h = static_cast<Handle*>(hp);
     return guarded([&] { with_batch<void>(h, [&](auto& b) { b.grads_into(mats, gaps); }); });
 }
 
+// K sets: the K matrices row-major, then K rows of go_a, ge_a, go_b, ge_b (K = 1: n*n + 4).
 static void params_out(const AlignParams& g, double* out) {
     const size_t nn = static_cast<size_t>(g.matrix.size()) * static_cast<size_t>(g.matrix.size());
-    std::memcpy(out, g.matrix.data(), nn * sizeof(double));
-    out[nn] = g.gap_open_a;
-    out[nn + 1] = g.gap_extend_a;
-    out[nn + 2] = g.gap_open_b;
-    out[nn + 3] = g.gap_extend_b;
+    const size_t K = static_cast<size_t>(g.num_sets());
+    std::memcpy(out, g.matrix.data(), K * nn * sizeof(double));
+    const std::vector<double> gaps = g.all_gaps();
+    std::memcpy(out + K * nn, gaps.data(), gaps.size() * sizeof(double));
 }
 
-// out: n*n + 4 doubles (matrix row-major, then go_a, ge_a, go_b, ge_b)
+// out: K*(n*n + 4) doubles (see params_out)
 int nwgref_weighted_grad(void* hp, const double* w, long n, double* out) {
     Handle* h = static_cast<Handle*>(hp);
     return guarded([&] {

@@ -3,6 +3,7 @@ sources (oracle/_ref/libnwgref.so; see oracle/Makefile.ref).  Run in the build c
 
     make -f oracle/Makefile.ref && python tests/golden/make_golden.py        # all sets
     python tests/golden/make_golden.py long                                  # the long-pair sets only
+    python tests/golden/make_golden.py tracks                                # the tracked sets only
 
 Each fixture is an .npz (no pickles) holding the inputs -- alphabet, matrix, gaps, problem
 type, encoded pairs -- and the reference's outputs: per-pair scores and gradients, and for the
@@ -83,9 +84,86 @@ def long_sets():
          run_ref(w, "float32", "soft", soft_precision="float32"))
 
 
+def track_sets():
+    """Tracked batches: K = 3 parameter sets, one per residue of A (segment tracks: runs of
+    one set), explicit or default gap tracks, every problem type, hard and score-only.  Sets
+    0 and 2 share gap_extend_b, so the column-0 border's runs of equal extends cross sets."""
+    K = 3
+    for alpha, name, lens in ((wl.AA, "prot", ((0, 90), (0, 140))), ("ACGT", "dna", ((0, 30), (0, 60)))):
+        n = len(alpha)
+        r = wl.rng(f"golden_tracks_{name}", "params")
+        if n > 4:
+            mats = np.stack([wl.random_matrix(r, n, -3, 2, 2, 6) for _ in range(K)])
+        else:
+            mats = np.stack([np.where(np.eye(4) > 0, 2.0, -1.0), np.where(np.eye(4) > 0, 1.0, -2.0),
+                             np.where(np.eye(4) > 0, 3.0, -1.0)])
+        gaps_aff = np.array([[10.5, 0.75, 9.25, 1.5], [8.0, 1.25, 11.0, 0.5], [12.0, 2.0, 6.5, 1.5]])
+        if n == 4:
+            gaps_aff = np.array([[2.0, 1.0, 3.0, 1.0], [1.0, 2.0, 2.0, 0.5], [3.0, 0.5, 1.0, 1.0]])
+        for gm in ("affine", "linear"):
+            gaps = gaps_aff if gm == "affine" else gaps_aff * np.array([0.0, 1.0, 0.0, 1.0])
+            for mode in ("global", "local", "glocal"):
+                w = wl.random_pairs(f"golden_tracks_{name}_{gm}_{mode}", 24, alpha, lens[0], lens[1],
+                                    gap_model=gm, mode=mode, related=0.5)
+                rt = wl.rng(w.name, "tracks")
+                tracks = []
+                for q in range(w.n_pairs):   # segment tracks: runs of one set
+                    m = int(w.off_a[q + 1] - w.off_a[q])
+                    t = np.zeros(m, np.uint16)
+                    pos = 0
+                    while pos < m:
+                        ln = int(rt.integers(1, 12))
+                        t[pos:pos + ln] = rt.integers(0, K)
+                        pos += ln
+                    tracks.append(t)
+                tracks = np.concatenate(tracks) if tracks else np.zeros(0, np.uint16)
+                explicit = mode != "glocal"
+                gtracks = (rt.integers(0, K, size=int(w.off_a[-1]) + w.n_pairs).astype(np.uint16)
+                           if explicit else None)
+                for prec in ("float32", "float64"):
+                    for gd in ("hard", "none"):
+                        if gd == "none" and prec == "float64":
+                            continue
+                        res = run_ref_tracked(w, mats, gaps, tracks, gtracks, prec, gd)
+                        extra = {"matrices": mats, "gaps_k": gaps, "tracks": tracks}
+                        if gtracks is not None:
+                            extra["gap_tracks"] = gtracks
+                        tag = "hard" if gd == "hard" else "score"
+                        save_tracked(f"{tag}_tracks_{name}_{gm}_{mode}_{prec}", w, prec, gd, res, extra)
+
+
+def run_ref_tracked(w, mats, gaps, tracks, gtracks, precision, grad_mode):
+    sa, sb = w.strings()
+    b = ref.RefBatch(sa, sb, w.alphabet, mats, gaps, double=(precision == "float64"), gap_model=w.gap_model,
+                     mode=w.mode, grad_mode=grad_mode, n_threads=1, tracks=tracks, gap_tracks=gtracks)
+    total = b.score_and_grad()
+    scores = b.scores()
+    out = {"total": np.array(total), "scores": scores}
+    if grad_mode != "none":
+        m, g = b.grads()
+        out["grads"] = np.concatenate([m.reshape(len(scores), -1), g.reshape(len(scores), -1)], axis=1)
+        out["compute_grad"] = b.compute_grad()
+        wts = wl.rng(w.name, "weights").normal(size=len(scores))
+        out["weights"] = wts
+        out["weighted_grad"] = b.weighted_grad(wts)
+    return out
+
+
+def save_tracked(name, w, precision, grad_mode, res, extra):
+    path = os.path.join(OUT, name + ".npz")
+    np.savez_compressed(path, alphabet=np.array(w.alphabet), matrix=extra["matrices"][0], gaps=extra["gaps_k"][0],
+                        gap_model=np.array(w.gap_model), mode=np.array(w.mode), precision=np.array(precision),
+                        grad_mode=np.array(grad_mode), codes_a=w.codes_a, off_a=w.off_a, codes_b=w.codes_b,
+                        off_b=w.off_b, **res, **extra)
+    print("wrote", path, len(w.off_a) - 1, "pairs")
+
+
 def main():
     if sys.argv[1:] == ["long"]:
         long_sets()
+        return
+    if sys.argv[1:] == ["tracks"]:
+        track_sets()
         return
     for gm in ("affine", "linear"):
         for mode in ("global", "local", "glocal"):

@@ -18,7 +18,7 @@ def _args(z):
     return (z["matrix"], z["gaps"], z["codes_a"], z["off_a"], z["codes_b"], z["off_b"])
 
 
-@pytest.mark.parametrize("path", [p for p in GOLDEN if "/hard_" in p or "/score_" in p],
+@pytest.mark.parametrize("path", [p for p in GOLDEN if ("/hard_" in p or "/score_" in p) and "_tracks_" not in p],
                          ids=lambda p: os.path.basename(p))
 def test_hard_matches_reference_golden(path):
     """Bit-exact: scores, per-pair counts, compute_grad and weighted_grad of the reference."""
@@ -193,3 +193,65 @@ def test_long_pairs_match_reference_engine():
         np.testing.assert_array_equal(s, rb.scores())
         mats, gaps = rb.grads()
         np.testing.assert_array_equal(g, np.concatenate([mats.reshape(2, -1), gaps], 1))
+
+
+TRACKED = [p for p in GOLDEN if "_tracks_" in p]
+
+
+@pytest.mark.parametrize("path", TRACKED, ids=lambda p: os.path.basename(p))
+def test_tracked_matches_reference_golden(path):
+    """Tracked batches (K = 3 sets, segment tracks, explicit or default gap tracks): the
+    oracle's restatement against the reference engine, bit-exact, scores and every count."""
+    z = np.load(path)
+    gd = str(z["grad_mode"])
+    gt = z["gap_tracks"] if "gap_tracks" in z else None
+    s, g = oracle.hard_tracked(z["matrices"], z["gaps_k"], z["codes_a"], z["off_a"], z["codes_b"], z["off_b"],
+                               z["tracks"], gt, gap_model=str(z["gap_model"]), mode=str(z["mode"]),
+                               precision=str(z["precision"]), with_grad=gd == "hard")
+    np.testing.assert_array_equal(s, z["scores"])
+    if gd == "hard":
+        np.testing.assert_array_equal(g, z["grads"])
+        np.testing.assert_array_equal(oracle.weighted_sum(g), z["compute_grad"])
+        np.testing.assert_array_equal(oracle.weighted_sum(g, z["weights"]), z["weighted_grad"])
+
+
+@pytest.mark.parametrize("gm", ["affine", "linear"])
+@pytest.mark.parametrize("mode", ["global", "local", "glocal"])
+def test_tracks_with_equal_sets_reduce_to_untracked(gm, mode):
+    """K copies of one parameter set under any tracks: the same scores as the untracked
+    problem, and per-set counts that sum to the untracked counts (the reference's invariance,
+    aligner.hpp:1820-1826: the run-wise column-0 border equals the closed form)."""
+    w = wl.random_pairs(f"trk_equal_{gm}_{mode}", 40, wl.AA, (0, 70), (0, 90), gap_model=gm, mode=mode,
+                        related=0.5, gaps=(10.5, 0.75, 9.25, 1.5) if gm == "affine" else (0, 0.75, 0, 1.5))
+    K, nn = 3, 400
+    r = wl.rng(w.name, "tracks")
+    tracks = r.integers(0, K, size=int(w.off_a[-1])).astype(np.uint16)
+    gtr = r.integers(0, K, size=int(w.off_a[-1]) + w.n_pairs).astype(np.uint16)
+    for prec in ("float32", "float64"):
+        s0, g0 = oracle.hard(w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, gap_model=gm, mode=mode,
+                             precision=prec)
+        s1, g1 = oracle.hard_tracked(np.stack([w.matrix] * K), np.stack([w.gaps] * K), w.codes_a, w.off_a,
+                                     w.codes_b, w.off_b, tracks, gtr, gap_model=gm, mode=mode, precision=prec)
+        np.testing.assert_array_equal(s1, s0)
+        mats = g1[:, :K * nn].reshape(-1, K, nn).sum(axis=1)
+        gaps = g1[:, K * nn:].reshape(-1, K, 4).sum(axis=1)
+        np.testing.assert_array_equal(np.concatenate([mats, gaps], 1), g0)
+        # one set, default tracks: the untracked problem exactly
+        s2, g2 = oracle.hard_tracked(w.matrix[None], w.gaps[None], w.codes_a, w.off_a, w.codes_b, w.off_b,
+                                     np.zeros(int(w.off_a[-1]), np.uint16), gap_model=gm, mode=mode, precision=prec)
+        np.testing.assert_array_equal(s2, s0)
+        np.testing.assert_array_equal(g2, g0)
+
+
+def test_tracked_score_is_counts_times_params():
+    """Integral parameters: each pair's score equals its K-set counts times the K sets, so a
+    count attributed to the wrong set (or the wrong gap field) changes the sum."""
+    w = wl.random_pairs("trk_identity", 40, wl.AA, (1, 70), (1, 90), related=0.5)
+    K = 3
+    r = wl.rng(w.name, "params")
+    mats = np.stack([wl.random_matrix(r) for _ in range(K)])
+    gaps = np.array([[10.0, 1.0, 9.0, 2.0], [8.0, 2.0, 11.0, 1.0], [12.0, 3.0, 6.0, 2.0]])
+    tracks = r.integers(0, K, size=int(w.off_a[-1])).astype(np.uint16)
+    s, g = oracle.hard_tracked(mats, gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, tracks)
+    theta = np.concatenate([mats.ravel(), gaps.ravel()])
+    np.testing.assert_array_equal(g @ theta, s)
