@@ -89,6 +89,22 @@ int nwg_batch_query(nwg_batch *b, int *done);
  * batch's precision on upload as static_cast<T>(double).  Invalidates cached results.
  * Replaces BatchEngine::set_params (batch_engine.hpp:597-606). */
 int nwg_batch_set_params(nwg_batch *b, const double *matrix, const double *gaps4);
+/* K parameter sets: `matrices` K blocks of n_alpha*n_alpha, `gaps` K rows of go_a, ge_a,
+ * go_b, ge_b.  K > 1 makes the batch tracked (default tracks: every residue set 0) and the
+ * per-pair gradient rows K * (n_alpha^2 + 4) doubles: K matrices, then K gap rows.  Hard and
+ * score-only batches.  Replaces BatchEngine::set_params with an AlignParams of K sets
+ * (batch_engine.hpp:597-606; align_params.hpp:55-97). */
+int nwg_batch_set_params_k(nwg_batch *b, int K, const double *matrices, const double *gaps);
+/* Tracks (per-row parameter sets): `tracks` one set per residue of A, pair p at
+ * off_a[p] .. off_a[p+1]; `gap_tracks` one per DP row 0..m, pair p at off_a[p] + p (m + 1
+ * entries).  NULL: the reference's defaults (set 0; a gap track following the track).  Sets
+ * must lie in [0, K).  Row i scores with matrix tracks[i-1] and pays gaps in B from that set,
+ * gaps in A from set gap_tracks[i]; each gradient count lands in its row's set.  Hard and
+ * score-only batches.  Replaces BatchEngine's tracks / gap_tracks arguments
+ * (batch_engine.hpp:284-350, 1467-1500; aligner.hpp:1800-2008). */
+int nwg_batch_set_tracks(nwg_batch *b, const uint16_t *tracks, const uint16_t *gap_tracks);
+/* The batch's parameter-set count and whether it is tracked. */
+int nwg_batch_num_sets(nwg_batch *b, int *K, int *tracked);
 /* The same from a device buffer of n_alpha*n_alpha + 4 doubles (matrix, then go_a, ge_a,
  * go_b, ge_b), copied on the context's stream without synchronising: the parameter update of
  * a training step stays on the device (and inside a captured graph).  Soft batches only; no

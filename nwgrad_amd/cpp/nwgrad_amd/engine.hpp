@@ -95,10 +95,18 @@ private:
 // An N x N substitution matrix over an interned alphabet, row-major (subst_matrix.hpp:29-33).
 class SubstMatrix {
 public:
-    SubstMatrix(const double* src, const Alphabet& a) : alpha_(&a), blk_(src, src + a.size() * a.size()) {}
-    explicit SubstMatrix(const Alphabet& a) : alpha_(&a), blk_(static_cast<size_t>(a.size() * a.size()), 0.0) {}
+    // nsets blocks of N x N, back to back (subst_matrix.hpp:29-33: set k at k*N*N).
+    SubstMatrix(const double* src, const Alphabet& a, int nsets = 1)
+        : alpha_(&a), sets_(nsets), blk_(src, src + static_cast<size_t>(nsets) * a.size() * a.size()) {}
+    explicit SubstMatrix(const Alphabet& a, int nsets = 1)
+        : alpha_(&a), sets_(nsets), blk_(static_cast<size_t>(nsets) * a.size() * a.size(), 0.0) {}
     const Alphabet& alphabet() const noexcept { return *alpha_; }
     int size() const noexcept { return alpha_->size(); }
+    int sets() const noexcept { return sets_; }
+    // set k alone, as a one-set matrix
+    SubstMatrix slice(int k) const {
+        return SubstMatrix(blk_.data() + static_cast<size_t>(k) * size() * size(), *alpha_);
+    }
     const double* data() const noexcept { return blk_.data(); }
     double* data() noexcept { return blk_.data(); }
     double at(int i, int j) const { return blk_[static_cast<size_t>(i * size() + j)]; }
@@ -115,6 +123,9 @@ public:
         if (alpha_ != o.alpha_)
             throw std::invalid_argument("nwgrad: cannot combine matrices over different alphabets (\"" +
                                         alpha_->symbols() + "\" vs \"" + o.alpha_->symbols() + "\")");
+        if (sets_ != o.sets_)
+            throw std::invalid_argument("nwgrad: cannot combine params with " + std::to_string(sets_) + " and " +
+                                        std::to_string(o.sets_) + " parameter sets");
     }
     SubstMatrix& operator+=(const SubstMatrix& o) {
         check_same(o);
@@ -128,30 +139,85 @@ public:
 
 private:
     const Alphabet* alpha_;
+    int sets_ = 1;
     std::vector<double> blk_;
 };
 
-// Parameter point and gradient (align_params.hpp:10-43: gap fields of a gradient are <= 0).
+// Parameter point and gradient (align_params.hpp:10-43: gap fields of a gradient are <= 0),
+// with K parameter sets (align_params.hpp:55-152): the matrix holds K blocks, the plain gap
+// fields are set 0's and `more_gaps` holds sets 1..K-1 (go_a, ge_a, go_b, ge_b each).
 struct AlignParams {
     SubstMatrix matrix;
     double gap_open_a = 0.0, gap_extend_a = 0.0, gap_open_b = 0.0, gap_extend_b = 0.0;
+    std::vector<double> more_gaps;
+
+    enum : int { kOpenA = 0, kExtendA = 1, kOpenB = 2, kExtendB = 3 };
 
     explicit AlignParams(const SubstMatrix& m, double goa = 0, double gea = 0, double gob = 0, double geb = 0)
-        : matrix(m), gap_open_a(goa), gap_extend_a(gea), gap_open_b(gob), gap_extend_b(geb) {}
-    explicit AlignParams(const Alphabet& a) : matrix(a) {}
+        : matrix(m), gap_open_a(goa), gap_extend_a(gea), gap_open_b(gob), gap_extend_b(geb),
+          more_gaps(static_cast<size_t>(4 * (m.sets() - 1)), 0.0) {}
+    // K sets: `stack` the K matrices, `gaps` K rows of go_a, ge_a, go_b, ge_b
+    AlignParams(const SubstMatrix& stack, const double* gaps)
+        : matrix(stack), gap_open_a(gaps[0]), gap_extend_a(gaps[1]), gap_open_b(gaps[2]), gap_extend_b(gaps[3]),
+          more_gaps(gaps + 4, gaps + 4 * stack.sets()) {}
+    explicit AlignParams(const Alphabet& a, int nsets = 1)
+        : matrix(a, nsets), more_gaps(static_cast<size_t>(4 * (nsets - 1)), 0.0) {}
 
     const Alphabet& alphabet() const noexcept { return matrix.alphabet(); }
+    int num_sets() const noexcept { return matrix.sets(); }
     void gaps4(double* g) const { g[0] = gap_open_a; g[1] = gap_extend_a; g[2] = gap_open_b; g[3] = gap_extend_b; }
+    double gap(int k, int f) const { return const_cast<AlignParams*>(this)->gap_ref(k, f); }
+    double& gap_ref(int k, int f) {
+        if (k == 0) return f == kOpenA ? gap_open_a : f == kExtendA ? gap_extend_a : f == kOpenB ? gap_open_b
+                                                                                                : gap_extend_b;
+        return more_gaps[static_cast<size_t>(4 * (k - 1) + f)];
+    }
+    // every set's gap fields, K rows of 4
+    std::vector<double> all_gaps() const {
+        std::vector<double> g(4);
+        gaps4(g.data());
+        g.insert(g.end(), more_gaps.begin(), more_gaps.end());
+        return g;
+    }
+    // set k as a one-set AlignParams
+    AlignParams set(int k) const {
+        return AlignParams(matrix.slice(k), gap(k, 0), gap(k, 1), gap(k, 2), gap(k, 3));
+    }
+    void assign_set(int k, const AlignParams& p) {
+        if (&p.alphabet() != &alphabet())
+            throw std::invalid_argument("nwgrad: the set's alphabet does not match the params'");
+        if (p.num_sets() != 1) throw std::invalid_argument("nwgrad: assign one parameter set at a time");
+        const size_t nn = static_cast<size_t>(matrix.size()) * static_cast<size_t>(matrix.size());
+        std::memcpy(matrix.data() + static_cast<size_t>(k) * nn, p.matrix.data(), nn * sizeof(double));
+        for (int f = 0; f < 4; ++f) gap_ref(k, f) = p.gap(0, f);
+    }
+    // the sets of `parts`, in order, as one params (align_params.hpp:83-96)
+    static AlignParams stack(const std::vector<AlignParams>& parts) {
+        if (parts.empty()) throw std::invalid_argument("nwgrad: stack() needs at least one AlignParams");
+        const Alphabet& a = parts[0].alphabet();
+        int K = 0;
+        for (const AlignParams& p : parts) {
+            if (&p.alphabet() != &a) throw std::invalid_argument("nwgrad: stack() needs params over one alphabet");
+            K += p.num_sets();
+        }
+        AlignParams r(a, K);
+        int k = 0;
+        for (const AlignParams& p : parts)
+            for (int s = 0; s < p.num_sets(); ++s) r.assign_set(k++, p.set(s));
+        return r;
+    }
 
     AlignParams& operator+=(const AlignParams& o) {
         matrix += o.matrix;
         gap_open_a += o.gap_open_a; gap_extend_a += o.gap_extend_a;
         gap_open_b += o.gap_open_b; gap_extend_b += o.gap_extend_b;
+        for (size_t k = 0; k < more_gaps.size(); ++k) more_gaps[k] += o.more_gaps[k];
         return *this;
     }
     AlignParams& operator*=(double s) noexcept {
         matrix *= s;
         gap_open_a *= s; gap_extend_a *= s; gap_open_b *= s; gap_extend_b *= s;
+        for (auto& v : more_gaps) v *= s;
         return *this;
     }
     AlignParams operator+(const AlignParams& o) const { AlignParams r(*this); r += o; return r; }
@@ -253,8 +319,14 @@ public:
         if (&p.alphabet() != alpha_ && N_ > 0)
             throw std::invalid_argument("nwgrad: params alphabet \"" + p.alphabet().symbols() +
                                         "\" does not match the batch's \"" + alpha_->symbols() + "\"");
+        AlignParams old = params_;
         params_ = p;
-        upload_params_();
+        try {
+            upload_params_();
+        } catch (...) {
+            params_ = old;
+            throw;
+        }
     }
     const AlignParams& params() const noexcept { return params_; }
 
@@ -326,6 +398,24 @@ public:
     }
     AlignParams compute_grad() const { return reduce_(alphabet(), nullptr); }
 
+    // Tracks (batch_engine.hpp:284-350): `tracks` one parameter set per residue of A, pair p at
+    // off_a[p] .. off_a[p+1]; `gap_tracks` one per DP row 0..m, pair p at off_a[p] + p; either may
+    // be null (the reference's defaults).  The batch is then tracked: grads() has num_sets() sets.
+    void set_tracks(const uint16_t* tracks, const uint16_t* gap_tracks) {
+        check_(nwg_batch_set_tracks(b_, tracks, gap_tracks));
+        valid_ = false;
+    }
+    int num_sets() const {
+        int K = 1, tr = 0;
+        check_(nwg_batch_num_sets(b_, &K, &tr));
+        return K;
+    }
+    bool tracked() const {
+        int K = 1, tr = 0;
+        check_(nwg_batch_num_sets(b_, &K, &tr));
+        return tr != 0;
+    }
+
     // Device-side plumbing for autograd and serving.
     nwg_batch* handle() const noexcept { return b_; }
     const double* scores_dev() const {
@@ -389,19 +479,17 @@ private:
         upload_params_();
     }
     void upload_params_() {
-        double g[4];
-        params_.gaps4(g);
-        check_(nwg_batch_set_params(b_, params_.matrix.data(), g));
+        const std::vector<double> g = params_.all_gaps();
+        check_(nwg_batch_set_params_k(b_, params_.num_sets(), params_.matrix.data(), g.data()));
         valid_ = false;
     }
     AlignParams reduce_(const Alphabet& a, const double* w) const {
         if (gd_ == GradMode::None) throw std::logic_error("nwgrad: grad_mode \"none\" computes no gradient");
         if (!valid_) throw std::logic_error("nwgrad: gradient not valid; compute an alignment first");
-        const int nn = a.size() * a.size();
-        std::vector<double> out(static_cast<size_t>(nn + 4));
+        const int nn = a.size() * a.size(), K = num_sets();
+        std::vector<double> out(static_cast<size_t>(K) * (nn + 4));
         check_(nwg_batch_weighted_grad(b_, w, w ? static_cast<int64_t>(size()) : 0, out.data()));
-        AlignParams r(SubstMatrix(out.data(), a), out[nn], out[nn + 1], out[nn + 2], out[nn + 3]);
-        return r;
+        return AlignParams(SubstMatrix(out.data(), a, K), out.data() + static_cast<size_t>(K) * nn);
     }
     void check_(int rc) const {
         if (rc != NWG_OK) throw_status(rc, nwg_last_error(ctx_));

@@ -29,6 +29,19 @@
 
 namespace nwg {
 
+// Tracks: K parameter sets chosen per DP row (aligner.hpp:1800-1826).  Row i (1..m) of pair p
+// takes its matrix and its gaps in B from set trk[off_a[p] + i - 1] and its gaps in A from set
+// gtrk[off_a[p] + p + i] (row 0 included); bx0[off_a[p] + p + i] is its column-0 border, the
+// reference's run-wise sum (setup_tracks_, aligner.hpp:1981-2007), formed on the host.
+template <typename T>
+struct TrackArgs {
+    const uint16_t* trk;
+    const uint16_t* gtrk;
+    const T* bx0;
+    const T* gaps;    // K x 4: go_a, ge_a, go_b, ge_b, each static_cast<T>(double)
+    int32_t K;
+};
+
 template <typename T>
 struct HardArgs {
     const uint8_t* codes_a;
@@ -40,12 +53,13 @@ struct HardArgs {
     int32_t n_tiles;
     int32_t tile_base;        // plain launch: the first tile of this launch
     int32_t* queue;           // persistent launch: next-tile counter (nullptr: plain)
-    const T* sub;             // n_alpha * n_alpha, the matrix cast to T
+    const T* sub;             // K * n_alpha * n_alpha, the matrices cast to T (K = 1 untracked)
     int32_t na;
     int32_t hist_slots;       // LDS histogram slots per wave
     T go_a, ge_a, go_b, ge_b;
+    TrackArgs<T> tr;          // tracked batches (the TR kernels) only
     double* scores;           // per pair
-    double* grads;            // per pair, na*na + 4 (hard only)
+    double* grads;            // per pair, K * (na*na + 4) (hard only)
     uint32_t* table;          // pointer-code scratch (hard only)
     int64_t slot_dwords;      // dwords per table slot
 };
@@ -170,11 +184,12 @@ __device__ __forceinline__ void local_update(Cand<T>& best, const T (&M)[S], con
 }
 
 // The lane's end-cell candidate once row m is done (aligner.hpp:4054-4066, 3851-3860; Glocal
-// 6121-6134).  col0: this lane also offers column 0 (the first lane of the pair's first panel).
+// 6121-6134).  col0: this lane also offers column 0 (the first lane of the pair's first panel),
+// whose value at row m is col0v (0 when m = 0).
 template <typename T, int GM, int AM, int S>
 __device__ __forceinline__ Cand<T> end_candidate(const T (&M)[S], const T (&X)[S], const T (&Y)[S],
                                                  const Cand<T>& best, int m, int n, int j0, int ncols, bool col0,
-                                                 T go_b, T ge_b) {
+                                                 T col0v) {
     Cand<T> c;
     c.v = T(0); c.i = -1; c.j = 0; c.st = 0;
     if constexpr (AM == kLocal) {
@@ -183,9 +198,8 @@ __device__ __forceinline__ Cand<T> end_candidate(const T (&M)[S], const T (&X)[S
         if (n == 0) {
             if (col0) {
                 c.i = m; c.j = 0;
-                if (m == 0) { c.v = T(0); c.st = 0; }
-                else if constexpr (GM == kAffine) { c.v = -(go_b + (T)m * ge_b); c.st = 1; }
-                else { c.v = (-(T)m) * ge_b; c.st = 0; }
+                c.v = col0v;
+                c.st = (m > 0 && GM == kAffine) ? 1 : 0;
             }
         } else if (n - 1 >= j0 && n - 1 < j0 + S) {
             const int kn = n - 1 - j0;
@@ -203,9 +217,8 @@ __device__ __forceinline__ Cand<T> end_candidate(const T (&M)[S], const T (&X)[S
     } else {   // Glocal: row m, the first column with the strictly largest value
         if (col0) {   // column 0 first
             c.i = m; c.j = 0;
-            if (m == 0) { c.v = T(0); c.st = 0; }
-            else if constexpr (GM == kAffine) { c.v = -(go_b + (T)m * ge_b); c.st = 1; }
-            else { c.v = (-(T)m) * ge_b; c.st = 0; }
+            c.v = col0v;
+            c.st = (m > 0 && GM == kAffine) ? 1 : 0;
         }
 #pragma unroll
         for (int k = 0; k < S; ++k) {
@@ -221,7 +234,16 @@ __device__ __forceinline__ Cand<T> end_candidate(const T (&M)[S], const T (&X)[S
     return c;
 }
 
-template <typename T, int GM, int AM, int S, bool HARD>
+// The column-0 cell of row i >= 1 (Global / Glocal; aligner.hpp:4009-4018, 3823-3826): the
+// closed form untracked, the row's precomputed border tracked.
+template <typename T, int GM, bool TR>
+__device__ __forceinline__ T col0_border(const TrackArgs<T>& tr, int64_t r0, int i, T go_b, T ge_b) {
+    if constexpr (TR) return tr.bx0[r0 + i];
+    else if constexpr (GM == kAffine) return -(go_b + (T)i * ge_b);
+    else return (-(T)i) * ge_b;
+}
+
+template <typename T, int GM, int AM, int S, bool HARD, bool TR>
 __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint16_t* __restrict__ hist,
                           int tile, int64_t slot, int lane) {
     static_assert(S % 4 == 0, "S is a multiple of 4");
@@ -230,20 +252,31 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
     const int lg = __builtin_ctz(G);
     const int q = lane >> lg, g = lane & (G - 1);
     const int p = q < td.npairs ? A.order[td.first + q] : -1;
-    const int na = A.na, nn = na * na, gs = nn + 4;
+    const int na = A.na, nn = na * na;
+    const int K = TR ? A.tr.K : 1, knn = K * nn, gs = K * (nn + 4);
     int m = 0, n = 0;
+    int64_t r0 = 0;   // tracked: pair p's first entry of the per-row arrays (gtrk, bx0)
     const uint8_t* ap = A.codes_a;
     const uint8_t* bp = A.codes_b;
+    const uint16_t* tk = A.tr.trk;
     if (p >= 0) {
         const int64_t a0 = A.off_a[p], b0 = A.off_b[p];
         m = (int)(A.off_a[p + 1] - a0);
         n = (int)(A.off_b[p + 1] - b0);
         ap += a0;
         bp += b0;
+        if constexpr (TR) { tk += a0; r0 = a0 + p; }
     }
     const int j0 = g * S;
     const int ncols = n - j0 < 0 ? 0 : (n - j0 > S ? S : n - j0);
-    const T go_a = A.go_a, ge_a = A.ge_a, go_b = A.go_b, ge_b = A.ge_b;
+    T go_a = A.go_a, ge_a = A.ge_a, go_b = A.go_b, ge_b = A.ge_b;
+    const T* __restrict__ gT = ssub + knn;   // tracked: the K x 4 gap table in LDS
+    if constexpr (TR) {   // row 0's gaps in A: set gtrk of row 0
+        if (p >= 0) {
+            const int s0 = A.tr.gtrk[r0];
+            go_a = gT[4 * s0]; ge_a = gT[4 * s0 + 1];
+        }
+    }
     const T NI = ninf<T>();
 
     int coff[S];
@@ -284,24 +317,35 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
     const int steps = td.max_m + G - 1;
     for (int t = 0; t < steps; ++t) {
         const int i = t - g + 1;
+        const bool active = (p >= 0) && i >= 1 && i <= m && ncols > 0;
+        int rs = 0;   // tracked: the row's set (matrix, gaps in B)
+        if constexpr (TR) {
+            if (p >= 0 && i >= 1 && i <= m) {
+                rs = tk[i - 1];
+                const int gsi = A.tr.gtrk[r0 + i];
+                go_a = gT[4 * gsi]; ge_a = gT[4 * gsi + 1];
+                go_b = gT[4 * rs + 2]; ge_b = gT[4 * rs + 3];
+            }
+        }
         T iM = shift_up(oM);
         T iX, iY;
         if constexpr (GM == kAffine) { iX = shift_up(oX); iY = shift_up(oY); }
         else { iX = NI; iY = NI; }
         if (g == 0) {   // column 0 of row i (aligner.hpp:4009-4018, 3823-3826)
-            const T it = (T)i;
+            T border = NI;
+            if constexpr (AM != kLocal) {
+                if (!TR || (p >= 0 && i >= 1 && i <= m)) border = col0_border<T, GM, TR>(A.tr, r0, i, go_b, ge_b);
+            }
             if constexpr (GM == kAffine) {
                 if (AM == kLocal) { iM = T(0); iX = NI; }
-                else { iM = NI; iX = -(go_b + it * ge_b); }
+                else { iM = NI; iX = border; }
                 iY = NI;
             } else {
-                iM = (AM == kLocal) ? T(0) : (-it) * ge_b;
+                iM = (AM == kLocal) ? T(0) : border;
             }
         }
-        const bool active = (p >= 0) && i >= 1 && i <= m && ncols > 0;
         if (active) {
-            const int rowoff = (int)ap[i - 1] * na;
-            const T* __restrict__ srow = ssub + rowoff;
+            const T* __restrict__ srow = ssub + ((TR ? rs * na : 0) + (int)ap[i - 1]) * na;
             uint32_t cw[S / 4];
             T rowmax;
             strip_row<T, GM, AM, S, HARD>(M, X, Y, iM, iX, iY, pM, pX, pY, srow, coff, ncols, go_a, ge_a, go_b, ge_b,
@@ -320,61 +364,86 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
 
     Cand<T> c;
     c.v = T(0); c.i = -1; c.j = 0; c.st = 0;
-    if (p >= 0) c = end_candidate<T, GM, AM, S>(M, X, Y, best, m, n, j0, ncols, g == 0, go_b, ge_b);
+    if (p >= 0) {
+        const T col0v = (m == 0 || AM == kLocal) ? T(0) : col0_border<T, GM, TR>(A.tr, r0, m, A.go_b, A.ge_b);
+        c = end_candidate<T, GM, AM, S>(M, X, Y, best, m, n, j0, ncols, g == 0, col0v);
+    }
     c = reduce_slot(c, G);
 
+    // The walk.  Untracked: gap counts in registers.  Tracked: every count in its row's set,
+    // K x 4 gap counters per pair slot in LDS after the slots' histograms.
     int c_goa = 0, c_gea = 0, c_gob = 0, c_geb = 0;
+    int* const gall = reinterpret_cast<int*>(reinterpret_cast<unsigned char*>(hist) +
+                                             ((A.hist_slots * knn * 2 + 15) & ~15));
+    int* gcnt = gall + q * 4 * K;
     if (g == 0 && p >= 0) {
         A.scores[p] = (double)c.v;
         if constexpr (HARD) {
-            uint16_t* h = hist + q * nn;
+            uint16_t* h = hist + q * knn;
             int i = c.i, j = c.j, st = c.st;
             const uint32_t* tr = tb + q * G;
+            // count a gap move: f 0 go_a, 1 ge_a (set gtrk of row i), 2 go_b, 3 ge_b (set trk)
+            auto gap = [&](int f, int row, int v) {
+                if constexpr (TR) {
+                    const int sset = f < 2 ? A.tr.gtrk[r0 + row] : tk[row - 1];
+                    gcnt[4 * sset + f] += v;
+                } else {
+                    if (f == 0) c_goa += v;
+                    else if (f == 1) c_gea += v;
+                    else if (f == 2) c_gob += v;
+                    else c_geb += v;
+                }
+            };
             for (;;) {
                 if (i == 0 && j == 0) break;
                 if (i == 0 || j == 0) {   // borders: runs (Global / Glocal column 0) or a stop
                     if (AM == kLocal) break;
                     if (i == 0) {
                         if (AM == kGlocal) break;
-                        c_gea += j;
-                        if (GM == kAffine) c_goa += 1;
+                        gap(1, 0, j);
+                        if (GM == kAffine) gap(0, 0, 1);
                         break;
                     }
-                    c_geb += i;
-                    if (GM == kAffine) c_gob += 1;
+                    if constexpr (TR) {
+                        for (int r = i; r >= 1; --r) gap(3, r, 1);
+                    } else {
+                        gap(3, i, i);
+                    }
+                    if (GM == kAffine) gap(2, 1, 1);
                     break;
                 }
                 const int jj = j - 1;
                 const int go = jj / S, k = jj - go * S;
                 const uint32_t w = tr[((int64_t)(i - 1 + go) * (S / 4) + (k >> 2)) * kWave + go];
                 const uint32_t cc = (w >> (8 * (k & 3))) & 0xffu;
+                const int mrow = ((TR ? (int)tk[i - 1] * na : 0) + (int)ap[i - 1]) * na + (int)bp[j - 1];
                 if constexpr (GM == kAffine) {
                     if (st == 0) {
                         if (AM == kLocal && (cc & 3u) == 3u) break;
                         const uint32_t kk = cc & 3u;
-                        h[(int)ap[i - 1] * na + (int)bp[j - 1]] += 1;
+                        h[mrow] += 1;
                         st = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
                         --i; --j;
                     } else if (st == 1) {
                         const uint32_t kk = (cc >> 2) & 3u;
                         const int pv = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
-                        c_geb += 1;
-                        c_gob += (pv != 1);
+                        gap(3, i, 1);
+                        if (pv != 1) gap(2, i, 1);
                         st = pv;
                         --i;
                     } else {
                         const uint32_t kk = cc >> 4;
                         const int pv = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
-                        c_gea += 1;
-                        c_goa += (pv != 2);
+                        gap(1, i, 1);
+                        if (pv != 2) gap(0, i, 1);
                         st = pv;
                         --j;
                     }
                 } else {
                     if (cc == 3u) break;
-                    if (cc == 0u) { h[(int)ap[i - 1] * na + (int)bp[j - 1]] += 1; --i; --j; }
-                    else if (cc == 1u) { c_geb += 1; --i; }
-                    else { c_gea += 1; --j; }
+                    if (cc == 0u) { h[mrow] += 1; --i; --j; }
+                    else if (cc == 1u) { gap(3, i, 1); --i; }
+                    else { gap(1, i, 1); --j; }
                 }
             }
         }
@@ -382,13 +451,20 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
     if constexpr (HARD) {
         wave_sync();
         const int np = td.npairs;
-        for (int idx = lane; idx < np * nn; idx += kWave) {
-            const int qq = idx / nn, e = idx - qq * nn;
+        for (int idx = lane; idx < np * knn; idx += kWave) {
+            const int qq = idx / knn, e = idx - qq * knn;
             const int pp = A.order[td.first + qq];
-            A.grads[(int64_t)pp * gs + e] = (double)hist[qq * nn + e];
-            hist[qq * nn + e] = 0;
+            A.grads[(int64_t)pp * gs + e] = (double)hist[qq * knn + e];
+            hist[qq * knn + e] = 0;
         }
-        if (g == 0 && p >= 0) {
+        if constexpr (TR) {
+            for (int idx = lane; idx < np * 4 * K; idx += kWave) {
+                const int qq = idx / (4 * K), e = idx - qq * 4 * K;
+                const int pp = A.order[td.first + qq];
+                A.grads[(int64_t)pp * gs + knn + e] = 0.0 - (double)gall[qq * 4 * K + e];
+                gall[qq * 4 * K + e] = 0;
+            }
+        } else if (g == 0 && p >= 0) {
             double* gr = A.grads + (int64_t)p * gs + nn;
             gr[0] = 0.0 - (double)c_goa;
             gr[1] = 0.0 - (double)c_gea;
@@ -399,29 +475,45 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
     }
 }
 
+// LDS of the hard / score-only fills: the K matrices and (tracked) the K x 4 gap table, then
+// (hard) per pair slot a histogram of K * n_alpha^2 uint16 counts, then (tracked) K x 4 int
+// gap counters per slot.
+template <typename T>
+__host__ __device__ constexpr int hard_lds_sub_bytes(int na, int K, bool tr) {
+    return ((K * na * na + (tr ? 4 * K : 0)) * (int)sizeof(T) + 15) & ~15;
+}
+__host__ __device__ constexpr int hard_lds_hist_bytes(int na, int K, bool tr, int slots) {
+    return ((slots * K * na * na * 2 + 15) & ~15) + (tr ? slots * 4 * K * 4 : 0);
+}
+
 // One wavefront per block.  Plain launch: block b fills tile tile_base + b.  Persistent
 // launch: each block pulls tiles from the queue until it is drained; every wave reaches
 // the exit test once the counter passes n_tiles.
-template <typename T, int GM, int AM, int S, bool HARD>
+template <typename T, int GM, int AM, int S, bool HARD, bool TR>
 __global__ __launch_bounds__(64) void hard_fill_kernel(HardArgs<T> A) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int na = A.na, nn = na * na;
+    const int K = TR ? A.tr.K : 1;
     T* ssub = reinterpret_cast<T*>(lds);
-    uint16_t* hist = reinterpret_cast<uint16_t*>(lds + ((nn * (int)sizeof(T) + 15) & ~15));
+    uint16_t* hist = reinterpret_cast<uint16_t*>(lds + hard_lds_sub_bytes<T>(na, K, TR));
     const int lane = threadIdx.x;
-    for (int k = lane; k < nn; k += kWave) ssub[k] = A.sub[k];
-    if constexpr (HARD)
-        for (int k = lane; k < A.hist_slots * nn; k += kWave) hist[k] = 0;
+    for (int k = lane; k < K * nn; k += kWave) ssub[k] = A.sub[k];
+    if constexpr (TR)
+        for (int k = lane; k < 4 * K; k += kWave) ssub[K * nn + k] = A.tr.gaps[k];
+    if constexpr (HARD) {
+        const int words = hard_lds_hist_bytes(na, K, TR, A.hist_slots) / 2;
+        for (int k = lane; k < words; k += kWave) hist[k] = 0;
+    }
     __syncthreads();
     if (A.queue == nullptr) {
         const int tile = A.tile_base + (int)blockIdx.x;
-        if (tile < A.n_tiles) hard_tile<T, GM, AM, S, HARD>(A, ssub, hist, tile, tile, lane);
+        if (tile < A.n_tiles) hard_tile<T, GM, AM, S, HARD, TR>(A, ssub, hist, tile, tile, lane);
         return;
     }
     for (;;) {
         const int tile = wave_uniform(atomicAdd(A.queue, lane == 0 ? 1 : 0));
         if (tile >= A.n_tiles) break;
-        hard_tile<T, GM, AM, S, HARD>(A, ssub, hist, tile, (int64_t)blockIdx.x, lane);
+        hard_tile<T, GM, AM, S, HARD, TR>(A, ssub, hist, tile, (int64_t)blockIdx.x, lane);
     }
 }
 

@@ -65,9 +65,10 @@ struct LongArgs {
     int32_t n_items;
     int32_t* queue;
     const LongPair* lpairs;
-    const T* sub;
+    const T* sub;         // K matrices cast to T (K = 1 untracked)
     int32_t na;
     T go_a, ge_a, go_b, ge_b;
+    TrackArgs<T> tr;      // tracked batches (the TR kernels) only
     int32_t* progress;    // per item: rows of its right column published
     T* bnd;               // right boundary columns
     CandRec<T>* cand;     // per item: the panel's end-cell candidate
@@ -78,7 +79,7 @@ struct LongArgs {
     uint64_t wait_ticks;  // bound of one wait, in s_memrealtime ticks
 };
 
-template <typename T, int GM, int AM, int S, bool HARD, int RB>
+template <typename T, int GM, int AM, int S, bool HARD, int RB, bool TR>
 __device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int it, int lane) {
     static_assert(RB >= 1 && RB <= kWave, "a row block fits the lanes of a wave");
     const PanelItem item = A.items[it];
@@ -90,7 +91,15 @@ __device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int
     const int g = lane;
     const int j0 = item.panel * (kWave * S) + g * S;
     const int ncols = n - j0 < 0 ? 0 : (n - j0 > S ? S : n - j0);
-    const T go_a = A.go_a, ge_a = A.ge_a, go_b = A.go_b, ge_b = A.ge_b;
+    T go_a = A.go_a, ge_a = A.ge_a, go_b = A.go_b, ge_b = A.ge_b;
+    const int nn = na * na, K = TR ? A.tr.K : 1;
+    const T* __restrict__ gT = ssub + K * nn;   // tracked: the K x 4 gap table in LDS
+    const uint16_t* tk = A.tr.trk + (TR ? a0 : 0);
+    const int64_t r0 = TR ? a0 + p : 0;
+    if constexpr (TR) {   // row 0's gaps in A
+        const int s0 = A.tr.gtrk[r0];
+        go_a = gT[4 * s0]; ge_a = gT[4 * s0 + 1];
+    }
     const T NI = ninf<T>();
     const bool first = item.bnd_in < 0, last = item.bnd_out < 0;
 
@@ -137,6 +146,15 @@ __device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int
     const int steps = m + kWave - 1;
     for (int t = 0; t < steps; ++t) {
         const int i = t - g + 1;
+        int rs = 0;   // tracked: the row's set (matrix, gaps in B)
+        if constexpr (TR) {
+            if (i >= 1 && i <= m) {
+                rs = tk[i - 1];
+                const int gsi = A.tr.gtrk[r0 + i];
+                go_a = gT[4 * gsi]; ge_a = gT[4 * gsi + 1];
+                go_b = gT[4 * rs + 2]; ge_b = gT[4 * rs + 3];
+            }
+        }
         if (!first && t < m && t % RB == 0) {
             // Rows t+1 .. min(t+RB, m) of the previous panel's right column.
             const int need = t + RB < m ? t + RB : m;
@@ -174,18 +192,21 @@ __device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int
             if constexpr (GM == kAffine) { lX = __shfl(bX, sl, kWave); lY = __shfl(bY, sl, kWave); }
             if (g == 0) { iM = lM; iX = lX; iY = lY; }
         } else if (g == 0) {   // column 0 of row i (aligner.hpp:4009-4018, 3823-3826)
-            const T itv = (T)i;
+            T border = NI;
+            if constexpr (AM != kLocal) {
+                if (!TR || (i >= 1 && i <= m)) border = col0_border<T, GM, TR>(A.tr, r0, i, go_b, ge_b);
+            }
             if constexpr (GM == kAffine) {
                 if (AM == kLocal) { iM = T(0); iX = NI; }
-                else { iM = NI; iX = -(go_b + itv * ge_b); }
+                else { iM = NI; iX = border; }
                 iY = NI;
             } else {
-                iM = (AM == kLocal) ? T(0) : (-itv) * ge_b;
+                iM = (AM == kLocal) ? T(0) : border;
             }
         }
         const bool active = i >= 1 && i <= m && ncols > 0;
         if (active) {
-            const T* __restrict__ srow = ssub + (int)ap[i - 1] * na;
+            const T* __restrict__ srow = ssub + ((TR ? rs * na : 0) + (int)ap[i - 1]) * na;
             uint32_t cw[S / 4];
             T rowmax;
             strip_row<T, GM, AM, S, HARD>(M, X, Y, iM, iX, iY, pM, pX, pY, srow, coff, ncols, go_a, ge_a, go_b, ge_b,
@@ -208,7 +229,8 @@ __device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int
         pM = iM; pX = iX; pY = iY;
     }
 
-    Cand<T> c = end_candidate<T, GM, AM, S>(M, X, Y, best, m, n, j0, ncols, item.panel == 0 && g == 0, go_b, ge_b);
+    const T col0v = (m == 0 || AM == kLocal) ? T(0) : col0_border<T, GM, TR>(A.tr, r0, m, A.go_b, A.ge_b);
+    Cand<T> c = end_candidate<T, GM, AM, S>(M, X, Y, best, m, n, j0, ncols, item.panel == 0 && g == 0, col0v);
     c = reduce_slot(c, kWave);
     if (lane == 0) {
         CandRec<T> r;
@@ -219,29 +241,38 @@ __device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int
 
 // One wavefront per block, pulling (pair, panel) items until the queue is drained; every
 // wave reaches the exit test once the counter passes n_items.
-template <typename T, int GM, int AM, int S, bool HARD, int RB>
+template <typename T, int GM, int AM, int S, bool HARD, int RB, bool TR>
 __global__ __launch_bounds__(64) void long_fill_kernel(LongArgs<T> A) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const int nn = A.na * A.na;
+    const int nn = A.na * A.na, K = TR ? A.tr.K : 1;
     T* ssub = reinterpret_cast<T*>(lds);
     const int lane = threadIdx.x;
-    for (int k = lane; k < nn; k += kWave) ssub[k] = A.sub[k];
+    for (int k = lane; k < K * nn; k += kWave) ssub[k] = A.sub[k];
+    if constexpr (TR)
+        for (int k = lane; k < 4 * K; k += kWave) ssub[K * nn + k] = A.tr.gaps[k];
     __syncthreads();
     for (;;) {
         const int it = wave_uniform(atomicAdd(A.queue, lane == 0 ? 1 : 0));
         if (it >= A.n_items) break;
-        long_panel<T, GM, AM, S, HARD, RB>(A, ssub, it, lane);
+        long_panel<T, GM, AM, S, HARD, RB, TR>(A, ssub, it, lane);
     }
+}
+
+// LDS of the long-pair walk: K * n_alpha^2 uint32 match counts and (tracked) K x 4 int gap
+// counters, then the table window.
+__host__ __device__ constexpr int long_walk_count_bytes(int na, int K, bool tr) {
+    return ((K * na * na + (tr ? 4 * K : 0)) * 4 + 15) & ~15;
 }
 
 // Block b: pair lpairs[b].  Combines its panels' candidates into the end cell, writes the
 // score, and (hard) walks the codes back, counting matches in an LDS histogram and gap moves
-// in registers (every thread runs the walk with the same values; thread 0 writes).
-template <typename T, int GM, int AM, int S, bool HARD>
+// in registers, or (tracked) every count in its row's set in LDS.  Every thread runs the walk
+// with the same values; thread 0 writes the counts.
+template <typename T, int GM, int AM, int S, bool HARD, bool TR>
 __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int tid = threadIdx.x, lane = tid & (kWave - 1);
-    const int na = A.na, nn = na * na, gs = nn + 4;
+    const int na = A.na, nn = na * na, K = TR ? A.tr.K : 1, knn = K * nn, gs = K * (nn + 4);
     const LongPair lp = A.lpairs[blockIdx.x];
     const int p = lp.pair;
 
@@ -264,16 +295,32 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
     if (tid == 0) A.scores[p] = (double)c.v;
     if constexpr (HARD) {
         uint32_t* hist = reinterpret_cast<uint32_t*>(lds);
-        uint32_t* win = reinterpret_cast<uint32_t*>(lds + ((nn * 4 + 15) & ~15));
-        for (int k = tid; k < nn; k += kWalkThreads) hist[k] = 0u;
+        int* gcnt = reinterpret_cast<int*>(hist + knn);   // tracked: K x 4
+        uint32_t* win = reinterpret_cast<uint32_t*>(lds + long_walk_count_bytes(na, K, TR));
+        for (int k = tid; k < knn + (TR ? 4 * K : 0); k += kWalkThreads) hist[k] = 0u;
         __syncthreads();
         const int64_t a0 = A.off_a[p], b0 = A.off_b[p];
         const int m = (int)(A.off_a[p + 1] - a0), n = (int)(A.off_b[p + 1] - b0);
         const uint8_t* ap = A.codes_a + a0;
         const uint8_t* bp = A.codes_b + b0;
+        const uint16_t* tk = A.tr.trk + (TR ? a0 : 0);
+        const int64_t r0 = TR ? a0 + p : 0;
         constexpr int PW = kWave * S;
         constexpr int SD = (S / 4) * kWave;   // dwords per table step
         int c_goa = 0, c_gea = 0, c_gob = 0, c_geb = 0;
+        // count a gap move: f 0 go_a, 1 ge_a (set gtrk of row `row`), 2 go_b, 3 ge_b (set trk)
+        auto gap = [&](int f, int row, int v) {
+            if constexpr (TR) {
+                const int sset = f < 2 ? A.tr.gtrk[r0 + row] : tk[row - 1];
+                if (tid == 0)
+                    __hip_atomic_fetch_add(&gcnt[4 * sset + f], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else {
+                if (f == 0) c_goa += v;
+                else if (f == 1) c_gea += v;
+                else if (f == 2) c_gob += v;
+                else c_geb += v;
+            }
+        };
         int i = c.i, j = c.j, st = c.st;
         if (i < 0 || i > m || j < 0 || j > n || st < 0 || st > 2 ||
             __hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
@@ -289,12 +336,16 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
                 if (AM == kLocal) break;
                 if (i == 0) {
                     if (AM == kGlocal) break;
-                    c_gea += j;
-                    if (GM == kAffine) c_goa += 1;
+                    gap(1, 0, j);
+                    if (GM == kAffine) gap(0, 0, 1);
                     break;
                 }
-                c_geb += i;
-                if (GM == kAffine) c_gob += 1;
+                if constexpr (TR) {
+                    for (int r = i; r >= 1; --r) gap(3, r, 1);
+                } else {
+                    gap(3, i, i);
+                }
+                if (GM == kAffine) gap(2, 1, 1);
                 break;
             }
             const int jj = j - 1;
@@ -328,27 +379,27 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
             // flow scalar
             const uint32_t w = (uint32_t)wave_uniform((int)win[(t - wlo) * SD + (k >> 2) * kWave + go]);
             const uint32_t cc = (w >> (8 * (k & 3))) & 0xffu;
+            const int mrow = ((TR ? (int)tk[i - 1] * na : 0) + (int)ap[i - 1]) * na + (int)bp[j - 1];
             if constexpr (GM == kAffine) {
                 if (st == 0) {
                     if (AM == kLocal && (cc & 3u) == 3u) break;
                     const uint32_t kk = cc & 3u;
                     if (tid == 0)
-                        __hip_atomic_fetch_add(&hist[(int)ap[i - 1] * na + (int)bp[j - 1]], 1u, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __hip_atomic_fetch_add(&hist[mrow], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     st = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
                     --i; --j;
                 } else if (st == 1) {
                     const uint32_t kk = (cc >> 2) & 3u;
                     const int pv = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
-                    c_geb += 1;
-                    c_gob += (pv != 1);
+                    gap(3, i, 1);
+                    if (pv != 1) gap(2, i, 1);
                     st = pv;
                     --i;
                 } else {
                     const uint32_t kk = cc >> 4;
                     const int pv = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
-                    c_gea += 1;
-                    c_goa += (pv != 2);
+                    gap(1, i, 1);
+                    if (pv != 2) gap(0, i, 1);
                     st = pv;
                     --j;
                 }
@@ -356,17 +407,18 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
                 if (cc == 3u) break;
                 if (cc == 0u) {
                     if (tid == 0)
-                        __hip_atomic_fetch_add(&hist[(int)ap[i - 1] * na + (int)bp[j - 1]], 1u, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __hip_atomic_fetch_add(&hist[mrow], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     --i; --j;
-                } else if (cc == 1u) { c_geb += 1; --i; }
-                else { c_gea += 1; --j; }
+                } else if (cc == 1u) { gap(3, i, 1); --i; }
+                else { gap(1, i, 1); --j; }
             }
         }
         __syncthreads();
         double* gr = A.grads + (int64_t)p * gs;
-        for (int k = tid; k < nn; k += kWalkThreads) gr[k] = (double)hist[k];
-        if (tid == 0) {
+        for (int k = tid; k < knn; k += kWalkThreads) gr[k] = (double)hist[k];
+        if constexpr (TR) {
+            for (int k = tid; k < 4 * K; k += kWalkThreads) gr[knn + k] = 0.0 - (double)gcnt[k];
+        } else if (tid == 0) {
             gr[nn + 0] = 0.0 - (double)c_goa;
             gr[nn + 1] = 0.0 - (double)c_gea;
             gr[nn + 2] = 0.0 - (double)c_gob;

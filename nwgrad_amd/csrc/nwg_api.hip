@@ -199,9 +199,17 @@ struct nwg_batch {
     std::vector<uint8_t> host_ok;
     int64_t soft_reruns = 0;
     // params
-    std::vector<double> matrix;
-    double gaps[4] = {0, 0, 0, 0};
+    std::vector<double> matrix;           // K matrices, n_alpha^2 each
+    std::vector<double> gaps = std::vector<double>(4, 0.0);   // K x (go_a, ge_a, go_b, ge_b)
+    int K = 1;                            // parameter sets
     bool have_params = false;
+    // tracks: set per residue of A (index off_a[p] + i - 1) and per DP row (off_a[p] + p + i)
+    bool tracked = false, bx_dirty = true;
+    std::vector<int64_t> hoa;             // off_a, host copy
+    std::vector<uint16_t> htrk, hgtrk;
+    DevPtr<uint16_t> dtrk, dgtrk;
+    DevPtr<float> bx32, gt32;             // column-0 borders per DP row; the K x 4 gap table
+    DevPtr<double> bx64, gt64;
     DevPtr<float> sub_f32;
     DevPtr<double> sub_f64;
     // reduction scratch
@@ -553,6 +561,117 @@ int build_soft_plans(nwg_batch* b, const std::vector<int32_t>& pairs, SoftPlan& 
     return upload_soft_plan(b, wide, wbins, wb);
 }
 
+// LDS slots of the hard walk's histograms (K sets of uint16 counts per pair slot, plus K x 4
+// gap counters when tracked) and of the soft pass (n_alpha x (n_alpha + 1) doubles): the slot
+// count fixes the narrowest lane group.  Returns whether it changed.
+bool set_slots(nwg_batch* b) {
+    const int na = b->na, nn = na * na;
+    int gmin = 1, slots = 64;
+    if (b->prob.grad_mode == 2) {
+        slots = std::max(1, std::min(nwg::kTilePairs, kSoftHistBytes / (na * (na + 1) * 8)));
+        while (64 / gmin > slots) ++gmin;
+    } else {
+        const int per_slot = b->prob.grad_mode == 1 ? b->K * nn * 2 + (b->tracked ? 16 * b->K : 0) : 0;
+        while (gmin < kMaxG && (64 / gmin) * per_slot > kHistBytes) gmin <<= 1;
+        slots = 64 / gmin;
+    }
+    const bool changed = gmin != b->gmin || slots != b->hist_slots;
+    b->gmin = gmin;
+    b->hist_slots = slots;
+    return changed;
+}
+
+// Tracks on the host: `trk` one set per residue of A (index off_a[p] + i - 1), `gtrk` one
+// per DP row (index off_a[p] + p + i); NULL: the reference's defaults (setup_tracks_,
+// aligner.hpp:1956-1967: set 0; row 0 the first residue's set, row i the set of residue i).
+int set_tracks_host(nwg_batch* b, const uint16_t* trk, const uint16_t* gtrk) {
+    nwg_ctx* ctx = b->ctx;
+    const int64_t N = b->N, la = N ? b->hoa[(size_t)N] : 0;
+    std::vector<uint16_t> t((size_t)std::max<int64_t>(la, 1), 0), g((size_t)std::max<int64_t>(la + N, 1), 0);
+    for (int64_t p = 0; p < N; ++p) {
+        const int64_t a0 = b->hoa[(size_t)p], m = b->mlen[(size_t)p];
+        for (int64_t i = 0; i < m; ++i) {
+            const uint16_t v = trk ? trk[a0 + i] : 0;
+            if (v >= b->K)
+                return fail(ctx, NWG_EINVAL, "nwgrad: track[%lld] = %d of pair %lld is not a parameter set (params has %d)",
+                            (long long)i, (int)v, (long long)p, b->K);
+            t[(size_t)(a0 + i)] = v;
+        }
+        for (int64_t i = 0; i <= m; ++i) {
+            uint16_t v;
+            if (gtrk) v = gtrk[a0 + p + i];
+            else v = i == 0 ? (m ? t[(size_t)a0] : 0) : t[(size_t)(a0 + i - 1)];
+            if (v >= b->K)
+                return fail(ctx, NWG_EINVAL,
+                            "nwgrad: gap_track[%lld] = %d of pair %lld is not a parameter set (params has %d)",
+                            (long long)i, (int)v, (long long)p, b->K);
+            g[(size_t)(a0 + p + i)] = v;
+        }
+    }
+    b->htrk.swap(t);
+    b->hgtrk.swap(g);
+    NWG_HIP(ctx, b->dtrk.grow(b->htrk.size()));
+    NWG_HIP(ctx, b->dgtrk.grow(b->hgtrk.size()));
+    NWG_HIP(ctx, hipMemcpyAsync(b->dtrk.p, b->htrk.data(), 2 * b->htrk.size(), hipMemcpyHostToDevice, ctx->stream));
+    NWG_HIP(ctx, hipMemcpyAsync(b->dgtrk.p, b->hgtrk.data(), 2 * b->hgtrk.size(), hipMemcpyHostToDevice, ctx->stream));
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    b->tracked = true;
+    b->bx_dirty = true;
+    return NWG_OK;
+}
+
+// The per-row column-0 borders of every pair and the K x 4 gap table, in precision T, as
+// the reference forms them (setup_tracks_, aligner.hpp:1968-2007): gaps cast once per set;
+// the border of row i is -(go_b(1) + C_i), C_i the sum of ge_b over rows 1..i taken by runs of
+// bitwise-equal extends (linear: -C_i).
+template <typename T>
+int upload_track_rows(nwg_batch* b, DevPtr<T>& bx, DevPtr<T>& gt) {
+    nwg_ctx* ctx = b->ctx;
+    const int K = b->K;
+    std::vector<T> g4((size_t)4 * K);
+    for (int k = 0; k < 4 * K; ++k) g4[(size_t)k] = static_cast<T>(b->gaps[(size_t)k]);
+    const int64_t N = b->N;
+    std::vector<T> out(b->hgtrk.size(), T(0));
+    const bool linear = b->prob.gap_model == 0;
+    for (int64_t p = 0; p < N; ++p) {
+        const int64_t a0 = b->hoa[(size_t)p], m = b->mlen[(size_t)p];
+        if (m == 0) continue;
+        T* o = out.data() + a0 + p;
+        const uint16_t* tk = b->htrk.data() + a0;
+        const T go = g4[(size_t)4 * tk[0] + 2];
+        T csum = T(0);
+        int64_t s0 = 1;
+        for (int64_t i = 1; i <= m; ++i) {
+            const T ge = g4[(size_t)4 * tk[i - 1] + 3];
+            if (i > 1) {
+                const T gp = g4[(size_t)4 * tk[i - 2] + 3];
+                if (std::memcmp(&ge, &gp, sizeof(T)) != 0) {
+                    csum = (s0 == 1) ? static_cast<T>(i - 1) * gp : csum + static_cast<T>(i - s0) * gp;
+                    s0 = i;
+                }
+            }
+            if (linear) o[i] = (s0 == 1) ? -static_cast<T>(i) * ge : -(csum + static_cast<T>(i - s0 + 1) * ge);
+            else o[i] = (s0 == 1) ? -(go + static_cast<T>(i) * ge) : -(go + (csum + static_cast<T>(i - s0 + 1) * ge));
+        }
+    }
+    NWG_HIP(ctx, bx.grow(std::max<size_t>(out.size(), 1)));
+    NWG_HIP(ctx, gt.grow((size_t)4 * K));
+    if (!out.empty())
+        NWG_HIP(ctx, hipMemcpyAsync(bx.p, out.data(), sizeof(T) * out.size(), hipMemcpyHostToDevice, ctx->stream));
+    NWG_HIP(ctx, hipMemcpyAsync(gt.p, g4.data(), sizeof(T) * g4.size(), hipMemcpyHostToDevice, ctx->stream));
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return NWG_OK;
+}
+
+int prepare_tracks(nwg_batch* b) {
+    if (!b->tracked || !b->bx_dirty) return NWG_OK;
+    const int rc = b->prob.precision == 0 ? upload_track_rows<float>(b, b->bx32, b->gt32)
+                                          : upload_track_rows<double>(b, b->bx64, b->gt64);
+    if (rc) return rc;
+    b->bx_dirty = false;
+    return NWG_OK;
+}
+
 // Validate and upload N pairs into the batch, (re)build the hard / score-only plan and mark
 // the soft plans stale.  Nothing of the batch changes when validation fails.  `keep`: grow
 // the device buffers with headroom and never shrink them (a reloaded batch).
@@ -585,7 +704,8 @@ int load_pairs(nwg_batch* b, int64_t N, const uint8_t* codes_a, const int64_t* o
     NWG_HIP(ctx, size(b->oa, (size_t)N + 1));
     NWG_HIP(ctx, size(b->ob, (size_t)N + 1));
     NWG_HIP(ctx, size(b->scores, (size_t)std::max<int64_t>(N, 1)));
-    if (b->prob.grad_mode != 0) NWG_HIP(ctx, size(b->grads, (size_t)std::max<int64_t>(N, 1) * (nn + 4)));
+    if (b->prob.grad_mode != 0)
+        NWG_HIP(ctx, size(b->grads, (size_t)std::max<int64_t>(N, 1) * (size_t)b->K * (nn + 4)));
     if (la) NWG_HIP(ctx, hipMemcpyAsync(b->ca.p, codes_a, (size_t)la, hipMemcpyHostToDevice, ctx->stream));
     if (lb) NWG_HIP(ctx, hipMemcpyAsync(b->cb.p, codes_b, (size_t)lb, hipMemcpyHostToDevice, ctx->stream));
     const int64_t zero[1] = {0};
@@ -607,6 +727,11 @@ int load_pairs(nwg_batch* b, int64_t N, const uint8_t* codes_a, const int64_t* o
     b->mlen.swap(mlen);
     b->nlen.swap(nlen);
     b->cells = cells;
+    b->hoa.assign(off_a ? off_a : zero, (off_a ? off_a : zero) + (N ? N + 1 : 1));
+    if (b->tracked) {   // new pairs: the default tracks (set 0) until the caller gives others
+        int rc = set_tracks_host(b, nullptr, nullptr);
+        if (rc) return rc;
+    }
     b->results_valid = false;
     b->soft_reruns = 0;
     for (SoftPlan* sp : {&b->sp_main, &b->sp_wide, &b->sq_main, &b->sq_wide}) sp->built = false;
@@ -682,23 +807,7 @@ int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t
     b->prob = *prob;
     b->na = n_alpha;
     b->N = N;
-    const int nn = n_alpha * n_alpha;
-    // The hard walk keeps uint16 match counts per pair slot in LDS, the soft pass double
-    // expected counts: the slot count, hence the narrowest lane group, follows the budget.
-    if (prob->grad_mode == 2) {
-        // soft: any width, floor(64 / G) pair slots of nn doubles
-        const int slots = std::max(1, std::min(nwg::kTilePairs, kSoftHistBytes / (n_alpha * (n_alpha + 1) * 8)));
-        int gmin = 1;
-        while (64 / gmin > slots) ++gmin;
-        b->gmin = gmin;
-        b->hist_slots = slots;
-    } else {
-        const int hist_entry = prob->grad_mode == 1 ? 2 : 0;
-        int gmin = 1;
-        while (gmin < kMaxG && (64 / gmin) * nn * hist_entry > kHistBytes) gmin <<= 1;
-        b->gmin = gmin;
-        b->hist_slots = 64 / gmin;
-    }
+    set_slots(b.get());
     NWG_HIP(ctx, b->queue.alloc(4));
     const int rc = load_pairs(b.get(), N, codes_a, off_a, codes_b, off_b, false);
     if (rc) return rc;
@@ -733,27 +842,89 @@ int nwg_batch_query(nwg_batch* b, int* done) {
 }
 
 int nwg_batch_set_params(nwg_batch* b, const double* matrix, const double* gaps4) {
-    if (!b || !matrix || !gaps4) return NWG_EINVAL;
+    return nwg_batch_set_params_k(b, 1, matrix, gaps4);
+}
+
+int nwg_batch_set_params_k(nwg_batch* b, int K, const double* matrices, const double* gaps) {
+    if (!b || !matrices || !gaps) return NWG_EINVAL;
     nwg_ctx* ctx = b->ctx;
+    if (K < 1 || K > 4096) return fail(ctx, NWG_EINVAL, "nwgrad: the number of parameter sets must be in [1, 4096]");
+    if (K > 1 && b->prob.grad_mode == 2)
+        return fail(ctx, NWG_EINVAL,
+                    "nwgrad_amd: several parameter sets (tracks) are not available with grad_mode=\"soft\" on the "
+                    "GPU path; the hard and score-only paths take them");
     const int nn = b->na * b->na;
-    b->matrix.assign(matrix, matrix + nn);
-    std::memcpy(b->gaps, gaps4, sizeof b->gaps);
-    NWG_HIP(ctx, hipSetDevice(ctx->device));
-    if (b->prob.precision == 0) {
-        std::vector<float> s((size_t)nn);
-        for (int k = 0; k < nn; ++k) s[(size_t)k] = static_cast<float>(matrix[k]);
-        NWG_HIP(ctx, b->sub_f32.reserve((size_t)nn));
-        NWG_HIP(ctx, hipMemcpy(b->sub_f32.p, s.data(), sizeof(float) * (size_t)nn, hipMemcpyHostToDevice));
-    } else {
-        NWG_HIP(ctx, b->sub_f64.reserve((size_t)nn));
-        NWG_HIP(ctx, hipMemcpy(b->sub_f64.p, matrix, sizeof(double) * (size_t)nn, hipMemcpyHostToDevice));
+    if (b->tracked) {   // the tracks must stay within the new sets
+        for (uint16_t v : b->htrk)
+            if (v >= K) return fail(ctx, NWG_EINVAL, "nwgrad: a track names set %d; params has %d sets", (int)v, K);
+        for (uint16_t v : b->hgtrk)
+            if (v >= K) return fail(ctx, NWG_EINVAL, "nwgrad: a gap track names set %d; params has %d sets", (int)v, K);
     }
-    NWG_HIP(ctx, b->theta.reserve((size_t)nn + 4));
-    std::vector<double> th(b->matrix);
-    th.insert(th.end(), gaps4, gaps4 + 4);
-    NWG_HIP(ctx, hipMemcpy(b->theta.p, th.data(), sizeof(double) * th.size(), hipMemcpyHostToDevice));
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    const int oldK = b->K;
+    b->matrix.assign(matrices, matrices + (size_t)K * nn);
+    b->gaps.assign(gaps, gaps + (size_t)4 * K);
+    b->K = K;
+    const size_t kn = (size_t)K * nn;
+    if (b->prob.precision == 0) {
+        std::vector<float> sv(kn);
+        for (size_t k = 0; k < kn; ++k) sv[k] = static_cast<float>(matrices[k]);
+        NWG_HIP(ctx, b->sub_f32.reserve(kn));
+        NWG_HIP(ctx, hipMemcpy(b->sub_f32.p, sv.data(), sizeof(float) * kn, hipMemcpyHostToDevice));
+    } else {
+        NWG_HIP(ctx, b->sub_f64.reserve(kn));
+        NWG_HIP(ctx, hipMemcpy(b->sub_f64.p, matrices, sizeof(double) * kn, hipMemcpyHostToDevice));
+    }
+    if (K == 1) {
+        NWG_HIP(ctx, b->theta.reserve((size_t)nn + 4));
+        std::vector<double> th(b->matrix);
+        th.insert(th.end(), gaps, gaps + 4);
+        NWG_HIP(ctx, hipMemcpy(b->theta.p, th.data(), sizeof(double) * th.size(), hipMemcpyHostToDevice));
+    }
+    if (K != oldK && b->prob.grad_mode != 0) {
+        NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        NWG_HIP(ctx, b->grads.alloc((size_t)std::max<int64_t>(b->N, 1) * kn + (size_t)std::max<int64_t>(b->N, 1) * 4 * K));
+    }
+    if (K > 1 && !b->tracked) {   // several sets make the problem tracked (default tracks)
+        const int rc = set_tracks_host(b, nullptr, nullptr);
+        if (rc) return rc;
+    }
+    b->bx_dirty = true;
+    if (set_slots(b) && b->prob.grad_mode != 2) {
+        std::vector<int32_t> all((size_t)b->N);
+        std::iota(all.begin(), all.end(), 0);
+        const int rc = build_plans(b, all, b->plan, b->lpl);
+        if (rc) return rc;
+    }
     b->have_params = true;
     b->results_valid = false;
+    return NWG_OK;
+}
+
+int nwg_batch_set_tracks(nwg_batch* b, const uint16_t* tracks, const uint16_t* gap_tracks) {
+    if (!b) return NWG_EINVAL;
+    nwg_ctx* ctx = b->ctx;
+    if (b->prob.grad_mode == 2)
+        return fail(ctx, NWG_EINVAL,
+                    "nwgrad_amd: tracks are not available with grad_mode=\"soft\" on the GPU path; the hard and "
+                    "score-only paths take them");
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    const int rc = set_tracks_host(b, tracks, gap_tracks);
+    if (rc) return rc;
+    if (set_slots(b)) {
+        std::vector<int32_t> all((size_t)b->N);
+        std::iota(all.begin(), all.end(), 0);
+        const int rc2 = build_plans(b, all, b->plan, b->lpl);
+        if (rc2) return rc2;
+    }
+    b->results_valid = false;
+    return NWG_OK;
+}
+
+int nwg_batch_num_sets(nwg_batch* b, int* K, int* tracked) {
+    if (!b) return NWG_EINVAL;
+    if (K) *K = b->K;
+    if (tracked) *tracked = b->tracked ? 1 : 0;
     return NWG_OK;
 }
 
@@ -797,15 +968,41 @@ int nwg_batch_set_soft(nwg_batch* b, double temperature, int precision) {
 namespace {
 
 // ── hard and score-only fills ───────────────────────────────────────────────
+template <typename T> struct TrackBufs;
+template <> struct TrackBufs<float> {
+    static const float* bx(nwg_batch* b) { return b->bx32.p; }
+    static const float* gt(nwg_batch* b) { return b->gt32.p; }
+};
+template <> struct TrackBufs<double> {
+    static const double* bx(nwg_batch* b) { return b->bx64.p; }
+    static const double* gt(nwg_batch* b) { return b->gt64.p; }
+};
+template <typename T>
+nwg::TrackArgs<T> track_args(nwg_batch* b) {
+    nwg::TrackArgs<T> t{};
+    if (b->tracked) {
+        t.trk = b->dtrk.p;
+        t.gtrk = b->dgtrk.p;
+        t.bx0 = TrackBufs<T>::bx(b);
+        t.gaps = TrackBufs<T>::gt(b);
+        t.K = b->K;
+    }
+    return t;
+}
+
 template <typename T>
 int launch_hard(nwg_batch* b, const Plan& pl) {
     nwg_ctx* ctx = b->ctx;
-    const bool hard = b->prob.grad_mode == 1;
-    const int nn = b->na * b->na;
-    const size_t lds = (size_t)((nn * (int)sizeof(T) + 15) & ~15) + (hard ? (size_t)b->hist_slots * nn * 2 : 0);
+    const bool hard = b->prob.grad_mode == 1, tr = b->tracked;
+    const size_t lds = (size_t)nwg::hard_lds_sub_bytes<T>(b->na, b->K, tr) +
+                       (hard ? (size_t)nwg::hard_lds_hist_bytes(b->na, b->K, tr, b->hist_slots) : 0);
+    if (lds > 65536)
+        return fail(ctx, NWG_EINVAL, "nwgrad_amd: %d parameter sets of %d x %d do not fit the fill's local memory",
+                    b->K, b->na, b->na);
     const int gm = b->prob.gap_model, am = b->prob.mode;
-    auto pick = [&](auto gmc, auto amc, auto hardc) -> void (*)(nwg::HardArgs<T>) {
-        return nwg::hard_fill_kernel<T, decltype(gmc)::value, decltype(amc)::value, kS, decltype(hardc)::value>;
+    auto pick = [&](auto gmc, auto amc, auto hardc, auto trc) -> void (*)(nwg::HardArgs<T>) {
+        return nwg::hard_fill_kernel<T, decltype(gmc)::value, decltype(amc)::value, kS, decltype(hardc)::value,
+                                     decltype(trc)::value>;
     };
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
@@ -813,8 +1010,10 @@ int launch_hard(nwg_batch* b, const Plan& pl) {
     using BT = std::true_type;
     using BF = std::false_type;
     void (*kern)(nwg::HardArgs<T>) = nullptr;
-#define NWG_PICK(GMV, AMV) \
-    if (gm == GMV::value && am == AMV::value) kern = hard ? pick(GMV{}, AMV{}, BT{}) : pick(GMV{}, AMV{}, BF{});
+#define NWG_PICK(GMV, AMV)                                                                      \
+    if (gm == GMV::value && am == AMV::value)                                                   \
+        kern = hard ? (tr ? pick(GMV{}, AMV{}, BT{}, BT{}) : pick(GMV{}, AMV{}, BT{}, BF{}))     \
+                    : (tr ? pick(GMV{}, AMV{}, BF{}, BT{}) : pick(GMV{}, AMV{}, BF{}, BF{}));
     NWG_PICK(I0, I0) NWG_PICK(I0, I1) NWG_PICK(I0, I2) NWG_PICK(I1, I0) NWG_PICK(I1, I1) NWG_PICK(I1, I2)
 #undef NWG_PICK
     if (!kern) return fail(ctx, NWG_EINVAL, "nwgrad_hip: unsupported problem type");
@@ -841,6 +1040,7 @@ int launch_hard(nwg_batch* b, const Plan& pl) {
     A.ge_a = static_cast<T>(b->gaps[1]);
     A.go_b = static_cast<T>(b->gaps[2]);
     A.ge_b = static_cast<T>(b->gaps[3]);
+    A.tr = track_args<T>(b);
     A.scores = b->scores.p;
     A.grads = b->grads.p;
     A.table = b->table.p;
@@ -876,16 +1076,20 @@ template <typename T>
 int launch_long(nwg_batch* b) {
     nwg_ctx* ctx = b->ctx;
     LongPlan& L = b->lpl;
-    const bool hard = b->prob.grad_mode == 1;
-    const int nn = b->na * b->na;
+    const bool hard = b->prob.grad_mode == 1, tr = b->tracked;
     const int gm = b->prob.gap_model, am = b->prob.mode;
     using FillK = void (*)(nwg::LongArgs<T>);
     FillK fill = nullptr, walk = nullptr;
     auto pick = [&](auto gmc, auto amc, auto hardc) {
         constexpr int G = decltype(gmc)::value, AM = decltype(amc)::value;
         constexpr bool H = decltype(hardc)::value;
-        fill = nwg::long_fill_kernel<T, G, AM, kS, H, kRowBlock>;
-        walk = nwg::long_walk_kernel<T, G, AM, kS, H>;
+        if (tr) {
+            fill = nwg::long_fill_kernel<T, G, AM, kS, H, kRowBlock, true>;
+            walk = nwg::long_walk_kernel<T, G, AM, kS, H, true>;
+        } else {
+            fill = nwg::long_fill_kernel<T, G, AM, kS, H, kRowBlock, false>;
+            walk = nwg::long_walk_kernel<T, G, AM, kS, H, false>;
+        }
     };
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
@@ -897,9 +1101,13 @@ int launch_long(nwg_batch* b) {
     NWG_PICK(I0, I0) NWG_PICK(I0, I1) NWG_PICK(I0, I2) NWG_PICK(I1, I0) NWG_PICK(I1, I1) NWG_PICK(I1, I2)
 #undef NWG_PICK
     if (!fill) return fail(ctx, NWG_EINVAL, "nwgrad_hip: unsupported problem type");
-    const size_t lds_fill = (size_t)((nn * (int)sizeof(T) + 15) & ~15);
-    const size_t lds_walk =
-        hard ? (size_t)((nn * 4 + 15) & ~15) + (size_t)nwg::kWalkSteps * (kS / 4) * kMaxG * 4 : 16;
+    const size_t lds_fill = (size_t)nwg::hard_lds_sub_bytes<T>(b->na, b->K, tr);
+    const size_t lds_walk = hard ? (size_t)nwg::long_walk_count_bytes(b->na, b->K, tr) +
+                                       (size_t)nwg::kWalkSteps * (kS / 4) * kMaxG * 4
+                                 : 16;
+    if (lds_fill > 65536 || lds_walk > 65536)
+        return fail(ctx, NWG_EINVAL, "nwgrad_amd: %d parameter sets of %d x %d do not fit the long-pair kernels' "
+                    "local memory", b->K, b->na, b->na);
     DevPtr<T>& bnd = LongBnd<T>::get(b);
     const bool grow = b->lprog.n < (size_t)L.max_items || b->lcand.n < (size_t)L.max_items * sizeof(nwg::CandRec<T>) ||
                       bnd.n < (size_t)std::max<int64_t>(L.max_bnd, 1) ||
@@ -927,6 +1135,7 @@ int launch_long(nwg_batch* b) {
     A.ge_a = static_cast<T>(b->gaps[1]);
     A.go_b = static_cast<T>(b->gaps[2]);
     A.ge_b = static_cast<T>(b->gaps[3]);
+    A.tr = track_args<T>(b);
     A.progress = b->lprog.p;
     A.bnd = bnd.p;
     A.cand = reinterpret_cast<nwg::CandRec<T>*>(b->lcand.p);
@@ -1177,6 +1386,8 @@ int run_async(nwg_batch* b) {
         rc = launch_soft_batch(b);
         if (rc) return rc;
     } else {
+        rc = prepare_tracks(b);
+        if (rc) return rc;
         if (!b->plan.tiles.empty()) {
             rc = b->prob.precision == 0 ? launch_hard<float>(b, b->plan) : launch_hard<double>(b, b->plan);
             if (rc) return rc;
@@ -1193,7 +1404,7 @@ int run_async(nwg_batch* b) {
 
 int weighted_dev(nwg_batch* b, const double* w_dev, double* out_dev) {
     nwg_ctx* ctx = b->ctx;
-    const int gs = b->na * b->na + 4;
+    const int gs = b->K * (b->na * b->na + 4);
     const int64_t nblocks = (b->N + kSumBlock - 1) / kSumBlock;
     NWG_HIP(ctx, b->partial.reserve((size_t)(nblocks * gs)));
     const int threads = 64;
@@ -1286,15 +1497,17 @@ int nwg_batch_grads(nwg_batch* b, double* mats, double* gaps) {
     if (b->prob.grad_mode == 0) return fail(ctx, NWG_ESTATE, "nwgrad: grad_mode \"none\" computes no gradient");
     if (!b->results_valid) return fail(ctx, NWG_ESTATE, "nwgrad: gradient not valid; compute an alignment first");
     NWG_HIP(ctx, hipSetDevice(ctx->device));
-    const int nn = b->na * b->na, gs = nn + 4;
+    // per pair: K matrices (mats[(i*K + k)*nn ..]) and K gap rows (gaps[(i*K + k)*4 ..]),
+    // the reference's grads_into layout (batch_engine.hpp:719-746)
+    const int K = b->K, nn = b->na * b->na, knn = K * nn, gs = K * (nn + 4);
     if (b->N == 0) return NWG_OK;
     std::vector<double> tmp((size_t)b->N * gs);
     NWG_HIP(ctx, hipMemcpyAsync(tmp.data(), b->grads.p, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost,
                                 ctx->stream));
     NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (int64_t i = 0; i < b->N; ++i) {
-        std::memcpy(mats + i * nn, tmp.data() + i * gs, sizeof(double) * (size_t)nn);
-        std::memcpy(gaps + i * 4, tmp.data() + i * gs + nn, sizeof(double) * 4);
+        std::memcpy(mats + i * knn, tmp.data() + i * gs, sizeof(double) * (size_t)knn);
+        std::memcpy(gaps + i * 4 * K, tmp.data() + i * gs + knn, sizeof(double) * 4 * (size_t)K);
     }
     return NWG_OK;
 }
@@ -1310,7 +1523,7 @@ int nwg_batch_weighted_grad(nwg_batch* b, const double* w, int64_t n, double* ou
     if (b->prob.grad_mode == 0) return fail(ctx, NWG_ESTATE, "nwgrad: grad_mode \"none\" computes no gradient");
     if (!b->results_valid) return fail(ctx, NWG_ESTATE, "nwgrad: gradient not valid; compute an alignment first");
     NWG_HIP(ctx, hipSetDevice(ctx->device));
-    const int gs = b->na * b->na + 4;
+    const int gs = b->K * (b->na * b->na + 4);
     const double* wd = nullptr;
     if (w) {
         NWG_HIP(ctx, b->wdev.reserve((size_t)b->N));

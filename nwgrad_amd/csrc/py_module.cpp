@@ -140,6 +140,87 @@ py::array_t<double> mat_to_numpy(const SubstMatrix& m) {
     return out;
 }
 
+// A tracks argument (py_exports.cpp:187-250): None; one sequence per pair, an entry None for
+// that pair's default; or one flat 1-D integer array, every pair's entries concatenated.
+// Pair p takes lens[p] entries.  Returns the concatenation (the C ABI's layout) and, per pair,
+// whether it was given; `given` is false for None.
+std::vector<uint16_t> flat_tracks(const py::object& obj, const std::vector<int64_t>& lens, const char* what,
+                                  bool& given, std::vector<char>& pair_given) {
+    const size_t n = lens.size();
+    int64_t total = 0;
+    for (int64_t l : lens) total += l;
+    std::vector<uint16_t> out(static_cast<size_t>(total), 0);
+    pair_given.assign(n, 0);
+    given = !obj.is_none();
+    if (!given) return out;
+    auto put = [&](const py::handle& h, int64_t at, int64_t len, size_t p) {
+        auto arr = py::array_t<int64_t, py::array::c_style | py::array::forcecast>::ensure(h);
+        if (!arr || arr.ndim() != 1)
+            throw py::type_error(std::string("nwgrad: ") + what + " entries must be 1-D integer sequences");
+        if (arr.shape(0) != len)
+            throw py::value_error(std::string("nwgrad: ") + what + "[" + std::to_string(p) + "] has " +
+                                  std::to_string(arr.shape(0)) + " entries, expected " + std::to_string(len));
+        for (int64_t k = 0; k < len; ++k) {
+            const int64_t v = arr.data()[k];
+            if (v < 0 || v > 65535)
+                throw py::value_error(std::string("nwgrad: ") + what + "[" + std::to_string(k) + "] = " +
+                                      std::to_string(v) + " is not a parameter set index");
+            out[static_cast<size_t>(at + k)] = static_cast<uint16_t>(v);
+        }
+    };
+    if (py::isinstance<py::array>(obj)) {
+        py::array a = obj.cast<py::array>();
+        if (a.ndim() == 1 && a.dtype().kind() != 'O') {
+            if (a.shape(0) != total)
+                throw py::value_error(std::string("nwgrad: flat ") + what + " has " + std::to_string(a.shape(0)) +
+                                      " entries, expected " + std::to_string(total) +
+                                      " (every pair's entries, concatenated)");
+            put(obj, 0, total, 0);
+            pair_given.assign(n, 1);
+            return out;
+        }
+    }
+    const py::sequence seq = obj.cast<py::sequence>();
+    if (static_cast<size_t>(py::len(seq)) != n)
+        throw py::value_error(std::string("nwgrad: ") + what + " needs one entry per pair (got " +
+                              std::to_string(py::len(seq)) + " for " + std::to_string(n) + " pairs)");
+    int64_t at = 0;
+    for (size_t p = 0; p < n; ++p) {
+        const py::object e = seq[p];
+        if (!e.is_none()) {
+            put(e, at, lens[p], p);
+            pair_given[p] = 1;
+        }
+        at += lens[p];
+    }
+    return out;
+}
+
+// Apply tracks / gap_tracks to a batch of pairs of lengths la (len(A) per pair).
+template <class T>
+void apply_tracks(SeqPairBatchT<T>& b, const std::vector<int64_t>& la, const py::object& tracks,
+                  const py::object& gap_tracks) {
+    if (tracks.is_none() && gap_tracks.is_none()) return;
+    std::vector<int64_t> lg(la);
+    for (auto& v : lg) v += 1;
+    bool tg = false, gg = false;
+    std::vector<char> tpg, gpg;
+    std::vector<uint16_t> t = flat_tracks(tracks, la, "tracks", tg, tpg);
+    std::vector<uint16_t> g = flat_tracks(gap_tracks, lg, "gap_tracks", gg, gpg);
+    if (gg) {   // a pair whose gap track is None takes the default: row 0 the first residue's set
+        int64_t at = 0, ag = 0;
+        for (size_t p = 0; p < la.size(); ++p) {
+            if (!gpg[p]) {
+                g[static_cast<size_t>(ag)] = la[p] ? t[static_cast<size_t>(at)] : 0;
+                for (int64_t i = 1; i <= la[p]; ++i) g[static_cast<size_t>(ag + i)] = t[static_cast<size_t>(at + i - 1)];
+            }
+            at += la[p];
+            ag += la[p] + 1;
+        }
+    }
+    b.set_tracks(t.data(), gg ? g.data() : nullptr);
+}
+
 std::vector<std::string_view> views(const std::vector<std::string>& v) {
     return std::vector<std::string_view>(v.begin(), v.end());
 }
@@ -174,8 +255,6 @@ void bind_batch(py::module_& m, const char* name) {
                  if (kernel != "auto" && kernel != "gpu" && kernel != "gfx950")
                      throw py::value_error("nwgrad_amd: kernel \"" + kernel + "\": this build's only backend is the "
                                            "gfx950 GPU (\"auto\")");
-                 if (!tracks.is_none() || !gap_tracks.is_none())
-                     throw py::value_error("nwgrad_amd: tracks / gap_tracks are not part of this build's GPU path");
                  if (!free_ends.is_none())
                      throw py::value_error("nwgrad_amd: free_ends (semi-global) are not part of this build's GPU path");
                  auto pb = std::make_unique<PB>();
@@ -184,6 +263,9 @@ void bind_batch(py::module_& m, const char* name) {
                      pb->b = std::make_unique<SeqPairBatchT<T>>(views(seqs_a), views(seqs_b), params, gm, am, gd,
                                                                 n_threads, device);
                  }
+                 std::vector<int64_t> la(seqs_a.size());
+                 for (size_t i = 0; i < seqs_a.size(); ++i) la[i] = static_cast<int64_t>(seqs_a[i].size());
+                 apply_tracks(*pb->b, la, tracks, gap_tracks);
                  pb->o.traceback = traceback;
                  return pb;
              }),
@@ -200,7 +282,8 @@ void bind_batch(py::module_& m, const char* name) {
                py::array_t<int64_t, py::array::c_style | py::array::forcecast> off_a,
                py::array_t<uint8_t, py::array::c_style | py::array::forcecast> codes_b,
                py::array_t<int64_t, py::array::c_style | py::array::forcecast> off_b, const AlignParams& params,
-               const std::string& gap_model, const std::string& mode, const std::string& grad_mode, int device) {
+               const std::string& gap_model, const std::string& mode, const std::string& grad_mode, int device,
+               py::object tracks, py::object gap_tracks) {
                 const GapModel gm = parse_gap_model(gap_model);
                 const AlignMode am = parse_mode(mode);
                 const GradMode gd = parse_grad_mode(grad_mode);
@@ -217,11 +300,14 @@ void bind_batch(py::module_& m, const char* name) {
                     pb->b = std::make_unique<SeqPairBatchT<T>>(codes_a.data(), oa, codes_b.data(), ob, N, params, gm,
                                                                am, gd, device);
                 }
+                std::vector<int64_t> la(static_cast<size_t>(N));
+                for (int64_t i = 0; i < N; ++i) la[static_cast<size_t>(i)] = oa[i + 1] - oa[i];
+                apply_tracks(*pb->b, la, tracks, gap_tracks);
                 return pb;
             },
             py::arg("codes_a"), py::arg("off_a"), py::arg("codes_b"), py::arg("off_b"), py::arg("params"),
             py::kw_only(), py::arg("gap_model") = "affine", py::arg("mode") = "global", py::arg("grad_mode") = "hard",
-            py::arg("device") = 0,
+            py::arg("device") = 0, py::arg("tracks") = py::none(), py::arg("gap_tracks") = py::none(),
             "A batch from pre-encoded pairs: uint8 codes in alphabet order and int64 offsets (N+1), pair i\n"
             "being codes[off[i]:off[i+1]].")
         .def("__len__", [](const PB& s) { return s.b->size(); })
@@ -265,10 +351,14 @@ void bind_batch(py::module_& m, const char* name) {
             },
             py::arg("weights"))
         .def("grads", [](const PB& s) {
-            const size_t N = s.b->size();
+            // (N, n, n) and (N, 4); tracked: (N, K, n, n) and (N, K, 4) (py_exports.cpp:1003-1030)
+            const py::ssize_t N = static_cast<py::ssize_t>(s.b->size());
             const py::ssize_t n = s.b->alphabet().size();
-            py::array_t<double> mats({static_cast<py::ssize_t>(N), n, n});
-            py::array_t<double> gaps({static_cast<py::ssize_t>(N), static_cast<py::ssize_t>(4)});
+            const bool tr = s.b->tracked();
+            const py::ssize_t K = s.b->num_sets();
+            py::array_t<double> mats = tr ? py::array_t<double>({N, K, n, n}) : py::array_t<double>({N, n, n});
+            py::array_t<double> gaps = tr ? py::array_t<double>({N, K, static_cast<py::ssize_t>(4)})
+                                          : py::array_t<double>({N, static_cast<py::ssize_t>(4)});
             {
                 double* mp = mats.mutable_data();
                 double* gp = gaps.mutable_data();
@@ -277,6 +367,9 @@ void bind_batch(py::module_& m, const char* name) {
             }
             return py::make_tuple(mats, gaps);
         })
+        .def_property_readonly("tracked", [](const PB& s) { return s.b->tracked(); },
+                               "Whether the pairs carry tracks (tracks=... or K > 1 parameter sets).")
+        .def_property_readonly("num_sets", [](const PB& s) { return s.b->num_sets(); })
         .def_property(
             "fill", [](const PB& s) { return s.b->fill(); }, [](PB& s, const std::string& v) { s.b->set_fill(v); },
             "\"interpair\" (default): one persistent launch pulling length-sorted tiles from a device work\n"
@@ -315,8 +408,6 @@ void bind_batch(py::module_& m, const char* name) {
         .def_property_readonly("mode", [](const PB& s) { return am_name(s.b->align_mode()); })
         .def_property_readonly("grad_mode", [](const PB& s) { return gd_name(s.b->grad_mode()); })
         .def_property_readonly("traceback", [](const PB& s) { return s.o.traceback; })
-        .def_property_readonly("tracked", [](const PB&) { return false; })
-        .def_property_readonly("num_sets", [](const PB& s) { return s.b->size() ? 1 : 0; })
         .def_property_readonly("cells", [](const PB& s) { return s.b->cells(); },
                                "Sum of len(A)*len(B) over the pairs (the GCUPS numerator).")
         .def_property_readonly("tiles", [](const PB& s) { return s.b->tiles(); })
@@ -376,20 +467,85 @@ PYBIND11_MODULE(_nwgrad_amd, m) {
              }),
              py::arg("matrix"), py::arg("gap_open_a") = 0.0, py::arg("gap_extend_a") = 0.0,
              py::arg("gap_open_b") = 0.0, py::arg("gap_extend_b") = 0.0)
+        .def_static(
+            "from_arrays",
+            [](py::array_t<double, py::array::c_style | py::array::forcecast> mats,
+               py::array_t<double, py::array::c_style | py::array::forcecast> gaps, const std::string& alphabet) {
+                if (mats.ndim() != 3 || mats.shape(1) != mats.shape(2))
+                    throw std::invalid_argument("nwgrad: matrices must have shape (K, N, N)");
+                const py::ssize_t K = mats.shape(0), n = mats.shape(1);
+                if (static_cast<py::ssize_t>(alphabet.size()) != n)
+                    throw std::invalid_argument("nwgrad: alphabet length must match matrix size");
+                if (gaps.ndim() != 2 || gaps.shape(0) != K || gaps.shape(1) != 4)
+                    throw std::invalid_argument("nwgrad: gaps must have shape (K, 4) -- gap_open_a, gap_extend_a, "
+                                                "gap_open_b, gap_extend_b per set -- for K = " + std::to_string(K) +
+                                                " matrices");
+                if (K < 1 || K > 4096)
+                    throw std::invalid_argument("nwgrad: the number of parameter sets must be in [1, 4096]");
+                return AlignParams(SubstMatrix(mats.data(), Alphabet::get(alphabet), static_cast<int>(K)), gaps.data());
+            },
+            py::arg("matrices"), py::arg("gaps"), py::arg("alphabet") = std::string(kAAOrder),
+            "K parameter sets from arrays: matrices (K, N, N) and gaps (K, 4).  Set k is what a track\n"
+            "entry k selects (SeqPairBatch(..., tracks=...)).")
+        .def_static("stack", &AlignParams::stack, py::arg("sets"),
+                    "Stack params over one alphabet into one params with their sets in order.")
         .def_property(
-            "matrix", [](const AlignParams& p) { return p.matrix; },
+            "matrices",
+            [](const AlignParams& p) {
+                const py::ssize_t K = p.num_sets(), n = p.matrix.size();
+                py::array_t<double> out({K, n, n});
+                std::memcpy(out.mutable_data(), p.matrix.data(), sizeof(double) * static_cast<size_t>(K * n * n));
+                return out;
+            },
+            [](AlignParams& p, py::array_t<double, py::array::c_style | py::array::forcecast> mats) {
+                const py::ssize_t K = p.num_sets(), n = p.matrix.size();
+                if (mats.ndim() != 3 || mats.shape(0) != K || mats.shape(1) != n || mats.shape(2) != n)
+                    throw std::invalid_argument("nwgrad: matrices must have shape (K, N, N) = (" + std::to_string(K) +
+                                                ", " + std::to_string(n) + ", " + std::to_string(n) + ")");
+                std::memcpy(p.matrix.data(), mats.data(), sizeof(double) * static_cast<size_t>(K * n * n));
+            })
+        .def_property(
+            "gaps",
+            [](const AlignParams& p) {
+                const std::vector<double> g = p.all_gaps();
+                py::array_t<double> out({static_cast<py::ssize_t>(p.num_sets()), static_cast<py::ssize_t>(4)});
+                std::memcpy(out.mutable_data(), g.data(), sizeof(double) * g.size());
+                return out;
+            },
+            [](AlignParams& p, py::array_t<double, py::array::c_style | py::array::forcecast> g) {
+                const py::ssize_t K = p.num_sets();
+                if (g.ndim() != 2 || g.shape(0) != K || g.shape(1) != 4)
+                    throw std::invalid_argument("nwgrad: gaps must have shape (K, 4) = (" + std::to_string(K) + ", 4)");
+                for (int k = 0; k < K; ++k)
+                    for (int f = 0; f < 4; ++f) p.gap_ref(k, f) = g.data()[k * 4 + f];
+            })
+        .def("__getitem__", [](const AlignParams& p, int64_t k) {
+            const int64_t K = p.num_sets();
+            if (k < 0) k += K;
+            if (k < 0 || k >= K) throw py::index_error("nwgrad: parameter set index out of range");
+            return p.set(static_cast<int>(k));
+        })
+        .def("__setitem__", [](AlignParams& p, int64_t k, const AlignParams& v) {
+            const int64_t K = p.num_sets();
+            if (k < 0) k += K;
+            if (k < 0 || k >= K) throw py::index_error("nwgrad: parameter set index out of range");
+            p.assign_set(static_cast<int>(k), v);
+        })
+        .def_property(
+            "matrix", [](const AlignParams& p) { return p.matrix.slice(0); },
             [](AlignParams& p, const SubstMatrix& mat) {
                 if (&mat.alphabet() != &p.matrix.alphabet())
                     throw std::invalid_argument("nwgrad: cannot assign a matrix over alphabet \"" +
                                                 mat.alphabet().symbols() + "\" to params over \"" +
                                                 p.matrix.alphabet().symbols() + "\"");
-                p.matrix = mat;
+                const size_t nn = static_cast<size_t>(mat.size()) * static_cast<size_t>(mat.size());
+                std::memcpy(p.matrix.data(), mat.data(), sizeof(double) * nn);   // set 0
             })
         .def_readwrite("gap_open_a", &AlignParams::gap_open_a)
         .def_readwrite("gap_extend_a", &AlignParams::gap_extend_a)
         .def_readwrite("gap_open_b", &AlignParams::gap_open_b)
         .def_readwrite("gap_extend_b", &AlignParams::gap_extend_b)
-        .def_property_readonly("num_sets", [](const AlignParams&) { return 1; })
+        .def_property_readonly("num_sets", &AlignParams::num_sets)
         .def_property_readonly("alphabet", [](const AlignParams& p) { return p.alphabet().symbols(); })
         .def("to_dict", [](const AlignParams& p) {
             py::dict d;
@@ -399,6 +555,18 @@ PYBIND11_MODULE(_nwgrad_amd, m) {
             d["gap_extend_a"] = p.gap_extend_a;
             d["gap_open_b"] = p.gap_open_b;
             d["gap_extend_b"] = p.gap_extend_b;
+            const int K = p.num_sets();
+            if (K > 1) {   // every set too (py_exports.cpp:1488-1510)
+                const py::ssize_t n = p.matrix.size();
+                d["num_sets"] = K;
+                py::array_t<double> mats({static_cast<py::ssize_t>(K), n, n});
+                std::memcpy(mats.mutable_data(), p.matrix.data(), sizeof(double) * static_cast<size_t>(K * n * n));
+                d["matrices"] = mats;
+                const std::vector<double> g = p.all_gaps();
+                py::array_t<double> gg({static_cast<py::ssize_t>(K), static_cast<py::ssize_t>(4)});
+                std::memcpy(gg.mutable_data(), g.data(), sizeof(double) * g.size());
+                d["gaps"] = gg;
+            }
             return d;
         })
         .def("__add__", [](const AlignParams& a, const AlignParams& b) { return a + b; })

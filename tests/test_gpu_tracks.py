@@ -1,0 +1,135 @@
+"""GPU parity of tracked batches (K parameter sets chosen per DP row): bit-exact against the
+reference engine's tracked golden vectors and the oracle, short and long (column-panel) pairs,
+through the C ABI."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from nwgrad_amd import workloads as wl
+from tests.gpu_util import say
+
+pytestmark = pytest.mark.gpu
+
+TRACKED = sorted(p for p in glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")) if "_tracks_" in p)
+
+
+def _params_k(alphabet, mats, gaps):
+    import nwgrad_amd as nw
+    return nw.AlignParams.from_arrays(np.asarray(mats, np.float64), np.asarray(gaps, np.float64), str(alphabet))
+
+
+def _batch(alphabet, mats, gaps, ca, oa, cb, ob, tracks, gap_tracks=None, *, precision="float32", gap_model="affine",
+           mode="global", grad_mode="hard", fill="interpair"):
+    import nwgrad_amd as nw
+    cls = nw.SeqPairBatch if precision == "float32" else nw.SeqPairBatchDouble
+    b = cls.from_codes(ca, oa, cb, ob, _params_k(alphabet, mats, gaps), gap_model=gap_model, mode=mode,
+                       grad_mode=grad_mode, tracks=tracks, gap_tracks=gap_tracks)
+    b.fill = fill
+    return b
+
+
+def _rows(b):
+    mats, gaps = b.grads()
+    N = len(b)
+    return np.concatenate([mats.reshape(N, -1), gaps.reshape(N, -1)], axis=1)
+
+
+def _flat(p):
+    d = p.to_dict()
+    if "matrices" in d:
+        return np.concatenate([d["matrices"].ravel(), d["gaps"].ravel()])
+    return np.concatenate([d["matrix"].ravel(), [d["gap_open_a"], d["gap_extend_a"], d["gap_open_b"],
+                                                  d["gap_extend_b"]]])
+
+
+@pytest.mark.parametrize("fill", ["interpair", "striped"])
+@pytest.mark.parametrize("path", TRACKED, ids=lambda p: os.path.basename(p)[:-4])
+def test_tracked_golden_bit_exact(gpu, path, fill):
+    z = np.load(path)
+    gt = z["gap_tracks"] if "gap_tracks" in z else None
+    say("case", os.path.basename(path), fill)
+    b = _batch(z["alphabet"], z["matrices"], z["gaps_k"], z["codes_a"], z["off_a"], z["codes_b"], z["off_b"],
+               z["tracks"], gt, precision=str(z["precision"]), gap_model=str(z["gap_model"]), mode=str(z["mode"]),
+               grad_mode=str(z["grad_mode"]), fill=fill)
+    assert b.tracked and b.num_sets == z["matrices"].shape[0]
+    total = b.score_and_grad()
+    np.testing.assert_array_equal(b.scores(), z["scores"])
+    assert total == float(z["total"])
+    if str(z["grad_mode"]) == "hard":
+        np.testing.assert_array_equal(_rows(b), z["grads"])
+        np.testing.assert_array_equal(_flat(b.compute_grad()), z["compute_grad"])
+        np.testing.assert_array_equal(_flat(b.weighted_grad(np.ascontiguousarray(z["weights"]))),
+                                      z["weighted_grad"])
+
+
+@pytest.mark.parametrize("gm", ["affine", "linear"])
+@pytest.mark.parametrize("mode", ["global", "local", "glocal"])
+def test_tracked_long_pairs(gpu, gm, mode):
+    """Tracked pairs wider than 1,024 columns (two to four column panels) next to short ones,
+    segment tracks over K = 4 sets, float32 and float64, against the oracle."""
+    r = wl.rng(f"trk_long_{gm}_{mode}", "seqs")
+    lens_b = [1100, 2049, 3500, 40, 700, 1025]
+    A = [r.integers(0, 20, size=int(r.integers(0, 500))).astype(np.uint8) for _ in lens_b]
+    B = [r.integers(0, 20, size=n).astype(np.uint8) for n in lens_b]
+    ca, oa = wl._pack(A)
+    cb, ob = wl._pack(B)
+    K = 4
+    mats = np.stack([wl.random_matrix(r) for _ in range(K)])
+    gaps = np.array([[10.5, 0.75, 9.25, 1.5], [8.0, 1.25, 11.0, 0.5], [12.0, 2.0, 6.5, 1.5], [9.0, 1.0, 9.0, 1.0]])
+    if gm == "linear":
+        gaps = gaps * np.array([0.0, 1.0, 0.0, 1.0])
+    tracks = np.repeat(r.integers(0, K, size=int(oa[-1]) // 7 + 1), 7)[:int(oa[-1])].astype(np.uint16)
+    for precision in ("float32", "float64"):
+        say("case tracked long", gm, mode, precision)
+        s_ref, g_ref = oracle.hard_tracked(mats, gaps, ca, oa, cb, ob, tracks, gap_model=gm, mode=mode,
+                                           precision=precision)
+        b = _batch(wl.AA, mats, gaps, ca, oa, cb, ob, tracks, precision=precision, gap_model=gm, mode=mode)
+        b.score_and_grad()
+        np.testing.assert_array_equal(b.scores(), s_ref)
+        np.testing.assert_array_equal(_rows(b), g_ref)
+        b = _batch(wl.AA, mats, gaps, ca, oa, cb, ob, tracks, precision=precision, gap_model=gm, mode=mode,
+                   grad_mode="none")
+        b.score_and_grad()
+        np.testing.assert_array_equal(b.scores(), s_ref)
+
+
+def test_equal_sets_match_untracked(gpu):
+    """K copies of one set under random tracks: the untracked scores, and per-set counts that
+    sum to the untracked counts."""
+    from tests.gpu_util import batch, grads_rows
+    w = wl.random_pairs("trk_equal_gpu", 200, wl.AA, (0, 300), (0, 1300), related=0.5)
+    K = 3
+    tracks = wl.rng(w.name, "t").integers(0, K, size=int(w.off_a[-1])).astype(np.uint16)
+    b0 = batch(w.alphabet, w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b)
+    b0.score_and_grad()
+    b1 = _batch(w.alphabet, np.stack([w.matrix] * K), np.stack([w.gaps] * K), w.codes_a, w.off_a, w.codes_b,
+                w.off_b, tracks)
+    b1.score_and_grad()
+    np.testing.assert_array_equal(b1.scores(), b0.scores())
+    mats, gaps = b1.grads()
+    summed = np.concatenate([mats.sum(axis=1).reshape(len(b1), -1), gaps.sum(axis=1)], axis=1)
+    np.testing.assert_array_equal(summed, grads_rows(b0))
+
+
+def test_track_errors(gpu):
+    w = wl.random_pairs("trk_err", 5, wl.AA, (5, 30), (5, 30))
+    mats = np.stack([w.matrix] * 2)
+    gaps = np.stack([w.gaps] * 2)
+    bad = np.full(int(w.off_a[-1]), 2, np.uint16)       # set 2 of 2
+    with pytest.raises(ValueError):
+        _batch(w.alphabet, mats, gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, bad)
+    with pytest.raises(ValueError):                     # wrong length
+        _batch(w.alphabet, mats, gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, np.zeros(3, np.uint16))
+    with pytest.raises(ValueError):                     # soft batches take one set, untracked
+        _batch(w.alphabet, mats, gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, bad * 0, grad_mode="soft")
+    ok = np.zeros(int(w.off_a[-1]), np.uint16)
+    b = _batch(w.alphabet, mats, gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, ok)
+    import nwgrad_amd as nw
+    with pytest.raises(ValueError):   # params with fewer sets than a track names
+        b2 = _batch(w.alphabet, mats, gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, ok + 1)
+        b2.set_params(nw.AlignParams(w.matrix, alphabet=w.alphabet))
+    b.score_and_grad()
+    assert b.grads()[0].shape == (5, 2, 20, 20)
