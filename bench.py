@@ -35,7 +35,8 @@ HBM_PEAK_GBS = 8000.0           # MI355X HBM3E
 SOFT_BYTES_PER_CELL = 24
 
 WORKLOADS = ("protein_soft_homologs", "dna_local_short", "dna_local_short_score", "protein_global_hard",
-             "long_hard_36000", "long_hard_36000_double", "long_hard_4096", "long_soft_8000", "serving")
+             "long_hard_36000", "long_hard_36000_double", "long_hard_4096", "long_soft_8000", "serving",
+             "protein_soft_train")
 
 
 def measured_traffic(workload, fill):
@@ -187,6 +188,56 @@ def serving_leg(duration, rate=200.0, warm=3.0):
     return res
 
 
+def training_leg(steps, warmup, lr=1e-3, margin=20.0):
+    """The contrastive training loop on protein_soft_homologs: per step the soft pass
+    (torch.autograd.Function over the HIP kernel), the margin loss of homolog against decoy
+    scores, the adjoint (weighted-gradient reduction) and an SGD update of the parameter
+    vector, captured once as a HIP graph and replayed `steps` times.  Returns (seconds,
+    cells per step, final loss, pairs rerun in double)."""
+    import torch
+    import nwgrad_amd as nw
+    from nwgrad_amd import autograd as ag
+    from nwgrad_amd import workloads as wl
+    w = wl.protein_soft_homologs()
+    b = make_batch(w, "soft", "interpair")
+    n_pos = w.n_pairs // 2
+    p = nw.AlignParams(w.matrix, alphabet=w.alphabet, gap_open_a=w.gaps[0], gap_extend_a=w.gaps[1],
+                       gap_open_b=w.gaps[2], gap_extend_b=w.gaps[3])
+    theta = ag.params_vector(p).requires_grad_(True)
+    loss_buf = torch.zeros((), dtype=torch.float64, device="cuda")
+
+    def step():
+        s = ag.soft_scores(theta, b)
+        loss = ag.margin_loss(s, n_pos, margin)
+        (g,) = torch.autograd.grad(loss, theta)
+        with torch.no_grad():
+            theta.sub_(lr * g)
+            loss_buf.copy_(loss)
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(max(1, warmup)):
+            step()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    ag.finish(b)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        graph.replay()
+        if (k + 1) % 25 == 0:
+            torch.cuda.synchronize()
+            print(f"bench: training step {k + 1}, loss {float(loss_buf):.5f}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    reruns = ag.finish(b)
+    return dt, b.cells, float(loss_buf), reruns
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,6 +275,18 @@ def main():
               flush=True)
         return 0
     print(f"bench: workload {args.workload}", file=sys.stderr, flush=True)
+    if args.workload == "protein_soft_train":
+        steps = 100 if args.steps == 10 else args.steps
+        dt, cells, loss, reruns = training_leg(steps, args.warmup)
+        gcups = cells * steps / dt / 1e9
+        line = {"metric": "GCUPS soft-grad training steps (soft pass, margin loss, adjoint, SGD) in one HIP graph",
+                "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+                "ms_per_step": round(dt / steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                "config": {"workload": "protein_soft_train", "pairs": 65536, "cells": cells, "lr": 1e-3,
+                           "margin": 20.0, "final_loss": loss, "pairs_rerun_in_double": reruns}}
+        print(json.dumps(line), flush=True)
+        return 0
     if args.workload == "serving":
         res = serving_leg(60.0 if args.serve_seconds is None else args.serve_seconds)
         line = {"metric": "GCUPS delivered under Poisson arrivals; p50 / p99 latency", "value": res["gcups_delivered"],
