@@ -175,7 +175,12 @@ def serving_leg(duration, rate=200.0, warm=3.0):
     print(f"bench: serving warm-up, {len(warm_reqs)} requests", file=sys.stderr, flush=True)
     replay(srv, warm_reqs)
     print(f"bench: serving, {len(timed)} requests over {duration:.0f} s", file=sys.stderr, flush=True)
-    out = replay(srv, timed)
+
+    def report(st):
+        print(f"bench: serving t={st['t_s']:.0f} s: {st['submitted']}/{st['of']} submitted, {st['completed']} done, "
+              f"{st['waiting']} waiting, {st['running']} launches running", file=sys.stderr, flush=True)
+
+    out = replay(srv, timed, progress=report)
     offered = sum(q.cells for q in timed) / duration / 1e9
     res = {"workload": "serving", "rate_per_s": rate, "duration_s": duration, "requests": out["jobs"],
            "failed": out["failed"], "p50_ms": round(out["p50_ms"], 3), "p99_ms": round(out["p99_ms"], 3),
@@ -305,9 +310,10 @@ def main():
     gcups = cells * args.steps / dt / 1e9
     reruns = b.soft_reruns if grad_mode == "soft" else 0
     launch = b.launch_info if grad_mode == "soft" else None
+    wide = b.wide_info if grad_mode == "soft" else None
     print(f"bench: {args.fill}: {dt / args.steps * 1e3:.3f} ms/step, {gcups:.2f} GCUPS, kernels {kms:.3f} ms"
-          + (f", {reruns} pairs rerun in double, launch (grid, wide blocks, per CU) {launch}" if grad_mode == "soft"
-             else ""), file=sys.stderr, flush=True)
+          + (f", {reruns} pairs rerun in double, launch (grid, wide blocks, per CU) {launch}, wide tiles "
+             f"(pipelined, pooled, pool rows, lag) {wide}" if grad_mode == "soft" else ""), file=sys.stderr, flush=True)
     # the other launch mode, same batch shape
     other = "striped" if args.fill == "interpair" else "interpair"
     gcups_other = kms2 = None

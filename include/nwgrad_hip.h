@@ -162,6 +162,19 @@ int nwg_batch_soft_reruns(nwg_batch *b, int64_t *n);
  * pool has no grid): blocks of the persistent launch (0 for plain launches), blocks that
  * drain the wide-tile queue first, and resident blocks per CU from the occupancy query. */
 int nwg_batch_launch_info(nwg_batch *b, int64_t *grid, int64_t *wide_slots, int64_t *per_cu);
+/* Device memory the soft pass may take for its stored forward rows, per launch (0: the
+ * default, 96 GiB).  Half of it at most goes to the wide pairs' pool (never less than the
+ * largest wide tile); a smaller pool means fewer wide pairs in flight, never a different
+ * result.  No counterpart in the
+ * reference, whose CPU pass keeps one pair's rows per worker thread (soft_kernel_impl.inl
+ * row storage); it bounds the HBM a serving process spends on soft batches. */
+int nwg_batch_set_soft_scratch(nwg_batch *b, int64_t bytes);
+/* The last persistent soft launch's wide tiles (diagnostics, no counterpart in the reference):
+ * how many ran pipelined, as panel items across waves (0: none), how many took a region of
+ * the circular wide pool (the pipelined ones, or every wide tile when block slots for the
+ * whole ones would not fit), the pool's rows (0: none), and the queue lag, in tiles, of a
+ * pipelined tile's backward items behind its forward ones. */
+int nwg_batch_wide_info(nwg_batch *b, int64_t *tiles, int64_t *pooled, int64_t *pool_rows, int64_t *lag);
 
 #ifdef __cplusplus
 }

@@ -448,6 +448,13 @@ public:
         return c;
     }
     // grid, wide-queue blocks, resident blocks per CU of the last soft launch
+    // Soft scratch: device memory the soft pass may take for stored forward rows (0: default).
+    void set_soft_scratch(int64_t bytes) { check_(nwg_batch_set_soft_scratch(b_, bytes)); }
+    std::array<int64_t, 4> wide_info() const {
+        std::array<int64_t, 4> r{0, 0, 0, 0};
+        check_(nwg_batch_wide_info(b_, &r[0], &r[1], &r[2], &r[3]));
+        return r;
+    }
     std::array<int64_t, 3> launch_info() const {
         std::array<int64_t, 3> r{0, 0, 0};
         check_(nwg_batch_launch_info(b_, &r[0], &r[1], &r[2]));

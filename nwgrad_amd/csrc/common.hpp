@@ -68,4 +68,37 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// Waits (the whole wave) until *word >= need, then an agent-scope acquire fence.  Polls with
+// relaxed loads (an acquire per poll would invalidate the L2 each time) and sleeps between
+// polls.  Bounded in real time: past `ticks` s_memrealtime ticks (100 MHz), or once any wave
+// has raised *err, it raises `bit` in *err and returns false; nothing waits after that, and
+// the host fails the call.
+__device__ __forceinline__ bool bounded_wait(const int32_t* word, int need, int32_t* err, uint64_t ticks, int lane,
+                                             int bit) {
+    bool ok = true;
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    for (;;) {
+        const int have = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (have >= need) break;
+        const int flagged = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (flagged != 0 || __builtin_amdgcn_s_memrealtime() - t0 > ticks) {
+            if (lane == 0) __hip_atomic_fetch_or(err, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ok = false;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(8);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return ok;
+}
+
+// Makes the wave's earlier global writes visible at agent scope, then stores v to *word
+// (release): a bounded_wait on the word that sees v sees the writes.
+__device__ __forceinline__ void publish(int32_t* word, int v, int lane) {
+    __threadfence();
+    if (lane == 0) __hip_atomic_store(word, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 }  // namespace nwg

@@ -7,7 +7,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <limits>
+#include <map>
 #include <memory>
 #include <new>
 #include <numeric>
@@ -137,9 +139,21 @@ struct SoftPlan {
     std::vector<int32_t> order;
     int64_t rows = 1;        // max over tiles of fsteps + 1
     int panels = 1;          // max over tiles of the column panels
+    int64_t work = 0;        // wave-steps over all tiles: 2 x panels x fsteps (forward and backward)
+    std::vector<nwg::SoftTile> ptiles;   // the persistent order (longest first)
+    std::vector<int64_t> pwork;          // per tile of it: wave-steps
+    int64_t sync_words = 0;  // pipelined wide tiles: error word, done flags, each tile's words
     bool built = false;
     DevPtr<int32_t> d_order;
     DevPtr<nwg::SoftTile> d_tiles, d_ptiles;
+    // persistent wide tiles: the queue of items and the pool regions' wait lists, built for
+    // one count of pipelined tiles, pool size and lag
+    std::vector<int2> items;
+    std::vector<int32_t> waits;
+    DevPtr<int2> d_items;
+    DevPtr<int32_t> d_waits;
+    int64_t items_tiles = -1, items_pooled = -1, items_base = -1, items_pool = 0, items_lag = -1;
+    int64_t pipelined_panels = 0;
 };
 
 // Long pairs (len(B) > kMaxN): one item per (pair, column panel), in launch groups whose
@@ -194,6 +208,11 @@ struct nwg_batch {
     DevPtr<int32_t> pexp;
     DevPtr<int32_t> lqueue;
     int64_t soft_grid = 0, soft_lslots = 0, soft_occ = 0;   // the last soft launch's shape
+    int64_t soft_pool = 0, soft_lag = 0, soft_ptiles = 0, soft_qtiles = 0;   // and its wide pool's
+    DevPtr<double> lzpart, lpart;                            // pipelined wide tiles' parts
+    DevPtr<int32_t> lsync;                                   // and words
+    bool soft_piped = false;                                 // a launch since the last check
+    size_t soft_scratch = kScratchBudget;                    // row-segment scratch of one launch
     int soft_occ_cache[2] = {0, 0};                          // resident blocks per CU: float, double
     DevPtr<uint8_t> soft_ok;
     std::vector<uint8_t> host_ok;
@@ -404,8 +423,18 @@ int upload_soft_plan(nwg_batch* b, SoftPlan& pl, std::vector<SoftBin>& bins, std
     nwg_ctx* ctx = b->ctx;
     pl.tiles.clear();
     pl.order.clear();
+    pl.items.clear();
+    pl.waits.clear();
+    pl.items_tiles = -1;
+    pl.items_pooled = -1;
+    pl.items_base = -1;
+    pl.items_pool = 0;
+    pl.items_lag = -1;
+    pl.pipelined_panels = 0;
     pl.rows = 1;
     pl.panels = 1;
+    pl.work = 0;
+    pl.pwork.clear();
     for (SoftBin& sb : bins) {
         nwg::SoftTile t = sb.t;
         t.first = (int32_t)pl.order.size();
@@ -413,6 +442,8 @@ int upload_soft_plan(nwg_batch* b, SoftPlan& pl, std::vector<SoftBin>& bins, std
         for (int32_t i : sb.pairs) pl.order.push_back(i);
         pl.rows = std::max<int64_t>(pl.rows, (int64_t)t.fsteps + 1);
         pl.panels = std::max<int>(pl.panels, t.panels);
+        pl.work += 2 * (int64_t)t.panels * t.fsteps;
+        t.sync = 0;
         pl.tiles.push_back(t);
     }
     pl.buckets = std::move(buckets);
@@ -420,6 +451,19 @@ int upload_soft_plan(nwg_batch* b, SoftPlan& pl, std::vector<SoftBin>& bins, std
     std::stable_sort(ptiles.begin(), ptiles.end(), [](const nwg::SoftTile& a, const nwg::SoftTile& c) {
         return (int64_t)a.fsteps * a.panels > (int64_t)c.fsteps * c.panels;
     });
+    // the persistent order's words for pipelined wide tiles: after the error word and one
+    // done flag per tile, per tile: forward and backward steps done per panel, forward and
+    // backward items done
+    int64_t words = 1 + (int64_t)ptiles.size();
+    for (nwg::SoftTile& t : ptiles) {
+        t.sync = (int32_t)words;
+        words += 2 * (int64_t)t.panels + 2;
+        pl.pwork.push_back(2 * (int64_t)t.panels * t.fsteps);
+        t.rbase = 0;
+        t.wait0 = t.nwait = t.poff = t.pad = 0;
+    }
+    pl.sync_words = words;
+    pl.ptiles = ptiles;
     const size_t N = pl.order.size();
     NWG_HIP(ctx, pl.d_order.reserve(std::max<size_t>(N, 1)));
     NWG_HIP(ctx, pl.d_tiles.reserve(std::max<size_t>(pl.tiles.size(), 1)));
@@ -1207,11 +1251,107 @@ SoftKernel<SF> soft_kernel_for(int mode) {
                      : mode == 2 ? nwg::soft_kernel<SF, nwg::kGlocal, kSS> : nwg::soft_kernel<SF, nwg::kGlobal, kSS>;
 }
 
+// The wide pool's queue and regions: tiles 0 .. Tq-1 of the persistent order (longest first),
+// the first Tp of them pipelined.  Tile t takes the next panels x (fsteps + 1) rows of the
+// circular pool [base, base + pool), from its start again when they do not fit, and waits
+// for the tiles that hold rows of its region to be done.  A pipelined tile's forward items
+// (panels 0 .. NP-1) come at the tile's turn, its backward items (panels NP-1 .. 0) `lag`
+// tiles later; another tile is one whole item.  Every item comes after the items it waits
+// for: the tile's earlier forward panels, its forward items and later backward panels, and
+// the items that finish the tiles whose regions it reuses (a pipelined tile's backward items
+// are queued early for that when the lag would place them later).
+int build_soft_items(nwg_batch* b, SoftPlan& wl, int64_t Tp, int64_t Tq, int64_t base, int64_t pool, int64_t lag) {
+    nwg_ctx* ctx = b->ctx;
+    std::vector<nwg::SoftTile>& pt = wl.ptiles;
+    wl.items.clear();
+    wl.waits.clear();
+    std::map<int64_t, std::pair<int64_t, int32_t>> held;   // region start -> (end, tile)
+    std::deque<int32_t> pending;                          // pipelined tiles, backward not queued
+    auto queue_backward = [&](int32_t j) {
+        for (int pp = pt[(size_t)j].panels - 1; pp >= 0; --pp) wl.items.push_back(int2{j, 2 * pp + 1});
+    };
+    int64_t pos = 0;
+    int32_t poff = 0;
+    std::vector<int32_t> w;
+    for (int64_t t = 0; t < Tq; ++t) {
+        nwg::SoftTile& tt = pt[(size_t)t];
+        const int64_t size = (int64_t)tt.panels * (tt.fsteps + 1);
+        if (pos + size > pool) pos = 0;
+        const int64_t a = pos, e = pos + size;
+        pos = e;
+        w.clear();
+        auto it = held.lower_bound(a);
+        if (it != held.begin() && std::prev(it)->second.first > a) it = std::prev(it);
+        while (it != held.end() && it->first < e) {
+            const int64_t s0 = it->first, s1 = it->second.first;
+            const int32_t j = it->second.second;
+            w.push_back(j);
+            it = held.erase(it);
+            if (s0 < a) held[s0] = {a, j};
+            if (s1 > e) {
+                held[e] = {s1, j};
+                break;
+            }
+        }
+        held[a] = {e, (int32_t)t};
+        std::sort(w.begin(), w.end());
+        w.erase(std::unique(w.begin(), w.end()), w.end());
+        tt.rbase = base + a;
+        tt.wait0 = (int32_t)wl.waits.size();
+        tt.nwait = (int32_t)w.size();
+        wl.waits.insert(wl.waits.end(), w.begin(), w.end());
+        const int32_t last_wait = w.empty() ? -1 : w.back();
+        while (!pending.empty() && pending.front() <= last_wait) {
+            queue_backward(pending.front());
+            pending.pop_front();
+        }
+        if (t < Tp) {
+            tt.poff = poff;
+            poff += tt.panels;
+            for (int pp = 0; pp < tt.panels; ++pp) wl.items.push_back(int2{(int)t, 2 * pp});
+            pending.push_back((int32_t)t);
+        } else {
+            tt.poff = 0;
+            wl.items.push_back(int2{(int)t, -1});
+        }
+        while (!pending.empty() && pending.front() <= t - lag) {
+            queue_backward(pending.front());
+            pending.pop_front();
+        }
+    }
+    while (!pending.empty()) {
+        queue_backward(pending.front());
+        pending.pop_front();
+    }
+    NWG_HIP(ctx, wl.d_items.reserve(std::max<size_t>(wl.items.size(), 1)));
+    NWG_HIP(ctx, wl.d_waits.reserve(std::max<size_t>(wl.waits.size(), 1)));
+    NWG_HIP(ctx, hipMemcpyAsync(wl.d_items.p, wl.items.data(), sizeof(int2) * wl.items.size(), hipMemcpyHostToDevice,
+                                ctx->stream));
+    if (!wl.waits.empty())
+        NWG_HIP(ctx, hipMemcpyAsync(wl.d_waits.p, wl.waits.data(), sizeof(int32_t) * wl.waits.size(),
+                                    hipMemcpyHostToDevice, ctx->stream));
+    NWG_HIP(ctx, hipMemcpyAsync(wl.d_ptiles.p, pt.data(), sizeof(nwg::SoftTile) * pt.size(), hipMemcpyHostToDevice,
+                                ctx->stream));
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    wl.items_tiles = Tp;
+    wl.items_pooled = Tq;
+    wl.items_base = base;
+    wl.items_pool = pool;
+    wl.items_lag = lag;
+    wl.pipelined_panels = poff;
+    return NWG_OK;
+}
+
 // One soft pass over the tiles of `pl` (one panel) and `wl` (column panels).  Persistent:
-// one launch whose blocks below n_lslots first drain the wide tiles' queue, then blocks below
-// n_mslots the main queue.  Plain: one launch per bucket, main plan then wide plan.
+// one launch.  A wide tile whose panels in sequence would take one wave longer than the
+// launch's work per resident wave is pipelined: its panel items spread it over as many waves
+// as it has panels (the longest tiles, so a prefix of the longest-first order).  The other
+// wide tiles run whole, one wave each, in the block's own slot, with no waits.  Blocks below
+// n_pwaves first drain the pipelined items, blocks below n_lslots the whole wide tiles, then
+// blocks below n_mslots the main queue.  Plain: one launch per bucket, main plan then wide
+// plan (whole tiles).
 template <typename SF>
-int launch_soft(nwg_batch* b, const SoftPlan& pl, const SoftPlan& wl) {
+int launch_soft(nwg_batch* b, const SoftPlan& pl, SoftPlan& wl) {
     using SS = SoftScratch<SF>;
     using LY = nwg::Seg<SF, kSS>;
     nwg_ctx* ctx = b->ctx;
@@ -1231,35 +1371,117 @@ int launch_soft(nwg_batch* b, const SoftPlan& pl, const SoftPlan& wl) {
     b->soft_occ = occ;
     const int64_t resident = (int64_t)occ * ctx->n_cu;
     const bool persistent = b->fill == 0;
-    // Slots: the main pool holds one panel region of pl.rows rows per slot; the wide pool
-    // wl.panels regions of wl.rows rows.  Plain launches use one slot per tile of a launch.
+    // Slots: the main pool holds one panel region of pl.rows rows per slot (one per block
+    // that runs main tiles); the wide pool one slot of panels x rows per block that runs whole
+    // wide tiles (the largest whole tile's) and, persistent, after them the circular pool of
+    // the pipelined tiles (or of every wide tile), each taking its own panels x (fsteps + 1)
+    // rows.
     const size_t row_bytes = (size_t)LY::row * sizeof(SF);
-    const size_t mslot = (size_t)pl.rows * row_bytes, lslot = (size_t)wl.panels * (size_t)wl.rows * row_bytes;
+    const size_t mslot = (size_t)pl.rows * row_bytes;
+    const size_t budget = b->soft_scratch;
+    const int64_t wrows_cap = (int64_t)(budget / 2 / row_bytes);   // the wide pool: half the budget
+    const int64_t T = (int64_t)wl.tiles.size();
+    int64_t Tp = 0, Tq = 0, pool = 0, lslots = 0, pwaves = 0, lag = 0;
+    int64_t wpanels = wl.panels, wrows = wl.rows;   // a whole tile's block slot
+    if (T > 0 && persistent) {
+        // Pipelined: the tiles longer than the launch's work per resident wave.
+        const double share = (double)(pl.work + wl.work) / (double)resident;
+        while (Tp < T && (double)wl.pwork[(size_t)Tp] > share) ++Tp;
+        auto size_of = [&](int64_t t) {
+            const nwg::SoftTile& tt = wl.ptiles[(size_t)t];
+            return (int64_t)tt.panels * (tt.fsteps + 1);
+        };
+        int64_t need_p = 0, big_p = 0, need = 0, big = 0, work_w = 0;
+        wpanels = 1;
+        wrows = 1;
+        for (int64_t t = 0; t < T; ++t) {
+            const int64_t size = size_of(t);
+            need += size;
+            big = std::max(big, size);
+            if (t < Tp) {
+                need_p += size;
+                big_p = std::max(big_p, size);
+            } else {
+                wpanels = std::max<int64_t>(wpanels, wl.ptiles[(size_t)t].panels);
+                wrows = std::max<int64_t>(wrows, (int64_t)wl.ptiles[(size_t)t].fsteps + 1);
+                work_w += wl.pwork[(size_t)t];
+            }
+        }
+        // Whole tiles run in block slots (no waits, no cache writeback), beside the pipelined
+        // tiles' pool (all of it, or at least half the wide budget), when the slots that fit
+        // get through the whole tiles' work within the launch's work per resident wave (the
+        // other blocks run main tiles meanwhile).  Otherwise -- slots sized for the largest
+        // whole tile hold too few of them -- every wide tile goes to the pool, whose regions
+        // fit each tile's own size.
+        const int64_t wslot = wpanels * wrows;
+        const int64_t keep_p = Tp > 0 ? std::max(big_p, std::min(need_p, wrows_cap / 2)) : 0;
+        const int64_t n_b = std::min<int64_t>(std::min<int64_t>(T - Tp, resident),
+                                              std::max<int64_t>(0, wrows_cap - keep_p) / wslot);
+        if (Tp == T || (n_b >= 1 && (double)work_w / (double)n_b <= share)) {
+            lslots = Tp < T ? n_b : 0;
+            Tq = Tp;
+            pool = Tp > 0 ? std::max(big_p, std::min(need_p, wrows_cap - lslots * wslot)) : 0;
+        } else {
+            lslots = 0;
+            Tq = T;
+            need_p = need;
+            big_p = big;
+            pool = std::max(big, std::min(need, wrows_cap));
+        }
+        if (Tq > 0) {
+            // Blocks that take pool items: the tiles that fit in the pool at once, times their
+            // waves (a pipelined tile's forward and backward items both); a pipelined tile's
+            // backward items come half of that many tiles after its forward ones.
+            double waves = 0.0;
+            for (int64_t t = 0; t < Tq; ++t) waves += t < Tp ? 2.0 * wl.ptiles[(size_t)t].panels : 1.0;
+            const double fit = std::min((double)Tq, (double)pool * (double)Tq / (double)need_p);
+            pwaves = std::max<int64_t>(1, std::min<int64_t>(resident, (int64_t)std::ceil(fit * waves / (double)Tq)));
+            lag = std::max<int64_t>(0, (int64_t)std::ceil(fit / 2.0));
+            const int64_t base = lslots * wslot;
+            if (wl.items_tiles != Tp || wl.items_pooled != Tq || wl.items_base != base || wl.items_pool != pool ||
+                wl.items_lag != lag) {
+                const int rc = build_soft_items(b, wl, Tp, Tq, base, pool, lag);
+                if (rc) return rc;
+            }
+        }
+    } else if (T > 0) {
+        const int64_t wslot = wpanels * wrows;
+        lslots = std::max<int64_t>(1, std::min<int64_t>({T, resident, wrows_cap / wslot}));
+    }
+    const size_t wide_rows = (size_t)(lslots * wpanels * wrows + pool);
     int64_t mslots = pl.tiles.empty() ? 0 : std::min<int64_t>((int64_t)pl.tiles.size(), resident);
-    int64_t lslots = wl.tiles.empty() ? 0 : std::min<int64_t>((int64_t)wl.tiles.size(), resident);
-    // within the scratch budget: the wide pool first takes what it needs of half of it
-    if (lslots > 0) lslots = std::max<int64_t>(1, std::min<int64_t>(lslots, (int64_t)(kScratchBudget / 2 / lslot)));
-    if (mslots > 0)
-        mslots = std::max<int64_t>(
-            1, std::min<int64_t>(mslots, (int64_t)((kScratchBudget - (size_t)lslots * lslot) / std::max<size_t>(mslot, 1))));
-    const size_t mseg = (size_t)mslots * pl.rows * LY::row, lseg = (size_t)lslots * wl.panels * wl.rows * LY::row;
-    const size_t bnd = (size_t)lslots * wl.panels * wl.rows * nwg::kTilePairs;
-    if (SS::seg(b).n < mseg || SS::lseg(b).n < lseg || SS::pb(b).n < bnd * 6 || b->pexp.n < bnd * 2) {
+    if (mslots > 0) {
+        const size_t used = wide_rows * row_bytes;
+        const size_t left = budget > used ? budget - used : 0;
+        mslots = std::max<int64_t>(1, std::min<int64_t>(mslots, (int64_t)(left / std::max<size_t>(mslot, 1))));
+    }
+    const bool piped = Tq > 0;
+    const size_t mseg = (size_t)mslots * pl.rows * LY::row;
+    const size_t lseg = wide_rows * LY::row;
+    const size_t bnd = wide_rows * nwg::kTilePairs;
+    const size_t zpart = piped ? (size_t)wl.pipelined_panels * nwg::kTilePairs * 4 : 0;
+    const size_t part = piped ? (size_t)wl.pipelined_panels * b->hist_slots * (nn + 6) : 0;
+    const size_t nsync = piped ? (size_t)wl.sync_words : 0;
+    if (SS::seg(b).n < mseg || SS::lseg(b).n < lseg || SS::pb(b).n < bnd * 6 || b->pexp.n < bnd * 2 ||
+        b->lzpart.n < zpart || b->lpart.n < part || b->lsync.n < nsync) {
         NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
         NWG_HIP(ctx, SS::seg(b).reserve(std::max<size_t>(mseg, 1)));
         NWG_HIP(ctx, SS::lseg(b).reserve(std::max<size_t>(lseg, 1)));
         NWG_HIP(ctx, SS::pb(b).reserve(std::max<size_t>(bnd * 6, 1)));
         NWG_HIP(ctx, b->pexp.reserve(std::max<size_t>(bnd * 2, 1)));
+        NWG_HIP(ctx, b->lzpart.reserve(std::max<size_t>(zpart, 1)));
+        NWG_HIP(ctx, b->lpart.reserve(std::max<size_t>(part, 1)));
+        NWG_HIP(ctx, b->lsync.reserve(std::max<size_t>(nsync, 1)));
     }
     NWG_HIP(ctx, b->soft_ok.reserve((size_t)std::max<int64_t>(b->N, 1)));
-    NWG_HIP(ctx, b->lqueue.reserve(1));
+    NWG_HIP(ctx, b->lqueue.reserve(2));
     nwg::SoftArgs<SF> A{};
     A.codes_a = b->ca.p;
     A.codes_b = b->cb.p;
     A.off_a = b->oa.p;
     A.off_b = b->ob.p;
     A.view = nwg::SoftView<SF>{SS::seg(b).p, SS::pb(b).p, b->pexp.p, pl.rows, 1};
-    A.lview = nwg::SoftView<SF>{SS::lseg(b).p, SS::pb(b).p, b->pexp.p, wl.rows, wl.panels};
+    A.lview = nwg::SoftView<SF>{SS::lseg(b).p, SS::pb(b).p, b->pexp.p, wrows, (int32_t)wpanels};
     A.psub = dps.p;
     A.na = b->na;
     A.hist_slots = b->hist_slots;
@@ -1275,10 +1497,10 @@ int launch_soft(nwg_batch* b, const SoftPlan& pl, const SoftPlan& wl) {
     // the weights (and, persistent, the queue counters) from the device params
     hipLaunchKernelGGL(nwg::soft_weights_kernel<SF>, dim3(1), dim3(256), 0, ctx->stream, b->theta.p, nn,
                        1.0 / b->temperature, lin ? 1 : 0, dps.p, SS::gw(b).p, persistent ? b->queue.p : nullptr,
-                       persistent ? b->lqueue.p : nullptr);
+                       persistent ? b->lqueue.p : nullptr, piped ? b->lsync.p : nullptr, (int64_t)nsync);
     NWG_HIP(ctx, hipGetLastError());
     if (persistent) {
-        const int64_t grid = std::max<int64_t>(mslots, lslots);
+        const int64_t grid = std::max<int64_t>({mslots, lslots, pwaves});
         A.order = pl.d_order.p;
         A.tiles = pl.d_ptiles.p;
         A.n_tiles = (int32_t)pl.tiles.size();
@@ -1286,11 +1508,27 @@ int launch_soft(nwg_batch* b, const SoftPlan& pl, const SoftPlan& wl) {
         A.n_mslots = (int32_t)mslots;
         A.lorder = wl.d_order.p;
         A.ltiles = wl.d_ptiles.p;
-        A.n_ltiles = (int32_t)wl.tiles.size();
+        A.n_ltiles = (int32_t)T;
+        A.n_ptiles = (int32_t)Tq;
         A.n_lslots = (int32_t)lslots;
+        A.n_pwaves = (int32_t)pwaves;
         A.lqueue = b->lqueue.p;
+        if (piped) {
+            A.litems = wl.d_items.p;
+            A.n_litems = (int32_t)wl.items.size();
+            A.lwait = wl.d_waits.p;
+            A.lsync = b->lsync.p;
+            A.lzpart = b->lzpart.p;
+            A.lpart = b->lpart.p;
+            A.wait_ticks = kWaitTicks;
+            b->soft_piped = true;
+        }
         b->soft_grid = grid;
-        b->soft_lslots = lslots;
+        b->soft_lslots = std::max(lslots, pwaves);
+        b->soft_pool = pool;
+        b->soft_lag = lag;
+        b->soft_ptiles = Tp;
+        b->soft_qtiles = Tq;
         hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64), lds, ctx->stream, A);
         NWG_HIP(ctx, hipGetLastError());
         return NWG_OK;
@@ -1300,6 +1538,10 @@ int launch_soft(nwg_batch* b, const SoftPlan& pl, const SoftPlan& wl) {
     A.lqueue = nullptr;
     b->soft_grid = 0;
     b->soft_lslots = 0;
+    b->soft_pool = 0;
+    b->soft_lag = 0;
+    b->soft_ptiles = 0;
+    b->soft_qtiles = 0;
     for (int w = 0; w < 2; ++w) {
         const SoftPlan& P = w == 0 ? pl : wl;
         const int64_t slots = w == 0 ? mslots : lslots;
@@ -1340,13 +1582,30 @@ int launch_soft_batch(nwg_batch* b) {
 // After a soft pass: a pair whose float32 pass lost range or mass beyond 2^-20 reruns in
 // double, and one that fails in double too fails the call (the reference's policy,
 // aligner.hpp:6554-6586: float32, then double, then std::domain_error under "scaled").
+// The pipelined wide tiles' error word, after a synchronisation: a wait that passed its bound
+// (the results of that launch are not to be used) fails the call.
+int check_soft_err(nwg_batch* b, const int32_t& word) {
+    if (!b->soft_piped) return NWG_OK;
+    b->soft_piped = false;
+    if (word == 0) return NWG_OK;
+    b->results_valid = false;
+    return fail(b->ctx, NWG_EDEVICE,
+                "nwgrad_hip: soft pass: a wide pair's panel waited past its bound for another panel (error word %d)",
+                (int)word);
+}
+
 int soft_finish(nwg_batch* b) {
     nwg_ctx* ctx = b->ctx;
     b->soft_reruns = 0;
     if (b->N == 0) return NWG_OK;
     b->host_ok.resize((size_t)b->N);
+    int32_t serr = 0;
     NWG_HIP(ctx, hipMemcpyAsync(b->host_ok.data(), b->soft_ok.p, (size_t)b->N, hipMemcpyDeviceToHost, ctx->stream));
+    if (b->soft_piped)
+        NWG_HIP(ctx, hipMemcpyAsync(&serr, b->lsync.p, sizeof serr, hipMemcpyDeviceToHost, ctx->stream));
     NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int rc = check_soft_err(b, serr);
+    if (rc) return rc;
     std::vector<int32_t> refused;
     for (int64_t i = 0; i < b->N; ++i)
         if (!b->host_ok[(size_t)i]) refused.push_back((int32_t)i);
@@ -1354,12 +1613,16 @@ int soft_finish(nwg_batch* b) {
     const bool ran_f32 = b->prob.precision == 0 && b->soft_precision == 0;
     if (ran_f32) {
         b->soft_reruns = (int64_t)refused.size();
-        int rc = build_soft_plans(b, refused, b->rr_main, b->rr_wide, b->fill == 0);
+        rc = build_soft_plans(b, refused, b->rr_main, b->rr_wide, b->fill == 0);
         if (rc) return rc;
         rc = launch_soft<double>(b, b->rr_main, b->rr_wide);
         if (rc) return rc;
         NWG_HIP(ctx, hipMemcpyAsync(b->host_ok.data(), b->soft_ok.p, (size_t)b->N, hipMemcpyDeviceToHost, ctx->stream));
+        if (b->soft_piped)
+            NWG_HIP(ctx, hipMemcpyAsync(&serr, b->lsync.p, sizeof serr, hipMemcpyDeviceToHost, ctx->stream));
         NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        rc = check_soft_err(b, serr);
+        if (rc) return rc;
         refused.clear();
         for (int64_t i = 0; i < b->N; ++i)
             if (!b->host_ok[(size_t)i]) refused.push_back((int32_t)i);
@@ -1584,6 +1847,22 @@ int nwg_batch_launch_info(nwg_batch* b, int64_t* grid, int64_t* wide_slots, int6
 int nwg_batch_soft_reruns(nwg_batch* b, int64_t* n) {
     if (!b || !n) return NWG_EINVAL;
     *n = b->soft_reruns;
+    return NWG_OK;
+}
+
+int nwg_batch_set_soft_scratch(nwg_batch* b, int64_t bytes) {
+    if (!b) return NWG_EINVAL;
+    if (bytes < 0) return fail(b->ctx, NWG_EINVAL, "nwgrad_hip: soft scratch bytes must be >= 0 (0: the default)");
+    b->soft_scratch = bytes == 0 ? kScratchBudget : (size_t)bytes;
+    return NWG_OK;
+}
+
+int nwg_batch_wide_info(nwg_batch* b, int64_t* tiles, int64_t* pooled, int64_t* pool_rows, int64_t* lag) {
+    if (!b) return NWG_EINVAL;
+    if (tiles) *tiles = b->soft_ptiles;
+    if (pooled) *pooled = b->soft_qtiles;
+    if (pool_rows) *pool_rows = b->soft_pool;
+    if (lag) *lag = b->soft_lag;
     return NWG_OK;
 }
 

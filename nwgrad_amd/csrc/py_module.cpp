@@ -411,6 +411,14 @@ void bind_batch(py::module_& m, const char* name) {
         .def_property_readonly("cells", [](const PB& s) { return s.b->cells(); },
                                "Sum of len(A)*len(B) over the pairs (the GCUPS numerator).")
         .def_property_readonly("tiles", [](const PB& s) { return s.b->tiles(); })
+        .def_property_readonly("wide_info", [](const PB& s) { auto r = s.b->wide_info(); return py::make_tuple(r[0], r[1], r[2], r[3]); },
+                               "(pipelined, pooled, pool_rows, lag) of the last persistent soft launch's wide tiles: "
+                               "how many ran pipelined, as panel items across waves, how many took a region of the "
+                               "circular wide pool, the pool's rows and the lag, in tiles, of a pipelined tile's "
+                               "backward items behind its forward ones")
+        .def("set_soft_scratch", [](PB& s, int64_t bytes) { s.b->set_soft_scratch(bytes); }, py::arg("bytes"),
+             "Device memory the soft pass may take for stored forward rows per launch (0: 96 GiB); "
+             "fewer bytes run fewer wide pairs at once, never a different result")
         .def_property_readonly("launch_info", [](const PB& s) { auto r = s.b->launch_info(); return py::make_tuple(r[0], r[1], r[2]); },
                                "(grid, wide-queue blocks, resident blocks per CU) of the last soft launch")
         .def_property_readonly("soft_reruns", [](const PB& s) { return s.b->soft_reruns(); },

@@ -48,6 +48,8 @@ namespace nwg {
 
 constexpr int kTilePairs = 16;  // pairs per soft tile at most
 constexpr int kCK = 4;          // backward bands: one forward row in kCK stored in full
+constexpr int kPipeChunk = 64;  // pipelined wide tiles: steps between progress waits / stores
+static_assert(kPipeChunk % (2 * kCK) == 0, "a chunk is whole backward loop trips");
 
 // One soft tile: pairs order[first .. first + npairs), pair q on lanes lo[q] .. lo[q]+G[q]-1.
 struct SoftTile {
@@ -59,17 +61,26 @@ struct SoftTile {
     int16_t panels;    // column panels (every pair of the tile runs the same count)
     uint8_t G[kTilePairs];
     uint8_t lo[kTilePairs];
+    // Persistent wide tiles (the queue's order): the tile's words in SoftArgs::lsync, its
+    // region of the wide pool (first row index; panels x (fsteps + 1) rows), the tiles whose
+    // regions it reuses (lwait[wait0 .. wait0 + nwait)), and (pipelined) the index of its
+    // first panel among the pipelined tiles' panels.
+    int32_t sync;
+    int64_t rbase;
+    int32_t wait0, nwait;
+    int32_t poff;
+    int32_t pad;
 };
-static_assert(sizeof(SoftTile) == 52, "SoftTile is 52 bytes");
+static_assert(sizeof(SoftTile) == 80, "SoftTile is 80 bytes");
 
 // One scratch pool: row segments and panel-boundary columns per wave slot.
 template <typename SF>
 struct SoftView {
-    SF* seg;          // row segments: per slot, panels x rows x (NV x 64) vectors
-    SF* pbnd;         // panel boundary columns: per slot, panel, row, pair: M, X, Y, BM, BY
+    SF* seg;          // row segments: per row index, (NV x 64) vectors
+    SF* pbnd;         // panel boundary columns: per row index, pair: M, X, Y, BM, BY
     int32_t* pexp;    // their exponents: forward, backward
-    int64_t rows;     // rows of one panel region (at least fsteps + 1 over the pool's tiles)
-    int32_t panels;   // panel regions per slot
+    int64_t rows;     // slots: rows of one panel region (at least fsteps + 1 over the pool's tiles)
+    int32_t panels;   // slots: panel regions per slot
 };
 
 template <typename SF>
@@ -86,13 +97,27 @@ struct SoftArgs {
     int32_t* queue;
     int32_t n_mslots;        // blocks below this own a main-pool slot and drain the main queue
     SoftView<SF> view;
-    // wide queue: tiles in column panels; blocks below n_lslots drain it first
+    // Wide tiles (column panels).  Persistent: longest first.  Tiles below n_ptiles have their
+    // items in the item queue, each tile in its own region of the circular wide pool: (tile,
+    // 2 * panel + direction) one panel's forward (0) or backward (1) of a pipelined tile, (tile,
+    // -1) a whole tile (see soft_tile).  The others run whole on one wave, from the wide queue,
+    // in the block's wide slot.  Blocks below n_pwaves drain the item queue first, blocks below
+    // n_lslots the wide queue.  Plain: whole tiles in block slots.
     const int32_t* lorder;
     const SoftTile* ltiles;
     int32_t n_ltiles;
+    int32_t n_ptiles;
     int32_t n_lslots;
-    int32_t* lqueue;
+    int32_t n_pwaves;
+    int32_t* lqueue;         // the item queue's counter, then the wide queue's
     SoftView<SF> lview;
+    const int2* litems;
+    int32_t n_litems;
+    const int32_t* lwait;    // the tiles whose pool regions a tile reuses
+    int32_t* lsync;          // [0] error word, [1 + t] tile t done, then each tile's words
+    double* lzpart;          // per pipelined panel, pair slot: forward Z part (acc, E) and flag
+    double* lpart;           // per pipelined panel, histogram slot: n_alpha^2 counts, 5 sums, flag
+    uint64_t wait_ticks;     // bound of one wait, s_memrealtime ticks
     const SF* psub;          // exp(matrix / T), n_alpha * n_alpha (LDS adds a row of ones)
     int32_t na;
     int32_t hist_slots;
@@ -277,10 +302,23 @@ __device__ __forceinline__ void load_rec(const SF* __restrict__ base, int lane, 
     }
 }
 
+// One tile in a block slot (idir -1: every panel's forward, Z, every panel's backward, the
+// results), or one item of the wide pool, in the tile's own region of it: idir 2 runs the
+// whole tile; idir 0 runs panel ipp's forward of a pipelined tile and leaves the pair's Z part
+// and range flag; idir 1 runs panel ipp's backward from Z formed out of every panel's part,
+// leaves its histogram, sums and flag, and the tile's last backward item to finish adds the
+// parts of all panels, in panel order, into the results.  An item waits, bounded in real
+// time, only for items queued before it: a tile's first item for the tiles whose pool
+// regions it reuses to be done, a forward item for the previous panel's forward rows, a
+// backward item for every forward item of its tile and for the next panel's backward rows.
+// A wave that holds an item is running, so every wait ends.  A tile in the pool is done when
+// its last item has made its writes visible at agent scope: another XCD's L2 then holds no
+// dirty line of the region that a later tile reuses.
 template <typename SF, int AM, int S>
 __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int32_t* __restrict__ order,
                           const SoftTile* __restrict__ tdp, int64_t slot, const SF* __restrict__ spsub,
-                          double* __restrict__ hist, double* __restrict__ lacc, int lane) {
+                          double* __restrict__ hist, double* __restrict__ lacc, int lane, int tix, int ipp,
+                          int idir) {
     using R = SoftRange<SF>;
     using LY = Seg<SF, S>;
     constexpr bool L = AM == kLocal, GL = AM == kGlocal;
@@ -308,12 +346,33 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
     }
     const int NP = td.panels;
     const int PW = G * S;   // columns of one panel of this pair
+    // the panels whose forward and backward this call runs
+    const bool whole = idir == -1 || idir == 2;
+    const int pf0 = whole ? 0 : ipp, pf1 = whole ? NP : (idir == 0 ? ipp + 1 : ipp);
+    const int pb0 = whole ? 0 : ipp, pb1 = whole ? NP : (idir == 1 ? ipp + 1 : ipp);
+    // pipelined: the tile's words -- forward steps done per panel, backward steps done per
+    // panel, forward items done, backward items done -- and the launch's error word
+    int32_t* const sw = idir < 0 ? nullptr : A.lsync + tdp->sync;
+    int32_t* const serr = A.lsync;
+    const int64_t poff = (idir == 0 || idir == 1) ? tdp->poff : 0;
+    // a tile's first item: the tiles whose pool regions it reuses are done
+    if (idir == 2 || (idir == 0 && ipp == 0)) {
+        const int w0 = tdp->wait0, nw = tdp->nwait;
+        for (int k = 0; k < nw; ++k) bounded_wait(A.lsync + 1 + A.lwait[w0 + k], 1, serr, A.wait_ticks, lane, 1);
+    }
     const SF oa = A.gapw[0], ea = A.gapw[1], ob = A.gapw[2], eb = A.gapw[3];
-    const int64_t rows = V.rows;
-    const int64_t pbase = slot * (int64_t)V.panels;   // this slot's first panel region
-    // panel boundary column of row i of region r, this pair: 6 values, 2 exponents
-    auto bnd = [&](int64_t r, int i) { return V.pbnd + ((r * rows + i) * kTilePairs + (q < 0 ? 0 : q)) * 6; };
-    auto bex = [&](int64_t r, int i) { return V.pexp + ((r * rows + i) * kTilePairs + (q < 0 ? 0 : q)) * 2; };
+    // Panel pp's row i is row index rbase + pp * rows + i of the pool: a block slot (the
+    // pool's row count per panel), or a pipelined tile's own region (its own rows).
+    const bool own = idir >= 0;
+    const int64_t rows = own ? (int64_t)td.fsteps + 1 : V.rows;
+    const int64_t rbase = own ? tdp->rbase : slot * (int64_t)V.panels * V.rows;
+    // panel boundary column of row i of panel pp, this pair: 6 values, 2 exponents
+    auto bnd = [&](int pp, int i) {
+        return V.pbnd + ((rbase + pp * rows + i) * kTilePairs + (q < 0 ? 0 : q)) * 6;
+    };
+    auto bex = [&](int pp, int i) {
+        return V.pexp + ((rbase + pp * rows + i) * kTilePairs + (q < 0 ? 0 : q)) * 2;
+    };
     bool bad = false;
 
     // ── forward ──────────────────────────────────────────────────────────────
@@ -323,14 +382,28 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
     ExpAcc zacc{0.0, 0};
     SF corner = SF(0);                       // Global: the (m, n) cell, held by the lane of column n
     int cornerE = 0;
-    for (int pp = 0; pp < NP; ++pp) {
+    for (int pp = pf0; pp < pf1; ++pp) {
+        // Pipelined: panel 0 waits until the tile that last used this scratch slot is done,
+        // panel pp > 0 for the previous panel's first steps (and, every kPipeChunk steps, for
+        // the next chunk's).  The boundary row that lane 0 reads at step t was written by the
+        // previous panel's last lane (G <= 64) at its step t + G, so a chunk of steps
+        // t .. t+kPipeChunk-1 needs t + kPipeChunk + 64 of its steps done.
+        int32_t* fin = nullptr;
+        int32_t* fout = nullptr;
+        if (idir == 0) {
+            if (pp > 0) {
+                fin = sw + pp - 1;
+                bounded_wait(fin, min(kPipeChunk + kWave, td.fsteps), serr, A.wait_ticks, lane, 1);
+            }
+            if (pp + 1 < NP) fout = sw + pp;
+        }
         const int j0 = pp * PW + g * S;
         const int ncols = n - j0 < 0 ? 0 : (n - j0 > S ? S : n - j0);
         const bool live = p >= 0 && (ncols > 0 || j0 == 0);   // the lane of column 0 always computes
         int coff[S];
 #pragma unroll
         for (int k = 0; k < S; ++k) coff[k] = (k < ncols) ? (int)bp[j0 + k] : na;   // na: no column
-        SF* __restrict__ seg = V.seg + (pbase + pp) * rows * LY::row;
+        SF* __restrict__ seg = V.seg + (rbase + pp * rows) * LY::row;
         E = 0;
         if (L || GL) {
 #pragma unroll
@@ -419,11 +492,14 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         SF qM = SF(0), qX = SF(0), qY = SF(0);
         int qE = 0;
         if (pin && live && m >= 1) {
-            const SF* bq = bnd(pbase + pp - 1, 1);
-            qM = bq[0]; qX = bq[1]; qY = bq[2]; qE = bex(pbase + pp - 1, 1)[0];
+            const SF* bq = bnd(pp - 1, 1);
+            qM = bq[0]; qX = bq[1]; qY = bq[2]; qE = bex(pp - 1, 1)[0];
         }
         const int fsteps = td.fsteps;
-        for (int t = 0; t < fsteps; ++t) {
+        // One wavefront step (row i = t - g + 1 on lane g); the loop runs it twice per trip,
+        // which lets the compiler keep the step's values in place instead of copying them
+        // at the loop's back edge.
+        auto fstep = [&](const int t) {
             const int i = t - g + 1;
             SF lM = wshift_up(oM), lX = wshift_up(oX), lY = wshift_up(oY);
             int lE = wshift_up(oE);
@@ -433,8 +509,8 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 if (pin) {
                     lM = qM; lX = qX; lY = qY; lE = qE;
                     if (i < m) {
-                        const SF* bq = bnd(pbase + pp - 1, i + 1);
-                        qM = bq[0]; qX = bq[1]; qY = bq[2]; qE = bex(pbase + pp - 1, i + 1)[0];
+                        const SF* bq = bnd(pp - 1, i + 1);
+                        qM = bq[0]; qX = bq[1]; qY = bq[2]; qE = bex(pp - 1, i + 1)[0];
                     }
                 }
                 int Ec = E + kpend;                 // this row's units
@@ -501,12 +577,26 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 else store_rec<SF, S>(seg + (int64_t)(i + g) * LY::row, lane, cM, cX, cY, E);
                 oM = M[S - 1]; oX = X[S - 1]; oY = Y[S - 1]; oE = E;
                 if (pout) {
-                    SF* bo = bnd(pbase + pp, i);
+                    SF* bo = bnd(pp, i);
                     bo[0] = oM; bo[1] = oX; bo[2] = oY;
-                    bex(pbase + pp, i)[0] = oE;
+                    bex(pp, i)[0] = oE;
                 }
             }
+        };
+        int t = 0;
+        for (; t + 1 < fsteps; t += 2) {
+            if (fin != nullptr && t > 0 && t % kPipeChunk == 0)
+                bounded_wait(fin, min(t + kPipeChunk + kWave, fsteps), serr, A.wait_ticks, lane, 1);
+            fstep(t);
+            fstep(t + 1);
+            if (fout != nullptr && (t + 2) % kPipeChunk == 0) publish(fout, t + 2, lane);
         }
+        if (t < fsteps) {
+            if (fin != nullptr && t > 0 && t % kPipeChunk == 0)
+                bounded_wait(fin, min(t + kPipeChunk + kWave, fsteps), serr, A.wait_ticks, lane, 1);
+            fstep(t);
+        }
+        if (fout != nullptr) publish(fout, fsteps, lane);
         // Global's end cell: row m of column n, in the panel that holds column n (a narrow
         // pair in a wide tile ends before the last panel)
         if constexpr (!L && !GL) {
@@ -524,7 +614,17 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
     }
 
     // ── Z ───────────────────────────────────────────────────────────────────
-    if constexpr (!L && !GL) {
+    if (idir == 1) {
+        // every panel's part, in panel order, once every forward item of the tile is done
+        bounded_wait(sw + 2 * NP, NP, serr, A.wait_ticks, lane, 1);
+        ExpAcc z{0.0, 0};
+        if (q >= 0)
+            for (int k = 0; k < NP; ++k) {
+                const double* zp = A.lzpart + ((poff + k) * kTilePairs + q) * 4;
+                expacc_add(z, zp[0], (int)zp[1]);
+            }
+        zacc = z;
+    } else if constexpr (!L && !GL) {
         const int gn = n == 0 ? 0 : ((n - 1) % PW) / S;   // the lane of column n
         zacc.acc = __shfl((double)corner, lo + gn, kWave);
         zacc.E = __shfl(cornerE, lo + gn, kWave);
@@ -536,6 +636,23 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         }
         zacc.acc = __shfl(zacc.acc, lo, kWave);
         zacc.E = __shfl(zacc.E, lo, kWave);
+    }
+    if (idir == 0) {
+        // a forward item: the pair's Z part (this panel's) and range flag, then the tile's count
+        int fb = bad ? 1 : 0;
+        for (int off = 1; off < kWave; off <<= 1) {
+            const int o = __shfl_down(fb, off, kWave);
+            if (g + off < G) fb |= o;
+        }
+        if (g == 0 && q >= 0) {
+            double* zp = A.lzpart + ((poff + ipp) * kTilePairs + q) * 4;
+            zp[0] = zacc.acc;
+            zp[1] = (double)zacc.E;
+            zp[2] = (double)fb;
+        }
+        __threadfence();
+        if (lane == 0) __hip_atomic_fetch_add(sw + 2 * NP, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        return;
     }
     const bool zbad = !(zacc.acc > 0.0) || !(zacc.acc <= 1.7976931348623157e308);
     // normalise: Z = zr * 2^ze, zr in [1, 2)
@@ -558,7 +675,22 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
     const bool col0_opens = !L;           // column-0 X opens count (kmin0 == 0)
     const int hrow = na + 1;   // histogram rows carry a spill column for the no-column letter
     double* __restrict__ hq = hist + (q < 0 ? 0 : q) * na * hrow;
-    for (int pp = NP - 1; pp >= 0; --pp) {
+    // Rows run m .. 0 at steps u = G-1-g .. G-1-g+m; the step count is rounded up to two
+    // bands (steps in which no lane is active) so that the loop body holds both parities.
+    const int bsteps = (td.fsteps + 2 * kCK) / (2 * kCK) * (2 * kCK);
+    for (int pp = pb1 - 1; pp >= pb0; --pp) {
+        // Pipelined: the next panel's backward steps, as the forward waits (mirrored: the
+        // row that lane G-1 reads at step u was written by the next panel's lane 0 at its
+        // step u + G).
+        int32_t* bin = nullptr;
+        int32_t* bout = nullptr;
+        if (idir == 1) {
+            if (pp + 1 < NP) {
+                bin = sw + NP + pp + 1;
+                bounded_wait(bin, min(kPipeChunk + kWave, bsteps), serr, A.wait_ticks, lane, 1);
+            }
+            if (pp > 0) bout = sw + NP + pp;
+        }
         const int j0 = pp * PW + g * S;
         const int ncols = n - j0 < 0 ? 0 : (n - j0 > S ? S : n - j0);
         const bool live = p >= 0 && (ncols > 0 || j0 == 0);
@@ -566,7 +698,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
 #pragma unroll
         for (int k = 0; k < S; ++k) coff[k] = (k < ncols) ? (int)bp[j0 + k] : na;   // na: no column
         coff[S] = (j0 + S < n) ? (int)bp[j0 + S] : na;
-        const SF* __restrict__ seg = V.seg + (pbase + pp) * rows * LY::row;
+        const SF* __restrict__ seg = V.seg + (rbase + pp * rows) * LY::row;
         // Row i+1 of the lane's columns.  Row m starts from a virtual row m+1 whose profile
         // is 1 (letter index na) and whose BM holds the end weights one column to the right,
         // so that row m's d = P BM(m+1, j+1) is its end weight: Global 1 at column n only,
@@ -624,8 +756,8 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         SF qM = SF(0), qY = SF(0);
         int qE = 0;
         if (pin && live) {
-            const SF* bq = bnd(pbase + pp + 1, m);
-            qM = bq[3]; qY = bq[4]; qE = bex(pbase + pp + 1, m)[1];
+            const SF* bq = bnd(pp + 1, m);
+            qM = bq[3]; qY = bq[4]; qE = bex(pp + 1, m)[1];
         }
         int acode = na;                // code of row i+1 (row m: the virtual row)
         SF pubM = SF(0), pubY = SF(0);
@@ -672,8 +804,8 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             if (pin) {
                 rM = qM; rY = qY; rE = qE;
                 if (i >= 1) {
-                    const SF* bq = bnd(pbase + pp + 1, i - 1);
-                    qM = bq[3]; qY = bq[4]; qE = bex(pbase + pp + 1, i - 1)[1];
+                    const SF* bq = bnd(pp + 1, i - 1);
+                    qM = bq[3]; qY = bq[4]; qE = bex(pp + 1, i - 1)[1];
                 }
             }
             const SF* __restrict__ pn = spsub + ac * (na + 1);   // row i+1's profile
@@ -795,9 +927,9 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             bzero = !(mx > SF(0));
             pubM = BM[0]; pubE = EB;
             if (pout) {
-                SF* bo = bnd(pbase + pp, i);
+                SF* bo = bnd(pp, i);
                 bo[3] = pubM; bo[4] = pubY;
-                bex(pbase + pp, i)[1] = pubE;
+                bex(pp, i)[1] = pubE;
             }
         };
 
@@ -827,15 +959,16 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 step(i, rM, rY, rE, Sb);
             }
         };
-        // Rows run m .. 0 at steps u = G-1-g .. G-1-g+m; the step count is rounded up to two
-        // bands (steps in which no lane is active) so that the loop body holds both parities.
-        const int bsteps = (td.fsteps + 2 * kCK) / (2 * kCK) * (2 * kCK);
         for (int u = 0; u < bsteps; u += 2 * kCK) {
+            if (bin != nullptr && u > 0 && u % kPipeChunk == 0)
+                bounded_wait(bin, min(u + kPipeChunk + kWave, bsteps), serr, A.wait_ticks, lane, 1);
 #pragma unroll
             for (int p = 0; p < kCK; ++p) phase(u + p, p, Fa, Fb);
 #pragma unroll
             for (int p = 0; p < kCK; ++p) phase(u + kCK + p, p, Fb, Fa);
+            if (bout != nullptr && (u + 2 * kCK) % kPipeChunk == 0) publish(bout, u + 2 * kCK, lane);
         }
+        if (bout != nullptr) publish(bout, bsteps, lane);
         if (NP > 1) __threadfence();   // the panel's boundary column is read by the previous panel
     }
 
@@ -853,9 +986,68 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         }
     }
     wave_sync();
+    const int hsz = na * hrow;
+    const double logz = zbad ? 0.0 : ::log(zr) + (double)ze * 0.69314718055994530942;
+    if (idir == 1) {
+        // A backward item: this panel's histogram, sums and flag as the tile's part, then the
+        // tile's count; the last backward item adds the parts of every panel in panel order.
+        const int np = td.npairs, pst = nn + 6;
+        const int64_t pstride = (int64_t)A.hist_slots * pst;
+        double* __restrict__ P0 = A.lpart + (poff + ipp) * pstride;
+        for (int idx = lane; idx < np * hsz; idx += kWave) {
+            const int qq = idx / hsz, r = idx - qq * hsz;
+            const int a = r / hrow, c = r - a * hrow;
+            if (c < na) P0[qq * pst + a * na + c] = hist[idx];
+            hist[idx] = 0.0;
+        }
+        if (g == 0 && q >= 0) {
+            double* pq = P0 + q * pst + nn;
+            pq[0] = g_oa; pq[1] = g_ea; pq[2] = g_ob; pq[3] = g_eb; pq[4] = loss;
+            pq[5] = bad ? 1.0 : 0.0;
+        }
+        __threadfence();
+        const int prev = wave_uniform(
+            __hip_atomic_fetch_add(sw + 2 * NP + 1, lane == 0 ? 1 : 0, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT));
+        if (prev != NP - 1) {
+            wave_sync();
+            return;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const double* __restrict__ PB = A.lpart + poff * pstride;
+        unsigned nonfinite = 0u;
+        for (int idx = lane; idx < np * nn; idx += kWave) {
+            const int qq = idx / nn, r = idx - qq * nn;
+            double v = 0.0;
+            for (int k = 0; k < NP; ++k) v += PB[k * pstride + qq * pst + r];
+            A.grads[(int64_t)order[td.first + qq] * gs + r] = v;
+            if (!(v <= 1.0e300)) nonfinite |= 1u << qq;
+        }
+        for (int off = 1; off < kWave; off <<= 1) nonfinite |= (unsigned)__shfl_xor((int)nonfinite, off, kWave);
+        if (g == 0 && p >= 0) {
+            double s5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            bool bb = false;
+            for (int k = 0; k < NP; ++k) {
+                const double* pq = PB + k * pstride + q * pst + nn;
+#pragma unroll
+                for (int c = 0; c < 5; ++c) s5[c] += pq[c];
+                const double* zp = A.lzpart + ((poff + k) * kTilePairs + q) * 4;
+                bb = bb || pq[5] != 0.0 || zp[2] != 0.0;
+            }
+            const bool okp = !bb && !zbad && s5[4] <= A.tol && !((nonfinite >> q) & 1u);
+            A.ok[p] = okp ? 1 : 0;
+            A.scores[p] = logz * A.temperature;
+            double* gr = A.grads + (int64_t)p * gs + nn;
+            gr[0] = A.linear ? 0.0 : -s5[0];
+            gr[1] = -s5[1];
+            gr[2] = A.linear ? 0.0 : -s5[2];
+            gr[3] = -s5[3];
+        }
+        publish(A.lsync + 1 + tix, 1, lane);
+        return;
+    }
     // Expected match counts out of the histogram (and the spill column cleared); a count that
     // left the double range marks its pair.
-    const int np = td.npairs, hsz = na * hrow;
+    const int np = td.npairs;
     unsigned nonfinite = 0u;
     for (int idx = lane; idx < np * hsz; idx += kWave) {
         const int qq = idx / hsz, r = idx - qq * hsz;
@@ -872,7 +1064,6 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
     if (g == 0 && p >= 0) {
         const bool okp = !bad && !zbad && loss <= A.tol && !((nonfinite >> q) & 1u);
         A.ok[p] = okp ? 1 : 0;
-        const double logz = zbad ? 0.0 : ::log(zr) + (double)ze * 0.69314718055994530942;
         A.scores[p] = logz * A.temperature;
         double* gr = A.grads + (int64_t)p * gs + nn;
         gr[0] = A.linear ? 0.0 : -g_oa;
@@ -880,23 +1071,21 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         gr[2] = A.linear ? 0.0 : -g_ob;
         gr[3] = -g_eb;
     }
+    if (idir == 2) publish(A.lsync + 1 + tix, 1, lane);   // its pool region is free
     wave_sync();
 }
 
-// Persistent: blocks below n_lslots first take wide tiles (column panels) from the wide
-// queue, then blocks below n_mslots take tiles from the main queue; each block works in
-// its own scratch slot.  Plain (no queue): block b runs tile tile_base + b.  One call site of
-// the tile body, so the kernel holds one copy of it.
 // The soft weights from the parameter vector on the device, as the reference forms them
 // (aligner.hpp:6870-6885): exp(s / T) per matrix entry and exp(-(go+ge)/T), exp(-ge/T) per
 // sequence (linear gaps: open = extend), each in double and cast once.  Also zeroes the
-// persistent launch's queue counters (a kernel, not a memset, so that a captured step holds
-// no memset node).
+// persistent launch's queue counters and the pipelined wide tiles' words (a kernel, not a
+// memset, so that a captured step holds no memset node).
 template <typename SF>
 __global__ __launch_bounds__(256) void soft_weights_kernel(const double* __restrict__ theta, int nn, double it,
                                                            int linear, SF* __restrict__ psub, SF* __restrict__ gapw,
-                                                           int32_t* q0, int32_t* q1) {
+                                                           int32_t* q0, int32_t* q1, int32_t* sync, int64_t nsync) {
     for (int k = threadIdx.x; k < nn; k += blockDim.x) psub[k] = (SF)::exp(theta[k] * it);
+    for (int64_t k = threadIdx.x; k < nsync; k += blockDim.x) sync[k] = 0;
     if (threadIdx.x == 0) {
         const double go_a = theta[nn], ge_a = theta[nn + 1], go_b = theta[nn + 2], ge_b = theta[nn + 3];
         const double ea = ::exp(-ge_a * it), eb = ::exp(-ge_b * it);
@@ -905,10 +1094,18 @@ __global__ __launch_bounds__(256) void soft_weights_kernel(const double* __restr
         gapw[2] = (SF)(linear ? eb : ::exp(-(go_b + ge_b) * it));
         gapw[3] = (SF)eb;
         if (q0) *q0 = 0;
-        if (q1) *q1 = 0;
+        if (q1) {
+            q1[0] = 0;
+            q1[1] = 0;
+        }
     }
 }
 
+// Persistent: blocks below n_pwaves first take items of pipelined wide tiles (each tile in its
+// own region of the wide pool), then blocks below n_lslots whole wide tiles, then blocks
+// below n_mslots one-panel tiles, each of the last two in the block's own scratch slot.
+// Plain (no queue): block b runs tile tile_base + b in slot b.  One call site of the tile
+// body, so the kernel holds one copy of it.
 // Occupancy: the float32 pass is held to two waves per SIMD (256 VGPRs); the double pass, twice
 // the register bytes, to one.
 template <typename SF, int AM, int S>
@@ -930,30 +1127,52 @@ void soft_kernel(SoftArgs<SF> A) {
     for (int k = lane; k < A.hist_slots * hsz; k += kWave) hist[k] = 0.0;
     __syncthreads();
     const int blk = (int)blockIdx.x;
-    int phase = A.queue == nullptr ? 2 : ((A.lqueue != nullptr && blk < A.n_lslots) ? 0 : 1);
+    int phase = A.queue == nullptr ? 3 : 0;
     for (;;) {
-        int tile = 0;
+        int tile = 0, ipp = 0, idir = -1;
         bool wide = false;
-        if (phase == 2) {
+        if (phase == 3) {
             tile = A.tile_base + blk;
             if (tile >= A.n_tiles) break;
-            phase = 3;   // one tile only
-        } else if (phase == 3) {
+            phase = 4;   // one tile only
+        } else if (phase == 4) {
             break;
         } else {
-            if (phase == 0) {
-                tile = wave_uniform(atomicAdd(A.lqueue, lane == 0 ? 1 : 0));
-                if (tile < A.n_ltiles) wide = true;
-                else phase = 1;
+            if (phase == 0) {   // items of pipelined wide tiles
+                phase = 1;
+                if (A.litems != nullptr && blk < A.n_pwaves) {
+                    const int k = wave_uniform(atomicAdd(A.lqueue, lane == 0 ? 1 : 0));
+                    if (k < A.n_litems) {
+                        const int2 it = A.litems[k];
+                        tile = wave_uniform(it.x);
+                        const int code = wave_uniform(it.y);
+                        ipp = code < 0 ? 0 : code >> 1;
+                        idir = code < 0 ? 2 : code & 1;
+                        wide = true;
+                        phase = 0;
+                    }
+                }
             }
-            if (phase == 1) {
+            if (phase == 1) {   // whole wide tiles
+                phase = 2;
+                if (blk < A.n_lslots) {
+                    const int k = wave_uniform(atomicAdd(A.lqueue + 1, lane == 0 ? 1 : 0));
+                    if (A.n_ptiles + k < A.n_ltiles) {
+                        tile = A.n_ptiles + k;
+                        wide = true;
+                        phase = 1;
+                    }
+                }
+            }
+            if (phase == 2) {   // one-panel tiles
                 if (blk >= A.n_mslots) break;
                 tile = wave_uniform(atomicAdd(A.queue, lane == 0 ? 1 : 0));
                 if (tile >= A.n_tiles) break;
             }
         }
         soft_tile<SF, AM, S>(A, wide ? A.lview : A.view, wide ? A.lorder : A.order,
-                             (wide ? A.ltiles : A.tiles) + tile, (int64_t)blk, spsub, hist, lacc, lane);
+                             (wide ? A.ltiles : A.tiles) + tile, (int64_t)blk, spsub, hist, lacc, lane, tile, ipp,
+                             idir);
     }
 }
 

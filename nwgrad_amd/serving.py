@@ -117,16 +117,20 @@ class Server:
                 raise TimeoutError("nwgrad_amd: the server did not drain in time")
 
 
-def replay(server: Server, requests, *, idle_sleep=2e-4, timeout=900.0):
+def replay(server: Server, requests, *, idle_sleep=2e-4, timeout=900.0, progress=None, progress_every=10.0):
     """Submit ``requests`` (objects with ``t`` (s from start), ``kind`` and encoded pairs) at
     their arrival times, polling the server in between; wait for all of them.  Returns a dict:
     per-job latency (submit to completion) p50 / p99 / max in ms, jobs, failed jobs, cells, the
-    span from the first submit to the last completion, GCUPS over it, launches."""
+    span from the first submit to the last completion, GCUPS over it, launches.
+    ``progress``: called every ``progress_every`` seconds with a dict of the replay's state
+    (seconds since start, jobs submitted and completed, jobs waiting, launches running)."""
     order = sorted(range(len(requests)), key=lambda k: requests[k].t)
     ids = []
     cells = 0
+    done = 0
     t0 = time.perf_counter()
     t_end = t0 + timeout
+    t_report = progress_every
     i = 0
     while i < len(order) or server.waiting or server.running_launches:
         now = time.perf_counter() - t0
@@ -135,7 +139,13 @@ def replay(server: Server, requests, *, idle_sleep=2e-4, timeout=900.0):
             ids.append(server.submit(q.kind, q.codes_a, q.off_a, q.codes_b, q.off_b))
             cells += q.cells
             i += 1
-        if server.poll() == 0:
+        if progress is not None and now >= t_report:
+            progress({"t_s": now, "submitted": i, "of": len(order), "completed": done, "waiting": server.waiting,
+                      "running": server.running_launches})
+            t_report += progress_every
+        finished = server.poll()
+        done += finished
+        if finished == 0:
             nxt = requests[order[i]].t - (time.perf_counter() - t0) if i < len(order) else idle_sleep
             time.sleep(max(0.0, min(idle_sleep, nxt)))
         if time.perf_counter() > t_end:

@@ -138,3 +138,91 @@ def test_column_panels(gpu, mode):
         for fill in ("interpair", "striped"):
             b = batch(wl.AA, mat, gaps, ca, oa, cb, ob, precision=prec, mode=mode, grad_mode="soft", fill=fill)
             check(b, s_ref, g_ref, tol, (np.diff(oa), np.diff(ob)))
+
+
+@pytest.mark.parametrize("alphabet", ["protein", "dna"])
+def test_wide_pipeline_pool(gpu, alphabet):
+    """Pairs wider than 512 columns run (persistent fill) as panel items pipelined across waves:
+    a panel's forward and its backward are separate items, waiting only for items queued before
+    them.  With the soft scratch held small the wide tiles go round a small circular pool -- a
+    tile's first item waits for the tiles whose rows it reuses -- down to a pool of the largest
+    tile alone.  The results do not depend on the pool: bit for bit the same as with the full
+    scratch, and within tolerance of the oracle.  DNA packs several wide pairs into one tile."""
+    r = wl.rng(f"soft_ring_{alphabet}", "lens")
+    na_alpha = 20 if alphabet == "protein" else 4
+    n = 36
+    nb = np.concatenate([r.integers(513, 2600, size=n), r.integers(1, 400, size=12)])
+    la = np.concatenate([r.integers(0, 260, size=n), r.integers(0, 300, size=12)])
+    A = [r.integers(0, na_alpha, size=int(x)).astype(np.uint8) for x in la]
+    B = [r.integers(0, na_alpha, size=int(x)).astype(np.uint8) for x in nb]
+    for k in range(0, len(A), 2):   # half related: A is a mutated prefix of B
+        if len(A[k]):
+            A[k] = wl.mutate(r, B[k][:len(A[k])], na_alpha, 0.2, 0.02, 3.0)
+    ca, oa = wl._pack(A)
+    cb, ob = wl._pack(B)
+    mat = wl.random_matrix(r, na_alpha)
+    gaps = np.array([10.0, 1.0, 9.0, 1.5])
+    alpha = wl.AA if alphabet == "protein" else wl.DNA
+    say("case soft wide pool", alphabet, len(nb), "pairs")
+    s_ref, g_ref = oracle.soft_log(mat, gaps, ca, oa, cb, ob)
+    lens = (np.diff(oa), np.diff(ob))
+    for prec, tol in (("float32", 1e-5), ("float64", 1e-10)):
+        b = batch(alpha, mat, gaps, ca, oa, cb, ob, precision=prec, grad_mode="soft")
+        check(b, s_ref, g_ref, tol, lens)
+        tiles, pooled, pool, lag = b.wide_info
+        say(f"  {prec}: {tiles} pipelined tiles, pool {pool} rows, lag {lag}, launch {b.launch_info}")
+        assert pooled == tiles
+        assert tiles >= 1 and pool >= 1   # a small batch: every wide tile is pipelined
+        s0, g0 = b.scores(), grads_rows(b)
+        for scratch in (40 << 20, 1):   # a pool of a few tiles, then of the largest tile alone
+            b.set_soft_scratch(scratch)
+            b.score_and_grad()
+            tiles2, _, pool2, lag2 = b.wide_info
+            say(f"  scratch {scratch}: pool {pool2} rows, lag {lag2}")
+            assert tiles2 == tiles and 1 <= pool2 < pool
+            np.testing.assert_array_equal(b.scores(), s0)
+            np.testing.assert_array_equal(grads_rows(b), g0)
+        b.set_soft_scratch(0)
+
+
+def test_wide_pipeline_mixed(gpu):
+    """Pipelined and whole wide tiles in one launch: a wide tile whose panels in sequence would
+    outlast the launch's work per resident wave (here the two 1,500 x 1,800 pairs) runs as
+    panel items across waves, the short wide pairs (20 x 600) run whole, next to 2,500
+    one-panel pairs.  The wide pairs against the oracle, the one-panel ones against a batch of
+    their own; unchanged bit for bit with the smallest pool."""
+    r = wl.rng("soft_mixed_wide", "lens")
+    la = np.concatenate([r.integers(150, 250, size=2500), np.full(300, 20), [1500, 1500]])
+    lb = np.concatenate([r.integers(250, 350, size=2500), np.full(300, 600), [1800, 1800]])
+    A = [r.integers(0, 20, size=int(x)).astype(np.uint8) for x in la]
+    B = [wl.mutate(r, a, 20, 0.25, 0.02, 3.0) for a in A]
+    B = [np.concatenate([b_, r.integers(0, 20, size=max(0, int(n) - len(b_))).astype(np.uint8)])[:int(n)]
+         for b_, n in zip(B, lb)]
+    ca, oa = wl._pack(A)
+    cb, ob = wl._pack(B)
+    mat = wl.random_matrix(r)
+    gaps = np.array([11.0, 1.0, 11.0, 1.0])
+    say("case soft mixed wide", len(A), "pairs")
+    # the oracle on the wide pairs (the one-panel ones against a batch of their own)
+    wide = np.arange(2500, len(A))
+    cwa, owa = wl._pack([A[k] for k in wide])
+    cwb, owb = wl._pack([B[k] for k in wide])
+    s_w, g_w = oracle.soft_log(mat, gaps, cwa, owa, cwb, owb, mode="glocal")
+    na_ = wl._pack(A[:2500])
+    nb_ = wl._pack(B[:2500])
+    bn = batch(wl.AA, mat, gaps, na_[0], na_[1], nb_[0], nb_[1], mode="glocal", grad_mode="soft")
+    bn.score_and_grad()
+    s_ref = np.concatenate([bn.scores(), s_w])
+    g_ref = np.concatenate([grads_rows(bn), g_w])
+    lens = (np.diff(oa), np.diff(ob))
+    b = batch(wl.AA, mat, gaps, ca, oa, cb, ob, mode="glocal", grad_mode="soft")
+    check(b, s_ref, g_ref, 1e-5, lens)
+    tiles, pooled, pool, lag = b.wide_info
+    say(f"  {tiles} pipelined tiles, {pooled} in the pool, pool {pool} rows, lag {lag}, launch {b.launch_info}")
+    assert tiles == 2 and pooled == 2   # the short wide pairs run whole in block slots
+    s0, g0 = b.scores(), grads_rows(b)
+    b.set_soft_scratch(1)
+    b.score_and_grad()
+    assert b.wide_info[2] < pool
+    np.testing.assert_array_equal(b.scores(), s0)
+    np.testing.assert_array_equal(grads_rows(b), g0)
