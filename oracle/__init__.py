@@ -86,6 +86,11 @@ def lib():
                                        P(ctypes.c_int64), P(ctypes.c_uint8), P(ctypes.c_int64), P(ctypes.c_uint16),
                                        P(ctypes.c_uint16), ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)]
         L.nwo_hard_tracked.restype = ctypes.c_int
+        L.nwo_soft_log_tracked.argtypes = [P(_TParams), ctypes.c_double, ctypes.c_int64, P(ctypes.c_uint8),
+                                           P(ctypes.c_int64), P(ctypes.c_uint8), P(ctypes.c_int64),
+                                           P(ctypes.c_uint16), P(ctypes.c_uint16), P(ctypes.c_double),
+                                           P(ctypes.c_double)]
+        L.nwo_soft_log_tracked.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -209,3 +214,29 @@ def hard_tracked(matrices, gaps, codes_a, off_a, codes_b, off_b, tracks, gap_tra
     if rc:
         raise MemoryError("oracle: allocation failed")
     return scores, (grads if with_grad else None)
+
+
+def soft_log_tracked(matrices, gaps, codes_a, off_a, codes_b, off_b, tracks, gap_tracks=None, *,
+                     gap_model="affine", mode="global", temperature=1.0):
+    """Tracked log-space double forward-backward: K parameter sets (matrices (K, na, na), gaps
+    (K, 4)), tracks / gap_tracks as for hard_tracked.  Scores T*logZ (N,) and expected counts
+    (N, K*(na*na + 4)): K matrix blocks, then K rows of go_a, ge_a, go_b, ge_b."""
+    mats = np.ascontiguousarray(matrices, np.float64)
+    K, na = mats.shape[0], mats.shape[1]
+    g4 = np.ascontiguousarray(gaps, np.float64).reshape(K, 4)
+    p = _TParams(GAP_MODELS[gap_model], MODES[mode], na, K, _ptr(mats, ctypes.c_double), _ptr(g4, ctypes.c_double))
+    ca, oa, cb, ob = _pairs(codes_a, off_a, codes_b, off_b)
+    N = len(oa) - 1
+    trk = np.ascontiguousarray(tracks, np.uint16)
+    gtrk = default_gap_tracks(trk, oa) if gap_tracks is None else np.ascontiguousarray(gap_tracks, np.uint16)
+    if trk.size == 0:
+        trk = np.zeros(1, np.uint16)
+    scores = np.zeros(N, np.float64)
+    grads = np.zeros((N, K * (na * na + 4)), np.float64)
+    rc = lib().nwo_soft_log_tracked(ctypes.byref(p), float(temperature), N, _ptr(ca, ctypes.c_uint8),
+                                    _ptr(oa, ctypes.c_int64), _ptr(cb, ctypes.c_uint8), _ptr(ob, ctypes.c_int64),
+                                    _ptr(trk, ctypes.c_uint16), _ptr(gtrk, ctypes.c_uint16),
+                                    _ptr(scores, ctypes.c_double), _ptr(grads, ctypes.c_double))
+    if rc:
+        raise MemoryError("oracle: allocation failed")
+    return scores, grads

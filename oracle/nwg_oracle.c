@@ -841,3 +841,227 @@ int nwo_hard_tracked(const nwo_tparams *p, int precision, int64_t N, const uint8
     free(D); free(rows); free(rowp); free(blk32); free(blk64);
     return 0;
 }
+
+/* ── tracked soft: the log-space pass of aligner.hpp:6231-6524 with each row's penalties
+   (drow_go_a / drow_ge_a of gap set gtrk[i], drow_go_b / drow_ge_b and the matrix of set
+   trk[i-1], aligner.hpp:1903-1916, as the tracked scaled job takes them, aligner.hpp:
+   6845-6881), the record kept per row (soft_record_log_, aligner.hpp:6647-6700) and added
+   into the sets of its rows (add_soft_record_, aligner.hpp:6628-6645): matches of row i into
+   set trk[i-1]'s block, row i's gap-in-A counts into set gtrk[i], its gap-in-B counts into
+   set trk[i-1].  Column 0 of Global / Glocal is the vertical chain of row penalties.
+   grads: K blocks of na*na, then K rows of go_a, ge_a, go_b, ge_b. */
+static void soft_affine_pair_tr(const nwo_tparams *p, double Tm, const uint8_t *a, const uint16_t *trk,
+                                const uint16_t *gtrk, int m, const uint8_t *b, int n, const double *goa,
+                                const double *gea, const double *gob, const double *geb, double *buf, double *score,
+                                double *g) {
+    const int AM = p->mode, na = p->n_alpha, nn = na * na, K = p->K;
+    const double it = 1.0 / Tm;
+    const size_t st = (size_t)n + 1, sz = (size_t)(m + 1) * st;
+    double *FM = buf, *FX = buf + sz, *FY = buf + 2 * sz, *BM = buf + 3 * sz, *BX = buf + 4 * sz,
+           *BY = buf + 5 * sz;
+#define AT(t, i, j) (t)[(size_t)(i) * st + (size_t)(j)]
+#define SUB(i, j) (p->matrices[((size_t)trk[(i) - 1] * na + a[(i) - 1]) * na + b[(j) - 1]] * it)
+#define GOA(i) (goa[i] * it)
+#define GEA(i) (gea[i] * it)
+#define GOB(i) (gob[i] * it)
+#define GEB(i) (geb[i] * it)
+    for (size_t k = 0; k < sz; ++k) { FM[k] = FX[k] = FY[k] = -INFINITY; }
+    if (AM != 1) {
+        AT(FM, 0, 0) = 0.0;
+        for (int i = 1; i <= m; ++i) AT(FX, i, 0) = (i == 1) ? -(GOB(1) + GEB(1)) : AT(FX, i - 1, 0) - GEB(i);
+        if (AM == 0) for (int j = 1; j <= n; ++j) AT(FY, 0, j) = -(GOA(0) + (double)j * GEA(0));
+        if (AM == 2) for (int j = 0; j <= n; ++j) AT(FM, 0, j) = 0.0;
+    } else {
+        for (int i = 0; i <= m; ++i) AT(FM, i, 0) = 0.0;
+        for (int j = 0; j <= n; ++j) AT(FM, 0, j) = 0.0;
+    }
+    for (int i = 1; i <= m; ++i)
+        for (int j = 1; j <= n; ++j) {
+            const double diag = lse3(AT(FM, i - 1, j - 1), AT(FX, i - 1, j - 1), AT(FY, i - 1, j - 1));
+            double mv = diag + SUB(i, j);
+            const double xv = lse3(AT(FM, i - 1, j) - GOB(i) - GEB(i), AT(FX, i - 1, j) - GEB(i),
+                                   AT(FY, i - 1, j) - GOB(i) - GEB(i));
+            const double yv = lse3(AT(FM, i, j - 1) - GOA(i) - GEA(i), AT(FX, i, j - 1) - GOA(i) - GEA(i),
+                                   AT(FY, i, j - 1) - GEA(i));
+            if (AM == 1) mv = lse2(mv, 0.0);
+            AT(FM, i, j) = mv; AT(FX, i, j) = xv; AT(FY, i, j) = yv;
+        }
+    double lz;
+    if (AM == 0) lz = lse3(AT(FM, m, n), AT(FX, m, n), AT(FY, m, n));
+    else if (AM == 2) {
+        lz = -INFINITY;
+        for (int j = 0; j <= n; ++j) lz = lse2(lz, lse3(AT(FM, m, j), AT(FX, m, j), AT(FY, m, j)));
+    } else {
+        lz = -INFINITY;
+        for (int i = 0; i <= m; ++i)
+            for (int j = 0; j <= n; ++j) lz = lse2(lz, lse3(AT(FM, i, j), AT(FX, i, j), AT(FY, i, j)));
+    }
+    const double e0 = (AM == 1) ? 0.0 : -INFINITY;
+    for (size_t k = 0; k < sz; ++k) { BM[k] = BX[k] = BY[k] = e0; }
+    if (AM == 0) { AT(BM, m, n) = AT(BX, m, n) = AT(BY, m, n) = 0.0; }
+    if (AM == 2) for (int j = 0; j <= n; ++j) { AT(BM, m, j) = AT(BX, m, j) = AT(BY, m, j) = 0.0; }
+    for (int i = m; i >= 0; --i)
+        for (int j = n; j >= 0; --j) {
+            const double bm = AT(BM, i, j), bx = AT(BX, i, j), by = AT(BY, i, j);
+            if (bm != -INFINITY && i > 0 && j > 0) {
+                const double c = bm + SUB(i, j);
+                AT(BM, i - 1, j - 1) = lse2(AT(BM, i - 1, j - 1), c);
+                AT(BX, i - 1, j - 1) = lse2(AT(BX, i - 1, j - 1), c);
+                AT(BY, i - 1, j - 1) = lse2(AT(BY, i - 1, j - 1), c);
+            }
+            if (bx != -INFINITY && i > 0) {
+                AT(BM, i - 1, j) = lse2(AT(BM, i - 1, j), bx - GOB(i) - GEB(i));
+                AT(BX, i - 1, j) = lse2(AT(BX, i - 1, j), bx - GEB(i));
+                AT(BY, i - 1, j) = lse2(AT(BY, i - 1, j), bx - GOB(i) - GEB(i));
+            }
+            if (by != -INFINITY && j > 0) {
+                AT(BM, i, j - 1) = lse2(AT(BM, i, j - 1), by - GOA(i) - GEA(i));
+                AT(BX, i, j - 1) = lse2(AT(BX, i, j - 1), by - GOA(i) - GEA(i));
+                AT(BY, i, j - 1) = lse2(AT(BY, i, j - 1), by - GEA(i));
+            }
+        }
+    const int gmin_j = (AM == 1) ? 1 : 0, gmin_i = (AM == 0) ? 0 : 1;
+    double *gg = g + (size_t)K * nn;
+    memset(g, 0, (size_t)K * (nn + 4) * sizeof(double));
+    for (int i = 1; i <= m; ++i)
+        for (int j = 1; j <= n; ++j) {
+            const double bm = AT(BM, i, j);
+            if (bm == -INFINITY) continue;
+            const double pred = lse3(AT(FM, i - 1, j - 1), AT(FX, i - 1, j - 1), AT(FY, i - 1, j - 1));
+            g[((size_t)trk[i - 1] * na + a[i - 1]) * na + b[j - 1]] += exp(pred + SUB(i, j) + bm - lz);
+        }
+    for (int i = 1; i <= m; ++i)
+        for (int j = 0; j <= n; ++j) {
+            const double fx = AT(FX, i, j), bx = AT(BX, i, j);
+            if (fx == -INFINITY || bx == -INFINITY) continue;
+            gg[4 * trk[i - 1] + 3] -= exp(fx + bx - lz);
+        }
+    for (int i = 0; i <= m; ++i)
+        for (int j = 1; j <= n; ++j) {
+            const double fy = AT(FY, i, j), by = AT(BY, i, j);
+            if (fy == -INFINITY || by == -INFINITY) continue;
+            gg[4 * gtrk[i] + 1] -= exp(fy + by - lz);
+        }
+    for (int i = 1; i <= m; ++i)
+        for (int j = gmin_j; j <= n; ++j) {
+            const double bx = AT(BX, i, j);
+            if (bx == -INFINITY) continue;
+            const double lo = lse2(AT(FM, i - 1, j), AT(FY, i - 1, j)) - GOB(i) - GEB(i) + bx - lz;
+            if (lo > -700) gg[4 * trk[i - 1] + 2] -= exp(lo);
+        }
+    for (int i = gmin_i; i <= m; ++i)
+        for (int j = 1; j <= n; ++j) {
+            const double by = AT(BY, i, j);
+            if (by == -INFINITY) continue;
+            const double lo = lse2(AT(FM, i, j - 1), AT(FX, i, j - 1)) - GOA(i) - GEA(i) + by - lz;
+            if (lo > -700) gg[4 * gtrk[i] + 0] -= exp(lo);
+        }
+    *score = lz * Tm;
+#undef AT
+#undef SUB
+}
+
+static void soft_linear_pair_tr(const nwo_tparams *p, double Tm, const uint8_t *a, const uint16_t *trk,
+                                const uint16_t *gtrk, int m, const uint8_t *b, int n, const double *gea,
+                                const double *geb, double *buf, double *score, double *g) {
+    const int AM = p->mode, na = p->n_alpha, nn = na * na, K = p->K;
+    const double it = 1.0 / Tm;
+    const size_t st = (size_t)n + 1, sz = (size_t)(m + 1) * st;
+    double *F = buf, *B = buf + sz;
+#define AT(t, i, j) (t)[(size_t)(i) * st + (size_t)(j)]
+#define SUB(i, j) (p->matrices[((size_t)trk[(i) - 1] * na + a[(i) - 1]) * na + b[(j) - 1]] * it)
+    for (size_t k = 0; k < sz; ++k) F[k] = -INFINITY;
+    if (AM != 1) {
+        AT(F, 0, 0) = 0.0;
+        for (int i = 1; i <= m; ++i) AT(F, i, 0) = AT(F, i - 1, 0) - GEB(i);
+        if (AM == 0) for (int j = 1; j <= n; ++j) AT(F, 0, j) = -(double)j * GEA(0);
+        if (AM == 2) for (int j = 0; j <= n; ++j) AT(F, 0, j) = 0.0;
+    } else {
+        for (int i = 0; i <= m; ++i) AT(F, i, 0) = 0.0;
+        for (int j = 0; j <= n; ++j) AT(F, 0, j) = 0.0;
+    }
+    for (int i = 1; i <= m; ++i)
+        for (int j = 1; j <= n; ++j) {
+            double v = lse3(AT(F, i - 1, j - 1) + SUB(i, j), AT(F, i - 1, j) - GEB(i), AT(F, i, j - 1) - GEA(i));
+            if (AM == 1) v = lse2(v, 0.0);
+            AT(F, i, j) = v;
+        }
+    double lz;
+    if (AM == 0) lz = AT(F, m, n);
+    else if (AM == 2) { lz = -INFINITY; for (int j = 0; j <= n; ++j) lz = lse2(lz, AT(F, m, j)); }
+    else {
+        lz = -INFINITY;
+        for (int i = 0; i <= m; ++i) for (int j = 0; j <= n; ++j) lz = lse2(lz, AT(F, i, j));
+    }
+    const double e0 = (AM == 1) ? 0.0 : -INFINITY;
+    for (size_t k = 0; k < sz; ++k) B[k] = e0;
+    if (AM == 0) AT(B, m, n) = 0.0;
+    if (AM == 2) for (int j = 0; j <= n; ++j) AT(B, m, j) = 0.0;
+    for (int i = m; i >= 0; --i)
+        for (int j = n; j >= 0; --j) {
+            const double bv = AT(B, i, j);
+            if (bv == -INFINITY) continue;
+            if (i > 0 && j > 0) AT(B, i - 1, j - 1) = lse2(AT(B, i - 1, j - 1), bv + SUB(i, j));
+            if (i > 0) AT(B, i - 1, j) = lse2(AT(B, i - 1, j), bv - GEB(i));
+            if (j > 0) AT(B, i, j - 1) = lse2(AT(B, i, j - 1), bv - GEA(i));
+        }
+    const int gmin_j = (AM == 1) ? 1 : 0, gmin_i = (AM == 0) ? 0 : 1;
+    double *gg = g + (size_t)K * nn;
+    memset(g, 0, (size_t)K * (nn + 4) * sizeof(double));
+    for (int i = 1; i <= m; ++i)
+        for (int j = 1; j <= n; ++j) {
+            const double bv = AT(B, i, j);
+            if (bv == -INFINITY) continue;
+            g[((size_t)trk[i - 1] * na + a[i - 1]) * na + b[j - 1]] += exp(AT(F, i - 1, j - 1) + SUB(i, j) + bv - lz);
+        }
+    for (int i = 1; i <= m; ++i)
+        for (int j = gmin_j; j <= n; ++j) {
+            const double bv = AT(B, i, j), fv = AT(F, i - 1, j);
+            if (bv == -INFINITY || fv == -INFINITY) continue;
+            gg[4 * trk[i - 1] + 3] -= exp(fv - GEB(i) + bv - lz);
+        }
+    for (int i = gmin_i; i <= m; ++i)
+        for (int j = 1; j <= n; ++j) {
+            const double bv = AT(B, i, j), fv = AT(F, i, j - 1);
+            if (bv == -INFINITY || fv == -INFINITY) continue;
+            gg[4 * gtrk[i] + 1] -= exp(fv - GEA(i) + bv - lz);
+        }
+    *score = lz * Tm;
+#undef AT
+#undef SUB
+#undef GOA
+#undef GEA
+#undef GOB
+#undef GEB
+}
+
+/* Tracked batch soft entry (log space, double).  Returns 0, or -1 on allocation failure. */
+int nwo_soft_log_tracked(const nwo_tparams *p, double temperature, int64_t N, const uint8_t *ca, const int64_t *oa,
+                         const uint8_t *cb, const int64_t *ob, const uint16_t *trk, const uint16_t *gtrk,
+                         double *scores, double *grads) {
+    const int na = p->n_alpha, K = p->K;
+    const size_t gs = (size_t)K * (na * na + 4);
+    size_t maxcells = 1, maxm = 1;
+    for (int64_t q = 0; q < N; ++q) {
+        const size_t m = (size_t)(oa[q + 1] - oa[q]), n = (size_t)(ob[q + 1] - ob[q]);
+        if ((m + 1) * (n + 1) > maxcells) maxcells = (m + 1) * (n + 1);
+        if (m + 1 > maxm) maxm = m + 1;
+    }
+    double *buf = (double *)malloc(6 * maxcells * sizeof(double));
+    double *rowp = (double *)malloc(5 * maxm * sizeof(double));
+    if (!buf || !rowp) { free(buf); free(rowp); return -1; }
+    for (int64_t q = 0; q < N; ++q) {
+        const uint8_t *a = ca + oa[q], *b = cb + ob[q];
+        const int m = (int)(oa[q + 1] - oa[q]), n = (int)(ob[q + 1] - ob[q]);
+        const uint16_t *tq = trk + oa[q], *gq = gtrk + oa[q] + q;
+        double *goa = rowp, *gea = rowp + maxm, *gob = rowp + 2 * maxm, *geb = rowp + 3 * maxm, *bx0 = rowp + 4 * maxm;
+        track_rows_f64(p, tq, gq, m, goa, gea, gob, geb, bx0);
+        double *g = grads + (size_t)q * gs;
+        if (p->gap_model == 1) soft_affine_pair_tr(p, temperature, a, tq, gq, m, b, n, goa, gea, gob, geb, buf,
+                                                   &scores[q], g);
+        else soft_linear_pair_tr(p, temperature, a, tq, gq, m, b, n, gea, geb, buf, &scores[q], g);
+    }
+    free(buf);
+    free(rowp);
+    return 0;
+}

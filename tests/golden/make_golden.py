@@ -4,6 +4,7 @@ sources (oracle/_ref/libnwgref.so; see oracle/Makefile.ref).  Run in the build c
     make -f oracle/Makefile.ref && python tests/golden/make_golden.py        # all sets
     python tests/golden/make_golden.py long                                  # the long-pair sets only
     python tests/golden/make_golden.py tracks                                # the tracked sets only
+    python tests/golden/make_golden.py soft_tracks                           # the tracked soft sets only
 
 Each fixture is an .npz (no pickles) holding the inputs -- alphabet, matrix, gaps, problem
 type, encoded pairs -- and the reference's outputs: per-pair scores and gradients, and for the
@@ -132,10 +133,64 @@ def track_sets():
                         save_tracked(f"{tag}_tracks_{name}_{gm}_{mode}_{prec}", w, prec, gd, res, extra)
 
 
+def soft_track_sets():
+    """Tracked soft batches: K = 3 parameter sets over segment tracks (explicit gap tracks in
+    Global and Local, the default in Glocal), protein and DNA, both gap models, every mode,
+    float32 and float64.  A is a mutated copy of B (the scaled pass needs related pairs)."""
+    K = 3
+    for alpha, name, lens in ((wl.AA, "prot", (0, 120)), ("ACGT", "dna", (0, 60))):
+        n = len(alpha)
+        r = wl.rng(f"golden_soft_tracks_{name}", "params")
+        if n > 4:
+            mats = np.stack([wl.random_matrix(r, n, -3, 2, 2, 6) for _ in range(K)])
+        else:
+            mats = np.stack([np.where(np.eye(4) > 0, 2.0, -1.0), np.where(np.eye(4) > 0, 1.0, -2.0),
+                             np.where(np.eye(4) > 0, 3.0, -1.0)])
+        gaps_aff = np.array([[10.5, 0.75, 9.25, 1.5], [8.0, 1.25, 11.0, 0.5], [12.0, 2.0, 6.5, 1.5]])
+        if n == 4:
+            gaps_aff = np.array([[2.0, 1.0, 3.0, 1.0], [1.0, 2.0, 2.0, 0.5], [3.0, 0.5, 1.0, 1.0]])
+        for gm in ("affine", "linear"):
+            gaps = gaps_aff if gm == "affine" else gaps_aff * np.array([0.0, 1.0, 0.0, 1.0])
+            for mode in ("global", "local", "glocal"):
+                rs = wl.rng(f"golden_soft_tracks_{name}_{gm}_{mode}", "seqs")
+                A, B = [], []
+                for _ in range(24):
+                    b_ = rs.integers(0, n, size=int(rs.integers(lens[0], lens[1] + 1))).astype(np.uint8)
+                    a_ = wl.mutate(rs, b_, n, 0.2, 0.02, 2.0) if len(b_) else b_
+                    A.append(a_)
+                    B.append(b_)
+                ca, oa = wl._pack(A)
+                cb, ob = wl._pack(B)
+                w = wl.Workload(f"golden_soft_tracks_{name}_{gm}_{mode}", alpha, mats[0], gaps[0], gm, mode, "soft",
+                                ca, oa, cb, ob)
+                rt = wl.rng(w.name, "tracks")
+                tracks = []
+                for q in range(w.n_pairs):   # segment tracks: runs of one set
+                    m = int(w.off_a[q + 1] - w.off_a[q])
+                    t = np.zeros(m, np.uint16)
+                    pos = 0
+                    while pos < m:
+                        ln = int(rt.integers(1, 12))
+                        t[pos:pos + ln] = rt.integers(0, K)
+                        pos += ln
+                    tracks.append(t)
+                tracks = np.concatenate(tracks) if tracks else np.zeros(0, np.uint16)
+                gtracks = (rt.integers(0, K, size=int(w.off_a[-1]) + w.n_pairs).astype(np.uint16)
+                           if mode != "glocal" else None)
+                for prec in ("float32", "float64"):
+                    res = run_ref_tracked(w, mats, gaps, tracks, gtracks, prec, "soft")
+                    extra = {"matrices": mats, "gaps_k": gaps, "tracks": tracks}
+                    if gtracks is not None:
+                        extra["gap_tracks"] = gtracks
+                    save_tracked(f"soft_tracks_{name}_{gm}_{mode}_{prec}", w, prec, "soft", res, extra)
+
+
 def run_ref_tracked(w, mats, gaps, tracks, gtracks, precision, grad_mode):
     sa, sb = w.strings()
     b = ref.RefBatch(sa, sb, w.alphabet, mats, gaps, double=(precision == "float64"), gap_model=w.gap_model,
                      mode=w.mode, grad_mode=grad_mode, n_threads=1, tracks=tracks, gap_tracks=gtracks)
+    if grad_mode == "soft":
+        b.set_soft(temperature=1.0, precision=precision)
     total = b.score_and_grad()
     scores = b.scores()
     out = {"total": np.array(total), "scores": scores}
@@ -164,6 +219,9 @@ def main():
         return
     if sys.argv[1:] == ["tracks"]:
         track_sets()
+        return
+    if sys.argv[1:] == ["soft_tracks"]:
+        soft_track_sets()
         return
     for gm in ("affine", "linear"):
         for mode in ("global", "local", "glocal"):

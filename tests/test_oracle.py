@@ -34,7 +34,8 @@ def test_hard_matches_reference_golden(path):
         np.testing.assert_array_equal(oracle.weighted_sum(g, z["weights"]), z["weighted_grad"])
 
 
-@pytest.mark.parametrize("path", [p for p in GOLDEN if "/soft_" in p], ids=lambda p: os.path.basename(p))
+@pytest.mark.parametrize("path", [p for p in GOLDEN if "/soft_" in p and "_tracks_" not in p],
+                         ids=lambda p: os.path.basename(p))
 def test_soft_matches_reference_golden(path):
     """Log-space double vs the reference's scaled pass: 1e-12 against its double pass, and
     against its float32 pass within the reference's own float32 tolerance (2e-4 rel / 1e-5 abs,
@@ -195,7 +196,8 @@ def test_long_pairs_match_reference_engine():
         np.testing.assert_array_equal(g, np.concatenate([mats.reshape(2, -1), gaps], 1))
 
 
-TRACKED = [p for p in GOLDEN if "_tracks_" in p]
+TRACKED = [p for p in GOLDEN if "_tracks_" in p and "/soft_" not in p]
+SOFT_TRACKED = [p for p in GOLDEN if "/soft_tracks_" in p]
 
 
 @pytest.mark.parametrize("path", TRACKED, ids=lambda p: os.path.basename(p))
@@ -255,3 +257,77 @@ def test_tracked_score_is_counts_times_params():
     s, g = oracle.hard_tracked(mats, gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, tracks)
     theta = np.concatenate([mats.ravel(), gaps.ravel()])
     np.testing.assert_array_equal(g @ theta, s)
+
+
+@pytest.mark.parametrize("path", SOFT_TRACKED, ids=lambda p: os.path.basename(p))
+def test_soft_tracked_matches_reference_golden(path):
+    """Tracked soft batches (K = 3 sets, segment tracks, explicit or default gap tracks): the
+    oracle's log-space restatement against the reference engine's float64 pass within 1e-9,
+    and its float32 scaled pass within 1e-5 (norm-relative per pair)."""
+    z = np.load(path)
+    gt = z["gap_tracks"] if "gap_tracks" in z else None
+    s, g = oracle.soft_log_tracked(z["matrices"], z["gaps_k"], z["codes_a"], z["off_a"], z["codes_b"], z["off_b"],
+                                   z["tracks"], gt, gap_model=str(z["gap_model"]), mode=str(z["mode"]))
+    tol = 1e-9 if str(z["precision"]) == "float64" else 1e-5
+    assert np.max(np.abs(s - z["scores"]) / np.maximum(1.0, np.abs(z["scores"]))) <= tol
+    scale = np.maximum(np.max(np.abs(z["grads"]), axis=1), 1e-300)
+    assert np.max(np.max(np.abs(g - z["grads"]), axis=1) / scale) <= tol
+
+
+@pytest.mark.parametrize("gm", ["affine", "linear"])
+@pytest.mark.parametrize("mode", ["global", "local", "glocal"])
+def test_soft_tracks_with_equal_sets_reduce_to_untracked(gm, mode):
+    """K copies of one set under any tracks: the untracked soft scores, and per-set counts that
+    sum to the untracked counts (to double rounding)."""
+    w = wl.random_pairs(f"soft_trk_equal_{gm}_{mode}", 30, wl.AA, (0, 60), (0, 80), gap_model=gm, mode=mode,
+                        related=0.5, gaps=(10.5, 0.75, 9.25, 1.5) if gm == "affine" else (0, 0.75, 0, 1.5))
+    K, nn = 3, 400
+    r = wl.rng(w.name, "tracks")
+    tracks = r.integers(0, K, size=int(w.off_a[-1])).astype(np.uint16)
+    gtr = r.integers(0, K, size=int(w.off_a[-1]) + w.n_pairs).astype(np.uint16)
+    s0, g0 = oracle.soft_log(w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, gap_model=gm, mode=mode)
+    s1, g1 = oracle.soft_log_tracked(np.stack([w.matrix] * K), np.stack([w.gaps] * K), w.codes_a, w.off_a,
+                                     w.codes_b, w.off_b, tracks, gtr, gap_model=gm, mode=mode)
+    np.testing.assert_allclose(s1, s0, rtol=1e-12, atol=1e-12)
+    mats = g1[:, :K * nn].reshape(-1, K, nn).sum(axis=1)
+    gaps = g1[:, K * nn:].reshape(-1, K, 4).sum(axis=1)
+    np.testing.assert_allclose(np.concatenate([mats, gaps], 1), g0, rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize("gm", ["affine", "linear"])
+@pytest.mark.parametrize("mode", ["global", "local", "glocal"])
+def test_soft_tracked_counts_are_score_derivatives(gm, mode):
+    """Each tracked soft count is the derivative of T log Z with respect to its parameter of its
+    set (matches +, gap penalties - the expected gaps): central differences on every gap field
+    of every set and on matrix entries of every set, so a count routed to the wrong set, row or
+    field fails."""
+    r = wl.rng(f"soft_trk_fd_{gm}_{mode}", "all")
+    K, na = 3, 4
+    mats = np.stack([np.where(np.eye(na) > 0, 2.0, -1.0) + r.normal(0, 0.3, (na, na)) for _ in range(K)])
+    gaps = np.array([[2.0, 1.0, 3.0, 0.5], [1.0, 0.75, 2.5, 1.0], [3.0, 0.5, 1.5, 1.25]])
+    if gm == "linear":
+        gaps = gaps * np.array([0.0, 1.0, 0.0, 1.0])
+    A = [r.integers(0, na, size=int(r.integers(1, 9))).astype(np.uint8) for _ in range(3)]
+    B = [r.integers(0, na, size=int(r.integers(1, 9))).astype(np.uint8) for _ in range(3)]
+    ca, oa = wl._pack(A)
+    cb, ob = wl._pack(B)
+    tracks = r.integers(0, K, size=int(oa[-1])).astype(np.uint16)
+    gtr = r.integers(0, K, size=int(oa[-1]) + 3).astype(np.uint16)
+    T = 1.7
+    s, g = oracle.soft_log_tracked(mats, gaps, ca, oa, cb, ob, tracks, gtr, gap_model=gm, mode=mode, temperature=T)
+    theta = np.concatenate([mats.ravel(), gaps.ravel()])
+    fields = list(range(K * na * na, len(theta))) + list(r.choice(K * na * na, size=12, replace=False))
+    h = 1e-5
+    for f in fields:
+        if gm == "linear" and f >= K * na * na and (f - K * na * na) % 2 == 0:
+            continue   # linear: no open fields
+        tp, tm = theta.copy(), theta.copy()
+        tp[f] += h
+        tm[f] -= h
+        sp, _ = oracle.soft_log_tracked(tp[:K * na * na].reshape(K, na, na), tp[K * na * na:].reshape(K, 4), ca, oa,
+                                        cb, ob, tracks, gtr, gap_model=gm, mode=mode, temperature=T)
+        sm, _ = oracle.soft_log_tracked(tm[:K * na * na].reshape(K, na, na), tm[K * na * na:].reshape(K, 4), ca, oa,
+                                        cb, ob, tracks, gtr, gap_model=gm, mode=mode, temperature=T)
+        # score = T log Z(theta / T): d score / d theta_f = the expected count (gaps: minus it)
+        fd = (sp - sm) / (2 * h)
+        np.testing.assert_allclose(g[:, f], fd, rtol=1e-6, atol=1e-7)
