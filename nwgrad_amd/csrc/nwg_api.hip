@@ -43,6 +43,7 @@ constexpr int kMaxG = 64;              // lanes per pair at most: one wavefront
 constexpr int kMaxN = kS * kMaxG;      // longest B a lane group holds (hard / score-only); wider: column panels
 constexpr int kHistBytes = 16384;      // LDS histogram budget per wave (hard walk, uint16 counts)
 constexpr int kSoftHistBytes = 10240;  // LDS histogram budget per wave (soft, double expected counts)
+constexpr size_t kSoftLdsMax = 65536;  // LDS of one soft wavefront, at most (tracked: K sets)
 constexpr int kBlocksPerCU = 8;        // persistent grid of the hard / score fills: waves per CU
 constexpr int kSoftBlocksPerCU = 8;    // fallback persistent grid of the soft pass
 constexpr int64_t kSumBlock = 4096;    // the reference's WEIGHTED_GRAD_BLOCK
@@ -213,7 +214,7 @@ struct nwg_batch {
     DevPtr<int32_t> lsync;                                   // and words
     bool soft_piped = false;                                 // a launch since the last check
     size_t soft_scratch = kScratchBudget;                    // row-segment scratch of one launch
-    int soft_occ_cache[2] = {0, 0};                          // resident blocks per CU: float, double
+    int soft_occ_cache[4] = {0, 0, 0, 0};   // resident blocks per CU: float, double; untracked, tracked
     DevPtr<uint8_t> soft_ok;
     std::vector<uint8_t> host_ok;
     int64_t soft_reruns = 0;
@@ -612,7 +613,8 @@ bool set_slots(nwg_batch* b) {
     const int na = b->na, nn = na * na;
     int gmin = 1, slots = 64;
     if (b->prob.grad_mode == 2) {
-        slots = std::max(1, std::min(nwg::kTilePairs, kSoftHistBytes / (na * (na + 1) * 8)));
+        const int K = b->tracked ? b->K : 1;
+        slots = std::max(1, std::min(nwg::kTilePairs, kSoftHistBytes / (K * na * (na + 1) * 8)));
         while (64 / gmin > slots) ++gmin;
     } else {
         const int per_slot = b->prob.grad_mode == 1 ? b->K * nn * 2 + (b->tracked ? 16 * b->K : 0) : 0;
@@ -622,6 +624,12 @@ bool set_slots(nwg_batch* b) {
     const bool changed = gmin != b->gmin || slots != b->hist_slots;
     b->gmin = gmin;
     b->hist_slots = slots;
+    if (b->prob.grad_mode == 2) {
+        // the soft tiles hold at most `slots` pairs and the LDS follows K: plan and query again
+        if (changed)
+            for (SoftPlan* sp : {&b->sp_main, &b->sp_wide, &b->sq_main, &b->sq_wide}) sp->built = false;
+        for (int& o : b->soft_occ_cache) o = 0;
+    }
     return changed;
 }
 
@@ -893,10 +901,6 @@ int nwg_batch_set_params_k(nwg_batch* b, int K, const double* matrices, const do
     if (!b || !matrices || !gaps) return NWG_EINVAL;
     nwg_ctx* ctx = b->ctx;
     if (K < 1 || K > 4096) return fail(ctx, NWG_EINVAL, "nwgrad: the number of parameter sets must be in [1, 4096]");
-    if (K > 1 && b->prob.grad_mode == 2)
-        return fail(ctx, NWG_EINVAL,
-                    "nwgrad_amd: several parameter sets (tracks) are not available with grad_mode=\"soft\" on the "
-                    "GPU path; the hard and score-only paths take them");
     const int nn = b->na * b->na;
     if (b->tracked) {   // the tracks must stay within the new sets
         for (uint16_t v : b->htrk)
@@ -919,10 +923,10 @@ int nwg_batch_set_params_k(nwg_batch* b, int K, const double* matrices, const do
         NWG_HIP(ctx, b->sub_f64.reserve(kn));
         NWG_HIP(ctx, hipMemcpy(b->sub_f64.p, matrices, sizeof(double) * kn, hipMemcpyHostToDevice));
     }
-    if (K == 1) {
-        NWG_HIP(ctx, b->theta.reserve((size_t)nn + 4));
+    {   // the soft pass's parameter vector: K matrices, then K rows of gaps
+        NWG_HIP(ctx, b->theta.reserve((size_t)K * (nn + 4)));
         std::vector<double> th(b->matrix);
-        th.insert(th.end(), gaps, gaps + 4);
+        th.insert(th.end(), gaps, gaps + 4 * (size_t)K);
         NWG_HIP(ctx, hipMemcpy(b->theta.p, th.data(), sizeof(double) * th.size(), hipMemcpyHostToDevice));
     }
     if (K != oldK && b->prob.grad_mode != 0) {
@@ -948,14 +952,10 @@ int nwg_batch_set_params_k(nwg_batch* b, int K, const double* matrices, const do
 int nwg_batch_set_tracks(nwg_batch* b, const uint16_t* tracks, const uint16_t* gap_tracks) {
     if (!b) return NWG_EINVAL;
     nwg_ctx* ctx = b->ctx;
-    if (b->prob.grad_mode == 2)
-        return fail(ctx, NWG_EINVAL,
-                    "nwgrad_amd: tracks are not available with grad_mode=\"soft\" on the GPU path; the hard and "
-                    "score-only paths take them");
     NWG_HIP(ctx, hipSetDevice(ctx->device));
     const int rc = set_tracks_host(b, tracks, gap_tracks);
     if (rc) return rc;
-    if (set_slots(b)) {
+    if (set_slots(b) && b->prob.grad_mode != 2) {
         std::vector<int32_t> all((size_t)b->N);
         std::iota(all.begin(), all.end(), 0);
         const int rc2 = build_plans(b, all, b->plan, b->lpl);
@@ -977,11 +977,10 @@ int nwg_batch_set_params_dev(nwg_batch* b, const double* theta_dev) {
     nwg_ctx* ctx = b->ctx;
     if (b->prob.grad_mode != 2)
         return fail(ctx, NWG_EINVAL, "nwgrad_hip: set_params_dev serves soft batches (grad_mode=\"soft\") only");
-    const int nn = b->na * b->na;
+    const size_t th = (size_t)b->K * (b->na * b->na + 4);   // K matrices, then K rows of gaps
     NWG_HIP(ctx, hipSetDevice(ctx->device));
-    if (b->theta.n < (size_t)nn + 4) NWG_HIP(ctx, b->theta.reserve((size_t)nn + 4));
-    NWG_HIP(ctx, hipMemcpyAsync(b->theta.p, theta_dev, sizeof(double) * ((size_t)nn + 4), hipMemcpyDeviceToDevice,
-                                ctx->stream));
+    if (b->theta.n < th) NWG_HIP(ctx, b->theta.reserve(th));
+    NWG_HIP(ctx, hipMemcpyAsync(b->theta.p, theta_dev, sizeof(double) * th, hipMemcpyDeviceToDevice, ctx->stream));
     b->have_params = true;
     b->results_valid = false;
     return NWG_OK;
@@ -1246,9 +1245,14 @@ template <typename SF>
 using SoftKernel = void (*)(nwg::SoftArgs<SF>);
 
 template <typename SF>
-SoftKernel<SF> soft_kernel_for(int mode) {
-    return mode == 1 ? nwg::soft_kernel<SF, nwg::kLocal, kSS>
-                     : mode == 2 ? nwg::soft_kernel<SF, nwg::kGlocal, kSS> : nwg::soft_kernel<SF, nwg::kGlobal, kSS>;
+SoftKernel<SF> soft_kernel_for(int mode, bool tracked) {
+    if (tracked)
+        return mode == 1 ? nwg::soft_kernel<SF, nwg::kLocal, kSS, true>
+                         : mode == 2 ? nwg::soft_kernel<SF, nwg::kGlocal, kSS, true>
+                                     : nwg::soft_kernel<SF, nwg::kGlobal, kSS, true>;
+    return mode == 1 ? nwg::soft_kernel<SF, nwg::kLocal, kSS, false>
+                     : mode == 2 ? nwg::soft_kernel<SF, nwg::kGlocal, kSS, false>
+                                 : nwg::soft_kernel<SF, nwg::kGlobal, kSS, false>;
 }
 
 // The wide pool's queue and regions: tiles 0 .. Tq-1 of the persistent order (longest first),
@@ -1359,13 +1363,18 @@ int launch_soft(nwg_batch* b, const SoftPlan& pl, SoftPlan& wl) {
     const bool lin = b->prob.gap_model == 0;
     if (pl.tiles.empty() && wl.tiles.empty()) return NWG_OK;
     DevPtr<SF>& dps = SS::pf(b);
-    NWG_HIP(ctx, dps.reserve((size_t)nn));
-    NWG_HIP(ctx, SS::gw(b).reserve(4));
-    SoftKernel<SF> kern = soft_kernel_for<SF>(b->prob.mode);
-    const size_t lds = (size_t)b->hist_slots * b->na * (b->na + 1) * sizeof(double) + 5 * 64 * sizeof(double) +
-                       (size_t)(b->na + 1) * (b->na + 1) * sizeof(SF);
+    const bool tr = b->tracked;
+    const int K = tr ? b->K : 1;
+    NWG_HIP(ctx, dps.reserve((size_t)K * nn));
+    NWG_HIP(ctx, SS::gw(b).reserve((size_t)4 * K));
+    SoftKernel<SF> kern = soft_kernel_for<SF>(b->prob.mode, tr);
+    const size_t lds = nwg::soft_lds_bytes<SF>(b->na, K, b->hist_slots);
+    if (lds > kSoftLdsMax)
+        return fail(ctx, NWG_EINVAL,
+                    "nwgrad_amd: the soft pass with %d parameter sets over a %d-letter alphabet needs %zu bytes of LDS "
+                    "per wavefront (at most %zu)", K, b->na, lds, kSoftLdsMax);
     // resident blocks per CU: queried once per kernel and LDS size (no query inside a captured step)
-    int& occ = b->soft_occ_cache[std::is_same<SF, float>::value ? 0 : 1];
+    int& occ = b->soft_occ_cache[(std::is_same<SF, float>::value ? 0 : 1) + (tr ? 2 : 0)];
     if (occ <= 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64, lds) != hipSuccess || occ <= 0))
         occ = kSoftBlocksPerCU;
     b->soft_occ = occ;
@@ -1460,7 +1469,7 @@ int launch_soft(nwg_batch* b, const SoftPlan& pl, SoftPlan& wl) {
     const size_t lseg = wide_rows * LY::row;
     const size_t bnd = wide_rows * nwg::kTilePairs;
     const size_t zpart = piped ? (size_t)wl.pipelined_panels * nwg::kTilePairs * 4 : 0;
-    const size_t part = piped ? (size_t)wl.pipelined_panels * b->hist_slots * (nn + 6) : 0;
+    const size_t part = piped ? (size_t)wl.pipelined_panels * b->hist_slots * ((size_t)K * (nn + 4) + 2) : 0;
     const size_t nsync = piped ? (size_t)wl.sync_words : 0;
     if (SS::seg(b).n < mseg || SS::lseg(b).n < lseg || SS::pb(b).n < bnd * 6 || b->pexp.n < bnd * 2 ||
         b->lzpart.n < zpart || b->lpart.n < part || b->lsync.n < nsync) {
@@ -1486,6 +1495,11 @@ int launch_soft(nwg_batch* b, const SoftPlan& pl, SoftPlan& wl) {
     A.na = b->na;
     A.hist_slots = b->hist_slots;
     A.gapw = SS::gw(b).p;
+    if (tr) {
+        A.trk = b->dtrk.p;
+        A.gtrk = b->dgtrk.p;
+        A.K = K;
+    }
     A.linear = lin ? 1 : 0;
     A.temperature = b->temperature;
     A.scores = b->scores.p;
@@ -1495,7 +1509,7 @@ int launch_soft(nwg_batch* b, const SoftPlan& pl, SoftPlan& wl) {
     A.vmin = (double)std::numeric_limits<SF>::min();
     A.vtiny = (double)std::numeric_limits<SF>::denorm_min();
     // the weights (and, persistent, the queue counters) from the device params
-    hipLaunchKernelGGL(nwg::soft_weights_kernel<SF>, dim3(1), dim3(256), 0, ctx->stream, b->theta.p, nn,
+    hipLaunchKernelGGL(nwg::soft_weights_kernel<SF>, dim3(1), dim3(256), 0, ctx->stream, b->theta.p, nn, K,
                        1.0 / b->temperature, lin ? 1 : 0, dps.p, SS::gw(b).p, persistent ? b->queue.p : nullptr,
                        persistent ? b->lqueue.p : nullptr, piped ? b->lsync.p : nullptr, (int64_t)nsync);
     NWG_HIP(ctx, hipGetLastError());

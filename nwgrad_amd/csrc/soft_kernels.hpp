@@ -28,8 +28,9 @@
 // Boundary values from a neighbour are converted by an exact power-of-two product; a segment
 // without mass of its own takes units in which the neighbour's boundary values are of order
 // one, and one whose neighbour's values would not fit below 2^headroom of its own units moves
-// to the neighbour's units minus the headroom (its old values then flush towards zero, as in
-// the reference's row scaling).
+// to the boundary's magnitude (the neighbour's units, or above them when its values exceed
+// one) minus the headroom (its old values then flush towards zero, as in the reference's row
+// scaling).
 //
 // Storage.  The forward stores every row segment of every lane -- M, X, Y of its S columns,
 // the boundary column it received, the exponent and the segment maximum -- as 16-byte
@@ -118,10 +119,15 @@ struct SoftArgs {
     double* lzpart;          // per pipelined panel, pair slot: forward Z part (acc, E) and flag
     double* lpart;           // per pipelined panel, histogram slot: n_alpha^2 counts, 5 sums, flag
     uint64_t wait_ticks;     // bound of one wait, s_memrealtime ticks
-    const SF* psub;          // exp(matrix / T), n_alpha * n_alpha (LDS adds a row of ones)
+    const SF* psub;          // exp(matrix / T), K x n_alpha x n_alpha (LDS adds a row of ones)
     int32_t na;
     int32_t hist_slots;
-    const SF* gapw;          // oa, ea, ob, eb: exp(-(go+ge)/T), exp(-ge/T) per sequence
+    const SF* gapw;          // per set: oa, ea, ob, eb: exp(-(go+ge)/T), exp(-ge/T) per sequence
+    // Tracks (the TR kernels): K sets; DP row i of pair p scores with set trk[off_a[p] + i - 1]
+    // (its matrix and its gaps in B) and charges gaps in A to set gtrk[off_a[p] + p + i].
+    const uint16_t* trk;
+    const uint16_t* gtrk;
+    int32_t K;
     int32_t linear;          // 1: report the open fields as 0
     double temperature;
     double* scores;
@@ -314,9 +320,10 @@ __device__ __forceinline__ void load_rec(const SF* __restrict__ base, int lane, 
 // A wave that holds an item is running, so every wait ends.  A tile in the pool is done when
 // its last item has made its writes visible at agent scope: another XCD's L2 then holds no
 // dirty line of the region that a later tile reuses.
-template <typename SF, int AM, int S>
+template <typename SF, int AM, int S, bool TR>
 __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int32_t* __restrict__ order,
                           const SoftTile* __restrict__ tdp, int64_t slot, const SF* __restrict__ spsub,
+                          const SF* __restrict__ sgw,
                           double* __restrict__ hist, double* __restrict__ lacc, int lane, int tix, int ipp,
                           int idir) {
     using R = SoftRange<SF>;
@@ -333,7 +340,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
     }
     const int g = lane - lo;
     const int p = q >= 0 ? order[td.first + q] : -1;
-    const int na = A.na, nn = na * na, gs = nn + 4;
+    const int na = A.na, nn = na * na, gs = (TR ? A.K : 1) * (nn + 4);
     int m = 0, n = 0;
     const uint8_t* ap = A.codes_a;
     const uint8_t* bp = A.codes_b;
@@ -360,7 +367,28 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         const int w0 = tdp->wait0, nw = tdp->nwait;
         for (int k = 0; k < nw; ++k) bounded_wait(A.lsync + 1 + A.lwait[w0 + k], 1, serr, A.wait_ticks, lane, 1);
     }
-    const SF oa = A.gapw[0], ea = A.gapw[1], ob = A.gapw[2], eb = A.gapw[3];
+    // set 0's gap weights (the untracked kernels read them as before, from the argument buffer)
+    const SF oa = TR ? sgw[0] : A.gapw[0], ea = TR ? sgw[1] : A.gapw[1];
+    const SF ob = TR ? sgw[2] : A.gapw[2], eb = TR ? sgw[3] : A.gapw[3];
+    // Tracks: the pair's row sets.  Row i's profile is set ts's (profile row ts * (na + 1) +
+    // its letter), its gaps in B (ob, eb) set ts's, its gaps in A (oa, ea) set gs's.
+    const uint16_t* tk = A.trk;
+    const uint16_t* gk = A.gtrk;
+    if constexpr (TR) {
+        if (p >= 0) {
+            tk += A.off_a[p];
+            gk += A.off_a[p] + p;
+        }
+    }
+    const int KS = TR ? A.K : 1;   // sets
+    SF oa0 = oa, ea0 = ea;         // row 0's gaps in A
+    if constexpr (TR) {
+        if (p >= 0) {
+            const int g0 = gk[0];
+            oa0 = sgw[4 * g0];
+            ea0 = sgw[4 * g0 + 1];
+        }
+    }
     // Panel pp's row i is row index rbase + pp * rows + i of the pool: a block slot (the
     // pool's row count per panel), or a pipelined tile's own region (its own rows).
     const bool own = idir >= 0;
@@ -418,7 +446,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             }
             // v = oa * ea^(j0 - 1) as vm * 2^ve (the boundary column; column 1 when j0 == 0)
             int ve = 0, be = 0;
-            double vm = (double)oa, bm = (double)ea;
+            double vm = (double)oa0, bm = (double)ea0;
             if (j0 > 0) {
                 for (int r = j0 - 1; r > 0; r >>= 1) {
                     if (r & 1) {
@@ -439,7 +467,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             {
                 double cm = vm;
                 int ce = ve;
-                const double em = (double)ea;
+                const double em = (double)ea0;
 #pragma unroll
                 for (int k = 0; k < S; ++k) {
                     if (j0 > 0 || k > 0) {
@@ -489,6 +517,13 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         const bool pin = g == 0 && j0 > 0;                            // reads the previous panel's column
         const bool pout = g == G - 1 && pp + 1 < NP;                  // writes this panel's last column
         int acode = (live && m >= 1) ? (int)ap[0] : 0;                // code of the lane's next row
+        int tnext = 0, gnext = 0;                                     // and (tracks) its sets
+        if constexpr (TR) {
+            if (live && m >= 1) {
+                tnext = tk[0];
+                gnext = gk[1];
+            }
+        }
         SF qM = SF(0), qX = SF(0), qY = SF(0);
         int qE = 0;
         if (pin && live && m >= 1) {
@@ -504,8 +539,20 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             SF lM = wshift_up(oM), lX = wshift_up(oX), lY = wshift_up(oY);
             int lE = wshift_up(oE);
             if (live && i >= 1 && i <= m) {
-                const int ac = acode;
+                int ac = acode;
                 if (i < m) acode = (int)ap[i];   // one row ahead
+                // the row's weights: its sets' (tracks), else the constants
+                SF r_oa = oa, r_ea = ea, r_ob = ob, r_eb = eb;
+                if constexpr (TR) {
+                    const int ts = tnext, gs = gnext;
+                    if (i < m) {
+                        tnext = tk[i];
+                        gnext = gk[i + 1];
+                    }
+                    ac += ts * (na + 1);
+                    r_oa = sgw[4 * gs]; r_ea = sgw[4 * gs + 1];
+                    r_ob = sgw[4 * ts + 2]; r_eb = sgw[4 * ts + 3];
+                }
                 if (pin) {
                     lM = qM; lX = qX; lY = qY; lE = qE;
                     if (i < m) {
@@ -522,13 +569,21 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                         const SF ml = dmax(lM, dmax(lX, lY));
                         Ec = ml > SF(0) ? lE + silogb(ml) : lE;
                         s = SF(1);
-                    } else if (lE - R::headroom > Ec) {
-                        Ec = lE - R::headroom;
-                        s = sldexp(SF(1), E - Ec);
+                    } else {
+                        // Headroom, on the boundary's magnitude (its units, or above them when
+                        // its values exceed 1): a neighbour's row can outgrow the units this
+                        // lane carries from its previous row -- the gap chain of a row whose
+                        // gap weights (tracks) decay slower than the row before's.
+                        const SF ml = dmax(lM, dmax(lX, lY));
+                        const int lmag = ml > SF(1) ? lE + silogb(ml) : lE;
+                        if (lmag - R::headroom > Ec) {
+                            Ec = lmag - R::headroom;
+                            s = sldexp(SF(1), E - Ec);
+                        }
                     }
                     nM = sldexp(lM, lE - Ec); nX = sldexp(lX, lE - Ec); nY = sldexp(lY, lE - Ec);
                 }
-                const SF ebs = eb * s, obs = ob * s;
+                const SF ebs = r_eb * s, obs = r_ob * s;
                 if (j0 == 0) {   // column 0 of row i (soft_kernel_impl.inl:669-674)
                     if constexpr (L) { nM = sldexp(SF(1), -Ec); nX = SF(0); nY = SF(0); }
                     else { nM = SF(0); nY = SF(0); nX = sfma(cX, ebs, (cM + cY) * obs); }
@@ -547,7 +602,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                     const SF uM = M[k], uX = X[k], uY = Y[k];
                     const SF mv = sfma(dT, pw[k] * s, fr);
                     const SF xv = sfma(uX, ebs, (uM + uY) * obs);
-                    const SF yv = sfma(ly, ea, (lm + lx) * oa);
+                    const SF yv = sfma(ly, r_ea, (lm + lx) * r_oa);
                     dT = T[k];
                     const SF tv = mv + xv + yv;
                     M[k] = mv; X[k] = xv; Y[k] = yv; T[k] = tv;
@@ -670,11 +725,11 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
     // gap counts oa, ea, ob, eb and the lost-mass bound.
     double* __restrict__ acc = lacc + lane;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) acc[c * kWave] = 0.0;
+    for (int c = 0; c < 4 * KS + 1; ++c) acc[c * kWave] = 0.0;
     const int kmin = (L || GL) ? 1 : 0;   // first row whose Y opens count
     const bool col0_opens = !L;           // column-0 X opens count (kmin0 == 0)
     const int hrow = na + 1;   // histogram rows carry a spill column for the no-column letter
-    double* __restrict__ hq = hist + (q < 0 ? 0 : q) * na * hrow;
+    double* __restrict__ hq = hist + (q < 0 ? 0 : q) * KS * na * hrow;   // K sets of na rows
     // Rows run m .. 0 at steps u = G-1-g .. G-1-g+m; the step count is rounded up to two
     // bands (steps in which no lane is active) so that the loop body holds both parities.
     const int bsteps = (td.fsteps + 2 * kCK) / (2 * kCK) * (2 * kCK);
@@ -729,11 +784,13 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         // (loaded during the previous band), W receives the recomputed rows.
         SF Fa[LY::vals], Fb[LY::vals];
         SF rec[kCK - 1][4];     // records of the rows above the band's full row, in order
-        int rcode[kCK - 1];     // their codes
+        int rcode[kCK - 1];     // their profile rows
+        int rsets[kCK - 1];     // (tracks) their sets: ts | gs << 16
 #pragma unroll
         for (int j = 0; j < kCK - 1; ++j) {
             rec[j][0] = rec[j][1] = rec[j][2] = rec[j][3] = SF(0);
             rcode[j] = 0;
+            rsets[j] = 0;
         }
         auto load_recs = [&](int src, int last) {   // the records of rows src+1 .. last
 #pragma unroll
@@ -742,6 +799,11 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 if (r <= last) {
                     load_rec<SF, S>(seg + (int64_t)(r + g) * LY::row, lane, rec[j]);
                     rcode[j] = (int)ap[r - 1];
+                    if constexpr (TR) {
+                        const int ts = tk[r - 1];
+                        rcode[j] += ts * (na + 1);
+                        rsets[j] = ts | (int)gk[r] << 16;
+                    }
                 }
             }
         };
@@ -760,15 +822,29 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             qM = bq[3]; qY = bq[4]; qE = bex(pp + 1, m)[1];
         }
         int acode = na;                // code of row i+1 (row m: the virtual row)
+        // (tracks) row i+1's set, and row i's sets, loaded one row ahead
+        int btn = 0, bti = 0, bgi = 0;
+        if constexpr (TR) {
+            if (live) {
+                bti = m >= 1 ? (int)tk[m - 1] : 0;
+                bgi = gk[m];
+            }
+        }
         SF pubM = SF(0), pubY = SF(0);
         int pubE = 0;
 
         // Recompute forward row i (i >= 1) into Fd from row i-1 in Fs and row i's record: the
         // forward's row step (same operations, same order) in the record's units.
-        auto recompute = [&](const SF(&Fs)[LY::vals], SF(&Fd)[LY::vals], const SF(&rc)[4], int code) {
+        auto recompute = [&](const SF(&Fs)[LY::vals], SF(&Fd)[LY::vals], const SF(&rc)[4], int code, int sets) {
             const int Ec = em_E(rc[3]);
             const SF s = sldexp(SF(1), em_E(Fs[LY::iE]) - Ec);   // row i-1's values into row i's units
-            const SF ebs = eb * s, obs = ob * s;
+            SF r_oa = oa, r_ea = ea, r_ob = ob, r_eb = eb;
+            if constexpr (TR) {
+                const int ts = sets & 0xffff, gs = sets >> 16;
+                r_oa = sgw[4 * gs]; r_ea = sgw[4 * gs + 1];
+                r_ob = sgw[4 * ts + 2]; r_eb = sgw[4 * ts + 3];
+            }
+            const SF ebs = r_eb * s, obs = r_ob * s;
             const SF fr = L ? sldexp(SF(1), -Ec) : SF(0);
             const SF* __restrict__ prow = spsub + code * (na + 1);
             SF pw[S];
@@ -783,7 +859,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 const SF uM = Fs[k], uX = Fs[LY::iX + k], uY = Fs[LY::iY + k];
                 const SF mv = sfma(dT, pw[k] * s, fr);
                 const SF xv = sfma(uX, ebs, (uM + uY) * obs);
-                const SF yv = sfma(ly, ea, (lm + lx) * oa);
+                const SF yv = sfma(ly, r_ea, (lm + lx) * r_oa);
                 dT = uM + uX + uY;
                 const SF tv = mv + xv + yv;
                 Fd[k] = mv; Fd[LY::iX + k] = xv; Fd[LY::iY + k] = yv;
@@ -801,6 +877,20 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         auto step = [&](int i, SF rM, SF rY, int rE, SF(&Fc)[LY::vals]) {
             const int ac = acode;
             if (i >= 1) acode = (int)ap[i - 1];   // one row ahead
+            // The step's weights.  Tracks: row i+1's set tn (its profile, its gaps in B, its
+            // match counts and X opens), row i's sets ti (its X moves) and gi (its gaps in A).
+            int tn = 0, ti = 0, gi = 0;
+            SF r_oa = oa, r_ea = ea, r_ob = ob, r_eb = eb;
+            if constexpr (TR) {
+                tn = btn;
+                ti = bti;
+                gi = bgi;
+                bti = i >= 2 ? (int)tk[i - 2] : 0;
+                if (i >= 1) bgi = gk[i - 1];
+                btn = ti;
+                r_oa = sgw[4 * gi]; r_ea = sgw[4 * gi + 1];
+                r_ob = sgw[4 * tn + 2]; r_eb = sgw[4 * tn + 3];
+            }
             if (pin) {
                 rM = qM; rY = qY; rE = qE;
                 if (i >= 1) {
@@ -808,7 +898,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                     qM = bq[3]; qY = bq[4]; qE = bex(pp + 1, i - 1)[1];
                 }
             }
-            const SF* __restrict__ pn = spsub + ac * (na + 1);   // row i+1's profile
+            const SF* __restrict__ pn = spsub + (ac + tn * (na + 1)) * (na + 1);   // row i+1's profile
             SF pw[S + 1];
 #pragma unroll
             for (int k = 0; k <= S; ++k) pw[k] = pn[coff[k]];
@@ -825,14 +915,19 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 const SF mr = dmax(rM, rY);
                 Ec = mr > SF(0) ? rE + silogb(mr) : rE;
                 s = SF(1);
-            } else if (has_right && rE - R::headroom > Ec) {
-                Ec = rE - R::headroom;
-                s = sldexp(SF(1), EB - Ec);
+            } else if (has_right) {
+                // headroom on the boundary's magnitude, as the forward's
+                const SF mr = dmax(rM, rY);
+                const int rmag = mr > SF(1) ? rE + silogb(mr) : rE;
+                if (rmag - R::headroom > Ec) {
+                    Ec = rmag - R::headroom;
+                    s = sldexp(SF(1), EB - Ec);
+                }
             }
             const SF rYc = has_right ? sldexp(rY, rE - Ec) : SF(0);   // BY(i, j0+S+1)
             const SF rMc = has_right ? sldexp(rM, rE - Ec) : SF(0);   // BM(i, j0+S+1)
             const SF init = L ? sldexp(SF(1), -Ec) : SF(0);
-            const SF obs = ob * s, ebs = eb * s;
+            const SF obs = r_ob * s, ebs = r_eb * s;
             // posterior factors: of row i's units, and of row i+1's (the late sums); none of
             // the late sums exist at row m
             const double fnew = zbad ? 0.0 : __builtin_amdgcn_ldexp(izr, efc + Ec - ze);
@@ -844,7 +939,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             SF mx = SF(0);
             SF sx = SF(0), sy = SF(0), so = SF(0), sob = SF(0);
             // match posteriors go into the pair's histogram row a_{i+1} (row m: none, added as 0)
-            double* __restrict__ h = hq + (ac < na ? ac : 0) * hrow;
+            double* __restrict__ h = hq + (ac < na ? tn * na + ac : 0) * hrow;
 #pragma unroll
             for (int kk = 0; kk < S; ++kk) {
                 const int k = S - 1 - kk;
@@ -857,8 +952,8 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 // (i+1, j), and the match posterior of (i+1, j+1)
                 sob = sfma(Fc[k] + Fc[LY::iY + k], v, sob);
                 atomicAdd(h + coff[k + 1], (double)((Fc[k] + Fc[LY::iX + k] + Fc[LY::iY + k]) * pd) * fmatch);
-                const SF byn = sfma(ea, byr, b1);
-                const SF hh = byr * oa;
+                const SF byn = sfma(r_ea, byr, b1);
+                const SF hh = byr * r_oa;
                 // Columns past len(B) hold exact zeros in Global and Glocal: their profile entry is
                 // 0 and no end weight lies there.  Local's start term reaches every cell, so its
                 // columns past len(B) are cleared.
@@ -885,7 +980,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 const SF b2 = sfma(v, ebs, d);
                 if (col0_opens) sob = sfma(Fc[LY::iC] + Fc[LY::iC + 2], v, sob);
                 atomicAdd(h + coff[0], (double)((Fc[LY::iC] + Fc[LY::iC + 1] + Fc[LY::iC + 2]) * pd) * fmatch);
-                const SF hh = byr * oa;
+                const SF hh = byr * r_oa;
                 c0M = b1 + hh;
                 c0X = b2 + hh;
                 mx = dmax(mx, dmax(c0M, c0X));
@@ -900,20 +995,20 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             if (!zbad && (mx > SF(0) || mfc > SF(0))) {
                 const double lr = 32.0 * (double)(ncols + 1) * A.vmin * (3.0 * (double)mx + (double)mfc) * fnew +
                                   8.0 * (double)(ncols + 1) * A.vtiny * (fnew + fold);
-                atomicAdd(acc + 4 * kWave, (lr <= 1.0e300) ? lr : 1.0e300);
+                atomicAdd(acc + 4 * KS * kWave, (lr <= 1.0e300) ? lr : 1.0e300);
             }
             if (!zbad) {
                 // every term is a sum of non-negative products: one range check covers them
                 const double c1 = i >= 1 ? (double)sx * fnew : 0.0;   // no X move ends in row 0
                 const double c2 = (double)sy * fnew;
-                const double c3 = i >= kmin ? (double)so * (double)oa * fnew : 0.0;
-                const double c4 = (double)sob * (double)ob * fold;
+                const double c3 = i >= kmin ? (double)so * (double)r_oa * fnew : 0.0;
+                const double c4 = (double)sob * (double)r_ob * fold;
                 if (!(c1 + c2 + c3 + c4 <= 1.0e300)) bad = true;
                 else {
-                    atomicAdd(acc, c3);
-                    atomicAdd(acc + kWave, c2);
-                    atomicAdd(acc + 2 * kWave, c4);
-                    atomicAdd(acc + 3 * kWave, c1);
+                    atomicAdd(acc + (4 * gi) * kWave, c3);
+                    atomicAdd(acc + (4 * gi + 1) * kWave, c2);
+                    atomicAdd(acc + (4 * tn + 2) * kWave, c4);
+                    atomicAdd(acc + (4 * ti + 3) * kWave, c1);
                 }
             }
             if (!(mx <= R::fmax)) bad = true;
@@ -944,10 +1039,10 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             if (p < kCK - 1) {
                 const int c = i - (kCK - 1 - p);
                 const int nr = i - (c > 0 ? c : 0);   // rows to recompute, ending at row i
-                if (nr >= 1) recompute(Sb, Wb, rec[0], rcode[0]);
+                if (nr >= 1) recompute(Sb, Wb, rec[0], rcode[0], rsets[0]);
 #pragma unroll
                 for (int j = 1; j < kCK - 1 - p; ++j)
-                    if (nr > j) recompute(Wb, Wb, rec[j], rcode[j]);
+                    if (nr > j) recompute(Wb, Wb, rec[j], rcode[j], rsets[j]);
                 if (nr == 0) copy_row(Sb, Wb);
                 step(i, rM, rY, rE, Wb);
             } else {
@@ -973,25 +1068,49 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
     }
 
     // ── per-pair results ────────────────────────────────────────────────────
+    // The pair's sums: untracked, the tree over its lanes; tracks, each of the K x 4 gap sums
+    // and the loss bound added lane by lane from LDS (pair_sum, on the pair's first lane).
     wave_sync();
-    double g_oa = acc[0], g_ea = acc[kWave], g_ob = acc[2 * kWave], g_eb = acc[3 * kWave], loss = acc[4 * kWave];
-    for (int off = 1; off < kWave; off <<= 1) {
-        const double o1 = __shfl_down(g_oa, off, kWave), o2 = __shfl_down(g_ea, off, kWave);
-        const double o3 = __shfl_down(g_ob, off, kWave), o4 = __shfl_down(g_eb, off, kWave);
-        const double o5 = __shfl_down(loss, off, kWave);
-        const int o6 = __shfl_down((int)bad, off, kWave);
-        if (g + off < G) {
-            g_oa += o1; g_ea += o2; g_ob += o3; g_eb += o4; loss += o5;
-            bad = bad || o6 != 0;
+    double g_oa = 0.0, g_ea = 0.0, g_ob = 0.0, g_eb = 0.0, loss = 0.0;
+    if constexpr (!TR) {
+        g_oa = acc[0]; g_ea = acc[kWave]; g_ob = acc[2 * kWave]; g_eb = acc[3 * kWave]; loss = acc[4 * kWave];
+        for (int off = 1; off < kWave; off <<= 1) {
+            const double o1 = __shfl_down(g_oa, off, kWave), o2 = __shfl_down(g_ea, off, kWave);
+            const double o3 = __shfl_down(g_ob, off, kWave), o4 = __shfl_down(g_eb, off, kWave);
+            const double o5 = __shfl_down(loss, off, kWave);
+            const int o6 = __shfl_down((int)bad, off, kWave);
+            if (g + off < G) {
+                g_oa += o1; g_ea += o2; g_ob += o3; g_eb += o4; loss += o5;
+                bad = bad || o6 != 0;
+            }
+        }
+    } else {
+        for (int off = 1; off < kWave; off <<= 1) {
+            const int o6 = __shfl_down((int)bad, off, kWave);
+            if (g + off < G) bad = bad || o6 != 0;
         }
     }
+    auto pair_sum = [&](int c) -> double {   // accumulator c: 4 * set + field, or 4K the loss
+        if constexpr (TR) {
+            double v = 0.0;
+            for (int l = lo; l < lo + G; ++l) v += lacc[c * kWave + l];
+            return v;
+        } else {
+            return c == 0 ? g_oa : (c == 1 ? g_ea : (c == 2 ? g_ob : (c == 3 ? g_eb : loss)));
+        }
+    };
+    const int nsum = 4 * KS + 1;   // the gap sums of every set, then the loss bound
+    if constexpr (TR) {
+        if (g == 0 && q >= 0) loss = pair_sum(4 * KS);
+    }
     wave_sync();
-    const int hsz = na * hrow;
+    const int hsz = KS * na * hrow;   // one pair slot's histogram
+    const int knn = KS * nn;
     const double logz = zbad ? 0.0 : ::log(zr) + (double)ze * 0.69314718055994530942;
     if (idir == 1) {
         // A backward item: this panel's histogram, sums and flag as the tile's part, then the
         // tile's count; the last backward item adds the parts of every panel in panel order.
-        const int np = td.npairs, pst = nn + 6;
+        const int np = td.npairs, pst = knn + nsum + 1;
         const int64_t pstride = (int64_t)A.hist_slots * pst;
         double* __restrict__ P0 = A.lpart + (poff + ipp) * pstride;
         for (int idx = lane; idx < np * hsz; idx += kWave) {
@@ -1001,9 +1120,9 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             hist[idx] = 0.0;
         }
         if (g == 0 && q >= 0) {
-            double* pq = P0 + q * pst + nn;
-            pq[0] = g_oa; pq[1] = g_ea; pq[2] = g_ob; pq[3] = g_eb; pq[4] = loss;
-            pq[5] = bad ? 1.0 : 0.0;
+            double* pq = P0 + q * pst + knn;
+            for (int c = 0; c < nsum; ++c) pq[c] = pair_sum(c);
+            pq[nsum] = bad ? 1.0 : 0.0;
         }
         __threadfence();
         const int prev = wave_uniform(
@@ -1015,8 +1134,8 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const double* __restrict__ PB = A.lpart + poff * pstride;
         unsigned nonfinite = 0u;
-        for (int idx = lane; idx < np * nn; idx += kWave) {
-            const int qq = idx / nn, r = idx - qq * nn;
+        for (int idx = lane; idx < np * knn; idx += kWave) {
+            const int qq = idx / knn, r = idx - qq * knn;
             double v = 0.0;
             for (int k = 0; k < NP; ++k) v += PB[k * pstride + qq * pst + r];
             A.grads[(int64_t)order[td.first + qq] * gs + r] = v;
@@ -1024,23 +1143,23 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         }
         for (int off = 1; off < kWave; off <<= 1) nonfinite |= (unsigned)__shfl_xor((int)nonfinite, off, kWave);
         if (g == 0 && p >= 0) {
-            double s5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
             bool bb = false;
+            double lsum = 0.0;
             for (int k = 0; k < NP; ++k) {
-                const double* pq = PB + k * pstride + q * pst + nn;
-#pragma unroll
-                for (int c = 0; c < 5; ++c) s5[c] += pq[c];
+                const double* pq = PB + k * pstride + q * pst + knn;
+                lsum += pq[nsum - 1];
                 const double* zp = A.lzpart + ((poff + k) * kTilePairs + q) * 4;
-                bb = bb || pq[5] != 0.0 || zp[2] != 0.0;
+                bb = bb || pq[nsum] != 0.0 || zp[2] != 0.0;
             }
-            const bool okp = !bb && !zbad && s5[4] <= A.tol && !((nonfinite >> q) & 1u);
+            const bool okp = !bb && !zbad && lsum <= A.tol && !((nonfinite >> q) & 1u);
             A.ok[p] = okp ? 1 : 0;
             A.scores[p] = logz * A.temperature;
-            double* gr = A.grads + (int64_t)p * gs + nn;
-            gr[0] = A.linear ? 0.0 : -s5[0];
-            gr[1] = -s5[1];
-            gr[2] = A.linear ? 0.0 : -s5[2];
-            gr[3] = -s5[3];
+            double* gr = A.grads + (int64_t)p * gs + knn;
+            for (int c = 0; c < nsum - 1; ++c) {
+                double v = 0.0;
+                for (int k = 0; k < NP; ++k) v += PB[k * pstride + q * pst + knn + c];
+                gr[c] = (A.linear && (c & 1) == 0) ? 0.0 : -v;
+            }
         }
         publish(A.lsync + 1 + tix, 1, lane);
         return;
@@ -1065,34 +1184,35 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         const bool okp = !bad && !zbad && loss <= A.tol && !((nonfinite >> q) & 1u);
         A.ok[p] = okp ? 1 : 0;
         A.scores[p] = logz * A.temperature;
-        double* gr = A.grads + (int64_t)p * gs + nn;
-        gr[0] = A.linear ? 0.0 : -g_oa;
-        gr[1] = -g_ea;
-        gr[2] = A.linear ? 0.0 : -g_ob;
-        gr[3] = -g_eb;
+        double* gr = A.grads + (int64_t)p * gs + knn;
+        for (int c = 0; c < nsum - 1; ++c) gr[c] = (A.linear && (c & 1) == 0) ? 0.0 : -pair_sum(c);
     }
     if (idir == 2) publish(A.lsync + 1 + tix, 1, lane);   // its pool region is free
     wave_sync();
 }
 
-// The soft weights from the parameter vector on the device, as the reference forms them
-// (aligner.hpp:6870-6885): exp(s / T) per matrix entry and exp(-(go+ge)/T), exp(-ge/T) per
-// sequence (linear gaps: open = extend), each in double and cast once.  Also zeroes the
+// The soft weights from the parameter vector on the device (K sets: K matrices, then K rows
+// of go_a, ge_a, go_b, ge_b), as the reference forms them (aligner.hpp:6870-6885; tracked
+// rows, aligner.hpp:6845-6866): exp(s / T) per matrix entry and exp(-(go+ge)/T), exp(-ge/T)
+// per sequence (linear gaps: open = extend), each in double and cast once.  Also zeroes the
 // persistent launch's queue counters and the pipelined wide tiles' words (a kernel, not a
 // memset, so that a captured step holds no memset node).
 template <typename SF>
-__global__ __launch_bounds__(256) void soft_weights_kernel(const double* __restrict__ theta, int nn, double it,
+__global__ __launch_bounds__(256) void soft_weights_kernel(const double* __restrict__ theta, int nn, int K, double it,
                                                            int linear, SF* __restrict__ psub, SF* __restrict__ gapw,
                                                            int32_t* q0, int32_t* q1, int32_t* sync, int64_t nsync) {
-    for (int k = threadIdx.x; k < nn; k += blockDim.x) psub[k] = (SF)::exp(theta[k] * it);
+    for (int k = threadIdx.x; k < K * nn; k += blockDim.x) psub[k] = (SF)::exp(theta[k] * it);
     for (int64_t k = threadIdx.x; k < nsync; k += blockDim.x) sync[k] = 0;
-    if (threadIdx.x == 0) {
-        const double go_a = theta[nn], ge_a = theta[nn + 1], go_b = theta[nn + 2], ge_b = theta[nn + 3];
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        const double* gp = theta + (size_t)K * nn + 4 * k;
+        const double go_a = gp[0], ge_a = gp[1], go_b = gp[2], ge_b = gp[3];
         const double ea = ::exp(-ge_a * it), eb = ::exp(-ge_b * it);
-        gapw[0] = (SF)(linear ? ea : ::exp(-(go_a + ge_a) * it));
-        gapw[1] = (SF)ea;
-        gapw[2] = (SF)(linear ? eb : ::exp(-(go_b + ge_b) * it));
-        gapw[3] = (SF)eb;
+        gapw[4 * k] = (SF)(linear ? ea : ::exp(-(go_a + ge_a) * it));
+        gapw[4 * k + 1] = (SF)ea;
+        gapw[4 * k + 2] = (SF)(linear ? eb : ::exp(-(go_b + ge_b) * it));
+        gapw[4 * k + 3] = (SF)eb;
+    }
+    if (threadIdx.x == 0) {
         if (q0) *q0 = 0;
         if (q1) {
             q1[0] = 0;
@@ -1108,22 +1228,34 @@ __global__ __launch_bounds__(256) void soft_weights_kernel(const double* __restr
 // body, so the kernel holds one copy of it.
 // Occupancy: the float32 pass is held to two waves per SIMD (256 VGPRs); the double pass, twice
 // the register bytes, to one.
-template <typename SF, int AM, int S>
+// LDS of one block: the histograms (pair slots x K x n_alpha x (n_alpha + 1) doubles), the
+// per-lane sums ((4K + 1) x 64 doubles), the K profiles ((n_alpha + 1)^2 each) and the K x 4
+// gap weights.  K = 1 without tracks.
+template <typename SF>
+__host__ __device__ constexpr size_t soft_lds_bytes(int na, int K, int hist_slots) {
+    return (size_t)hist_slots * K * na * (na + 1) * sizeof(double) + (size_t)(4 * K + 1) * kWave * sizeof(double) +
+           (size_t)K * (na + 1) * (na + 1) * sizeof(SF) + (size_t)4 * K * sizeof(SF);
+}
+
+template <typename SF, int AM, int S, bool TR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(SF) == 4 ? 2 : 1)))
 void soft_kernel(SoftArgs<SF> A) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const int na = A.na, nn = na * na;
+    const int na = A.na, nn = na * na, K = TR ? A.K : 1;
     double* hist = reinterpret_cast<double*>(lds);
-    const int hsz = na * (na + 1);
-    double* lacc = hist + (size_t)A.hist_slots * hsz;   // 5 x 64 per-lane sums
-    SF* spsub = reinterpret_cast<SF*>(lacc + 5 * kWave);
+    const int hsz = K * na * (na + 1);
+    double* lacc = hist + (size_t)A.hist_slots * hsz;   // (4K + 1) x 64 per-lane sums
+    SF* spsub = reinterpret_cast<SF*>(lacc + (4 * K + 1) * kWave);
+    SF* sgw = spsub + K * (na + 1) * (na + 1);
     const int lane = threadIdx.x;
-    // The profile with a row and a column added: column na (a column past len(B)) weighs 0 and
-    // row na (the backward's virtual row m+1) weighs 1 everywhere.
-    for (int k = lane; k < (na + 1) * (na + 1); k += kWave) {
-        const int a = k / (na + 1), c = k - a * (na + 1);
-        spsub[k] = a == na ? SF(1) : (c == na ? SF(0) : A.psub[a * na + c]);
+    // Each set's profile with a row and a column added: column na (a column past len(B))
+    // weighs 0 and row na (the backward's virtual row m+1) weighs 1 everywhere.
+    for (int k = lane; k < K * (na + 1) * (na + 1); k += kWave) {
+        const int set = k / ((na + 1) * (na + 1)), r = k - set * (na + 1) * (na + 1);
+        const int a = r / (na + 1), c = r - a * (na + 1);
+        spsub[k] = a == na ? SF(1) : (c == na ? SF(0) : A.psub[set * nn + a * na + c]);
     }
+    for (int k = lane; k < 4 * K; k += kWave) sgw[k] = A.gapw[k];
     for (int k = lane; k < A.hist_slots * hsz; k += kWave) hist[k] = 0.0;
     __syncthreads();
     const int blk = (int)blockIdx.x;
@@ -1170,9 +1302,9 @@ void soft_kernel(SoftArgs<SF> A) {
                 if (tile >= A.n_tiles) break;
             }
         }
-        soft_tile<SF, AM, S>(A, wide ? A.lview : A.view, wide ? A.lorder : A.order,
-                             (wide ? A.ltiles : A.tiles) + tile, (int64_t)blk, spsub, hist, lacc, lane, tile, ipp,
-                             idir);
+        soft_tile<SF, AM, S, TR>(A, wide ? A.lview : A.view, wide ? A.lorder : A.order,
+                                 (wide ? A.ltiles : A.tiles) + tile, (int64_t)blk, spsub, sgw, hist, lacc, lane, tile,
+                                 ipp, idir);
     }
 }
 

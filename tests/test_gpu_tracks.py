@@ -1,6 +1,7 @@
-"""GPU parity of tracked batches (K parameter sets chosen per DP row): bit-exact against the
-reference engine's tracked golden vectors and the oracle, short and long (column-panel) pairs,
-through the C ABI."""
+"""GPU parity of tracked batches (K parameter sets chosen per DP row): hard and score-only
+bit-exact against the reference engine's tracked golden vectors and the oracle, short and long
+(column-panel) pairs; soft within 1e-5 (float32) / 1e-10 (float64) of the oracle and of the
+reference's float32 pass, norm-relative per pair as in test_gpu_soft.py."""
 import glob
 import os
 
@@ -13,7 +14,10 @@ from tests.gpu_util import say
 
 pytestmark = pytest.mark.gpu
 
-TRACKED = sorted(p for p in glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")) if "_tracks_" in p)
+TRACKED = sorted(p for p in glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz"))
+                 if "_tracks_" in p and "/soft_" not in p)
+SOFT_TRACKED = sorted(p for p in glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz"))
+                      if "/soft_tracks_" in p)
 
 
 def _params_k(alphabet, mats, gaps):
@@ -123,8 +127,6 @@ def test_track_errors(gpu):
         _batch(w.alphabet, mats, gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, bad)
     with pytest.raises(ValueError):                     # wrong length
         _batch(w.alphabet, mats, gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, np.zeros(3, np.uint16))
-    with pytest.raises(ValueError):                     # soft batches take one set, untracked
-        _batch(w.alphabet, mats, gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, bad * 0, grad_mode="soft")
     ok = np.zeros(int(w.off_a[-1]), np.uint16)
     b = _batch(w.alphabet, mats, gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, ok)
     import nwgrad_amd as nw
@@ -133,3 +135,90 @@ def test_track_errors(gpu):
         b2.set_params(nw.AlignParams(w.matrix, alphabet=w.alphabet))
     b.score_and_grad()
     assert b.grads()[0].shape == (5, 2, 20, 20)
+
+
+def _rel(s, g, s_ref, g_ref):
+    es = np.max(np.abs(s - s_ref) / np.maximum(1.0, np.abs(s_ref)))
+    scale = np.maximum(np.max(np.abs(g_ref), axis=1), 1e-300)
+    return float(es), float(np.max(np.max(np.abs(g - g_ref), axis=1) / scale))
+
+
+@pytest.mark.parametrize("fill", ["interpair", "striped"])
+@pytest.mark.parametrize("path", SOFT_TRACKED, ids=lambda p: os.path.basename(p)[:-4])
+def test_soft_tracked_golden(gpu, path, fill):
+    """Tracked soft (K = 3 sets): against the reference's own result (its float32 scaled pass
+    or its float64 pass) and against the oracle."""
+    z = np.load(path)
+    gt = z["gap_tracks"] if "gap_tracks" in z else None
+    prec = str(z["precision"])
+    say("case", os.path.basename(path), fill)
+    s_log, g_log = oracle.soft_log_tracked(z["matrices"], z["gaps_k"], z["codes_a"], z["off_a"], z["codes_b"],
+                                           z["off_b"], z["tracks"], gt, gap_model=str(z["gap_model"]),
+                                           mode=str(z["mode"]))
+    b = _batch(z["alphabet"], z["matrices"], z["gaps_k"], z["codes_a"], z["off_a"], z["codes_b"], z["off_b"],
+               z["tracks"], gt, precision=prec, gap_model=str(z["gap_model"]), mode=str(z["mode"]),
+               grad_mode="soft", fill=fill)
+    assert b.tracked and b.num_sets == 3
+    b.score_and_grad()
+    s, g = b.scores(), _rows(b)
+    tol = 1e-5 if prec == "float32" else 1e-10
+    es, eg = _rel(s, g, s_log, g_log)
+    say(f"  vs oracle: score {es:.2e}, grad {eg:.2e}")
+    assert es <= tol and eg <= tol
+    es, eg = _rel(s, g, z["scores"], z["grads"])
+    say(f"  vs reference: score {es:.2e}, grad {eg:.2e}")
+    assert es <= max(tol, 1e-9) and eg <= max(tol, 1e-9)
+
+
+@pytest.mark.parametrize("gm", ["affine", "linear"])
+@pytest.mark.parametrize("mode", ["global", "local", "glocal"])
+def test_soft_tracked_wide_pairs(gpu, gm, mode):
+    """Tracked soft pairs from one to five column panels (the persistent launch pipelines the
+    longest across waves) beside short ones, K = 4 sets over segment tracks and random gap
+    tracks, float32 and float64, both fills, against the oracle."""
+    r = wl.rng(f"soft_trk_wide_{gm}_{mode}", "seqs")
+    lens_b = [40, 300, 700, 1100, 1600, 2049, 90, 513]
+    B = [r.integers(0, 20, size=n).astype(np.uint8) for n in lens_b]
+    # related pairs of about equal lengths: a gap run across most of a 2,000-column pair does
+    # not fit the scaled pass's range (in the reference either)
+    A = [wl.mutate(r, b_, 20, 0.2, 0.02, 2.0) for b_ in B]
+    ca, oa = wl._pack(A)
+    cb, ob = wl._pack(B)
+    K = 4
+    mats = np.stack([wl.random_matrix(r) for _ in range(K)])
+    gaps = np.array([[10.5, 0.75, 9.25, 1.5], [8.0, 1.25, 11.0, 0.5], [12.0, 2.0, 6.5, 1.5], [9.0, 1.0, 9.0, 1.0]])
+    if gm == "linear":
+        gaps = gaps * np.array([0.0, 1.0, 0.0, 1.0])
+    tracks = np.repeat(r.integers(0, K, size=int(oa[-1]) // 7 + 1), 7)[:int(oa[-1])].astype(np.uint16)
+    gtr = r.integers(0, K, size=int(oa[-1]) + len(A)).astype(np.uint16)
+    s_ref, g_ref = oracle.soft_log_tracked(mats, gaps, ca, oa, cb, ob, tracks, gtr, gap_model=gm, mode=mode)
+    for precision, tol in (("float32", 1e-5), ("float64", 1e-10)):
+        for fill in ("interpair", "striped"):
+            say("case soft tracked wide", gm, mode, precision, fill)
+            b = _batch(wl.AA, mats, gaps, ca, oa, cb, ob, tracks, gtr, precision=precision, gap_model=gm, mode=mode,
+                       grad_mode="soft", fill=fill)
+            b.score_and_grad()
+            es, eg = _rel(b.scores(), _rows(b), s_ref, g_ref)
+            say(f"  score {es:.2e}, grad {eg:.2e}, wide {b.wide_info if fill == 'interpair' else None}")
+            assert es <= tol and eg <= tol
+
+
+def test_soft_equal_sets_match_untracked(gpu):
+    """K copies of one set under random tracks: the untracked soft scores, and per-set counts
+    that sum to the untracked counts (within the soft tolerance)."""
+    from tests.gpu_util import batch, grads_rows
+    w = wl.random_pairs("soft_trk_equal_gpu", 120, wl.AA, (0, 300), (0, 900), related=1.0)
+    K = 3
+    r = wl.rng(w.name, "t")
+    tracks = r.integers(0, K, size=int(w.off_a[-1])).astype(np.uint16)
+    gtr = r.integers(0, K, size=int(w.off_a[-1]) + w.n_pairs).astype(np.uint16)
+    b0 = batch(w.alphabet, w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, grad_mode="soft")
+    b0.score_and_grad()
+    b1 = _batch(w.alphabet, np.stack([w.matrix] * K), np.stack([w.gaps] * K), w.codes_a, w.off_a, w.codes_b,
+                w.off_b, tracks, gtr, grad_mode="soft")
+    b1.score_and_grad()
+    mats, gaps = b1.grads()
+    summed = np.concatenate([mats.sum(axis=1).reshape(len(b1), -1), gaps.sum(axis=1)], axis=1)
+    es, eg = _rel(b1.scores(), summed, b0.scores(), grads_rows(b0))
+    say(f"  score {es:.2e}, grad {eg:.2e}")
+    assert es <= 1e-5 and eg <= 1e-5
