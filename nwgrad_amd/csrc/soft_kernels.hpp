@@ -28,9 +28,8 @@
 // Boundary values from a neighbour are converted by an exact power-of-two product; a segment
 // without mass of its own takes units in which the neighbour's boundary values are of order
 // one, and one whose neighbour's values would not fit below 2^headroom of its own units moves
-// to the boundary's magnitude (the neighbour's units, or above them when its values exceed
-// one) minus the headroom (its old values then flush towards zero, as in the reference's row
-// scaling).
+// to the neighbour's units (tracked: to the boundary's magnitude) minus the headroom (its old
+// values then flush towards zero, as in the reference's row scaling).
 //
 // Storage.  The forward stores every row segment of every lane -- M, X, Y of its S columns,
 // the boundary column it received, the exponent and the segment maximum -- as 16-byte
@@ -570,12 +569,17 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                         Ec = ml > SF(0) ? lE + silogb(ml) : lE;
                         s = SF(1);
                     } else {
-                        // Headroom, on the boundary's magnitude (its units, or above them when
-                        // its values exceed 1): a neighbour's row can outgrow the units this
-                        // lane carries from its previous row -- the gap chain of a row whose
-                        // gap weights (tracks) decay slower than the row before's.
-                        const SF ml = dmax(lM, dmax(lX, lY));
-                        const int lmag = ml > SF(1) ? lE + silogb(ml) : lE;
+                        // Headroom.  Untracked, on the neighbour's units: every row's gap chain
+                        // decays at the same rate, so the units a lane carries from its previous
+                        // row keep pace with the row.  Tracked, on the boundary's magnitude
+                        // (above its units when its values exceed 1): a row whose gap weights
+                        // decay slower than the row before's outgrows the carried units a few
+                        // lanes along it.
+                        int lmag = lE;
+                        if constexpr (TR) {
+                            const SF ml = dmax(lM, dmax(lX, lY));
+                            if (ml > SF(1)) lmag += silogb(ml);
+                        }
                         if (lmag - R::headroom > Ec) {
                             Ec = lmag - R::headroom;
                             s = sldexp(SF(1), E - Ec);
@@ -638,17 +642,22 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 }
             }
         };
+        // Pipelined: every kPipeChunk steps, publish the steps done and wait for the next
+        // chunk's input -- one uniform comparison per trip (sync_at: the next such step).
+        int sync_at = (fin != nullptr || fout != nullptr) ? kPipeChunk : 0x7fffffff;
+        auto sync = [&](int t) {
+            if (fout != nullptr) publish(fout, t, lane);
+            if (fin != nullptr) bounded_wait(fin, min(t + kPipeChunk + kWave, fsteps), serr, A.wait_ticks, lane, 1);
+            sync_at += kPipeChunk;
+        };
         int t = 0;
         for (; t + 1 < fsteps; t += 2) {
-            if (fin != nullptr && t > 0 && t % kPipeChunk == 0)
-                bounded_wait(fin, min(t + kPipeChunk + kWave, fsteps), serr, A.wait_ticks, lane, 1);
+            if (t == sync_at) sync(t);
             fstep(t);
             fstep(t + 1);
-            if (fout != nullptr && (t + 2) % kPipeChunk == 0) publish(fout, t + 2, lane);
         }
         if (t < fsteps) {
-            if (fin != nullptr && t > 0 && t % kPipeChunk == 0)
-                bounded_wait(fin, min(t + kPipeChunk + kWave, fsteps), serr, A.wait_ticks, lane, 1);
+            if (t == sync_at) sync(t);
             fstep(t);
         }
         if (fout != nullptr) publish(fout, fsteps, lane);
@@ -916,9 +925,12 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 Ec = mr > SF(0) ? rE + silogb(mr) : rE;
                 s = SF(1);
             } else if (has_right) {
-                // headroom on the boundary's magnitude, as the forward's
-                const SF mr = dmax(rM, rY);
-                const int rmag = mr > SF(1) ? rE + silogb(mr) : rE;
+                // headroom, as the forward's (tracked: on the boundary's magnitude)
+                int rmag = rE;
+                if constexpr (TR) {
+                    const SF mr = dmax(rM, rY);
+                    if (mr > SF(1)) rmag += silogb(mr);
+                }
                 if (rmag - R::headroom > Ec) {
                     Ec = rmag - R::headroom;
                     s = sldexp(SF(1), EB - Ec);
@@ -1054,14 +1066,18 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 step(i, rM, rY, rE, Sb);
             }
         };
+        // pipelined: publish and wait every kPipeChunk steps, as the forward
+        int bsync_at = (bin != nullptr || bout != nullptr) ? kPipeChunk : 0x7fffffff;
         for (int u = 0; u < bsteps; u += 2 * kCK) {
-            if (bin != nullptr && u > 0 && u % kPipeChunk == 0)
-                bounded_wait(bin, min(u + kPipeChunk + kWave, bsteps), serr, A.wait_ticks, lane, 1);
+            if (u == bsync_at) {
+                if (bout != nullptr) publish(bout, u, lane);
+                if (bin != nullptr) bounded_wait(bin, min(u + kPipeChunk + kWave, bsteps), serr, A.wait_ticks, lane, 1);
+                bsync_at += kPipeChunk;
+            }
 #pragma unroll
             for (int p = 0; p < kCK; ++p) phase(u + p, p, Fa, Fb);
 #pragma unroll
             for (int p = 0; p < kCK; ++p) phase(u + kCK + p, p, Fb, Fa);
-            if (bout != nullptr && (u + 2 * kCK) % kPipeChunk == 0) publish(bout, u + 2 * kCK, lane);
         }
         if (bout != nullptr) publish(bout, bsteps, lane);
         if (NP > 1) __threadfence();   // the panel's boundary column is read by the previous panel
