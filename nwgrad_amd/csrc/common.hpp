@@ -46,11 +46,7 @@ __device__ __forceinline__ double shift_down(double v) { return __shfl_down(v, 1
 __device__ __forceinline__ int shift_down(int v) { return __shfl_down(v, 1, kWave); }
 
 // The value of the wave's first active lane as a wave-uniform (scalar) value: branches on it
-// are uniform to the compiler.  Persistent loops fetch their next tile with it, from an
-// atomicAdd that every lane issues (lane 0 adding 1, the others 0) rather than one issued
-// under `if (lane == 0)`: with a lane-dependent branch in the loop the compiler has split the
-// loop into an outer loop for lane 0 (the fetch) and an inner one that the other lanes never
-// leave, each lane then running different tiles and the wave never reconverging.
+// are uniform to the compiler.
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ float wave_uniform(float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
@@ -60,6 +56,36 @@ __device__ __forceinline__ double wave_uniform(double v) {
     const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
     const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// The next index of a persistent queue, for the whole wave: one atomic add of 1, issued by
+// lane 0 alone, whose old value comes back wave-uniform.  The narrowing to lane 0 happens
+// inside one asm block (save exec, exec = lane 0, the returning atomic, wait for it, restore
+// exec), so the source holds no lane-dependent branch: a loop that fetches with it stays one
+// uniform loop with a scalar exit test.  Both simpler forms cost something:
+//  - `if (lane == 0) old = atomicAdd(...)` has let the compiler restructure the persistent
+//    loop so that its exit became an exec-mask exit lane 0 alone could take (the other lanes
+//    never left the loop);
+//  - atomicAdd(counter, lane == 0 ? 1 : 0) from every lane is safe, but its per-lane operand
+//    makes the compiler's atomic optimizer scan the 64 operands one lane at a time before its
+//    single atomic, on every fetch.
+// The caller reaches it with lane 0 active (the persistent loops run with all 64 lanes).
+// The s_nop 4 after each exec write keeps the next vector instruction clear of it.
+__device__ __forceinline__ int wave_fetch(int32_t* counter) {
+    int old;
+    uint64_t saved;
+    asm volatile(
+        "s_mov_b64 %1, exec\n\t"
+        "s_mov_b64 exec, 1\n\t"
+        "s_nop 4\n\t"
+        "global_atomic_add %0, %2, %3, %4 sc0\n\t"
+        "s_waitcnt vmcnt(0)\n\t"
+        "s_mov_b64 exec, %1\n\t"
+        "s_nop 4"
+        : "=&v"(old), "=&s"(saved)
+        : "v"(0), "v"(1), "s"(counter)
+        : "memory");
+    return __builtin_amdgcn_readfirstlane(old);
 }
 
 // Wavefront-local memory ordering for LDS handed between lanes of one wave.

@@ -511,7 +511,7 @@ __global__ __launch_bounds__(64) void hard_fill_kernel(HardArgs<T> A) {
         return;
     }
     for (;;) {
-        const int tile = wave_uniform(atomicAdd(A.queue, lane == 0 ? 1 : 0));
+        const int tile = wave_fetch(A.queue);
         if (tile >= A.n_tiles) break;
         hard_tile<T, GM, AM, S, HARD, TR>(A, ssub, hist, tile, (int64_t)blockIdx.x, lane);
     }

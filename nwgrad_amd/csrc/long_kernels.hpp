@@ -252,7 +252,7 @@ __global__ __launch_bounds__(64) void long_fill_kernel(LongArgs<T> A) {
         for (int k = lane; k < 4 * K; k += kWave) ssub[K * nn + k] = A.tr.gaps[k];
     __syncthreads();
     for (;;) {
-        const int it = wave_uniform(atomicAdd(A.queue, lane == 0 ? 1 : 0));
+        const int it = wave_fetch(A.queue);
         if (it >= A.n_items) break;
         long_panel<T, GM, AM, S, HARD, RB, TR>(A, ssub, it, lane);
     }

@@ -1289,7 +1289,7 @@ void soft_kernel(SoftArgs<SF> A) {
             if (phase == 0) {   // items of pipelined wide tiles
                 phase = 1;
                 if (A.litems != nullptr && blk < A.n_pwaves) {
-                    const int k = wave_uniform(atomicAdd(A.lqueue, lane == 0 ? 1 : 0));
+                    const int k = wave_fetch(A.lqueue);
                     if (k < A.n_litems) {
                         const int2 it = A.litems[k];
                         tile = wave_uniform(it.x);
@@ -1304,7 +1304,7 @@ void soft_kernel(SoftArgs<SF> A) {
             if (phase == 1) {   // whole wide tiles
                 phase = 2;
                 if (blk < A.n_lslots) {
-                    const int k = wave_uniform(atomicAdd(A.lqueue + 1, lane == 0 ? 1 : 0));
+                    const int k = wave_fetch(A.lqueue + 1);
                     if (A.n_ptiles + k < A.n_ltiles) {
                         tile = A.n_ptiles + k;
                         wide = true;
@@ -1314,7 +1314,7 @@ void soft_kernel(SoftArgs<SF> A) {
             }
             if (phase == 2) {   // one-panel tiles
                 if (blk >= A.n_mslots) break;
-                tile = wave_uniform(atomicAdd(A.queue, lane == 0 ? 1 : 0));
+                tile = wave_fetch(A.queue);
                 if (tile >= A.n_tiles) break;
             }
         }
