@@ -175,6 +175,11 @@ int nwg_batch_set_soft_scratch(nwg_batch *b, int64_t bytes);
  * whole ones would not fit), the pool's rows (0: none), and the queue lag, in tiles, of a
  * pipelined tile's backward items behind its forward ones. */
 int nwg_batch_wide_info(nwg_batch *b, int64_t *tiles, int64_t *pooled, int64_t *pool_rows, int64_t *lag);
+/* How nwg_batch_run has formed this batch's totals so far (diagnostics, no counterpart in the
+ * reference): from the device's sum, where every order of the additions is exact (all
+ * scores multiples of 2^q with the sum of magnitudes below 2^(53+q)), or by summing the
+ * scores in pair order on the host.  Both give the pair-order total. */
+int nwg_batch_total_info(nwg_batch *b, int64_t *from_device, int64_t *in_pair_order);
 
 #ifdef __cplusplus
 }

@@ -455,6 +455,12 @@ public:
         check_(nwg_batch_wide_info(b_, &r[0], &r[1], &r[2], &r[3]));
         return r;
     }
+    // Totals formed so far from the device's exact sum / in pair order on the host.
+    std::array<int64_t, 2> total_info() const {
+        std::array<int64_t, 2> r{0, 0};
+        check_(nwg_batch_total_info(b_, &r[0], &r[1]));
+        return r;
+    }
     std::array<int64_t, 3> launch_info() const {
         std::array<int64_t, 3> r{0, 0, 0};
         check_(nwg_batch_launch_info(b_, &r[0], &r[1], &r[2]));

@@ -234,8 +234,11 @@ struct nwg_batch {
     DevPtr<double> sub_f64;
     // reduction scratch
     DevPtr<double> partial, wdev, outdev;
+    DevPtr<nwg::ScoreStats> stats;   // the score total's parts, then the result
     double* host_scores = nullptr;   // pinned
     int64_t host_cap = 0;
+    nwg::ScoreStats* host_stats = nullptr;   // pinned
+    int64_t total_fast = 0, total_ordered = 0;   // totals taken from the device sum / in pair order
     // settings
     int fill = 0;
     double temperature = 1.0;
@@ -246,6 +249,7 @@ struct nwg_batch {
 
     ~nwg_batch() {
         if (host_scores) (void)hipHostFree(host_scores);
+        if (host_stats) (void)hipHostFree(host_stats);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
     }
@@ -1696,6 +1700,48 @@ int weighted_dev(nwg_batch* b, const double* w_dev, double* out_dev) {
     return NWG_OK;
 }
 
+// The sum of the scores in pair-index order, as the reference forms the total
+// (batch_engine.hpp:1754-1757), after the results are complete on the stream.  Where no
+// partial sum can round (ScoreStats), every order gives that double and the device's sum is
+// taken (32 bytes back); otherwise the scores come back and are summed in pair order here.
+int score_total(nwg_batch* b, double* total_out) {
+    nwg_ctx* ctx = b->ctx;
+    double t = 0.0;
+    if (b->N > 0) {
+        const int parts = (int)std::min<int64_t>((b->N + nwg::kStatsThreads - 1) / nwg::kStatsThreads, 1024);
+        NWG_HIP(ctx, b->stats.reserve((size_t)parts + 1));
+        if (!b->host_stats)
+            NWG_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&b->host_stats), sizeof(nwg::ScoreStats),
+                                       hipHostMallocDefault));
+        hipLaunchKernelGGL(nwg::score_stats_kernel, dim3(parts), dim3(nwg::kStatsThreads), 0, ctx->stream,
+                           b->scores.p, b->N, b->stats.p);
+        NWG_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(nwg::score_stats_final_kernel, dim3(1), dim3(nwg::kStatsThreads), 0, ctx->stream,
+                           b->stats.p, parts, b->stats.p + parts);
+        NWG_HIP(ctx, hipGetLastError());
+        NWG_HIP(ctx, hipMemcpyAsync(b->host_stats, b->stats.p + parts, sizeof(nwg::ScoreStats),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const nwg::ScoreStats st = *b->host_stats;
+        // st.mag is a sum of N non-negative doubles in some order: the true sum of magnitudes
+        // is below st.mag * (1 + 2N * 2^-53); the extra 4 * 2^-52 covers this product's rounding.
+        const double bound = st.mag * (1.0 + (double)(b->N + 4) * 0x1p-52);
+        const bool exact = st.bad == 0 && (st.qmin == nwg::kNoBit || bound < std::ldexp(1.0, 53 + st.qmin));
+        if (exact) {
+            t = st.sum + 0.0;
+            ++b->total_fast;
+        } else {
+            NWG_HIP(ctx, hipMemcpyAsync(b->host_scores, b->scores.p, sizeof(double) * (size_t)b->N,
+                                        hipMemcpyDeviceToHost, ctx->stream));
+            NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            for (int64_t i = 0; i < b->N; ++i) t += b->host_scores[i];
+            ++b->total_ordered;
+        }
+    }
+    *total_out = t;
+    return NWG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1727,16 +1773,7 @@ int nwg_batch_run(nwg_batch* b, double* total_out) {
         rc = soft_finish(b);
         if (rc) return rc;
     }
-    if (total_out) {
-        if (b->N)
-            NWG_HIP(ctx, hipMemcpyAsync(b->host_scores, b->scores.p, sizeof(double) * (size_t)b->N,
-                                        hipMemcpyDeviceToHost, ctx->stream));
-        NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        // Pair-index order, as the reference sums (batch_engine.hpp:1754-1757).
-        double t = 0.0;
-        for (int64_t i = 0; i < b->N; ++i) t += b->host_scores[i];
-        *total_out = t;
-    }
+    if (total_out) return score_total(b, total_out);
     return NWG_OK;
 }
 
@@ -1877,6 +1914,13 @@ int nwg_batch_wide_info(nwg_batch* b, int64_t* tiles, int64_t* pooled, int64_t* 
     if (pooled) *pooled = b->soft_qtiles;
     if (pool_rows) *pool_rows = b->soft_pool;
     if (lag) *lag = b->soft_lag;
+    return NWG_OK;
+}
+
+int nwg_batch_total_info(nwg_batch* b, int64_t* from_device, int64_t* in_pair_order) {
+    if (!b) return NWG_EINVAL;
+    if (from_device) *from_device = b->total_fast;
+    if (in_pair_order) *in_pair_order = b->total_ordered;
     return NWG_OK;
 }
 

@@ -416,6 +416,10 @@ void bind_batch(py::module_& m, const char* name) {
                                "how many ran pipelined, as panel items across waves, how many took a region of the "
                                "circular wide pool, the pool's rows and the lag, in tiles, of a pipelined tile's "
                                "backward items behind its forward ones")
+        .def_property_readonly("total_info", [](const PB& s) { auto r = s.b->total_info(); return py::make_tuple(r[0], r[1]); },
+                               "(from_device, in_pair_order): how score_and_grad() has formed the totals so far -- "
+                               "the device's sum, taken where no partial sum can round (every order gives the "
+                               "pair-order total), or the scores summed in pair order on the host")
         .def("set_soft_scratch", [](PB& s, int64_t bytes) { s.b->set_soft_scratch(bytes); }, py::arg("bytes"),
              "Device memory the soft pass may take for stored forward rows per launch (0: 96 GiB); "
              "fewer bytes run fewer wide pairs at once, never a different result")

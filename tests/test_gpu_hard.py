@@ -116,3 +116,53 @@ def test_set_params_and_errors(gpu):
     s_ref, g_ref = oracle.hard(mat2, [3, 0.5, 1, 2], w.codes_a, w.off_a, w.codes_b, w.off_b)
     np.testing.assert_array_equal(b.scores(), s_ref)
     np.testing.assert_array_equal(grads_rows(b), g_ref)
+
+
+def _total_is_exact_in_any_order(s):
+    """The certificate the library applies before it takes the device's sum as the total: all
+    scores finite, multiples of 2^q (q: the lowest set-bit exponent over the nonzero ones),
+    and the sum of magnitudes, with rounding margin, below 2^(53 + q)."""
+    s = np.asarray(s, np.float64)
+    if not np.all(np.isfinite(s)):
+        return False
+    nz = s[s != 0]
+    if nz.size == 0:
+        return True
+    m, e = np.frexp(np.abs(nz))
+    ints = (m * 2.0 ** 53).astype(np.int64)
+    low = np.log2((ints & -ints).astype(np.float64)).astype(np.int64)
+    q = int(np.min(low + e - 53))
+    bound = float(np.sum(np.abs(nz))) * (1.0 + (len(s) + 4) * 2.0 ** -52)
+    return bound < 2.0 ** (53 + q)
+
+
+def test_total_paths(gpu):
+    """score_and_grad()'s total is the pair-order sum of the scores (the reference's
+    left-to-right loop) whichever way it is formed: the device's sum where every order is
+    exact, the host's pair-order loop elsewhere.  Cases on both sides of the certificate,
+    checked against np.cumsum (sequential), and the path each took."""
+    w = wl.random_pairs("total_paths", 3000, wl.AA, (20, 200), (20, 200), related=0.5)
+    r = wl.rng("total_paths", "matrix")
+    frac = w.matrix + r.uniform(-0.5, 0.5, size=w.matrix.shape)   # arbitrary doubles
+    frac = (frac + frac.T) / 2
+    cases = [
+        ("integer matrix, float32", w.matrix, (11, 1, 11, 1), "float32", "global", True),
+        ("integer matrix, quarter gaps, float64", w.matrix, (4.5, 0.25, 3.75, 1.0), "float64", "global", True),
+        ("fractional matrix, float64", frac, (11.1, 0.9, 10.7, 1.3), "float64", "global", False),
+        ("all scores zero (local, negative matrix)", -np.abs(w.matrix) - 1, (11, 1, 11, 1), "float64", "local",
+         True),
+        # 2^-40 in one gap: every global score carries it; thousands of pairs of ~100 push the
+        # sum of magnitudes past 2^13
+        ("gap with a 2^-40 part, float64", w.matrix, (11, 1 + 2.0 ** -40, 11, 1), "float64", "global", False),
+    ]
+    for name, mat, gaps, precision, mode, exact in cases:
+        say("case total", name)
+        b = batch(wl.AA, mat, gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, precision=precision, mode=mode)
+        f0, o0 = b.total_info
+        total = b.score_and_grad()
+        s = b.scores()
+        assert _total_is_exact_in_any_order(s) == exact, name
+        f1, o1 = b.total_info
+        assert (f1 - f0, o1 - o0) == ((1, 0) if exact else (0, 1)), name
+        assert total == float(np.cumsum(s)[-1]), name
+        assert np.signbit(total) == np.signbit(np.cumsum(s)[-1]), name
