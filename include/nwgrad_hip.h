@@ -111,8 +111,9 @@ int nwg_batch_num_sets(nwg_batch *b, int *K, int *tracked);
  * counterpart in the reference, whose parameters live on the host. */
 int nwg_batch_set_params_dev(nwg_batch *b, const double *theta_dev);
 
-/* fill 0: one persistent launch pulling tiles from a device work queue (default);
- * fill 1: plain per-bucket launches.  Results do not depend on it
+/* fill 0 (default): the batch's tiles in one launch, most work first -- short hard / score
+ * tiles one per block in dispatch order, soft and long-pair work pulled from device queues by
+ * a persistent grid; fill 1: plain per-bucket launches.  Results do not depend on it
  * (the reference's `fill=` knob, py_exports.cpp:1180-1207). */
 int nwg_batch_set_fill(nwg_batch *b, int fill);
 
@@ -169,6 +170,17 @@ int nwg_batch_launch_info(nwg_batch *b, int64_t *grid, int64_t *wide_slots, int6
  * reference, whose CPU pass keeps one pair's rows per worker thread (soft_kernel_impl.inl
  * row storage); it bounds the HBM a serving process spends on soft batches. */
 int nwg_batch_set_soft_scratch(nwg_batch *b, int64_t bytes);
+/* Device memory the hard fill of pairs up to 1,024 columns may take for its pointer codes
+ * (0: the default, 16 GiB).  Fill 0 runs one plain launch, a code slot per tile, when that
+ * fits, else one persistent launch whose waves (one slot each) fit it; fill 1 splits each
+ * bucket into launches that fit it (never fewer than 1,024 tiles a launch).  Results never
+ * depend on it.  No counterpart in the reference, whose CPU walk keeps one pair's codes per
+ * worker thread. */
+int nwg_batch_set_table_budget(nwg_batch *b, int64_t bytes);
+/* The hard / score-only fill of pairs up to 1,024 columns under the current fill mode and
+ * table budget (diagnostics, no counterpart in the reference): its launches, whether fill 0
+ * runs persistent, and the grid of the last fill if it ran persistent (0 if plain). */
+int nwg_batch_hard_launches(nwg_batch *b, int64_t *launches, int64_t *persistent, int64_t *last_grid);
 /* The last persistent soft launch's wide tiles (diagnostics, no counterpart in the reference):
  * how many ran pipelined, as panel items across waves (0: none), how many took a region of
  * the circular wide pool (the pipelined ones, or every wide tile when block slots for the

@@ -330,8 +330,10 @@ public:
     }
     const AlignParams& params() const noexcept { return params_; }
 
-    // fill: "interpair" = one persistent launch over a device work queue; "striped" /
-    // "rowwise" = plain per-bucket launches.  Results are the same whichever runs.
+    // fill: "interpair" = the batch's tiles in one launch, most work first (short hard / score
+    // tiles one per block in dispatch order; soft and long-pair work from device queues on a
+    // persistent grid); "striped" / "rowwise" = plain per-bucket launches.  Results are the
+    // same whichever runs.
     void set_fill(const std::string& f) {
         int v;
         if (f == "interpair") v = 0;
@@ -450,6 +452,14 @@ public:
     // grid, wide-queue blocks, resident blocks per CU of the last soft launch
     // Soft scratch: device memory the soft pass may take for stored forward rows (0: default).
     void set_soft_scratch(int64_t bytes) { check_(nwg_batch_set_soft_scratch(b_, bytes)); }
+    // Pointer-code memory of the hard fill (0: default); its launches, whether fill 0 runs
+    // persistent, the last fill's persistent grid (0: plain).
+    void set_table_budget(int64_t bytes) { check_(nwg_batch_set_table_budget(b_, bytes)); }
+    std::array<int64_t, 3> hard_launches() const {
+        std::array<int64_t, 3> r{0, 0, 0};
+        check_(nwg_batch_hard_launches(b_, &r[0], &r[1], &r[2]));
+        return r;
+    }
     std::array<int64_t, 4> wide_info() const {
         std::array<int64_t, 4> r{0, 0, 0, 0};
         check_(nwg_batch_wide_info(b_, &r[0], &r[1], &r[2], &r[3]));

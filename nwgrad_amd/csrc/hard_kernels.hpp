@@ -51,7 +51,7 @@ struct HardArgs {
     const int32_t* order;     // pair indices, tile-major
     const TileDesc* tiles;
     int32_t n_tiles;
-    int32_t tile_base;        // plain launch: the first tile of this launch
+    int32_t tile_base;        // plain launch: block b fills tile tile_base + b (below n_tiles)
     int32_t* queue;           // persistent launch: next-tile counter (nullptr: plain)
     const T* sub;             // K * n_alpha * n_alpha, the matrices cast to T (K = 1 untracked)
     int32_t na;
@@ -486,9 +486,15 @@ __host__ __device__ constexpr int hard_lds_hist_bytes(int na, int K, bool tr, in
     return ((slots * K * na * na * 2 + 15) & ~15) + (tr ? slots * 4 * K * 4 : 0);
 }
 
-// One wavefront per block.  Plain launch: block b fills tile tile_base + b.  Persistent
-// launch: each block pulls tiles from the queue until it is drained; every wave reaches
-// the exit test once the counter passes n_tiles.
+// One wavefront per block.
+//  - Plain launch (queue == nullptr): block b fills tile tile_base + b into pointer-code slot
+//    b.  Over the tiles in their persistent order (most row steps first) this is a
+//    longest-first work queue without a counter: the dispatcher hands blocks to the CUs in
+//    index order as slots free up.  It needs a slot per tile.
+//  - Persistent launch: each block pulls tiles from the queue into its own slot until the
+//    queue is drained (every wave reaches the exit test once the counter passes n_tiles),
+//    for batches whose codes would not fit a slot per tile.
+// The slot histograms start at zero and each tile's flush clears what it used.
 template <typename T, int GM, int AM, int S, bool HARD, bool TR>
 __global__ __launch_bounds__(64) void hard_fill_kernel(HardArgs<T> A) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -507,7 +513,7 @@ __global__ __launch_bounds__(64) void hard_fill_kernel(HardArgs<T> A) {
     __syncthreads();
     if (A.queue == nullptr) {
         const int tile = A.tile_base + (int)blockIdx.x;
-        if (tile < A.n_tiles) hard_tile<T, GM, AM, S, HARD, TR>(A, ssub, hist, tile, tile, lane);
+        if (tile < A.n_tiles) hard_tile<T, GM, AM, S, HARD, TR>(A, ssub, hist, tile, (int64_t)blockIdx.x, lane);
         return;
     }
     for (;;) {

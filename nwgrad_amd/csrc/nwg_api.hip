@@ -49,6 +49,8 @@ constexpr int kSoftBlocksPerCU = 8;    // fallback persistent grid of the soft p
 constexpr int64_t kSumBlock = 4096;    // the reference's WEIGHTED_GRAD_BLOCK
 constexpr size_t kScratchBudget = size_t(96) << 30;   // soft row-segment scratch of one launch, at most
 constexpr size_t kTableBudget = size_t(64) << 30;     // long-pair pointer codes of one launch group, at most
+constexpr int64_t kHardTableBudget = int64_t(16) << 30;   // short-pair pointer codes of one fill (default)
+constexpr int64_t kHardMinLaunch = 1024;   // tiles of a per-bucket launch the budget cannot split further
 constexpr int kRowBlock = 64;          // long pairs: rows a panel publishes at a time
 constexpr uint64_t kWaitTicks = 200000000;   // long pairs: a row-block wait, at most (2 s of the 100 MHz s_memrealtime)
 
@@ -121,6 +123,13 @@ struct Bucket {
 // G each (G from len(B), at least gmin), pairs sorted by len(A) within a width so a
 // tile's rows are even; the plain order (bucket by bucket) and the persistent order
 // (longest tile first) of the same tiles.
+// One plain launch of the hard / score-only fill: tiles [begin, end) of the plain order, each
+// block's pointer-code slot sized for the longest of them.
+struct HardLaunch {
+    int32_t begin, end;
+    int64_t slot_dwords;
+};
+
 struct Plan {
     int S = kS;              // columns per lane the tiles were cut for
     std::vector<TileDesc> tiles;
@@ -131,6 +140,10 @@ struct Plan {
     bool built = false;
     DevPtr<int32_t> d_order;
     DevPtr<TileDesc> d_tiles, d_ptiles;
+    // the per-bucket fill's launches: each bucket in spans whose pointer codes fit the batch's
+    // table budget, and the table the largest needs
+    std::vector<HardLaunch> bucket_launches;
+    int64_t bucket_table_dwords = 0;
 };
 
 // The soft planner's output: tiles of one or more pairs side by side (nwg::SoftTile).
@@ -214,6 +227,8 @@ struct nwg_batch {
     DevPtr<int32_t> lsync;                                   // and words
     bool soft_piped = false;                                 // a launch since the last check
     size_t soft_scratch = kScratchBudget;                    // row-segment scratch of one launch
+    int64_t table_budget = kHardTableBudget;                 // short-pair pointer codes of one fill
+    int64_t hard_grid = 0;                                   // the last hard fill's persistent grid (0: plain)
     int soft_occ_cache[4] = {0, 0, 0, 0};   // resident blocks per CU: float, double; untracked, tracked
     DevPtr<uint8_t> soft_ok;
     std::vector<uint8_t> host_ok;
@@ -263,6 +278,42 @@ int group_width(int n, int S, int gmin, bool pow2) {
     int g = (n + S - 1) / S;
     if (pow2) g = next_pow2(g);
     return std::min(std::max(g, gmin), kMaxG);
+}
+
+// The per-bucket fill's launches: each bucket in spans whose pointer codes (one slot per tile,
+// sized for the span's longest tile) fit the batch's table budget, never fewer than
+// kHardMinLaunch tiles; score-only fills keep no codes and take each bucket whole.
+void split_launches(nwg_batch* b, Plan& pl) {
+    const bool codes = b->prob.grad_mode == 1;
+    const int S = pl.S;
+    pl.bucket_launches.clear();
+    pl.bucket_table_dwords = 0;
+    for (const Bucket& bk : pl.buckets) {
+        int c0 = bk.tile_begin;
+        while (c0 < bk.tile_end) {
+            int64_t steps = 1;
+            int c1 = c0;
+            while (c1 < bk.tile_end) {
+                const TileDesc& t = pl.tiles[(size_t)c1];
+                const int64_t s = std::max<int64_t>(steps, (int64_t)t.max_m + t.G - 1);
+                const int64_t bytes = (int64_t)(c1 - c0 + 1) * s * (S / 4) * 64 * 4;
+                if (codes && c1 - c0 >= kHardMinLaunch && bytes > b->table_budget) break;
+                steps = s;
+                ++c1;
+            }
+            const int64_t sd = steps * (S / 4) * 64;
+            pl.bucket_launches.push_back(HardLaunch{c0, c1, sd});
+            pl.bucket_table_dwords = std::max<int64_t>(pl.bucket_table_dwords, (int64_t)(c1 - c0) * sd);
+            c0 = c1;
+        }
+    }
+}
+
+// Whether the default fill of a plan runs persistent: its pointer codes, a slot per tile of
+// the plan's longest steps, exceed the batch's table budget.
+bool hard_persistent(const nwg_batch* b, const Plan& pl) {
+    const int64_t slot_bytes = pl.max_steps * (pl.S / 4) * 64 * 4;
+    return b->prob.grad_mode == 1 && (int64_t)pl.tiles.size() * slot_bytes > b->table_budget;
 }
 
 int build_plan(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& pl, int S, bool pow2) {
@@ -317,6 +368,7 @@ int build_plan(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& pl, int S,
     };
     std::stable_sort(ptiles.begin(), ptiles.end(),
                      [&](const TileDesc& a, const TileDesc& c) { return work(a) > work(c); });
+    split_launches(b, pl);
     NWG_HIP(ctx, pl.d_order.reserve(std::max<size_t>(pl.order.size(), 1)));
     NWG_HIP(ctx, pl.d_tiles.reserve(std::max<size_t>(pl.tiles.size(), 1)));
     NWG_HIP(ctx, pl.d_ptiles.reserve(std::max<size_t>(pl.tiles.size(), 1)));
@@ -1064,14 +1116,21 @@ int launch_hard(nwg_batch* b, const Plan& pl) {
     NWG_PICK(I0, I0) NWG_PICK(I0, I1) NWG_PICK(I0, I2) NWG_PICK(I1, I0) NWG_PICK(I1, I1) NWG_PICK(I1, I2)
 #undef NWG_PICK
     if (!kern) return fail(ctx, NWG_EINVAL, "nwgrad_hip: unsupported problem type");
+    // fill 0: one plain launch over the persistent order when a slot per tile fits the table
+    // budget, else one persistent launch with as many slots (waves) as fit it, at most the
+    // resident blocks; fill 1: the per-bucket launches
     const int64_t slot_dwords = pl.max_steps * (kS / 4) * 64;
-    const int64_t slots = std::max<int64_t>(
-        b->fill == 0 ? std::min<int64_t>((int64_t)pl.tiles.size(), resident_blocks(ctx, kern, lds, kBlocksPerCU))
-                     : (int64_t)pl.tiles.size(),
-        1);
-    if (hard && b->table.n < (size_t)(slots * slot_dwords)) {
+    const bool persistent = b->fill == 0 && hard_persistent(b, pl);
+    int64_t grid = 0;
+    if (persistent)
+        grid = std::max<int64_t>(1, std::min<int64_t>({(int64_t)pl.tiles.size(),
+                                                       resident_blocks(ctx, kern, lds, kBlocksPerCU),
+                                                       b->table_budget / (slot_dwords * 4)}));
+    const int64_t table_dwords = b->fill != 0 ? pl.bucket_table_dwords
+                                              : (persistent ? grid : (int64_t)pl.tiles.size()) * slot_dwords;
+    if (hard && b->table.n < (size_t)table_dwords) {
         NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        NWG_HIP(ctx, b->table.alloc((size_t)(slots * slot_dwords)));
+        NWG_HIP(ctx, b->table.alloc((size_t)table_dwords));
     }
     nwg::HardArgs<T> A{};
     A.codes_a = b->ca.p;
@@ -1091,25 +1150,29 @@ int launch_hard(nwg_batch* b, const Plan& pl) {
     A.scores = b->scores.p;
     A.grads = b->grads.p;
     A.table = b->table.p;
-    A.slot_dwords = slot_dwords;
-    if (pl.tiles.empty()) return NWG_OK;
+    b->hard_grid = grid;
     if (b->fill == 0) {
-        NWG_HIP(ctx, hipMemsetAsync(b->queue.p, 0, sizeof(int32_t), ctx->stream));
-        A.tiles = pl.d_ptiles.p;
-        A.n_tiles = (int32_t)pl.tiles.size();
-        A.queue = b->queue.p;
+        A.tiles = pl.d_ptiles.p;   // most row steps first
         A.tile_base = 0;
-        hipLaunchKernelGGL(kern, dim3((unsigned)slots), dim3(64), lds, ctx->stream, A);
-        NWG_HIP(ctx, hipGetLastError());
-    } else {
-        A.tiles = pl.d_tiles.p;
-        A.queue = nullptr;
-        for (const Bucket& bk : pl.buckets) {
-            A.tile_base = bk.tile_begin;
-            A.n_tiles = bk.tile_end;
-            hipLaunchKernelGGL(kern, dim3(bk.tile_end - bk.tile_begin), dim3(64), lds, ctx->stream, A);
+        A.n_tiles = (int32_t)pl.tiles.size();
+        A.slot_dwords = slot_dwords;
+        if (persistent) {
+            hipLaunchKernelGGL(nwg::clear_words_kernel, dim3(1), dim3(256), 0, ctx->stream, b->queue.p, (int64_t)1);
             NWG_HIP(ctx, hipGetLastError());
+            A.queue = b->queue.p;
         }
+        hipLaunchKernelGGL(kern, dim3((unsigned)(persistent ? grid : (int64_t)pl.tiles.size())), dim3(64), lds,
+                           ctx->stream, A);
+        NWG_HIP(ctx, hipGetLastError());
+        return NWG_OK;
+    }
+    A.tiles = pl.d_tiles.p;
+    for (const HardLaunch& L : pl.bucket_launches) {
+        A.tile_base = L.begin;
+        A.n_tiles = L.end;
+        A.slot_dwords = L.slot_dwords;
+        hipLaunchKernelGGL(kern, dim3((unsigned)(L.end - L.begin)), dim3(64), lds, ctx->stream, A);
+        NWG_HIP(ctx, hipGetLastError());
     }
     return NWG_OK;
 }
@@ -1198,8 +1261,11 @@ int launch_long(nwg_batch* b) {
         A.items = L.d_items.p + gr.item_begin;
         A.n_items = items;
         A.lpairs = L.d_lpairs.p + gr.pair_begin;
-        NWG_HIP(ctx, hipMemsetAsync(b->lprog.p, 0, sizeof(int32_t) * (size_t)items, ctx->stream));
-        NWG_HIP(ctx, hipMemsetAsync(A.queue, 0, sizeof(int32_t), ctx->stream));
+        hipLaunchKernelGGL(nwg::clear_words_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, ctx->stream,
+                           b->lprog.p, (int64_t)items);
+        NWG_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(nwg::clear_words_kernel, dim3(1), dim3(256), 0, ctx->stream, A.queue, (int64_t)1);
+        NWG_HIP(ctx, hipGetLastError());
         const int64_t grid = std::max<int64_t>(std::min<int64_t>(items, resident), 1);
         hipLaunchKernelGGL(fill, dim3((unsigned)grid), dim3(64), lds_fill, ctx->stream, A);
         NWG_HIP(ctx, hipGetLastError());
@@ -1905,6 +1971,22 @@ int nwg_batch_set_soft_scratch(nwg_batch* b, int64_t bytes) {
     if (!b) return NWG_EINVAL;
     if (bytes < 0) return fail(b->ctx, NWG_EINVAL, "nwgrad_hip: soft scratch bytes must be >= 0 (0: the default)");
     b->soft_scratch = bytes == 0 ? kScratchBudget : (size_t)bytes;
+    return NWG_OK;
+}
+
+int nwg_batch_set_table_budget(nwg_batch* b, int64_t bytes) {
+    if (!b) return NWG_EINVAL;
+    if (bytes < 0) return fail(b->ctx, NWG_EINVAL, "nwgrad_hip: table budget bytes must be >= 0 (0: the default)");
+    b->table_budget = bytes == 0 ? kHardTableBudget : bytes;
+    split_launches(b, b->plan);
+    return NWG_OK;
+}
+
+int nwg_batch_hard_launches(nwg_batch* b, int64_t* launches, int64_t* persistent, int64_t* last_grid) {
+    if (!b) return NWG_EINVAL;
+    if (launches) *launches = b->plan.tiles.empty() ? 0 : b->fill == 0 ? 1 : (int64_t)b->plan.bucket_launches.size();
+    if (persistent) *persistent = b->fill == 0 && hard_persistent(b, b->plan) ? 1 : 0;
+    if (last_grid) *last_grid = b->hard_grid;
     return NWG_OK;
 }
 

@@ -372,8 +372,9 @@ void bind_batch(py::module_& m, const char* name) {
         .def_property_readonly("num_sets", [](const PB& s) { return s.b->num_sets(); })
         .def_property(
             "fill", [](const PB& s) { return s.b->fill(); }, [](PB& s, const std::string& v) { s.b->set_fill(v); },
-            "\"interpair\" (default): one persistent launch pulling length-sorted tiles from a device work\n"
-            "queue; \"striped\" / \"rowwise\": plain per-bucket launches.  Results are identical.")
+            "\"interpair\" (default): the batch's tiles in one launch, most work first -- short hard /\n"
+            "score tiles one per block in dispatch order, soft and long-pair work pulled from device queues\n"
+            "by a persistent grid; \"striped\" / \"rowwise\": plain per-bucket launches.  Results are identical.")
         .def_property(
             "soft_temperature", [](const PB& s) { return s.b->soft_temperature(); },
             [](PB& s, double v) { s.b->set_soft_temperature(v); })
@@ -420,6 +421,14 @@ void bind_batch(py::module_& m, const char* name) {
                                "(from_device, in_pair_order): how score_and_grad() has formed the totals so far -- "
                                "the device's sum, taken where no partial sum can round (every order gives the "
                                "pair-order total), or the scores summed in pair order on the host")
+        .def("set_table_budget", [](PB& s, int64_t bytes) { s.b->set_table_budget(bytes); }, py::arg("bytes"),
+             "Device memory the hard fill (pairs up to 1,024 columns) may take for its pointer codes (0: 16 GiB). "
+             "Over it, fill \"interpair\" runs one persistent launch with as many waves as fit it and the "
+             "per-bucket fill splits its launches; results never change")
+        .def_property_readonly("hard_launches", [](const PB& s) { auto r = s.b->hard_launches(); return py::make_tuple(r[0], r[1], r[2]); },
+                               "(launches, persistent, last_grid) of the hard / score-only fill of pairs up to "
+                               "1,024 columns under the current fill mode and table budget: its launches, whether "
+                               "the default fill runs persistent, and the last fill's persistent grid (0: plain)")
         .def("set_soft_scratch", [](PB& s, int64_t bytes) { s.b->set_soft_scratch(bytes); }, py::arg("bytes"),
              "Device memory the soft pass may take for stored forward rows per launch (0: 96 GiB); "
              "fewer bytes run fewer wide pairs at once, never a different result")

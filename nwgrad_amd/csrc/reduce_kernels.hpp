@@ -31,6 +31,13 @@ __global__ __launch_bounds__(64) void block_sum_kernel(const double* __restrict_
     partial[blk * gs + k] = acc;
 }
 
+// p[0 .. n) = 0: counters and progress words cleared by a kernel rather than a memset, so the
+// pass can be captured in a graph without memset nodes.
+__global__ __launch_bounds__(256) void clear_words_kernel(int32_t* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 // The batch total without a pass in pair order, where no partial sum can round: the scores'
 // sum and sum of magnitudes (in any order), the lowest set-bit exponent over the nonzero
 // scores (each score is then a multiple of 2^qmin) and whether any score is NaN or infinite.

@@ -166,3 +166,39 @@ def test_total_paths(gpu):
         assert (f1 - f0, o1 - o0) == ((1, 0) if exact else (0, 1)), name
         assert total == float(np.cumsum(s)[-1]), name
         assert np.signbit(total) == np.signbit(np.cumsum(s)[-1]), name
+
+
+@pytest.mark.parametrize("fill", ["interpair", "striped"])
+def test_table_budget(gpu, fill):
+    """A pointer-code budget too small for a slot per tile: the default fill turns into one
+    persistent launch whose waves (one slot each) fit the budget, pulling tiles from the device
+    queue; the per-bucket fill splits its buckets into launches of 1,024 tiles.  Every score and
+    count unchanged, and the oracle's."""
+    w = wl.random_pairs("table_budget", 100000, wl.DNA, (10, 40), (10, 60), gaps=(3, 1, 3, 1), related=0.5)
+    s_ref, g_ref = oracle.hard(w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b)
+    b = from_workload(w, fill=fill)
+    launches, persistent, _ = b.hard_launches
+    assert persistent == 0 and launches >= 1 and (launches == 1 or fill == "striped")
+    budget = 1 << 20
+    b.set_table_budget(budget)
+    launches2, persistent2, _ = b.hard_launches
+    if fill == "interpair":
+        assert (launches2, persistent2) == (1, 1)
+    else:
+        assert persistent2 == 0 and launches2 >= -(-b.tiles // 1024) and launches2 > launches
+    total = b.score_and_grad()
+    grid = b.hard_launches[2]
+    say("case table budget", fill, b.tiles, "tiles", launches, "->", launches2, "launches, persistent grid", grid)
+    if fill == "interpair":
+        # slots of at least (10 + 1 - 1) rows of 1 KiB each: the budget holds at most 102 of them
+        assert 1 <= grid <= budget // (10 * 1024)
+    else:
+        assert grid == 0
+    np.testing.assert_array_equal(b.scores(), s_ref)
+    np.testing.assert_array_equal(grads_rows(b), g_ref)
+    assert total == float(np.cumsum(s_ref)[-1])
+    b.set_table_budget(0)
+    assert b.hard_launches[:2] == (launches, 0)
+    b.score_and_grad()
+    assert b.hard_launches[2] == 0
+    np.testing.assert_array_equal(grads_rows(b), g_ref)
