@@ -23,7 +23,11 @@
 // the smaller (i, j): the reference's row-major first rule), then walks the codes back.
 // The walk only ever moves to the same, the previous or the one-before-previous step of a
 // panel's table, so it reads the table through an LDS window of kWalkSteps steps that all
-// threads of the block refill together.
+// threads of the block refill together.  Within a panel the step only decreases, so the next
+// window is known: it is loaded into registers while the walk runs through the current one,
+// and a refill is a store to LDS.  The window also stages the codes of B in the panel's
+// columns and those of A (tracked: the row sets too) in the window's rows, so a walk step
+// reads LDS only.
 #pragma once
 
 #include "hard_kernels.hpp"
@@ -32,6 +36,7 @@ namespace nwg {
 
 constexpr int kWalkSteps = 48;     // table steps held in the walk's LDS window (48 KB)
 constexpr int kWalkThreads = 256;  // threads of a walk block (they refill the window together)
+constexpr int kWalkRows = 128;     // rows of A staged per window: at least kWalkSteps + kWave - 1
 
 struct PanelItem {
     int32_t pair;
@@ -259,9 +264,90 @@ __global__ __launch_bounds__(64) void long_fill_kernel(LongArgs<T> A) {
 }
 
 // LDS of the long-pair walk: K * n_alpha^2 uint32 match counts and (tracked) K x 4 int gap
-// counters, then the table window.
+// counters, then the table window, the panel's codes of B, and the window rows' codes of A
+// (tracked: their sets and gap sets, uint16).
 __host__ __device__ constexpr int long_walk_count_bytes(int na, int K, bool tr) {
     return ((K * na * na + (tr ? 4 * K : 0)) * 4 + 15) & ~15;
+}
+__host__ __device__ constexpr int long_walk_lds_bytes(int na, int K, bool tr, int S) {
+    return long_walk_count_bytes(na, K, tr) + kWalkSteps * (S / 4) * kWave * 4 + kWave * S + kWalkRows +
+           (tr ? 2 * kWalkRows * 2 : 0);
+}
+
+// The walk's prefetched window: a thread's share of kWalkSteps table steps, 12 vectors of 4
+// dwords (16-column panels: 48 steps x 64 vectors / 256 threads).  Separate fields of a plain
+// vector type, reached through compile-time indices, so the compiler keeps them in registers
+// while they are carried across walk steps (an array of uint4 there stayed in scratch).
+typedef uint32_t walk_u4 __attribute__((ext_vector_type(4)));   // a plain vector: promoted to registers
+struct WalkBuf {
+    walk_u4 v0, v1, v2, v3, v4, v5, v6, v7, v8, v9, v10, v11;
+};
+template <int R>
+__device__ __forceinline__ walk_u4& walk_field(WalkBuf& b) {
+    static_assert(R >= 0 && R < 12, "WalkBuf holds 12 vectors");
+    if constexpr (R == 0) return b.v0;
+    else if constexpr (R == 1) return b.v1;
+    else if constexpr (R == 2) return b.v2;
+    else if constexpr (R == 3) return b.v3;
+    else if constexpr (R == 4) return b.v4;
+    else if constexpr (R == 5) return b.v5;
+    else if constexpr (R == 6) return b.v6;
+    else if constexpr (R == 7) return b.v7;
+    else if constexpr (R == 8) return b.v8;
+    else if constexpr (R == 9) return b.v9;
+    else if constexpr (R == 10) return b.v10;
+    else return b.v11;
+}
+// Thread tid's share of n4 vectors from src (the last one repeated past n4: every load in flight
+// at once), and back out to dst below n4.
+template <int R>
+__device__ __forceinline__ void walk_load(WalkBuf& b, const walk_u4* __restrict__ src, int n4, int tid) {
+    if constexpr (R < 12) {
+        const int q = tid + R * kWalkThreads;
+        walk_field<R>(b) = src[q < n4 ? q : n4 - 1];
+        walk_load<R + 1>(b, src, n4, tid);
+    }
+}
+template <int R>
+__device__ __forceinline__ void walk_store(WalkBuf& b, walk_u4* dst, int n4, int tid) {
+    if constexpr (R < 12) {
+        const int q = tid + R * kWalkThreads;
+        if (q < n4) dst[q] = walk_field<R>(b);
+        walk_store<R + 1>(b, dst, n4, tid);
+    }
+}
+
+// Window [lo, hi] of a panel's table (its first dword at `panel`) into registers: the thread's
+// share of the steps, and for thread tid < kWalkRows the staged row alo + tid of A, alo =
+// max(0, lo - 63) (a step t of the window is row t - go + 1 for a lane go in 0 .. 63): its
+// code and, tracked, its set and gap set (gt: the pair's gap sets, one per DP row).
+template <bool TR, int SD>
+__device__ __forceinline__ void walk_fetch(WalkBuf& buf, int& ra, int& rtk, int& rgt, const uint32_t* panel,
+                                           int lo, int hi, int tid, int m, const uint8_t* ap, const uint16_t* tk,
+                                           const uint16_t* gt) {
+    walk_load<0>(buf, reinterpret_cast<const walk_u4*>(panel + (int64_t)lo * SD), (hi - lo + 1) * (SD / 4), tid);
+    const int alo = lo - (kWave - 1) > 0 ? lo - (kWave - 1) : 0;
+    const int x = alo + tid;
+    if (tid < kWalkRows && x <= hi && x < m) {
+        ra = ap[x];
+        if constexpr (TR) {
+            rtk = tk[x];
+            rgt = gt[x + 1];
+        }
+    }
+}
+// The fetched window [lo, hi] into LDS.
+template <bool TR, int SD>
+__device__ __forceinline__ void walk_put(WalkBuf& buf, int ra, int rtk, int rgt, uint32_t* win, uint8_t* sa,
+                                         uint16_t* stk, uint16_t* sgt, int lo, int hi, int tid) {
+    walk_store<0>(buf, reinterpret_cast<walk_u4*>(win), (hi - lo + 1) * (SD / 4), tid);
+    if (tid < kWalkRows) {
+        sa[tid] = (uint8_t)ra;
+        if constexpr (TR) {
+            stk[tid] = (uint16_t)rtk;
+            sgt[tid] = (uint16_t)rgt;
+        }
+    }
 }
 
 // Block b: pair lpairs[b].  Combines its panels' candidates into the end cell, writes the
@@ -328,8 +414,21 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
             if (tid == 0) __hip_atomic_fetch_or(A.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             i = 0; j = 0;
         }
+        uint8_t* sb = reinterpret_cast<uint8_t*>(win + kWalkSteps * SD);   // B codes of the window's panel
+        uint8_t* sa = sb + PW;                                               // A codes of rows alo ..
+        uint16_t* stk = reinterpret_cast<uint16_t*>(sa + kWalkRows);          // tracked: their sets
+        uint16_t* sgt = stk + kWalkRows;                                     // and gap sets (of row x + 1)
+        static_assert(12 * kWalkThreads == kWalkSteps * (SD / 4), "the window is 12 vectors a thread");
+        static_assert(kWalkRows >= kWalkSteps + kWave - 1 && kWalkRows <= kWalkThreads, "staged rows");
+        // Window [lo, hi] of panel P into registers: the table steps, and per thread tid <
+        // kWalkRows the staged row alo + tid (alo = max(0, lo - 63): a step t of the window is
+        // row t - go + 1 for a lane go in 0 .. 63).
+        WalkBuf buf;
+        int ra = 0, rtk = 0, rgt = 0;
         int wpanel = -1;
         int wlo = 1, whi = 0;   // the window holds steps [wlo, whi] of panel wpanel
+        int alo = 0;            // and rows alo .. of A
+        bool ahead = false;     // registers hold steps [wlo - kWalkSteps, wlo - 1] (clamped at 0)
         for (;;) {
             if (i <= 0 && j <= 0) break;
             if (i == 0 || j == 0) {   // borders: runs (Global / Glocal column 0) or a stop
@@ -354,32 +453,46 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
             const int t = i - 1 + go;
             if (P != wpanel || t < wlo || t > whi) {
                 __syncthreads();   // every read of the old window is done
-                wpanel = P;
-                whi = t;
-                wlo = t - (kWalkSteps - 1) > 0 ? t - (kWalkSteps - 1) : 0;
-                const uint4* src = reinterpret_cast<const uint4*>(A.table + A.items[lp.item + P].table_off +
-                                                                  (int64_t)wlo * SD);
-                uint4* dst = reinterpret_cast<uint4*>(win);
-                const int n4 = (whi - wlo + 1) * (SD / 4);
-                constexpr int per = kWalkSteps * (SD / 4) / kWalkThreads;
-                uint4 buf[per];
-#pragma unroll
-                for (int r = 0; r < per; ++r) {   // every load in flight at once (n4 >= 64)
-                    const int q = tid + r * kWalkThreads;
-                    buf[r] = src[q < n4 ? q : n4 - 1];
+                const int nlo = wlo - kWalkSteps > 0 ? wlo - kWalkSteps : 0;
+                if (P == wpanel && ahead && t >= nlo && t < wlo) {
+                    whi = wlo - 1;   // the prefetched window: t is at most 2 steps below wlo
+                    wlo = nlo;
+                } else {
+                    whi = t;
+                    wlo = t - (kWalkSteps - 1) > 0 ? t - (kWalkSteps - 1) : 0;
+                    walk_fetch<TR, SD>(buf, ra, rtk, rgt, A.table + A.items[lp.item + P].table_off, wlo, whi, tid, m,
+                                       ap, tk, A.tr.gtrk + r0);
+                    if (P != wpanel) {
+                        const int c0 = P * PW, cn = n - c0 < PW ? n - c0 : PW;
+                        for (int x = tid; x < cn; x += kWalkThreads) sb[x] = bp[c0 + x];
+                    }
+                    wpanel = P;
                 }
-#pragma unroll
-                for (int r = 0; r < per; ++r) {
-                    const int q = tid + r * kWalkThreads;
-                    if (q < n4) dst[q] = buf[r];
-                }
+                alo = wlo - (kWave - 1) > 0 ? wlo - (kWave - 1) : 0;
+                walk_put<TR, SD>(buf, ra, rtk, rgt, win, sa, stk, sgt, wlo, whi, tid);
                 __syncthreads();
+                ahead = wlo > 0;
+                if (ahead)
+                    walk_fetch<TR, SD>(buf, ra, rtk, rgt, A.table + A.items[lp.item + P].table_off,
+                                       wlo - kWalkSteps > 0 ? wlo - kWalkSteps : 0, wlo - 1, tid, m, ap, tk,
+                                       A.tr.gtrk + r0);
             }
             // every thread reads the same word: a wave-uniform value keeps the walk's control
             // flow scalar
             const uint32_t w = (uint32_t)wave_uniform((int)win[(t - wlo) * SD + (k >> 2) * kWave + go]);
             const uint32_t cc = (w >> (8 * (k & 3))) & 0xffu;
-            const int mrow = ((TR ? (int)tk[i - 1] * na : 0) + (int)ap[i - 1]) * na + (int)bp[j - 1];
+            const int ax = i - 1 - alo;   // row i's staged entries
+            const int mrow = ((TR ? (int)stk[ax] * na : 0) + (int)sa[ax]) * na + (int)sb[jr];
+            // a gap move of row i: f 0 go_a, 1 ge_a (its gap set), 2 go_b, 3 ge_b (its set)
+            auto gapw = [&](int f) {
+                if constexpr (TR) {
+                    const int sset = f < 2 ? (int)sgt[ax] : (int)stk[ax];
+                    if (tid == 0)
+                        __hip_atomic_fetch_add(&gcnt[4 * sset + f], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else {
+                    gap(f, i, 1);
+                }
+            };
             if constexpr (GM == kAffine) {
                 if (st == 0) {
                     if (AM == kLocal && (cc & 3u) == 3u) break;
@@ -391,15 +504,15 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
                 } else if (st == 1) {
                     const uint32_t kk = (cc >> 2) & 3u;
                     const int pv = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
-                    gap(3, i, 1);
-                    if (pv != 1) gap(2, i, 1);
+                    gapw(3);
+                    if (pv != 1) gapw(2);
                     st = pv;
                     --i;
                 } else {
                     const uint32_t kk = cc >> 4;
                     const int pv = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
-                    gap(1, i, 1);
-                    if (pv != 2) gap(0, i, 1);
+                    gapw(1);
+                    if (pv != 2) gapw(0);
                     st = pv;
                     --j;
                 }
@@ -409,8 +522,8 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
                     if (tid == 0)
                         __hip_atomic_fetch_add(&hist[mrow], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     --i; --j;
-                } else if (cc == 1u) { gap(3, i, 1); --i; }
-                else { gap(1, i, 1); --j; }
+                } else if (cc == 1u) { gapw(3); --i; }
+                else { gapw(1); --j; }
             }
         }
         __syncthreads();

@@ -1212,8 +1212,7 @@ int launch_long(nwg_batch* b) {
 #undef NWG_PICK
     if (!fill) return fail(ctx, NWG_EINVAL, "nwgrad_hip: unsupported problem type");
     const size_t lds_fill = (size_t)nwg::hard_lds_sub_bytes<T>(b->na, b->K, tr);
-    const size_t lds_walk = hard ? (size_t)nwg::long_walk_count_bytes(b->na, b->K, tr) +
-                                       (size_t)nwg::kWalkSteps * (kS / 4) * kMaxG * 4
+    const size_t lds_walk = hard ? (size_t)nwg::long_walk_lds_bytes(b->na, b->K, tr, kS)
                                  : 16;
     if (lds_fill > 65536 || lds_walk > 65536)
         return fail(ctx, NWG_EINVAL, "nwgrad_amd: %d parameter sets of %d x %d do not fit the long-pair kernels' "
