@@ -22,7 +22,7 @@
 // Walk.  One block per pair combines the panels' end-cell candidates (larger value, ties to
 // the smaller (i, j): the reference's row-major first rule), then walks the codes back.
 // The walk only ever moves to the same, the previous or the one-before-previous step of a
-// panel's table, so it reads the table through an LDS window of kWalkSteps steps that all
+// panel's table, so it reads the table through an LDS window of 48 KiB of steps that all
 // threads of the block refill together.  Within a panel the step only decreases, so the next
 // window is known: it is loaded into registers while the walk runs through the current one,
 // and a refill is a store to LDS.  The window also stages the codes of B in the panel's
@@ -34,9 +34,11 @@
 
 namespace nwg {
 
-constexpr int kWalkSteps = 48;     // table steps held in the walk's LDS window (48 KB)
+constexpr int kWalkWindowBytes = 48 << 10;   // the walk's LDS window of table steps (S * 64 bytes each)
 constexpr int kWalkThreads = 256;  // threads of a walk block (they refill the window together)
-constexpr int kWalkRows = 128;     // rows of A staged per window: at least kWalkSteps + kWave - 1
+constexpr int kWalkRows = 160;     // rows of A staged per window: at least its steps + kWave - 1
+template <int S>
+__host__ __device__ constexpr int walk_steps() { return kWalkWindowBytes / (S * kWave); }
 
 struct PanelItem {
     int32_t pair;
@@ -270,12 +272,12 @@ __host__ __device__ constexpr int long_walk_count_bytes(int na, int K, bool tr) 
     return ((K * na * na + (tr ? 4 * K : 0)) * 4 + 15) & ~15;
 }
 __host__ __device__ constexpr int long_walk_lds_bytes(int na, int K, bool tr, int S) {
-    return long_walk_count_bytes(na, K, tr) + kWalkSteps * (S / 4) * kWave * 4 + kWave * S + kWalkRows +
+    return long_walk_count_bytes(na, K, tr) + kWalkWindowBytes + kWave * S + kWalkRows +
            (tr ? 2 * kWalkRows * 2 : 0);
 }
 
-// The walk's prefetched window: a thread's share of kWalkSteps table steps, 12 vectors of 4
-// dwords (16-column panels: 48 steps x 64 vectors / 256 threads).  Separate fields of a plain
+// The walk's prefetched window: a thread's share of the window's table steps, 12 vectors of 4
+// dwords (48 KiB / 16 B / 256 threads, whatever the panel width).  Separate fields of a plain
 // vector type, reached through compile-time indices, so the compiler keeps them in registers
 // while they are carried across walk steps (an array of uint4 there stayed in scratch).
 typedef uint32_t walk_u4 __attribute__((ext_vector_type(4)));   // a plain vector: promoted to registers
@@ -414,12 +416,13 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
             if (tid == 0) __hip_atomic_fetch_or(A.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             i = 0; j = 0;
         }
-        uint8_t* sb = reinterpret_cast<uint8_t*>(win + kWalkSteps * SD);   // B codes of the window's panel
+        constexpr int WS = walk_steps<S>();   // table steps of the window
+        uint8_t* sb = reinterpret_cast<uint8_t*>(win + WS * SD);   // B codes of the window's panel
         uint8_t* sa = sb + PW;                                               // A codes of rows alo ..
         uint16_t* stk = reinterpret_cast<uint16_t*>(sa + kWalkRows);          // tracked: their sets
         uint16_t* sgt = stk + kWalkRows;                                     // and gap sets (of row x + 1)
-        static_assert(12 * kWalkThreads == kWalkSteps * (SD / 4), "the window is 12 vectors a thread");
-        static_assert(kWalkRows >= kWalkSteps + kWave - 1 && kWalkRows <= kWalkThreads, "staged rows");
+        static_assert(12 * kWalkThreads == WS * (SD / 4), "the window is 12 vectors a thread");
+        static_assert(kWalkRows >= WS + kWave - 1 && kWalkRows <= kWalkThreads, "staged rows");
         // Window [lo, hi] of panel P into registers: the table steps, and per thread tid <
         // kWalkRows the staged row alo + tid (alo = max(0, lo - 63): a step t of the window is
         // row t - go + 1 for a lane go in 0 .. 63).
@@ -428,7 +431,7 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
         int wpanel = -1;
         int wlo = 1, whi = 0;   // the window holds steps [wlo, whi] of panel wpanel
         int alo = 0;            // and rows alo .. of A
-        bool ahead = false;     // registers hold steps [wlo - kWalkSteps, wlo - 1] (clamped at 0)
+        bool ahead = false;     // registers hold steps [wlo - WS, wlo - 1] (clamped at 0)
         for (;;) {
             if (i <= 0 && j <= 0) break;
             if (i == 0 || j == 0) {   // borders: runs (Global / Glocal column 0) or a stop
@@ -453,13 +456,13 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
             const int t = i - 1 + go;
             if (P != wpanel || t < wlo || t > whi) {
                 __syncthreads();   // every read of the old window is done
-                const int nlo = wlo - kWalkSteps > 0 ? wlo - kWalkSteps : 0;
+                const int nlo = wlo - WS > 0 ? wlo - WS : 0;
                 if (P == wpanel && ahead && t >= nlo && t < wlo) {
                     whi = wlo - 1;   // the prefetched window: t is at most 2 steps below wlo
                     wlo = nlo;
                 } else {
                     whi = t;
-                    wlo = t - (kWalkSteps - 1) > 0 ? t - (kWalkSteps - 1) : 0;
+                    wlo = t - (WS - 1) > 0 ? t - (WS - 1) : 0;
                     walk_fetch<TR, SD>(buf, ra, rtk, rgt, A.table + A.items[lp.item + P].table_off, wlo, whi, tid, m,
                                        ap, tk, A.tr.gtrk + r0);
                     if (P != wpanel) {
@@ -474,7 +477,7 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
                 ahead = wlo > 0;
                 if (ahead)
                     walk_fetch<TR, SD>(buf, ra, rtk, rgt, A.table + A.items[lp.item + P].table_off,
-                                       wlo - kWalkSteps > 0 ? wlo - kWalkSteps : 0, wlo - 1, tid, m, ap, tk,
+                                       wlo - WS > 0 ? wlo - WS : 0, wlo - 1, tid, m, ap, tk,
                                        A.tr.gtrk + r0);
             }
             // every thread reads the same word: a wave-uniform value keeps the walk's control

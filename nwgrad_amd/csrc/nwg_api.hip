@@ -38,6 +38,7 @@ struct nwg_ctx {
 namespace {
 
 constexpr int kS = 16;                 // columns per lane: hard and score-only fills
+constexpr int kLS = 8;                 // columns per lane: long pairs (column panels of kLS * 64)
 constexpr int kSS = 8;                 // columns per lane: soft pass (float32 and double)
 constexpr int kMaxG = 64;              // lanes per pair at most: one wavefront
 constexpr int kMaxN = kS * kMaxG;      // longest B a lane group holds (hard / score-only); wider: column panels
@@ -392,8 +393,8 @@ int build_plan(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& pl, int S,
 int build_long_plan(nwg_batch* b, std::vector<int32_t> pairs, LongPlan& L) {
     nwg_ctx* ctx = b->ctx;
     const bool hard = b->prob.grad_mode == 1;
-    constexpr int PW = kS * kMaxG;
-    constexpr int64_t SD = (kS / 4) * kMaxG;   // table dwords per step
+    constexpr int PW = kLS * kMaxG;
+    constexpr int64_t SD = (kLS / 4) * kMaxG;   // table dwords per step
     L.items.clear();
     L.lpairs.clear();
     L.groups.clear();
@@ -1194,11 +1195,11 @@ int launch_long(nwg_batch* b) {
         constexpr int G = decltype(gmc)::value, AM = decltype(amc)::value;
         constexpr bool H = decltype(hardc)::value;
         if (tr) {
-            fill = nwg::long_fill_kernel<T, G, AM, kS, H, kRowBlock, true>;
-            walk = nwg::long_walk_kernel<T, G, AM, kS, H, true>;
+            fill = nwg::long_fill_kernel<T, G, AM, kLS, H, kRowBlock, true>;
+            walk = nwg::long_walk_kernel<T, G, AM, kLS, H, true>;
         } else {
-            fill = nwg::long_fill_kernel<T, G, AM, kS, H, kRowBlock, false>;
-            walk = nwg::long_walk_kernel<T, G, AM, kS, H, false>;
+            fill = nwg::long_fill_kernel<T, G, AM, kLS, H, kRowBlock, false>;
+            walk = nwg::long_walk_kernel<T, G, AM, kLS, H, false>;
         }
     };
     using I0 = std::integral_constant<int, 0>;
@@ -1212,7 +1213,7 @@ int launch_long(nwg_batch* b) {
 #undef NWG_PICK
     if (!fill) return fail(ctx, NWG_EINVAL, "nwgrad_hip: unsupported problem type");
     const size_t lds_fill = (size_t)nwg::hard_lds_sub_bytes<T>(b->na, b->K, tr);
-    const size_t lds_walk = hard ? (size_t)nwg::long_walk_lds_bytes(b->na, b->K, tr, kS)
+    const size_t lds_walk = hard ? (size_t)nwg::long_walk_lds_bytes(b->na, b->K, tr, kLS)
                                  : 16;
     if (lds_fill > 65536 || lds_walk > 65536)
         return fail(ctx, NWG_EINVAL, "nwgrad_amd: %d parameter sets of %d x %d do not fit the long-pair kernels' "

@@ -56,7 +56,7 @@ def test_long_pairs_modes(gpu, gm, mode):
 
 
 def test_long_many_panels(gpu):
-    """Twenty-panel pairs (len(B) 20,000) with short and empty len(A), and one len(A) = 6,000:
+    """Forty-panel pairs (len(B) 20,000, 512-column panels) with short and empty len(A), and one len(A) = 6,000:
     the panel pipeline over many waves, rows from 0 to 6,000."""
     r = wl.rng("long_many_panels", "seqs")
     A = [r.integers(0, 20, size=k).astype(np.uint8) for k in (0, 1, 2, 63, 64, 65, 300, 6000)]

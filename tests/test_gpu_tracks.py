@@ -72,7 +72,7 @@ def test_tracked_golden_bit_exact(gpu, path, fill):
 @pytest.mark.parametrize("gm", ["affine", "linear"])
 @pytest.mark.parametrize("mode", ["global", "local", "glocal"])
 def test_tracked_long_pairs(gpu, gm, mode):
-    """Tracked pairs wider than 1,024 columns (two to four column panels) next to short ones,
+    """Tracked pairs wider than 1,024 columns (three to seven 512-column panels) next to short ones,
     segment tracks over K = 4 sets, float32 and float64, against the oracle."""
     r = wl.rng(f"trk_long_{gm}_{mode}", "seqs")
     lens_b = [1100, 2049, 3500, 40, 700, 1025]
