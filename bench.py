@@ -260,8 +260,9 @@ def main():
                     help="only time a 1 GiB device-to-device copy (the known-size reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(pairs shard across GPUs with one gradient all-reduce per step; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: one GPU only; --gpus {args.gpus} is not supported (across GPUs the pairs "
+              "would shard by cells, each GPU running its share, with one all-reduce of the gradient "
+              "per step)", file=sys.stderr)
         return 2
     import torch
     sync = torch.cuda.synchronize
