@@ -222,3 +222,36 @@ def test_soft_equal_sets_match_untracked(gpu):
     es, eg = _rel(b1.scores(), summed, b0.scores(), grads_rows(b0))
     say(f"  score {es:.2e}, grad {eg:.2e}")
     assert es <= 1e-5 and eg <= 1e-5
+
+
+@pytest.mark.parametrize("gm", ["affine", "linear"])
+def test_tracked_table_budget(gpu, gm):
+    """Tracked short pairs under a pointer-code budget that holds only a few dozen slots: the
+    default fill runs persistent, each wave filling many tiles in turn, so every tile's flush
+    must leave its per-set histograms and gap counters at zero for the next.  Matrix tracks
+    and gap tracks over K = 3 sets, against the oracle, and unchanged from the plain launch."""
+    r = wl.rng(f"trk_budget_{gm}", "seqs")
+    A = [r.integers(0, 20, size=int(r.integers(1, 120))).astype(np.uint8) for _ in range(3000)]
+    B = [r.integers(0, 20, size=int(r.integers(1, 120))).astype(np.uint8) for _ in range(3000)]
+    ca, oa = wl._pack(A)
+    cb, ob = wl._pack(B)
+    K = 3
+    mats = np.stack([wl.random_matrix(r) for _ in range(K)])
+    gaps = np.array([[10.5, 0.75, 9.25, 1.5], [8.0, 1.25, 11.0, 0.5], [12.0, 2.0, 6.5, 1.5]])
+    if gm == "linear":
+        gaps = gaps * np.array([0.0, 1.0, 0.0, 1.0])
+    tracks = r.integers(0, K, size=int(oa[-1])).astype(np.uint16)
+    gtracks = r.integers(0, K, size=int(oa[-1]) + len(A)).astype(np.uint16)
+    s_ref, g_ref = oracle.hard_tracked(mats, gaps, ca, oa, cb, ob, tracks, gtracks, gap_model=gm)
+    b = _batch(wl.AA, mats, gaps, ca, oa, cb, ob, tracks, gtracks, gap_model=gm)
+    b.score_and_grad()
+    assert b.hard_launches == (1, 0, 0)
+    np.testing.assert_array_equal(b.scores(), s_ref)
+    np.testing.assert_array_equal(_rows(b), g_ref)
+    b.set_table_budget(1 << 20)
+    b.score_and_grad()
+    launches, persistent, grid = b.hard_launches
+    say("case tracked budget", gm, b.tiles, "tiles, persistent grid", grid)
+    assert (launches, persistent) == (1, 1) and 1 <= grid < b.tiles // 4
+    np.testing.assert_array_equal(b.scores(), s_ref)
+    np.testing.assert_array_equal(_rows(b), g_ref)
