@@ -202,3 +202,52 @@ def test_table_budget(gpu, fill):
     b.score_and_grad()
     assert b.hard_launches[2] == 0
     np.testing.assert_array_equal(grads_rows(b), g_ref)
+
+
+class _DevF64:
+    """A device buffer of n doubles at `ptr`, for torch.as_tensor (no copy)."""
+
+    def __init__(self, ptr, n):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": "<f8", "data": (ptr, False), "version": 3}
+
+
+@pytest.mark.parametrize("budget", [0, 1 << 20])
+def test_hard_pass_captured(gpu, budget):
+    """The hard pass captured in a graph: short tiles in one plain launch (budget 0) or in a
+    persistent launch whose queue counter a kernel clears (1 MiB budget), and column-panel pairs
+    whose progress words and queue counter kernels clear.  Before every replay the scores are
+    overwritten on the device, with eager work and synchronisation around it; every replay
+    gives the eager (and the oracle's) scores and counts again."""
+    import torch
+    r = wl.rng("hard_captured", "seqs")
+    A = [r.integers(0, 20, size=int(r.integers(0, 200))).astype(np.uint8) for _ in range(2000)]
+    B = [r.integers(0, 20, size=int(r.integers(0, 300))).astype(np.uint8) for _ in range(2000)]
+    A += [r.integers(0, 20, size=int(r.integers(100, 400))).astype(np.uint8) for _ in range(6)]
+    B += [r.integers(0, 20, size=int(r.integers(1500, 2600))).astype(np.uint8) for _ in range(6)]
+    ca, oa = wl._pack(A)
+    cb, ob = wl._pack(B)
+    mat = wl.random_matrix(wl.rng("hard_captured", "matrix"))
+    gaps = np.array([11.0, 1.0, 11.0, 1.0])
+    s_ref, g_ref = oracle.hard(mat, gaps, ca, oa, cb, ob)
+    b = batch(wl.AA, mat, gaps, ca, oa, cb, ob)
+    b.set_table_budget(budget)
+    stream = torch.cuda.Stream()
+    b._set_stream(stream.cuda_stream)
+    b.score_and_grad()   # sizes every device buffer outside the capture
+    assert b.hard_launches[1] == (1 if budget else 0)
+    np.testing.assert_array_equal(b.scores(), s_ref)
+    np.testing.assert_array_equal(grads_rows(b), g_ref)
+    say("case hard captured, budget", budget, "persistent grid", b.hard_launches[2])
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=stream):
+        b._run_async()
+    scores_dev = torch.as_tensor(_DevF64(b._scores_dev_ptr(), len(b)), device="cuda")
+    for k in range(3):
+        scores_dev.fill_(-7.0)
+        _ = (torch.ones(8, device="cuda") * 2).sum().item()   # eager work between replays
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(b.scores(), s_ref)
+        np.testing.assert_array_equal(grads_rows(b), g_ref)
