@@ -38,6 +38,7 @@ struct nwg_ctx {
 namespace {
 
 constexpr int kS = 16;                 // columns per lane: hard and score-only fills
+constexpr int kSN = 8;                 // ... narrow: batches with too few tiles at kS to fill the SIMDs
 constexpr int kLS = 8;                 // columns per lane: long pairs (column panels of kLS * 64)
 constexpr int kSS = 8;                 // columns per lane: soft pass (float32 and double)
 constexpr int kMaxG = 64;              // lanes per pair at most: one wavefront
@@ -462,6 +463,17 @@ int build_plans(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& main, Lon
     for (int32_t i : pairs) (b->nlen[(size_t)i] > kMaxN ? lp : sp).push_back(i);
     int rc = build_plan(b, sp, main, kS, true);
     if (rc) return rc;
+    // Fewer tiles than two per SIMD leave the fill latency-bound (one wave a SIMD has nothing
+    // to hide a step's latency behind): cut the tiles at kSN columns a lane, about twice as
+    // many waves, when every pair fits one wave at that width.  protein_global_hard (1,182
+    // tiles at 16 columns): 266 -> 328 GCUPS; dna_local_short (65,536 tiles) keeps 16 columns,
+    // where 8 measured 18 % slower.
+    int64_t widest = 0;
+    for (int32_t i : sp) widest = std::max<int64_t>(widest, b->nlen[(size_t)i]);
+    if (!main.tiles.empty() && (int64_t)main.tiles.size() < 2 * 4 * (int64_t)b->ctx->n_cu && widest <= kSN * kMaxG) {
+        rc = build_plan(b, sp, main, kSN, true);
+        if (rc) return rc;
+    }
     return build_long_plan(b, lp, wide);
 }
 
@@ -1101,8 +1113,9 @@ int launch_hard(nwg_batch* b, const Plan& pl) {
                     b->K, b->na, b->na);
     const int gm = b->prob.gap_model, am = b->prob.mode;
     auto pick = [&](auto gmc, auto amc, auto hardc, auto trc) -> void (*)(nwg::HardArgs<T>) {
-        return nwg::hard_fill_kernel<T, decltype(gmc)::value, decltype(amc)::value, kS, decltype(hardc)::value,
-                                     decltype(trc)::value>;
+        constexpr int G = decltype(gmc)::value, AM = decltype(amc)::value;
+        constexpr bool H = decltype(hardc)::value, TRC = decltype(trc)::value;
+        return pl.S == kSN ? nwg::hard_fill_kernel<T, G, AM, kSN, H, TRC> : nwg::hard_fill_kernel<T, G, AM, kS, H, TRC>;
     };
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
@@ -1120,7 +1133,7 @@ int launch_hard(nwg_batch* b, const Plan& pl) {
     // fill 0: one plain launch over the persistent order when a slot per tile fits the table
     // budget, else one persistent launch with as many slots (waves) as fit it, at most the
     // resident blocks; fill 1: the per-bucket launches
-    const int64_t slot_dwords = pl.max_steps * (kS / 4) * 64;
+    const int64_t slot_dwords = pl.max_steps * (pl.S / 4) * 64;
     const bool persistent = b->fill == 0 && hard_persistent(b, pl);
     int64_t grid = 0;
     if (persistent)
