@@ -144,6 +144,11 @@ int nwg_batch_grads(nwg_batch *b, double *mats, double *gaps);
 int nwg_batch_weighted_grad(nwg_batch *b, const double *w, int64_t n, double *out);
 /* The same on device buffers, asynchronous on the context's stream (for autograd). */
 int nwg_batch_weighted_grad_dev(nwg_batch *b, const double *w_dev, double *out_dev);
+/* Pair i's cached score and (grad non-NULL) gradient row, K * (n_alpha^2 + 4) doubles: K
+ * matrices, then K rows of go_a, ge_a, go_b, ge_b.  NWG_EINDEX outside [0, N).  Replaces the
+ * per-pair accessors behind SeqPairBatch.__getitem__ views (py_exports.cpp:895-905;
+ * seq_pair.hpp score() / grad()). */
+int nwg_batch_pair_result(nwg_batch *b, int64_t i, double *score, double *grad);
 /* Device pointer to the per-pair scores (N doubles), valid until the batch is freed. */
 int nwg_batch_scores_dev(nwg_batch *b, const double **out_dev);
 /* Device-to-device copy of the per-pair scores into dst_dev (N doubles), on the context's
@@ -159,6 +164,14 @@ int nwg_batch_last_kernel_ms(nwg_batch *b, double *ms);
  * 2^-20) and that were rerun in double -- the reference's kF32Refused fallback
  * (aligner.hpp:6554-6572, batch_engine.hpp:1285-1306). */
 int nwg_batch_soft_reruns(nwg_batch *b, int64_t *n);
+/* The float32 refusals of a soft batch's passes, as its asynchronous (or graph-captured)
+ * passes leave them: pair-passes refused, the first pass (1-based) that refused any (0: none)
+ * and passes run.  Until nwg_batch_soft_finish reruns them, a refused pair's gradient row is
+ * left out of nwg_batch_weighted_grad_dev, so a training step never applies it; this record
+ * says whether that happened.  reset: clear the refusal counts (the pass counter keeps
+ * counting).  No counterpart in the reference, whose calls rerun refusals before returning
+ * (aligner.hpp:6554-6586; batch_engine.hpp:1285-1306 kF32Refused). */
+int nwg_batch_soft_refusals(nwg_batch *b, int64_t *pair_passes, int64_t *first_pass, int64_t *passes, int reset);
 /* Shape of the last soft launch (diagnostics; no counterpart in the reference, whose worker
  * pool has no grid): blocks of the persistent launch (0 for plain launches), blocks that
  * drain the wide-tile queue first, and resident blocks per CU from the occupancy query. */

@@ -14,6 +14,7 @@
 
 #include <array>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -29,67 +30,92 @@ namespace nwgrad_amd {
 
 inline constexpr std::string_view kAAOrder = "ACDEFGHIKLMNPQRSTVWY";
 
-// Interned, immutable symbol set: two alphabets are equal iff they are the same object.
+// An ordered set of ASCII symbols; symbol k encodes as the byte k.  Alphabets are interned --
+// get() returns one object per distinct symbol string for the life of the process -- so two
+// alphabets are equal exactly when they are the same object, and params, matrices and batches
+// compare alphabets by address.
 class Alphabet {
 public:
     static const Alphabet& get(std::string_view symbols) {
-        static std::mutex mu;
-        static auto* registry = new std::map<std::string, const Alphabet*, std::less<>>();
-        std::lock_guard<std::mutex> lock(mu);
-        auto it = registry->find(symbols);
-        if (it != registry->end()) return *it->second;
-        const Alphabet* a = new Alphabet(symbols);
-        registry->emplace(std::string(symbols), a);
-        return *a;
+        static Interned table;
+        return table.find_or_add(symbols);
     }
-    int size() const noexcept { return n_; }
+    int size() const noexcept { return static_cast<int>(symbols_.size()); }
     const std::string& symbols() const noexcept { return symbols_; }
-    int index_of(char c) const noexcept { return lut_[static_cast<unsigned char>(c)]; }
-    // Validate and encode s into out (s.size() bytes); throws naming the character.
+    // the symbol's code, or -1 when c is not a symbol
+    int index_of(char c) const noexcept {
+        const uint8_t v = code_[static_cast<unsigned char>(c)];
+        return v == kNotASymbol ? -1 : static_cast<int>(v);
+    }
+    // Codes of s into out (s.size() bytes).  The whole string is translated first; a string
+    // with a character outside the alphabet then raises, naming the first such character.
     void encode_into(std::string_view s, uint8_t* out) const {
+        uint8_t miss = 0;
         for (size_t k = 0; k < s.size(); ++k) {
-            const int i = index_of(s[k]);
-            if (i < 0) throw_bad_char(s[k], k);
-            out[k] = static_cast<uint8_t>(i);
+            out[k] = code_[static_cast<unsigned char>(s[k])];
+            miss |= static_cast<uint8_t>(out[k] == kNotASymbol);
         }
+        if (!miss) return;
+        size_t at = 0;
+        while (out[at] != kNotASymbol) ++at;
+        char msg[96];
+        const auto u = static_cast<unsigned char>(s[at]);
+        std::snprintf(msg, sizeof msg, "nwgrad: character '%c' (0x%02x) at position %zu", s[at], u, at);
+        throw std::invalid_argument(std::string(msg) + " is not in alphabet \"" + symbols_ + "\"");
     }
     Alphabet(const Alphabet&) = delete;
     Alphabet& operator=(const Alphabet&) = delete;
 
 private:
-    explicit Alphabet(std::string_view symbols) : symbols_(symbols), n_(static_cast<int>(symbols.size())) {
-        if (symbols.empty()) throw std::invalid_argument("nwgrad: alphabet must be non-empty");
-        if (symbols.size() > 127) throw std::invalid_argument("nwgrad: alphabet may have at most 127 symbols");
-        for (int i = 0; i < 256; ++i) lut_[i] = -1;
-        for (int i = 0; i < n_; ++i) {
-            const char ch = symbols[static_cast<size_t>(i)];
-            const auto c = static_cast<unsigned char>(ch);
-            if (c >= 128)
-                throw std::invalid_argument("nwgrad: alphabet symbols must be ASCII; got a byte >= 0x80 "
-                                            "(a non-ASCII character encoded as UTF-8?)");
-            if (ch == '-')
-                throw std::invalid_argument("nwgrad: '-' cannot be an alphabet symbol; it is reserved as "
-                                            "the gap marker in aligned sequences");
-            if (lut_[c] >= 0)
-                throw std::invalid_argument(std::string("nwgrad: duplicate symbol '") + ch + "' in alphabet \"" +
-                                            symbols_ + "\"");
-            lut_[c] = static_cast<int16_t>(i);
+    static constexpr uint8_t kNotASymbol = 0xff;
+
+    // The process's alphabets, each built once; lookups under a mutex (alphabets are few and
+    // looked up once per params or batch, never per residue).
+    class Interned {
+    public:
+        const Alphabet& find_or_add(std::string_view symbols) {
+            std::lock_guard<std::mutex> hold(mu_);
+            for (const auto& a : made_)
+                if (a->symbols_ == symbols) return *a;
+            made_.emplace_back(new Alphabet(symbols));
+            return *made_.back();
         }
+
+    private:
+        std::mutex mu_;
+        std::vector<std::unique_ptr<const Alphabet>> made_;
+    };
+
+    // The reason `symbols` cannot be an alphabet, or an empty string when it can.  One pass in
+    // symbol order; the first offending symbol decides the message.
+    static std::string rejection(std::string_view symbols) {
+        if (symbols.empty()) return "nwgrad: alphabet must be non-empty";
+        if (symbols.size() > 127) return "nwgrad: alphabet may have at most 127 symbols";
+        std::array<bool, 128> seen{};
+        for (char ch : symbols) {
+            const auto u = static_cast<unsigned char>(ch);
+            if (u > 127)
+                return "nwgrad: alphabet symbols must be ASCII; got a byte >= 0x80 (a non-ASCII character encoded "
+                       "as UTF-8?)";
+            if (ch == '-')
+                return "nwgrad: '-' cannot be an alphabet symbol; it is reserved as the gap marker in aligned "
+                       "sequences";
+            if (seen[u]) return "nwgrad: duplicate symbol '" + std::string(1, ch) + "' in alphabet \"" +
+                                std::string(symbols) + "\"";
+            seen[u] = true;
+        }
+        return {};
     }
-    [[noreturn]] void throw_bad_char(char c, size_t pos) const {
-        static const char* hex = "0123456789abcdef";
-        const auto u = static_cast<unsigned char>(c);
-        std::string msg = "nwgrad: character '";
-        msg += c;
-        msg += "' (0x";
-        msg += hex[u >> 4];
-        msg += hex[u & 0xf];
-        msg += ") at position " + std::to_string(pos) + " is not in alphabet \"" + symbols_ + "\"";
-        throw std::invalid_argument(msg);
+
+    explicit Alphabet(std::string_view symbols) : symbols_(symbols) {
+        const std::string why = rejection(symbols);
+        if (!why.empty()) throw std::invalid_argument(why);
+        code_.fill(kNotASymbol);
+        for (size_t k = 0; k < symbols_.size(); ++k) code_[static_cast<unsigned char>(symbols_[k])] = static_cast<uint8_t>(k);
     }
+
     std::string symbols_;
-    int n_;
-    int16_t lut_[256];
+    std::array<uint8_t, 256> code_{};
 };
 
 // An N x N substitution matrix over an interned alphabet, row-major (subst_matrix.hpp:29-33).
@@ -405,6 +431,20 @@ public:
     // be null (the reference's defaults).  The batch is then tracked: grads() has num_sets() sets.
     void set_tracks(const uint16_t* tracks, const uint16_t* gap_tracks) {
         check_(nwg_batch_set_tracks(b_, tracks, gap_tracks));
+        const int64_t la = oa_.empty() ? 0 : oa_.back();
+        if (tracks) trk_.assign(tracks, tracks + la);
+        else trk_.assign(static_cast<size_t>(la), 0);
+        gtrk_.resize(static_cast<size_t>(la + N_));
+        for (int64_t p = 0; p < N_; ++p) {   // the reference's default gap track where none is given
+            const int64_t a0 = oa_[static_cast<size_t>(p)], m = oa_[static_cast<size_t>(p) + 1] - a0;
+            uint16_t* g = gtrk_.data() + a0 + p;
+            if (gap_tracks) {
+                std::copy(gap_tracks + a0 + p, gap_tracks + a0 + p + m + 1, g);
+            } else {
+                g[0] = m ? trk_[static_cast<size_t>(a0)] : 0;
+                for (int64_t i = 1; i <= m; ++i) g[i] = trk_[static_cast<size_t>(a0 + i - 1)];
+            }
+        }
         valid_ = false;
     }
     int num_sets() const {
@@ -416,6 +456,42 @@ public:
         int K = 1, tr = 0;
         check_(nwg_batch_num_sets(b_, &K, &tr));
         return tr != 0;
+    }
+
+    // Pair i (the reference's SeqPairT view, seq_pair.hpp): its sequences, its tracks, and its
+    // cached results.
+    std::string seq_a(size_t i) const { return decode_(ca_, oa_, i); }
+    std::string seq_b(size_t i) const { return decode_(cb_, ob_, i); }
+    int64_t len_a(size_t i) const { return oa_[i + 1] - oa_[i]; }
+    int64_t len_b(size_t i) const { return ob_[i + 1] - ob_[i]; }
+    const uint8_t* codes_a(size_t i) const { return ca_.data() + oa_[i]; }
+    const uint8_t* codes_b(size_t i) const { return cb_.data() + ob_[i]; }
+    std::vector<uint16_t> track(size_t i) const {
+        if (trk_.empty() && len_a(i) > 0) return std::vector<uint16_t>(static_cast<size_t>(len_a(i)), 0);
+        return std::vector<uint16_t>(trk_.begin() + oa_[i], trk_.begin() + oa_[i + 1]);
+    }
+    std::vector<uint16_t> gap_track(size_t i) const {
+        if (gtrk_.empty()) return std::vector<uint16_t>(static_cast<size_t>(len_a(i) + 1), 0);
+        const auto at = gtrk_.begin() + oa_[i] + static_cast<int64_t>(i);
+        return std::vector<uint16_t>(at, at + len_a(i) + 1);
+    }
+    bool results_valid() const noexcept { return valid_; }
+    double pair_score(size_t i) const {
+        double v = 0.0;
+        check_(nwg_batch_pair_result(b_, static_cast<int64_t>(i), &v, nullptr));
+        return v;
+    }
+    AlignParams pair_grad(size_t i) const {
+        const int nn = alpha_->size() * alpha_->size(), K = num_sets();
+        std::vector<double> row(static_cast<size_t>(K) * (nn + 4));
+        check_(nwg_batch_pair_result(b_, static_cast<int64_t>(i), nullptr, row.data()));
+        return AlignParams(SubstMatrix(row.data(), *alpha_, K), row.data() + static_cast<size_t>(K) * nn);
+    }
+    // Soft: the float32 refusals the batch's passes left (pair-passes, first pass, passes).
+    std::array<int64_t, 3> soft_refusals(bool reset) {
+        std::array<int64_t, 3> r{0, 0, 0};
+        check_(nwg_batch_soft_refusals(b_, &r[0], &r[1], &r[2], reset ? 1 : 0));
+        return r;
     }
 
     // Device-side plumbing for autograd and serving.
@@ -488,8 +564,20 @@ public:
     }
 
 private:
+    std::string decode_(const std::vector<uint8_t>& codes, const std::vector<int64_t>& off, size_t i) const {
+        if (i >= static_cast<size_t>(N_)) throw std::out_of_range("SeqPairBatch index out of range");
+        const std::string& sym = alpha_->symbols();
+        std::string out(static_cast<size_t>(off[i + 1] - off[i]), '\0');
+        for (size_t k = 0; k < out.size(); ++k) out[k] = sym[codes[static_cast<size_t>(off[i]) + k]];
+        return out;
+    }
     void create_(const uint8_t* ca, const int64_t* oa, const uint8_t* cb, const int64_t* ob, int64_t N, int device) {
         N_ = N;
+        // host copies: the views' sequences and tracks (the reference keeps its codes on the host too)
+        oa_.assign(oa, oa + N + 1);
+        ob_.assign(ob, ob + N + 1);
+        ca_.assign(ca, ca + oa_.back());
+        cb_.assign(cb, cb + ob_.back());
         ctx_ = Device::ctx(device);
         nwg_problem pr{};
         pr.precision = std::is_same<T, float>::value ? 0 : 1;
@@ -519,6 +607,9 @@ private:
     }
 
     const Alphabet* alpha_;
+    std::vector<uint8_t> ca_, cb_;
+    std::vector<int64_t> oa_, ob_;
+    std::vector<uint16_t> trk_, gtrk_;   // tracked: per residue of A / per DP row (pair p at oa_[p] + p)
     AlignParams params_;
     GapModel gm_;
     AlignMode am_;

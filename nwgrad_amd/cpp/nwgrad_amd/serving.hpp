@@ -17,6 +17,11 @@
 // and launches of different slots overlap on the device.  A pair's result does not depend on
 // the launch it lands in (hard and score-only: bit for bit; soft: the per-pair sums are
 // formed within the pair's tile, so up to float rounding of a different lane split).
+//
+// Params may hold K parameter sets (AlignParams::stack): the server is then tracked, every job
+// carries a set per residue of A (tracks) and per DP row (gap tracks; the reference's default
+// when not given, aligner.hpp:1956-1967), and a pair's gradient row is K matrices, then K gap
+// rows (the reference's grads_into layout, batch_engine.hpp:719-746).
 #pragma once
 
 #include <algorithm>
@@ -51,9 +56,10 @@ public:
         double bytes = 0.0;
         std::vector<uint8_t> ca, cb;
         std::vector<int64_t> oa, ob;
+        std::vector<uint16_t> trk, gtrk;   // tracked server: per residue of A, per DP row (pair p at oa[p] + p)
         int state = 0;   // 0 waiting, 1 running, 2 done, 3 failed
         double t_submit = 0.0, t_launch = 0.0, t_done = 0.0;
-        std::vector<double> scores, grads;   // grads: n rows of n_alpha^2 + 4 (hard and soft)
+        std::vector<double> scores, grads;   // grads: n rows of K * (n_alpha^2 + 4) (hard and soft)
         std::string error;
     };
 
@@ -64,6 +70,7 @@ public:
         if (!(opt_.hbm_budget_bytes > 0)) throw std::invalid_argument("nwgrad: hbm_budget_bytes must be > 0");
         if (!(opt_.soft_temperature > 0)) throw std::invalid_argument("nwgrad: soft_temperature must be > 0");
         na_ = static_cast<int>(params_.alphabet().size());
+        K_ = params_.num_sets();
         t0_ = std::chrono::steady_clock::now();
         for (int k = 0; k < 3; ++k)
             for (int s = 0; s < opt_.slots_per_kind; ++s) {
@@ -87,8 +94,13 @@ public:
     }
 
     // Queue a job of pre-encoded pairs (codes in [0, n_alpha), offsets of n + 1 entries from 0).
+    // A tracked server (K > 1 sets) takes `tracks`, one set per residue of A (oa[n] entries), and
+    // `gap_tracks`, one per DP row (oa[n] + n entries, pair p's at oa[p] + p); either may be null
+    // for the reference's defaults (every residue set 0; a gap track following the track).
     int64_t submit(JobKind kind, const uint8_t* ca, const int64_t* oa, const uint8_t* cb, const int64_t* ob,
-                   int64_t n) {
+                   int64_t n, const uint16_t* tracks = nullptr, const uint16_t* gap_tracks = nullptr) {
+        if ((tracks || gap_tracks) && K_ == 1)
+            throw std::invalid_argument("nwgrad: tracks need params with more than one parameter set");
         if (n < 0) throw std::invalid_argument("nwgrad: a job needs n >= 0 pairs");
         auto j = std::make_unique<Job>();
         j->kind = kind;
@@ -108,9 +120,29 @@ public:
             if (ca[k] >= na_) throw std::invalid_argument("nwgrad: job code out of range in A");
         for (int64_t k = 0; k < lb; ++k)
             if (cb[k] >= na_) throw std::invalid_argument("nwgrad: job code out of range in B");
-        const int gs = na_ * na_ + 4;
+        const int gs = K_ * (na_ * na_ + 4);
         j->bytes += static_cast<double>(la + lb) + 24.0 * static_cast<double>(n) +
                     (kind == JobKind::Score ? 0.0 : 8.0 * gs * static_cast<double>(n));
+        if (K_ > 1) {   // the job's tracks, defaults filled in
+            j->trk.assign(static_cast<size_t>(la), 0);
+            if (tracks) std::copy(tracks, tracks + la, j->trk.begin());
+            j->gtrk.assign(static_cast<size_t>(la + n), 0);
+            for (int64_t p = 0; p < n; ++p) {
+                const int64_t a0 = oa[p], m = oa[p + 1] - a0;
+                uint16_t* g = j->gtrk.data() + a0 + p;
+                if (gap_tracks) {
+                    std::copy(gap_tracks + a0 + p, gap_tracks + a0 + p + m + 1, g);
+                } else {
+                    g[0] = m ? j->trk[static_cast<size_t>(a0)] : 0;
+                    for (int64_t i = 1; i <= m; ++i) g[i] = j->trk[static_cast<size_t>(a0 + i - 1)];
+                }
+            }
+            for (uint16_t v : j->trk)
+                if (v >= K_) throw std::invalid_argument("nwgrad: a track names a set outside [0, K)");
+            for (uint16_t v : j->gtrk)
+                if (v >= K_) throw std::invalid_argument("nwgrad: a gap track names a set outside [0, K)");
+            j->bytes += 2.0 * static_cast<double>(2 * la + n) + 8.0 * static_cast<double>(la + n);
+        }
         j->ca.assign(ca, ca + la);
         j->cb.assign(cb, cb + lb);
         j->oa.assign(oa, oa + n + 1);
@@ -167,6 +199,7 @@ public:
     double bytes_in_flight() const { return in_flight_bytes_; }
     int64_t launches() const { return n_launches_; }
     int n_alpha() const { return na_; }
+    int num_sets() const { return K_; }
 
 private:
     struct Slot {
@@ -209,9 +242,11 @@ private:
         }
         if (take.empty()) return false;
         q.erase(q.begin(), q.begin() + static_cast<std::ptrdiff_t>(take.size()));
-        // concatenate the jobs' pairs
+        // concatenate the jobs' pairs (and tracks: concatenation keeps the per-row layout, pair P's
+        // gap track at oa[P] + P)
         std::vector<uint8_t> ca, cb;
         std::vector<int64_t> oa(1, 0), ob(1, 0);
+        std::vector<uint16_t> trk, gtrk;
         oa.reserve((size_t)pairs + 1);
         ob.reserve((size_t)pairs + 1);
         for (int64_t id : take) {
@@ -219,6 +254,8 @@ private:
             const int64_t ba = oa.back(), bb = ob.back();
             ca.insert(ca.end(), j.ca.begin(), j.ca.end());
             cb.insert(cb.end(), j.cb.begin(), j.cb.end());
+            trk.insert(trk.end(), j.trk.begin(), j.trk.end());
+            gtrk.insert(gtrk.end(), j.gtrk.begin(), j.gtrk.end());
             for (int64_t i = 1; i <= j.n; ++i) {
                 oa.push_back(ba + j.oa[(size_t)i]);
                 ob.push_back(bb + j.ob[(size_t)i]);
@@ -226,7 +263,7 @@ private:
         }
         const double t = now();
         try {
-            prepare_(*sl, ca, oa, cb, ob, pairs);
+            prepare_(*sl, ca, oa, cb, ob, pairs, trk, gtrk);
             check_(*sl, nwg_batch_run_async(sl->b));
         } catch (const std::exception& e) {
             for (int64_t id : take) {
@@ -251,7 +288,8 @@ private:
     }
 
     void prepare_(Slot& sl, const std::vector<uint8_t>& ca, const std::vector<int64_t>& oa,
-                  const std::vector<uint8_t>& cb, const std::vector<int64_t>& ob, int64_t n) {
+                  const std::vector<uint8_t>& cb, const std::vector<int64_t>& ob, int64_t n,
+                  const std::vector<uint16_t>& trk, const std::vector<uint16_t>& gtrk) {
         const uint8_t* pa = ca.empty() ? &kZero_ : ca.data();
         const uint8_t* pb = cb.empty() ? &kZero_ : cb.data();
         if (!sl.ctx) {
@@ -265,12 +303,15 @@ private:
             nwg_problem pr{f64_ ? 1 : 0, static_cast<int32_t>(gm_), static_cast<int32_t>(am_),
                            static_cast<int32_t>(sl.kind)};
             check_(sl, nwg_batch_create(sl.ctx, &pr, na_, n, pa, oa.data(), pb, ob.data(), &sl.b));
-            double gaps[4];
-            params_.gaps4(gaps);
-            check_(sl, nwg_batch_set_params(sl.b, params_.matrix.data(), gaps));
+            const std::vector<double> gaps = params_.all_gaps();
+            check_(sl, nwg_batch_set_params_k(sl.b, K_, params_.matrix.data(), gaps.data()));
             if (sl.kind == JobKind::Soft) check_(sl, nwg_batch_set_soft(sl.b, opt_.soft_temperature, f64_ ? 1 : 0));
         } else {
             check_(sl, nwg_batch_reload(sl.b, n, pa, oa.data(), pb, ob.data()));
+        }
+        if (K_ > 1) {
+            static const uint16_t kNone = 0;
+            check_(sl, nwg_batch_set_tracks(sl.b, trk.empty() ? &kNone : trk.data(), gtrk.empty() ? &kNone : gtrk.data()));
         }
     }
 
@@ -282,20 +323,20 @@ private:
         int64_t total = 0;
         for (int64_t id : sl.jobs) total += jobs_.at(id)->n;
         std::vector<double> scores((size_t)total), grads;
-        const int gs = na_ * na_ + 4;
+        const int nn = na_ * na_, knn = K_ * nn, gs = K_ * (nn + 4);
         std::string err;
         try {
             if (sl.kind == JobKind::Soft) check_(sl, nwg_batch_soft_finish(sl.b));
             if (total > 0) {
                 check_(sl, nwg_batch_scores(sl.b, scores.data()));
                 if (sl.kind != JobKind::Score) {
-                    std::vector<double> mats((size_t)total * na_ * na_), gaps((size_t)total * 4);
+                    std::vector<double> mats((size_t)total * knn), gaps((size_t)total * 4 * K_);
                     check_(sl, nwg_batch_grads(sl.b, mats.data(), gaps.data()));
                     grads.resize((size_t)total * gs);
                     for (int64_t i = 0; i < total; ++i) {
-                        std::copy(mats.begin() + i * na_ * na_, mats.begin() + (i + 1) * na_ * na_,
-                                  grads.begin() + i * gs);
-                        std::copy(gaps.begin() + i * 4, gaps.begin() + i * 4 + 4, grads.begin() + i * gs + na_ * na_);
+                        std::copy(mats.begin() + i * knn, mats.begin() + (i + 1) * knn, grads.begin() + i * gs);
+                        std::copy(gaps.begin() + i * 4 * K_, gaps.begin() + (i + 1) * 4 * K_,
+                                  grads.begin() + i * gs + knn);
                     }
                 }
             }
@@ -316,6 +357,8 @@ private:
             }
             std::vector<uint8_t>().swap(j.ca);
             std::vector<uint8_t>().swap(j.cb);
+            std::vector<uint16_t>().swap(j.trk);
+            std::vector<uint16_t>().swap(j.gtrk);
             at += j.n;
         }
         in_flight_bytes_ -= sl.bytes;
@@ -335,6 +378,7 @@ private:
     ServeOptions opt_;
     int device_;
     int na_ = 0;
+    int K_ = 1;
     std::chrono::steady_clock::time_point t0_;
     std::vector<std::unique_ptr<Slot>> slots_;
     std::deque<int64_t> waiting_[3];

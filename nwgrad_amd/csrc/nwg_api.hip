@@ -235,6 +235,7 @@ struct nwg_batch {
     DevPtr<uint8_t> soft_ok;
     std::vector<uint8_t> host_ok;
     int64_t soft_reruns = 0;
+    DevPtr<unsigned long long> srec;   // soft: refusal record of the main passes (reduce_kernels.hpp)
     // params
     std::vector<double> matrix;           // K matrices, n_alpha^2 each
     std::vector<double> gaps = std::vector<double>(4, 0.0);   // K x (go_a, ge_a, go_b, ge_b)
@@ -318,51 +319,56 @@ bool hard_persistent(const nwg_batch* b, const Plan& pl) {
     return b->prob.grad_mode == 1 && (int64_t)pl.tiles.size() * slot_bytes > b->table_budget;
 }
 
-int build_plan(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& pl, int S, bool pow2) {
-    nwg_ctx* ctx = b->ctx;
-    pl.S = S;
-    pl.tiles.clear();
-    pl.buckets.clear();
-    pl.max_steps = 1;
-    pl.panels = 1;
+// The tiles of `pairs` (lengths mlen / nlen by pair index): lane-group width G from len(B)
+// (S columns a lane, at least gmin lanes, a power of two with pow2), pairs sorted by G and then
+// longest len(A) first, cut into tiles of 64 / G pairs of one width.  Fills order, tiles and
+// buckets (the plain order), ptiles (the same tiles, most row steps first), max_steps and
+// panels.  Host work only: the batch planner and the no-allocation entry share it.
+void cut_tiles(const std::vector<int32_t>& mlen, const std::vector<int32_t>& nlen, const std::vector<int32_t>& pairs,
+               int S, int gmin, bool pow2, std::vector<int32_t>& order, std::vector<TileDesc>& tiles,
+               std::vector<Bucket>& buckets, std::vector<TileDesc>& ptiles, int64_t& max_steps, int& panels) {
+    tiles.clear();
+    buckets.clear();
+    max_steps = 1;
+    panels = 1;
     const int64_t N = (int64_t)pairs.size();
-    std::vector<int32_t> gsel((size_t)b->N > 0 ? (size_t)b->N : 1);
+    std::vector<int32_t> gsel(std::max<size_t>(mlen.size(), 1));
     for (int32_t i : pairs) {
-        gsel[(size_t)i] = group_width(b->nlen[(size_t)i], S, b->gmin, pow2);
-        pl.panels = std::max(pl.panels, (b->nlen[(size_t)i] + S * kMaxG - 1) / (S * kMaxG));
+        gsel[(size_t)i] = group_width(nlen[(size_t)i], S, gmin, pow2);
+        panels = std::max(panels, (nlen[(size_t)i] + S * kMaxG - 1) / (S * kMaxG));
     }
-    pl.order = pairs;
-    std::stable_sort(pl.order.begin(), pl.order.end(), [&](int32_t x, int32_t y) {
+    order = pairs;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
         if (gsel[(size_t)x] != gsel[(size_t)y]) return gsel[(size_t)x] < gsel[(size_t)y];
-        return b->mlen[(size_t)x] > b->mlen[(size_t)y];
+        return mlen[(size_t)x] > mlen[(size_t)y];
     });
     int64_t pos = 0;
     while (pos < N) {
-        const int G = gsel[(size_t)pl.order[(size_t)pos]];
-        Bucket bk{G, (int)pl.tiles.size(), 0};
-        while (pos < N && gsel[(size_t)pl.order[(size_t)pos]] == G) {
+        const int G = gsel[(size_t)order[(size_t)pos]];
+        Bucket bk{G, (int)tiles.size(), 0};
+        while (pos < N && gsel[(size_t)order[(size_t)pos]] == G) {
             const int per = 64 / G;
             TileDesc td{};
             td.first = (int32_t)pos;
             td.G = (int16_t)G;
             int np = 0, mm = 0, mn = 0;
-            while (np < per && pos < N && gsel[(size_t)pl.order[(size_t)pos]] == G) {
-                mm = std::max(mm, b->mlen[(size_t)pl.order[(size_t)pos]]);
-                mn = std::max(mn, b->nlen[(size_t)pl.order[(size_t)pos]]);
+            while (np < per && pos < N && gsel[(size_t)order[(size_t)pos]] == G) {
+                mm = std::max(mm, mlen[(size_t)order[(size_t)pos]]);
+                mn = std::max(mn, nlen[(size_t)order[(size_t)pos]]);
                 ++np;
                 ++pos;
             }
             td.npairs = (int16_t)np;
             td.max_m = mm;
             td.max_n = mn;
-            pl.max_steps = std::max<int64_t>(pl.max_steps, (int64_t)mm + G - 1);
-            pl.tiles.push_back(td);
+            max_steps = std::max<int64_t>(max_steps, (int64_t)mm + G - 1);
+            tiles.push_back(td);
         }
-        bk.tile_end = (int)pl.tiles.size();
-        pl.buckets.push_back(bk);
+        bk.tile_end = (int)tiles.size();
+        buckets.push_back(bk);
     }
     // persistent order: most row steps first (a wave's time is its steps times its panels)
-    std::vector<TileDesc> ptiles(pl.tiles);
+    ptiles = tiles;
     auto work = [S](const TileDesc& t) {
         const int pw = t.G * S;
         const int64_t np = t.max_n > pw ? (t.max_n + pw - 1) / pw : 1;
@@ -370,6 +376,15 @@ int build_plan(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& pl, int S,
     };
     std::stable_sort(ptiles.begin(), ptiles.end(),
                      [&](const TileDesc& a, const TileDesc& c) { return work(a) > work(c); });
+}
+
+int build_plan(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& pl, int S, bool pow2) {
+    nwg_ctx* ctx = b->ctx;
+    pl.S = S;
+    const int64_t N = (int64_t)pairs.size();
+    std::vector<TileDesc> ptiles;
+    cut_tiles(b->mlen, b->nlen, pairs, S, b->gmin, pow2, pl.order, pl.tiles, pl.buckets, ptiles, pl.max_steps,
+              pl.panels);
     split_launches(b, pl);
     NWG_HIP(ctx, pl.d_order.reserve(std::max<size_t>(pl.order.size(), 1)));
     NWG_HIP(ctx, pl.d_tiles.reserve(std::max<size_t>(pl.tiles.size(), 1)));
@@ -930,6 +945,10 @@ int nwg_batch_create(nwg_ctx* ctx, const nwg_problem* prob, int n_alpha, int64_t
     b->N = N;
     set_slots(b.get());
     NWG_HIP(ctx, b->queue.alloc(4));
+    if (prob->grad_mode == 2) {
+        NWG_HIP(ctx, b->srec.alloc(3));
+        NWG_HIP(ctx, hipMemset(b->srec.p, 0, 3 * sizeof(unsigned long long)));
+    }
     const int rc = load_pairs(b.get(), N, codes_a, off_a, codes_b, off_b, false);
     if (rc) return rc;
     NWG_HIP(ctx, hipEventCreate(&b->ev0));
@@ -978,30 +997,33 @@ int nwg_batch_set_params_k(nwg_batch* b, int K, const double* matrices, const do
             if (v >= K) return fail(ctx, NWG_EINVAL, "nwgrad: a gap track names set %d; params has %d sets", (int)v, K);
     }
     NWG_HIP(ctx, hipSetDevice(ctx->device));
+    // Work still queued on the context's stream (an asynchronous pass, a set_params_dev copy)
+    // reads these buffers: let it finish, then upload on the same stream and wait for the copies
+    // (their host sources are this call's own vectors).
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const int oldK = b->K;
     b->matrix.assign(matrices, matrices + (size_t)K * nn);
     b->gaps.assign(gaps, gaps + (size_t)4 * K);
     b->K = K;
     const size_t kn = (size_t)K * nn;
+    std::vector<float> sv;
     if (b->prob.precision == 0) {
-        std::vector<float> sv(kn);
+        sv.resize(kn);
         for (size_t k = 0; k < kn; ++k) sv[k] = static_cast<float>(matrices[k]);
         NWG_HIP(ctx, b->sub_f32.reserve(kn));
-        NWG_HIP(ctx, hipMemcpy(b->sub_f32.p, sv.data(), sizeof(float) * kn, hipMemcpyHostToDevice));
+        NWG_HIP(ctx, hipMemcpyAsync(b->sub_f32.p, sv.data(), sizeof(float) * kn, hipMemcpyHostToDevice, ctx->stream));
     } else {
         NWG_HIP(ctx, b->sub_f64.reserve(kn));
-        NWG_HIP(ctx, hipMemcpy(b->sub_f64.p, matrices, sizeof(double) * kn, hipMemcpyHostToDevice));
+        NWG_HIP(ctx, hipMemcpyAsync(b->sub_f64.p, matrices, sizeof(double) * kn, hipMemcpyHostToDevice, ctx->stream));
     }
-    {   // the soft pass's parameter vector: K matrices, then K rows of gaps
-        NWG_HIP(ctx, b->theta.reserve((size_t)K * (nn + 4)));
-        std::vector<double> th(b->matrix);
-        th.insert(th.end(), gaps, gaps + 4 * (size_t)K);
-        NWG_HIP(ctx, hipMemcpy(b->theta.p, th.data(), sizeof(double) * th.size(), hipMemcpyHostToDevice));
-    }
-    if (K != oldK && b->prob.grad_mode != 0) {
-        NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // the soft pass's parameter vector: K matrices, then K rows of gaps
+    NWG_HIP(ctx, b->theta.reserve((size_t)K * (nn + 4)));
+    std::vector<double> th(b->matrix);
+    th.insert(th.end(), gaps, gaps + 4 * (size_t)K);
+    NWG_HIP(ctx, hipMemcpyAsync(b->theta.p, th.data(), sizeof(double) * th.size(), hipMemcpyHostToDevice, ctx->stream));
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (K != oldK && b->prob.grad_mode != 0)
         NWG_HIP(ctx, b->grads.alloc((size_t)std::max<int64_t>(b->N, 1) * kn + (size_t)std::max<int64_t>(b->N, 1) * 4 * K));
-    }
     if (K > 1 && !b->tracked) {   // several sets make the problem tracked (default tracks)
         const int rc = set_tracks_host(b, nullptr, nullptr);
         if (rc) return rc;
@@ -1102,20 +1124,13 @@ nwg::TrackArgs<T> track_args(nwg_batch* b) {
     return t;
 }
 
+// The hard / score-only fill kernel of a problem type and lane width (kS or kSN columns).
 template <typename T>
-int launch_hard(nwg_batch* b, const Plan& pl) {
-    nwg_ctx* ctx = b->ctx;
-    const bool hard = b->prob.grad_mode == 1, tr = b->tracked;
-    const size_t lds = (size_t)nwg::hard_lds_sub_bytes<T>(b->na, b->K, tr) +
-                       (hard ? (size_t)nwg::hard_lds_hist_bytes(b->na, b->K, tr, b->hist_slots) : 0);
-    if (lds > 65536)
-        return fail(ctx, NWG_EINVAL, "nwgrad_amd: %d parameter sets of %d x %d do not fit the fill's local memory",
-                    b->K, b->na, b->na);
-    const int gm = b->prob.gap_model, am = b->prob.mode;
+void (*hard_kernel_for(int gm, int am, bool hard, bool tr, int S))(nwg::HardArgs<T>) {
     auto pick = [&](auto gmc, auto amc, auto hardc, auto trc) -> void (*)(nwg::HardArgs<T>) {
         constexpr int G = decltype(gmc)::value, AM = decltype(amc)::value;
         constexpr bool H = decltype(hardc)::value, TRC = decltype(trc)::value;
-        return pl.S == kSN ? nwg::hard_fill_kernel<T, G, AM, kSN, H, TRC> : nwg::hard_fill_kernel<T, G, AM, kS, H, TRC>;
+        return S == kSN ? nwg::hard_fill_kernel<T, G, AM, kSN, H, TRC> : nwg::hard_fill_kernel<T, G, AM, kS, H, TRC>;
     };
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
@@ -1129,6 +1144,19 @@ int launch_hard(nwg_batch* b, const Plan& pl) {
                     : (tr ? pick(GMV{}, AMV{}, BF{}, BT{}) : pick(GMV{}, AMV{}, BF{}, BF{}));
     NWG_PICK(I0, I0) NWG_PICK(I0, I1) NWG_PICK(I0, I2) NWG_PICK(I1, I0) NWG_PICK(I1, I1) NWG_PICK(I1, I2)
 #undef NWG_PICK
+    return kern;
+}
+
+template <typename T>
+int launch_hard(nwg_batch* b, const Plan& pl) {
+    nwg_ctx* ctx = b->ctx;
+    const bool hard = b->prob.grad_mode == 1, tr = b->tracked;
+    const size_t lds = (size_t)nwg::hard_lds_sub_bytes<T>(b->na, b->K, tr) +
+                       (hard ? (size_t)nwg::hard_lds_hist_bytes(b->na, b->K, tr, b->hist_slots) : 0);
+    if (lds > 65536)
+        return fail(ctx, NWG_EINVAL, "nwgrad_amd: %d parameter sets of %d x %d do not fit the fill's local memory",
+                    b->K, b->na, b->na);
+    void (*kern)(nwg::HardArgs<T>) = hard_kernel_for<T>(b->prob.gap_model, b->prob.mode, hard, tr, pl.S);
     if (!kern) return fail(ctx, NWG_EINVAL, "nwgrad_hip: unsupported problem type");
     // fill 0: one plain launch over the persistent order when a slot per tile fits the table
     // budget, else one persistent launch with as many slots (waves) as fit it, at most the
@@ -1241,11 +1269,12 @@ int launch_long(nwg_batch* b) {
         NWG_HIP(ctx, b->lcand.reserve((size_t)L.max_items * sizeof(nwg::CandRec<T>)));
         NWG_HIP(ctx, bnd.reserve((size_t)std::max<int64_t>(L.max_bnd, 1)));
         if (hard) NWG_HIP(ctx, b->ltable.reserve((size_t)std::max<int64_t>(L.max_table, 1)));
-        if (b->lerr.n < 1) {
-            NWG_HIP(ctx, b->lerr.alloc(1));
-            NWG_HIP(ctx, hipMemset(b->lerr.p, 0, sizeof(int32_t)));
-        }
+        if (b->lerr.n < 1) NWG_HIP(ctx, b->lerr.alloc(1));
     }
+    // A launch starts with a clear error word: a word left set would make every wait of this
+    // launch give up at once.  Cleared by a kernel, so a captured pass holds no memset node.
+    hipLaunchKernelGGL(nwg::clear_words_kernel, dim3(1), dim3(256), 0, ctx->stream, b->lerr.p, (int64_t)1);
+    NWG_HIP(ctx, hipGetLastError());
     nwg::LongArgs<T> A{};
     A.codes_a = b->ca.p;
     A.codes_b = b->cb.p;
@@ -1662,7 +1691,7 @@ int launch_soft(nwg_batch* b, const SoftPlan& pl, SoftPlan& wl) {
 
 // A batch's own soft pass: float32 or double, packed tiles (persistent) or per-bucket tiles
 // (plain), planned on first use.
-int launch_soft_batch(nwg_batch* b) {
+int launch_soft_batch(nwg_batch* b, bool record) {
     const bool packed = b->fill == 0;
     SoftPlan& mp = packed ? b->sp_main : b->sq_main;
     SoftPlan& wp = packed ? b->sp_wide : b->sq_wide;
@@ -1672,8 +1701,18 @@ int launch_soft_batch(nwg_batch* b) {
         const int rc = build_soft_plans(b, all, mp, wp, packed);
         if (rc) return rc;
     }
-    if (b->prob.precision == 0 && b->soft_precision == 0) return launch_soft<float>(b, mp, wp);
-    return launch_soft<double>(b, mp, wp);
+    const int rc = (b->prob.precision == 0 && b->soft_precision == 0) ? launch_soft<float>(b, mp, wp)
+                                                                       : launch_soft<double>(b, mp, wp);
+    if (rc || b->N == 0 || !record) return rc;
+    // an asynchronous pass's refusals into the batch's record (a captured step records them too)
+    nwg_ctx* ctx = b->ctx;
+    hipLaunchKernelGGL(nwg::soft_refusal_step_kernel, dim3(1), dim3(64), 0, ctx->stream, b->srec.p);
+    NWG_HIP(ctx, hipGetLastError());
+    const int64_t blocks = std::min<int64_t>((b->N + 255) / 256, 1024);
+    hipLaunchKernelGGL(nwg::soft_refusal_scan_kernel, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                       b->soft_ok.p, b->N, b->srec.p);
+    NWG_HIP(ctx, hipGetLastError());
+    return NWG_OK;
 }
 
 // After a soft pass: a pair whose float32 pass lost range or mass beyond 2^-20 reruns in
@@ -1729,11 +1768,14 @@ int soft_finish(nwg_batch* b) {
     return fail(ctx, NWG_ERANGE,
                 "nwgrad: soft path: %lld pair(s) (first: %lld) do not fit the scaled probability-space "
                 "forward-backward (mass lost to underflow could move the result by more than 2^-45, or a weight "
-                "overflows -- typically a very low temperature or extreme scores)",
+                "overflows -- typically a very low temperature or extreme scores); the reference's "
+                "soft_impl=\"scaled_or_log\" would rerun them in log space, which this GPU path does not have",
                 (long long)refused.size(), (long long)refused[0]);
 }
 
-int run_async(nwg_batch* b) {
+// record: the pass is asynchronous (its caller may consume results before any rerun), so its
+// float32 refusals go into the batch's refusal record.
+int run_async(nwg_batch* b, bool record) {
     nwg_ctx* ctx = b->ctx;
     if (!b->have_params) return fail(ctx, NWG_ESTATE, "nwgrad_hip: set_params() has not been called");
     NWG_HIP(ctx, hipSetDevice(ctx->device));
@@ -1743,7 +1785,7 @@ int run_async(nwg_batch* b) {
     if (timed) NWG_HIP(ctx, hipEventRecord(b->ev0, ctx->stream));
     int rc = NWG_OK;
     if (b->prob.grad_mode == 2) {
-        rc = launch_soft_batch(b);
+        rc = launch_soft_batch(b, record);
         if (rc) return rc;
     } else {
         rc = prepare_tracks(b);
@@ -1770,8 +1812,9 @@ int weighted_dev(nwg_batch* b, const double* w_dev, double* out_dev) {
     const int threads = 64;
     const int64_t total = nblocks * gs;
     const int grid = (int)((total + threads - 1) / threads);
+    const uint8_t* ok = b->prob.grad_mode == 2 ? b->soft_ok.p : nullptr;
     hipLaunchKernelGGL(nwg::block_sum_kernel, dim3(std::max(grid, 1)), dim3(threads), 0, ctx->stream, b->grads.p,
-                       w_dev, b->N, gs, kSumBlock, b->partial.p);
+                       w_dev, ok, b->N, gs, kSumBlock, b->partial.p);
     NWG_HIP(ctx, hipGetLastError());
     hipLaunchKernelGGL(nwg::block_total_kernel, dim3((gs + 63) / 64), dim3(64), 0, ctx->stream, b->partial.p, nblocks,
                        gs, out_dev);
@@ -1827,7 +1870,7 @@ extern "C" {
 
 int nwg_batch_run_async(nwg_batch* b) {
     if (!b) return NWG_EINVAL;
-    return run_async(b);
+    return run_async(b, true);
 }
 
 int nwg_batch_soft_finish(nwg_batch* b) {
@@ -1841,7 +1884,7 @@ int nwg_batch_soft_finish(nwg_batch* b) {
 int nwg_batch_run(nwg_batch* b, double* total_out) {
     if (!b) return NWG_EINVAL;
     nwg_ctx* ctx = b->ctx;
-    int rc = run_async(b);
+    int rc = run_async(b, false);
     if (rc) return rc;
     NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
@@ -1865,7 +1908,25 @@ int nwg_batch_scores(nwg_batch* b, double* out) {
         NWG_HIP(ctx, hipMemcpyAsync(out, b->scores.p, sizeof(double) * (size_t)b->N, hipMemcpyDeviceToHost,
                                     ctx->stream));
     NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return NWG_OK;
+    return check_long_err(b);   // an asynchronous pass's long-pair failure surfaces here
+}
+
+int nwg_batch_pair_result(nwg_batch* b, int64_t i, double* score, double* grad) {
+    if (!b) return NWG_EINVAL;
+    nwg_ctx* ctx = b->ctx;
+    if (i < 0 || i >= b->N)
+        return fail(ctx, NWG_EINDEX, "nwgrad: pair index %lld out of range for %lld pairs", (long long)i,
+                    (long long)b->N);
+    if (!b->results_valid) return fail(ctx, NWG_ESTATE, "nwgrad: score not valid; compute an alignment first");
+    if (grad && b->prob.grad_mode == 0) return fail(ctx, NWG_ESTATE, "nwgrad: grad_mode \"none\" computes no gradient");
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    const int gs = b->K * (b->na * b->na + 4);
+    if (score) NWG_HIP(ctx, hipMemcpyAsync(score, b->scores.p + i, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (grad)
+        NWG_HIP(ctx, hipMemcpyAsync(grad, b->grads.p + i * gs, sizeof(double) * (size_t)gs, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return check_long_err(b);
 }
 
 int nwg_batch_scores_dev(nwg_batch* b, const double** out_dev) {
@@ -1898,6 +1959,8 @@ int nwg_batch_grads(nwg_batch* b, double* mats, double* gaps) {
     NWG_HIP(ctx, hipMemcpyAsync(tmp.data(), b->grads.p, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost,
                                 ctx->stream));
     NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int rc = check_long_err(b);
+    if (rc) return rc;
     for (int64_t i = 0; i < b->N; ++i) {
         std::memcpy(mats + i * knn, tmp.data() + i * gs, sizeof(double) * (size_t)knn);
         std::memcpy(gaps + i * 4 * K, tmp.data() + i * gs + knn, sizeof(double) * 4 * (size_t)K);
@@ -1928,7 +1991,7 @@ int nwg_batch_weighted_grad(nwg_batch* b, const double* w, int64_t n, double* ou
     if (rc) return rc;
     NWG_HIP(ctx, hipMemcpyAsync(out, b->outdev.p, sizeof(double) * (size_t)gs, hipMemcpyDeviceToHost, ctx->stream));
     NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return NWG_OK;
+    return check_long_err(b);
 }
 
 int nwg_batch_weighted_grad_dev(nwg_batch* b, const double* w_dev, double* out_dev) {
@@ -1980,6 +2043,25 @@ int nwg_batch_soft_reruns(nwg_batch* b, int64_t* n) {
     return NWG_OK;
 }
 
+int nwg_batch_soft_refusals(nwg_batch* b, int64_t* pair_passes, int64_t* first_pass, int64_t* passes, int reset) {
+    if (!b) return NWG_EINVAL;
+    nwg_ctx* ctx = b->ctx;
+    if (b->prob.grad_mode != 2) return fail(ctx, NWG_EINVAL, "nwgrad_hip: refusal records belong to soft batches");
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    unsigned long long r[3] = {0, 0, 0};
+    NWG_HIP(ctx, hipMemcpyAsync(r, b->srec.p, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (pair_passes) *pair_passes = (int64_t)r[0];
+    if (first_pass) *first_pass = (int64_t)r[1];
+    if (passes) *passes = (int64_t)r[2];
+    if (reset) {   // refusal counts start again; the pass counter keeps counting
+        r[0] = r[1] = 0;
+        NWG_HIP(ctx, hipMemcpyAsync(b->srec.p, r, 2 * sizeof(unsigned long long), hipMemcpyHostToDevice, ctx->stream));
+        NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return NWG_OK;
+}
+
 int nwg_batch_set_soft_scratch(nwg_batch* b, int64_t bytes) {
     if (!b) return NWG_EINVAL;
     if (bytes < 0) return fail(b->ctx, NWG_EINVAL, "nwgrad_hip: soft scratch bytes must be >= 0 (0: the default)");
@@ -2020,3 +2102,4 @@ int nwg_batch_total_info(nwg_batch* b, int64_t* from_device, int64_t* in_pair_or
 }
 
 }  // extern "C"
+

@@ -74,15 +74,39 @@ void bind_server(py::module_& m) {
              py::arg("max_cells_per_launch") = int64_t(4000000000), py::arg("max_pairs_per_launch") = int64_t(1) << 20,
              py::arg("soft_temperature") = 1.0, py::arg("device") = 0)
         .def("submit",
-             [](AlignServer& s, const std::string& kind, U8Arr ca, I64Arr oa, U8Arr cb, I64Arr ob) {
+             [](AlignServer& s, const std::string& kind, U8Arr ca, I64Arr oa, U8Arr cb, I64Arr ob, py::object tracks,
+                py::object gap_tracks) {
                  if (oa.ndim() != 1 || ob.ndim() != 1 || oa.size() < 1 || oa.size() != ob.size())
                      throw py::value_error("nwgrad_amd: off_a and off_b need n + 1 entries each");
                  const int64_t n = static_cast<int64_t>(oa.size()) - 1;
                  if (oa.data()[n] > static_cast<int64_t>(ca.size()) || ob.data()[n] > static_cast<int64_t>(cb.size()))
                      throw py::value_error("nwgrad_amd: offsets run past the codes");
-                 return s.submit(parse_kind(kind), ca.data(), oa.data(), cb.data(), ob.data(), n);
+                 // flat tracks: oa[n] entries (one per residue of A), gap tracks oa[n] + n (one per DP row)
+                 auto flat = [&](const py::object& o, int64_t want, const char* what) {
+                     std::vector<uint16_t> v;
+                     if (o.is_none()) return v;
+                     auto a = py::array_t<int64_t, py::array::c_style | py::array::forcecast>::ensure(o);
+                     if (!a || a.ndim() != 1 || a.shape(0) != want)
+                         throw py::value_error(std::string("nwgrad_amd: ") + what + " must be a flat integer array of " +
+                                               std::to_string(want) + " entries");
+                     v.resize(static_cast<size_t>(want));
+                     for (int64_t k = 0; k < want; ++k) {
+                         const int64_t x = a.data()[k];
+                         if (x < 0 || x > 65535)
+                             throw py::value_error(std::string("nwgrad_amd: ") + what + " entry " + std::to_string(x) +
+                                                   " is not a parameter set index");
+                         v[static_cast<size_t>(k)] = static_cast<uint16_t>(x);
+                     }
+                     return v;
+                 };
+                 const int64_t la = oa.data()[n];
+                 const std::vector<uint16_t> t = flat(tracks, la, "tracks");
+                 const std::vector<uint16_t> g = flat(gap_tracks, la + n, "gap_tracks");
+                 return s.submit(parse_kind(kind), ca.data(), oa.data(), cb.data(), ob.data(), n,
+                                 tracks.is_none() ? nullptr : t.data(), gap_tracks.is_none() ? nullptr : g.data());
              },
-             py::arg("kind"), py::arg("codes_a"), py::arg("off_a"), py::arg("codes_b"), py::arg("off_b"))
+             py::arg("kind"), py::arg("codes_a"), py::arg("off_a"), py::arg("codes_b"), py::arg("off_b"),
+             py::arg("tracks") = py::none(), py::arg("gap_tracks") = py::none())
         .def("poll", &AlignServer::poll)
         .def("now", &AlignServer::now)
         .def("state", [](const AlignServer& s, int64_t id) {
@@ -103,7 +127,7 @@ void bind_server(py::module_& m) {
             if (!j.scores.empty()) std::memcpy(sc.mutable_data(), j.scores.data(), sizeof(double) * j.scores.size());
             d["scores"] = sc;
             if (!j.grads.empty()) {
-                const py::ssize_t gs = s.n_alpha() * s.n_alpha() + 4;
+                const py::ssize_t gs = s.num_sets() * (s.n_alpha() * s.n_alpha() + 4);
                 py::array_t<double> g({static_cast<py::ssize_t>(j.n), gs});
                 std::memcpy(g.mutable_data(), j.grads.data(), sizeof(double) * j.grads.size());
                 d["grads"] = g;
@@ -113,6 +137,7 @@ void bind_server(py::module_& m) {
             return d;
         })
         .def("release", &AlignServer::release)
+        .def_property_readonly("num_sets", &AlignServer::num_sets)
         .def_property_readonly("waiting", &AlignServer::waiting)
         .def_property_readonly("running_launches", &AlignServer::running_launches)
         .def_property_readonly("launches", &AlignServer::launches)
@@ -238,6 +263,53 @@ struct PyBatch {
     BatchOpts o;
 };
 
+// batch[i]: a view of pair i that keeps its batch alive (the reference's SeqPairT view,
+// py_exports.cpp:895-905 and :566-765).  Results are read from the batch's device arrays when
+// asked for, so a view always shows the batch's current results.
+template <class T>
+struct PairView {
+    py::object owner;
+    PyBatch<T>* pb = nullptr;
+    size_t i = 0;
+    SeqPairBatchT<T>& b() const { return *pb->b; }
+};
+
+template <class T>
+void bind_view(py::module_& m, const char* name) {
+    using PV = PairView<T>;
+    py::class_<PV>(m, name)
+        .def_property_readonly("score", [](const PV& v) -> py::object {
+            if (!v.b().results_valid()) return py::none();
+            return py::float_(v.b().pair_score(v.i));
+        }, "Alignment score, or None if not yet computed.")
+        .def_property_readonly("grad", [](const PV& v) -> py::object {
+            if (!v.b().results_valid() || v.b().grad_mode() == GradMode::None) return py::none();
+            return py::cast(v.b().pair_grad(v.i));
+        }, "Gradient as an AlignParams object, or None if not computed.")
+        .def_property_readonly("score_valid", [](const PV& v) { return v.b().results_valid(); })
+        .def_property_readonly("grad_valid", [](const PV& v) {
+            return v.b().results_valid() && v.b().grad_mode() != GradMode::None;
+        })
+        .def_property_readonly("seq_a", [](const PV& v) { return v.b().seq_a(v.i); })
+        .def_property_readonly("seq_b", [](const PV& v) { return v.b().seq_b(v.i); })
+        .def_property_readonly("track", [](const PV& v) -> py::object {
+            if (!v.b().tracked()) return py::none();
+            return py::cast(v.b().track(v.i));
+        }, "The pair's parameter set per residue of seq_a (None when not tracked).")
+        .def_property_readonly("gap_track", [](const PV& v) -> py::object {
+            if (!v.b().tracked()) return py::none();
+            return py::cast(v.b().gap_track(v.i));
+        }, "The pair's parameter set per DP row 0..len(seq_a) for gaps in seq_a (None when not tracked).")
+        .def_property_readonly("gap_model", [](const PV& v) { return gm_name(v.b().gap_model()); })
+        .def_property_readonly("mode", [](const PV& v) { return am_name(v.b().align_mode()); })
+        .def_property_readonly("grad_mode", [](const PV& v) { return gd_name(v.b().grad_mode()); })
+        .def_property_readonly("index", [](const PV& v) { return v.i; })
+        .def("__repr__", [name](const PV& v) {
+            return std::string(name) + "(batch pair " + std::to_string(v.i) + ", len(a)=" +
+                   std::to_string(v.b().len_a(v.i)) + ", len(b)=" + std::to_string(v.b().len_b(v.i)) + ")";
+        });
+}
+
 template <class T>
 void bind_batch(py::module_& m, const char* name) {
     using PB = PyBatch<T>;
@@ -311,6 +383,20 @@ void bind_batch(py::module_& m, const char* name) {
             "A batch from pre-encoded pairs: uint8 codes in alphabet order and int64 offsets (N+1), pair i\n"
             "being codes[off[i]:off[i+1]].")
         .def("__len__", [](const PB& s) { return s.b->size(); })
+        .def(
+            "__getitem__",
+            [](py::object self, int64_t i) {
+                PB& s = self.cast<PB&>();
+                const int64_t n = static_cast<int64_t>(s.b->size());
+                if (i < 0) i += n;
+                if (i < 0 || i >= n) throw py::index_error("SeqPairBatch index out of range");
+                PairView<T> v;
+                v.owner = self;
+                v.pb = &s;
+                v.i = static_cast<size_t>(i);
+                return v;
+            },
+            py::arg("i"), "Pair i as a view (score, grad, seq_a, seq_b, track, ...) that keeps the batch alive.")
         .def("set_params", [](PB& s, const AlignParams& p) { s.b->set_params(p); }, py::arg("params"))
         .def(
             "score_and_grad",
@@ -384,10 +470,18 @@ void bind_batch(py::module_& m, const char* name) {
         .def_property(
             "soft_impl", [](const PB& s) { return s.o.soft_impl; },
             [](PB& s, const std::string& v) {
-                if (v != "scaled" && v != "scaled_or_log" && v != "log")
+                if (v == "log")
+                    throw py::value_error("nwgrad_amd: soft_impl \"log\" (the log-space forward-backward) is not part "
+                                          "of the GPU path; \"scaled\" (float32, then double, then an error) and "
+                                          "\"scaled_or_log\" are");
+                if (v != "scaled" && v != "scaled_or_log")
                     throw py::value_error("nwgrad: unknown soft_impl \"" + v + "\"");
                 s.o.soft_impl = v;
-            })
+            },
+            "\"scaled\" (default): the scaled probability-space pass, a float32 refusal rerun in double, a\n"
+            "double refusal a ValueError.  \"scaled_or_log\": the same; where the reference would fall back\n"
+            "to its log-space pass this build raises the same ValueError (its GPU pass keeps per-segment\n"
+            "exponents, so it refuses far fewer pairs).  \"log\" is refused.")
         .def_property(
             "soft_guide", [](const PB& s) { return s.o.soft_guide; },
             [](PB& s, const std::string& v) {
@@ -446,6 +540,10 @@ void bind_batch(py::module_& m, const char* name) {
         .def("_scores_copy_dev", [](const PB& s, uintptr_t dst) { s.b->scores_copy_dev(reinterpret_cast<double*>(dst)); },
              py::arg("dst_dev"))
         .def("_soft_finish", [](PB& s) { s.b->soft_finish(); return s.b->soft_reruns(); })
+        .def("_soft_refusals", [](PB& s, bool reset) { auto r = s.b->soft_refusals(reset); return py::make_tuple(r[0], r[1], r[2]); },
+             py::arg("reset") = false,
+             "(pair_passes, first_pass, passes): float32 refusals the batch's passes left before any rerun -- "
+             "pair-passes refused, the first pass (1-based) with one (0: none), passes run.  reset clears the counts.")
         .def("_set_stream", [](PB& s, uintptr_t stream) { s.b->set_stream(reinterpret_cast<void*>(stream)); },
              py::arg("stream"))
         .def(
@@ -602,6 +700,8 @@ PYBIND11_MODULE(_nwgrad_amd, m) {
         .def("__isub__", [](AlignParams& a, const AlignParams& b) -> AlignParams& { a -= b; return a; },
              py::return_value_policy::reference);
 
+    bind_view<float>(m, "SeqPair");
+    bind_view<double>(m, "SeqPairDouble");
     bind_batch<float>(m, "SeqPairBatch");
     bind_batch<double>(m, "SeqPairBatchDouble");
     bind_server(m);

@@ -10,9 +10,13 @@ namespace nwg {
 // (w ? grads[i][k] * w[i] : grads[i][k]).  The product is rounded before the add: the
 // library is compiled with -ffp-contract=off, as the reference forms the product out of
 // line (scale_rows_) so no FMA fuses it.
+// ok (soft batches, may be null): a pair whose flag is 0 -- a float32 refusal not yet rerun in
+// double, as in an asynchronous or captured training step -- adds nothing.  After a finished
+// pass every flag is 1, and the sum is the reference's.
 __global__ __launch_bounds__(64) void block_sum_kernel(const double* __restrict__ grads,
-                                                       const double* __restrict__ w, int64_t N, int gs,
-                                                       int64_t block, double* __restrict__ partial) {
+                                                       const double* __restrict__ w, const uint8_t* __restrict__ ok,
+                                                       int64_t N, int gs, int64_t block,
+                                                       double* __restrict__ partial) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nblocks = (N + block - 1) / block;
     if (t >= nblocks * gs) return;
@@ -22,13 +26,40 @@ __global__ __launch_bounds__(64) void block_sum_kernel(const double* __restrict_
     double acc = 0.0;
     if (w) {
         for (int64_t i = i0; i < i1; ++i) {
+            if (ok && !ok[i]) continue;
             const double prod = grads[i * gs + k] * w[i];
             acc += prod;
         }
     } else {
-        for (int64_t i = i0; i < i1; ++i) acc += grads[i * gs + k];
+        for (int64_t i = i0; i < i1; ++i)
+            if (!ok || ok[i]) acc += grads[i * gs + k];
     }
     partial[blk * gs + k] = acc;
+}
+
+// The refusal record of a soft batch's main passes: rec[0] pair-passes the float32 pass refused,
+// rec[1] the first pass (1-based) with a refusal, 0 for none, rec[2] passes run.  Per pass,
+// soft_refusal_step_kernel advances rec[2], then this scan (the next kernel on the stream)
+// counts the refused flags and names the pass in rec[1] if it is the first with any.
+__global__ __launch_bounds__(256) void soft_refusal_scan_kernel(const uint8_t* __restrict__ ok, int64_t N,
+                                                                unsigned long long* __restrict__ rec) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < N; i += (int64_t)gridDim.x * 256)
+        mine += ok[i] ? 0 : 1;
+    if (mine) atomicAdd(&cnt, mine);
+    __syncthreads();
+    if (threadIdx.x == 0 && cnt > 0) {
+        atomicAdd(&rec[0], (unsigned long long)cnt);
+        atomicCAS(&rec[1], 0ull, rec[2]);
+    }
+}
+
+// Advances the pass counter rec[2] (one thread; launched before the scan of the same pass).
+__global__ void soft_refusal_step_kernel(unsigned long long* __restrict__ rec) {
+    if (threadIdx.x == 0) rec[2] += 1ull;
 }
 
 // p[0 .. n) = 0: counters and progress words cleared by a kernel rather than a memset, so the

@@ -26,7 +26,7 @@ KINDS = ("score", "hard", "soft")
 
 
 class Server:
-    """A continuous-batching server over ``params`` (an ``AlignParams``).
+    """A continuous-batching server over ``params`` (an ``AlignParams``, one or K parameter sets).
 
     ``double``: float64 (the ``SeqPairBatchDouble`` numerics) instead of float32.
     ``hbm_budget_bytes``: device memory the jobs in flight may hold (codes, results and hard
@@ -44,12 +44,19 @@ class Server:
                               max_pairs_per_launch=int(max_pairs_per_launch), soft_temperature=float(soft_temperature),
                               device=int(device))
 
-    def submit(self, kind, codes_a, off_a, codes_b, off_b) -> int:
-        """Queue a job of encoded pairs (``uint8`` codes, ``int64`` offsets from 0); its id."""
+    def submit(self, kind, codes_a, off_a, codes_b, off_b, tracks=None, gap_tracks=None) -> int:
+        """Queue a job of encoded pairs (``uint8`` codes, ``int64`` offsets from 0); its id.
+
+        With params of K > 1 parameter sets the server is tracked: ``tracks`` is one set per
+        residue of A (flat, ``off_a[-1]`` entries) and ``gap_tracks`` one per DP row (flat,
+        ``off_a[-1] + n`` entries, pair p's at ``off_a[p] + p``); either may be None for the
+        reference's defaults.  Gradient rows then hold K matrices, then K rows of gaps."""
         if kind not in KINDS:
             raise ValueError(f"nwgrad_amd: unknown job kind {kind!r} (expected one of {KINDS})")
         return self._s.submit(kind, np.ascontiguousarray(codes_a, np.uint8), np.ascontiguousarray(off_a, np.int64),
-                              np.ascontiguousarray(codes_b, np.uint8), np.ascontiguousarray(off_b, np.int64))
+                              np.ascontiguousarray(codes_b, np.uint8), np.ascontiguousarray(off_b, np.int64),
+                              tracks=None if tracks is None else np.ascontiguousarray(tracks, np.int64),
+                              gap_tracks=None if gap_tracks is None else np.ascontiguousarray(gap_tracks, np.int64))
 
     def submit_strings(self, kind, seqs_a, seqs_b) -> int:
         """Queue a job given as two lists of strings over the params' alphabet."""
@@ -90,6 +97,10 @@ class Server:
 
     def now(self) -> float:
         return self._s.now()
+
+    @property
+    def num_sets(self) -> int:
+        return self._s.num_sets
 
     @property
     def waiting(self) -> int:
@@ -136,7 +147,7 @@ def replay(server: Server, requests, *, idle_sleep=2e-4, timeout=900.0, progress
         now = time.perf_counter() - t0
         while i < len(order) and requests[order[i]].t <= now:
             q = requests[order[i]]
-            ids.append(server.submit(q.kind, q.codes_a, q.off_a, q.codes_b, q.off_b))
+            ids.append(server.submit(q.kind, q.codes_a, q.off_a, q.codes_b, q.off_b, tracks=getattr(q, "tracks", None)))
             cells += q.cells
             i += 1
         if progress is not None and now >= t_report:

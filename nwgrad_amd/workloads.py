@@ -44,6 +44,7 @@ class Workload:
     off_a: np.ndarray
     codes_b: np.ndarray
     off_b: np.ndarray
+    tracks: np.ndarray | None = None   # K > 1 sets: one set per residue of A, flat
 
     @property
     def n_pairs(self) -> int:
@@ -239,6 +240,7 @@ class Request:
     off_a: np.ndarray
     codes_b: np.ndarray
     off_b: np.ndarray
+    tracks: np.ndarray | None = None   # K > 1 sets: one set per residue of A, flat
 
     @property
     def n_pairs(self) -> int:
@@ -266,8 +268,10 @@ class ServingLoad:
     reference's does).  Request k is a deterministic function of (name, k)."""
 
     def __init__(self, rate: float = 200.0, duration: float = 60.0, max_pairs: int = 4096, lo: int = 30,
-                 hi: int = 5000, mix=(0.7, 0.2, 0.1), name: str = "serving"):
+                 hi: int = 5000, mix=(0.7, 0.2, 0.1), name: str = "serving", sets: int = 1,
+                 soft_homologs: bool = True):
         self.name, self.max_pairs, self.lo, self.hi = name, max_pairs, lo, hi
+        self.sets, self.soft_homologs = int(sets), bool(soft_homologs)
         r = rng(name, "arrivals")
         t, times = 0.0, []
         while True:
@@ -280,6 +284,13 @@ class ServingLoad:
         self.pool = rng(name, "pool").integers(0, 20, size=(1 << 25) + max_pairs * hi).astype(np.uint8)
         self.matrix = random_matrix(rng(name, "matrix"))
         self.gaps = np.array([11.0, 1.0, 11.0, 1.0])
+        # K parameter sets: set 0 the matrix above; sets 1.. random matrices of the same law with
+        # gaps drawn around 11 / 1
+        rs = rng(name, "sets")
+        self.matrices = np.stack([self.matrix] + [random_matrix(rs) for _ in range(self.sets - 1)])
+        self.set_gaps = np.stack([self.gaps] + [np.array([rs.integers(8, 14), rs.uniform(0.5, 1.5),
+                                                          rs.integers(8, 14), rs.uniform(0.5, 1.5)])
+                                                for _ in range(self.sets - 1)])
 
     def __len__(self) -> int:
         return len(self.times)
@@ -296,7 +307,7 @@ class ServingLoad:
         la = _log_uniform_int(r, self.lo, self.hi, n)
         kind = str(self.kinds[k])
         ca, oa = self._slices(r, la)
-        if kind == "soft":
+        if kind == "soft" and self.soft_homologs:
             cb = ca.copy()
             subs = r.random(len(cb)) < 0.3
             cb[subs] = (cb[subs] + r.integers(1, 20, size=int(subs.sum()))) % 20
@@ -304,4 +315,27 @@ class ServingLoad:
         else:
             lb = _log_uniform_int(r, self.lo, self.hi, n)
             cb, ob = self._slices(r, lb)
-        return Request(float(self.times[k]), kind, ca, oa, cb.astype(np.uint8), ob)
+        tracks = segment_tracks(r, oa, self.sets) if self.sets > 1 else None
+        return Request(float(self.times[k]), kind, ca, oa, cb.astype(np.uint8), ob, tracks)
+
+
+def segment_tracks(r: np.random.Generator, off_a: np.ndarray, sets: int, max_segments: int = 8) -> np.ndarray:
+    """Per-residue tracks of A in uniform-random contiguous segments: pair p's A (m residues) is
+    cut at U{1..min(max_segments, m)} - 1 uniform points in [1, m) (coinciding cuts merge), and
+    each segment takes a set drawn uniformly from [0, sets).  Flat, off_a[-1] entries."""
+    off = np.asarray(off_a, np.int64)
+    la = int(off[-1])
+    if la == 0:
+        return np.zeros(0, np.uint16)
+    m = np.diff(off)
+    starts = np.zeros(la, np.int64)          # 1 where a segment starts
+    nz = m > 0
+    starts[off[:-1][nz]] = 1
+    nseg = np.where(nz, r.integers(1, np.minimum(max_segments, np.maximum(m, 1)) + 1), 0)
+    ncut = np.maximum(nseg - 1, 0)
+    owner = np.repeat(np.arange(len(m)), ncut)
+    if owner.size:
+        pos = off[owner] + 1 + np.floor(r.random(owner.size) * (m[owner] - 1)).astype(np.int64)
+        starts[pos] = 1
+    seg = np.cumsum(starts) - 1
+    return r.integers(0, sets, size=int(seg[-1]) + 1).astype(np.uint16)[seg]

@@ -102,3 +102,44 @@ def test_poisson_replay(gpu):
     say(f"  {out}")
     assert out["failed"] == 0 and out["jobs"] == len(reqs)
     assert out["p99_ms"] >= out["p50_ms"] > 0
+
+
+def test_tracked_k4_alone_and_in_the_mix(gpu):
+    """The serving configuration's K = 4 arm: params of four sets, every job's A carrying a
+    per-residue track in uniform-random contiguous segments (workloads.segment_tracks).  18 jobs
+    of the three kinds in two waves; every job against the same pairs run alone in a tracked
+    batch, and one hard job against the oracle's tracked restatement."""
+    import nwgrad_amd as nw
+    import oracle
+    from nwgrad_amd.serving import Server
+    load = wl.ServingLoad(rate=100, duration=1.0, sets=4, name="serving_k4")
+    pk = nw.AlignParams.from_arrays(load.matrices, load.set_gaps, alphabet=wl.AA)
+    jobs = _jobs("serving_k4", 18, 30, (20, 1300))
+    r = wl.rng("serving_k4", "tracks")
+    tracks = [wl.segment_tracks(r, j[2], 4) for j in jobs]
+    srv = Server(pk, slots_per_kind=2, max_cells_per_launch=10_000_000)
+    assert srv.num_sets == 4
+    ids = []
+    for wave in range(2):
+        for j, t in list(zip(jobs, tracks))[wave * 9:(wave + 1) * 9]:
+            ids.append(srv.submit(*j, tracks=t))
+        for _ in range(3):
+            srv.poll()
+        say(f"  wave {wave}: {srv.running_launches} launches running, {srv.waiting} jobs waiting")
+    srv.drain(timeout=300)
+    say(f"  {srv.launches} launches for {len(jobs)} jobs")
+    assert srv.launches < len(jobs)
+    checked_oracle = False
+    for jid, j, t in zip(ids, jobs, tracks):
+        kind, ca, oa, cb, ob = j
+        b = nw.SeqPairBatch.from_codes(ca, oa, cb, ob, pk, grad_mode="none" if kind == "score" else kind, tracks=t)
+        b.score_and_grad()
+        alone = (b.scores(), None if kind == "score" else grads_rows(b))
+        res = srv.result(jid)
+        _check(kind, res, alone)
+        if kind == "hard" and not checked_oracle:
+            s_ref, g_ref = oracle.hard_tracked(load.matrices, load.set_gaps, ca, oa, cb, ob, t)
+            np.testing.assert_array_equal(res["scores"], s_ref)
+            np.testing.assert_array_equal(res["grads"], g_ref)
+            checked_oracle = True
+    assert checked_oracle
