@@ -206,6 +206,22 @@ int nwg_batch_wide_info(nwg_batch *b, int64_t *tiles, int64_t *pooled, int64_t *
  * scores in pair order on the host.  Both give the pair-order total. */
 int nwg_batch_total_info(nwg_batch *b, int64_t *from_device, int64_t *in_pair_order);
 
+/* A hard or score-only pass that allocates nothing: no counterpart in the reference.  It is
+ * for callers that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  The caller passes device buffers: the
+ * pairs (codes_*_dev in [0, n_alpha), offsets off_*_dev of N+1 entries), scores_dev (N
+ * doubles) and, hard, grads_dev (N rows of n_alpha^2 + 4 doubles, the batch path's layout);
+ * off_a / off_b are host copies of the offsets, for the planner.  Every other buffer (the
+ * plan, the matrix, the pointer codes) lives in a 64 MiB device array of the library, one
+ * per device: at most 2^20 pairs, len(B) at most 1,024, untracked params; a plan or tile
+ * that does not fit is refused (NWG_EINVAL).  Results equal nwg_batch_run's for the same
+ * pairs.  Calls on one device share the array: calls that overlap on a device race.  The
+ * call synchronises the context's stream before returning. */
+int nwg_hard_noalloc(nwg_ctx *ctx, const nwg_problem *prob, int n_alpha, int64_t n_pairs,
+                     const uint8_t *codes_a_dev, const int64_t *off_a_dev, const uint8_t *codes_b_dev,
+                     const int64_t *off_b_dev, const int64_t *off_a, const int64_t *off_b,
+                     const double *matrix, const double *gaps4, double *scores_dev, double *grads_dev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2103,3 +2103,4 @@ int nwg_batch_total_info(nwg_batch* b, int64_t* from_device, int64_t* in_pair_or
 
 }  // extern "C"
 
+#include "noalloc.hpp"
