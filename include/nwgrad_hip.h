@@ -133,6 +133,38 @@ int nwg_batch_run_async(nwg_batch *b);
  * float32 -> double -> error policy, aligner.hpp:6554-6586). */
 int nwg_batch_soft_finish(nwg_batch *b);
 
+/* Banded re-alignment around each pair's guide from the last pass (the training-loop step after
+ * a set_params): the pass of nwg_batch_run with only the cells of each row's band
+ * [max(1, g[i] - bw), max(g[i] + bw, g[i+1] - 1 + bw)] (Global's row m to n; bw = min(bandwidth,
+ * n)), the others -inf -- the reference's GuideBanded fill, bit for bit.  The pass writes new
+ * guides.  Hard batches; NWG_ESTATE before any pass, NWG_EINVAL for bandwidth < 0 (a width of 0
+ * follows the guide's own cells, as streaming align() does with guides and band 0).  Replaces
+ * BatchEngine::banded_grad (batch_engine.hpp:633-653, 2853-2982; aligner.hpp:2621-2730,
+ * 5991-6098). */
+int nwg_batch_run_banded(nwg_batch *b, int bandwidth, double *total_out);
+
+/* Guides of the last hard pass: per pair the column after each row of A (m + 1 entries, pair p's
+ * at off_a[p] + p, sum(m) + N in all), interpolated where a Local path leaves rows uncovered --
+ * the reference's guide_j (aligner.hpp:3155-3160, 2734-2749; store_guide_, batch_engine.hpp:
+ * 2045-2055).  They survive set_params.  Hard batches; NWG_ESTATE before any pass. */
+int nwg_batch_guides(nwg_batch *b, int32_t *out);
+int nwg_batch_pair_guide(nwg_batch *b, int64_t i, int32_t *out);
+/* Guides from the caller (the same layout), for a banded pass around given alignments; checked
+ * as the reference checks a supplied guide (validate_guide, aligner.hpp:2622-2642: every entry
+ * in [0, n], never decreasing).  Replaces the guides of streaming align(band, aligned_a,
+ * aligned_b) (batch_engine.hpp:748-1273). */
+int nwg_batch_set_guides(nwg_batch *b, const int32_t *guides);
+
+/* Stored paths (keep_paths): keep != 0 makes the following hard passes store each pair's path
+ * as run-length ops, (type << 30) | length with type 0 both advance, 1 a gap in B, 2 a gap in A
+ * (batch_engine.hpp:1415-1418, 2056-2092).  nwg_batch_paths copies them out: `ops` holds
+ * sum(m + n) entries, pair p's buffer of m + n at off_a[p] + off_b[p] with its n_ops[p] ops
+ * at the buffer's end, ends[2p], ends[2p+1] the cell the path ends at (the reference's
+ * alignment_end).  set_params and nwg_batch_drop_paths invalidate them (NWG_ESTATE after). */
+int nwg_batch_set_keep_paths(nwg_batch *b, int keep);
+int nwg_batch_paths(nwg_batch *b, uint32_t *ops, int32_t *n_ops, int32_t *ends);
+int nwg_batch_drop_paths(nwg_batch *b);
+
 /* Copies of the cached results (BatchEngine::scores / grads_into,
  * batch_engine.hpp:689-695, 722-746).  mats: N*n_alpha*n_alpha, gaps: N*4. */
 int nwg_batch_scores(nwg_batch *b, double *out);

@@ -13,6 +13,7 @@
 #pragma once
 
 #include <array>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -254,6 +255,22 @@ struct AlignParams {
     friend AlignParams operator*(double s, const AlignParams& p) { return p * s; }
 };
 
+// The guide of an alignment given as two gapped strings (the reference's guide_j_from_aligned,
+// aligner.hpp:159-180): entry i is the column reached once i residues of A are consumed; a
+// column of two residues or a gap in B consumes one of A, a gap in A moves the column only.
+inline std::vector<int> guide_from_aligned(std::string_view a, std::string_view b) {
+    if (a.size() != b.size()) throw std::invalid_argument("aligned strings must have equal length");
+    std::vector<int> g{0};
+    int col = 0;
+    for (size_t c = 0; c < a.size(); ++c) {
+        const bool ga = a[c] == '-', gb = b[c] == '-';
+        if (ga && gb) throw std::invalid_argument("aligned strings have a column with two gaps");
+        if (!gb) ++col;
+        if (!ga) g.push_back(col);
+    }
+    return g;
+}
+
 enum class GapModel { Linear = 0, Affine = 1 };
 enum class AlignMode { Global = 0, Local = 1, Glocal = 2 };
 enum class GradMode { None = 0, Hard = 1, Soft = 2 };
@@ -288,6 +305,13 @@ public:
         ctxs[device] = c;
         return c;
     }
+};
+
+// What a streaming align() returns (batch_result.hpp:69-77): one score per pair and the gradient
+// summed over them.
+struct BatchResult {
+    std::vector<double> scores;
+    AlignParams grad;
 };
 
 // SeqPairBatchT<T>: T = float (SeqPairBatch) or double (SeqPairBatchDouble).
@@ -340,7 +364,8 @@ public:
         n_threads_ = v;
     }
 
-    // batch_engine.hpp:597-606: a copy of params, over the batch's alphabet.
+    // batch_engine.hpp:597-606: a copy of params, over the batch's alphabet.  Clears scores,
+    // gradients and stored paths; guides stay (banded_grad() re-aligns around them).
     void set_params(const AlignParams& p) {
         if (&p.alphabet() != alpha_ && N_ > 0)
             throw std::invalid_argument("nwgrad: params alphabet \"" + p.alphabet().symbols() +
@@ -353,6 +378,8 @@ public:
             params_ = old;
             throw;
         }
+        stored_ = false;
+        paths_cached_ = false;
     }
     const AlignParams& params() const noexcept { return params_; }
 
@@ -392,12 +419,133 @@ public:
     }
     std::string soft_precision() const { return soft_f32_ ? "float32" : "double"; }
 
-    // batch_engine.hpp:615-629: every pair's score (and gradient); the sum in pair order.
-    double score_and_grad() {
+    // batch_engine.hpp:615-629: every pair's score (and gradient) and guide; the sum in pair
+    // order.  keep_paths: also each pair's path (aligned(), coordinates(), cigar()).  Hard
+    // batches write guides and paths in their own pass; score-only and soft batches take them
+    // from a hard pass over the same pairs under the params scored -- the reference's eager
+    // guide, the same hard fill -- run here when paths are kept, else on the first guide read.
+    double score_and_grad(bool keep_paths = false) {
         double total = 0.0;
+        if (gd_ == GradMode::Hard) check_(nwg_batch_set_keep_paths(b_, keep_paths ? 1 : 0));
         check_(nwg_batch_run(b_, &total));
         valid_ = true;
+        gparams_ = params_;
+        gfresh_ = false;
+        paths_cached_ = false;
+        stored_ = keep_paths;
+        guides_ = true;
+        if (keep_paths && gd_ != GradMode::Hard) resolve_guides_(true);
         return total;
+    }
+    // batch_engine.hpp:633-653: a banded re-alignment and gradient around every pair's guide,
+    // the training-loop step after set_params; new guides.  Hard batches.
+    double banded_grad(int bandwidth, bool keep_paths = false) {
+        if (N_ == 0) return 0.0;
+        if (bandwidth <= 0)
+            throw std::invalid_argument("nwgrad: banded_grad() needs bandwidth > 0 (got " + std::to_string(bandwidth) +
+                                        "); use score_and_grad() for full DP");
+        if (gd_ != GradMode::Hard)
+            throw std::invalid_argument("nwgrad_amd: banded_grad() runs on hard batches (grad_mode=\"hard\") on the "
+                                        "GPU path");
+        if (!guides_) throw std::logic_error("nwgrad: banded_grad() needs a guide path; run score_and_grad() first");
+        double total = 0.0;
+        check_(nwg_batch_set_keep_paths(b_, keep_paths ? 1 : 0));
+        check_(nwg_batch_run_banded(b_, bandwidth, &total));
+        valid_ = true;
+        gparams_ = params_;
+        paths_cached_ = false;
+        stored_ = keep_paths;
+        return total;
+    }
+    // Free the stored paths; scores, gradients and guides stay.
+    void drop_paths() {
+        if (gd_ == GradMode::Hard) check_(nwg_batch_drop_paths(b_));
+        if (gb_) gb_->drop_paths();
+        stored_ = false;
+        paths_cached_ = false;
+        std::vector<uint32_t>().swap(pops_);
+    }
+    bool path_valid() const noexcept { return guides_; }
+    bool stored_path_valid() const noexcept { return stored_; }
+    // Pair i's guide (the column after each row of A, m + 1 entries).
+    std::vector<int> guide_j(size_t i) {
+        if (i >= size()) throw std::out_of_range("SeqPairBatch index out of range");
+        if (!guides_) throw std::logic_error("nwgrad: alignment not computed; call align() first");
+        std::vector<int> g(static_cast<size_t>(len_a(i) + 1));
+        if (gd_ == GradMode::Hard) {
+            check_(nwg_batch_pair_guide(b_, static_cast<int64_t>(i), g.data()));
+            return g;
+        }
+        if (!gfresh_) resolve_guides_(false);
+        return gb_->guide_j(i);
+    }
+    // Stored paths (keep_paths), rendered as the reference renders them (batch_engine.hpp:
+    // 513-590): gapped strings, Biopython-style coordinates and SAM CIGARs (seq_a the query).
+    std::pair<std::string, std::string> aligned(size_t i) const {
+        const PathView v = path_(i);
+        const std::string& sym = alpha_->symbols();
+        std::string a, b;
+        a.reserve(static_cast<size_t>(v.cols));
+        b.reserve(static_cast<size_t>(v.cols));
+        const uint8_t* ca = codes_a(i);
+        const uint8_t* cb = codes_b(i);
+        int64_t x = v.i0, y = v.j0;
+        for (int64_t q = 0; q < v.n; ++q) {
+            const uint32_t t = v.ops[q] >> 30, len = v.ops[q] & 0x3fffffffu;
+            for (uint32_t k = 0; k < len; ++k) {
+                a.push_back(t == 2 ? '-' : sym[ca[x++]]);
+                b.push_back(t == 1 ? '-' : sym[cb[y++]]);
+            }
+        }
+        return {std::move(a), std::move(b)};
+    }
+    std::pair<std::vector<int64_t>, std::vector<int64_t>> coordinates(size_t i) const {
+        const PathView v = path_(i);
+        std::vector<int64_t> ci{v.i0}, cj{v.j0};
+        int64_t x = v.i0, y = v.j0;
+        for (int64_t q = 0; q < v.n; ++q) {
+            const uint32_t t = v.ops[q] >> 30, len = v.ops[q] & 0x3fffffffu;
+            x += t != 2 ? len : 0;
+            y += t != 1 ? len : 0;
+            ci.push_back(x);
+            cj.push_back(y);
+        }
+        if (v.n == 0) {   // an empty path still has its end column
+            ci.push_back(x);
+            cj.push_back(y);
+        }
+        return {std::move(ci), std::move(cj)};
+    }
+    std::string cigar(size_t i, bool extended = false, bool soft_clip = false) const {
+        const PathView v = path_(i);
+        const uint8_t* ca = codes_a(i);
+        const uint8_t* cb = codes_b(i);
+        std::string out;
+        char cur = 0;
+        uint64_t run = 0;
+        auto put = [&](char op, uint64_t cnt) {
+            if (cnt == 0) return;
+            if (op != cur && run) out += std::to_string(run) + cur;
+            if (op != cur) { cur = op; run = 0; }
+            run += cnt;
+        };
+        if (soft_clip) put('S', static_cast<uint64_t>(v.i0));
+        int64_t x = v.i0, y = v.j0;
+        for (int64_t q = 0; q < v.n; ++q) {
+            const uint32_t t = v.ops[q] >> 30, len = v.ops[q] & 0x3fffffffu;
+            if (t == 1) { put('I', len); x += len; }
+            else if (t == 2) { put('D', len); y += len; }
+            else if (!extended) { put('M', len); x += len; y += len; }
+            else for (uint32_t k = 0; k < len; ++k, ++x, ++y) put(ca[x] == cb[y] ? '=' : 'X', 1);
+        }
+        if (soft_clip) put('S', static_cast<uint64_t>(len_a(i) - x));
+        if (run) out += std::to_string(run) + cur;
+        return out;
+    }
+    std::vector<std::string> cigars(bool extended = false, bool soft_clip = false) const {
+        std::vector<std::string> out(size());
+        for (size_t i = 0; i < size(); ++i) out[i] = cigar(i, extended, soft_clip);
+        return out;
     }
     // Enqueue only (no total, no synchronisation): for device-side pipelines.
     void score_and_grad_async() {
@@ -563,7 +711,112 @@ public:
         return t;
     }
 
+    // Streaming align() (batch_engine.hpp:748-1273): the pairs (seqs_a[i], seqs_b[i]) under
+    // `params`, this batch's problem type and settings, nothing kept: every score and the
+    // gradient summed over the pairs (zero for grad_mode "none").  band > 0 (hard batches)
+    // bands every pair around its guide -- `guides` (one per pair, from aligned strings) or
+    // the diagonal round(i * n / m); guides with band 0 band at width 0.
+    BatchResult align(const std::vector<std::string_view>& seqs_a, const std::vector<std::string_view>& seqs_b,
+                       const AlignParams& params, int band, const std::vector<std::vector<int>>& guides,
+                       const uint16_t* tracks, const uint16_t* gap_tracks) const {
+        if (seqs_a.size() != seqs_b.size()) throw std::invalid_argument("nwgrad: align() needs seqs_a and seqs_b of equal length");
+        if (N_ > 0 && &params.alphabet() != alpha_)
+            throw std::invalid_argument("nwgrad: align() params over alphabet \"" + params.alphabet().symbols() +
+                                        "\" in a batch over alphabet \"" + alpha_->symbols() + "\"");
+        if (band < 0) throw std::invalid_argument("nwgrad: band must be >= 0, got " + std::to_string(band));
+        const bool banded = band > 0 || !guides.empty();
+        if (banded && gd_ != GradMode::Hard)
+            throw std::invalid_argument("nwgrad_amd: banded align() runs on hard batches (grad_mode=\"hard\") on the GPU "
+                                        "path");
+        if (!guides.empty() && guides.size() != seqs_a.size())
+            throw std::invalid_argument("nwgrad: aligned_a / aligned_b need one entry per pair");
+        BatchResult r{std::vector<double>(seqs_a.size(), 0.0), AlignParams(params.alphabet(), params.num_sets())};
+        if (seqs_a.empty()) return r;
+        SeqPairBatchT<T> tmp(seqs_a, seqs_b, params, gm_, am_, gd_, n_threads_, device_);
+        if (tracks || gap_tracks) tmp.set_tracks(tracks, gap_tracks);
+        if (gd_ == GradMode::Soft) {
+            tmp.set_soft_temperature(soft_temp_);
+            tmp.set_soft_precision(soft_precision());
+        }
+        if (banded) {
+            std::vector<int32_t> g;
+            for (size_t i = 0; i < seqs_a.size(); ++i) {
+                const int m = static_cast<int>(seqs_a[i].size()), n = static_cast<int>(seqs_b[i].size());
+                if (!guides.empty()) {
+                    if (guides[i].size() != static_cast<size_t>(m) + 1)
+                        throw std::invalid_argument("nwgrad: guide_j has " + std::to_string(guides[i].size()) +
+                                                    " entries, but sequence A of length " + std::to_string(m) +
+                                                    " needs " + std::to_string(m + 1) +
+                                                    " (do the aligned strings spell the input sequences?)");
+                    g.insert(g.end(), guides[i].begin(), guides[i].end());
+                } else {   // the diagonal (aligner.hpp:1781-1790)
+                    for (int k = 0; k <= m; ++k)
+                        g.push_back(m > 0 ? static_cast<int32_t>(std::llround(static_cast<double>(k) * n / m)) : 0);
+                }
+            }
+            tmp.check_(nwg_batch_set_guides(tmp.b_, g.data()));
+            double total = 0.0;
+            tmp.check_(nwg_batch_run_banded(tmp.b_, band, &total));
+            tmp.valid_ = true;
+        } else {
+            tmp.score_and_grad();
+        }
+        r.scores = tmp.scores();
+        if (gd_ != GradMode::None) r.grad = tmp.compute_grad();
+        return r;
+    }
+
 private:
+    // The stored path of pair i: its ops (in path order), their columns, and the start cell (the
+    // end cell less the path's moves, batch_engine.hpp:1441-1449).
+    struct PathView {
+        const uint32_t* ops;
+        int64_t n, cols, i0, j0;
+    };
+    PathView path_(size_t i) const {
+        if (i >= size()) throw std::out_of_range("SeqPairBatch index out of range");
+        if (!guides_) throw std::logic_error("nwgrad: alignment not computed; call align() first");
+        if (!stored_)
+            throw std::logic_error(
+                "nwgrad: stored path unavailable; use align(keep_paths=True) or banded_align(keep_paths=True)");
+        if (gd_ != GradMode::Hard) return gb_->path_(i);
+        if (!paths_cached_) {
+            int64_t cap = oa_.back() + ob_.back();
+            pops_.assign(static_cast<size_t>(std::max<int64_t>(cap, 1)), 0u);
+            pn_.assign(static_cast<size_t>(std::max<int64_t>(N_, 1)), 0);
+            pend_.assign(static_cast<size_t>(std::max<int64_t>(2 * N_, 1)), 0);
+            check_(nwg_batch_paths(b_, pops_.data(), pn_.data(), pend_.data()));
+            paths_cached_ = true;
+        }
+        PathView v{};
+        const int64_t end = oa_[i + 1] + ob_[i + 1];
+        v.n = pn_[i];
+        v.ops = pops_.data() + end - v.n;
+        int64_t x = pend_[2 * i], y = pend_[2 * i + 1], cols = 0;
+        for (int64_t q = 0; q < v.n; ++q) {
+            const uint32_t t = v.ops[q] >> 30, len = v.ops[q] & 0x3fffffffu;
+            x -= t != 2 ? len : 0;
+            y -= t != 1 ? len : 0;
+            cols += len;
+        }
+        v.i0 = x;
+        v.j0 = y;
+        v.cols = cols;
+        return v;
+    }
+    // The guides (and, keep, the paths) of a score-only or soft batch: a hard pass over the same
+    // pairs under the params of the last full pass.
+    void resolve_guides_(bool keep) {
+        if (!gb_) {
+            gb_ = std::make_unique<SeqPairBatchT<T>>(ca_.data(), oa_.data(), cb_.data(), ob_.data(), N_, gparams_, gm_,
+                                                     am_, GradMode::Hard, device_);
+            if (!trk_.empty()) gb_->set_tracks(trk_.data(), gtrk_.data());
+        } else {
+            gb_->set_params(gparams_);
+        }
+        gb_->score_and_grad(keep);
+        gfresh_ = true;
+    }
     std::string decode_(const std::vector<uint8_t>& codes, const std::vector<int64_t>& off, size_t i) const {
         if (i >= static_cast<size_t>(N_)) throw std::out_of_range("SeqPairBatch index out of range");
         const std::string& sym = alpha_->symbols();
@@ -573,6 +826,7 @@ private:
     }
     void create_(const uint8_t* ca, const int64_t* oa, const uint8_t* cb, const int64_t* ob, int64_t N, int device) {
         N_ = N;
+        device_ = device;
         // host copies: the views' sequences and tracks (the reference keeps its codes on the host too)
         oa_.assign(oa, oa + N + 1);
         ob_.assign(ob, ob + N + 1);
@@ -619,6 +873,16 @@ private:
     nwg_ctx* ctx_ = nullptr;
     nwg_batch* b_ = nullptr;
     bool valid_ = false;
+    int device_ = 0;
+    // guides and stored paths: whether the last full pass left guides / kept paths; the host
+    // copy of the paths; for score-only and soft batches the hard batch that holds them and the
+    // params it must run under
+    bool guides_ = false, stored_ = false, gfresh_ = false;
+    mutable bool paths_cached_ = false;
+    mutable std::vector<uint32_t> pops_;
+    mutable std::vector<int32_t> pn_, pend_;
+    std::unique_ptr<SeqPairBatchT<T>> gb_;
+    AlignParams gparams_{Alphabet::get("A")};
     std::string fill_ = "interpair";
     double soft_temp_ = 1.0;
     bool soft_f32_ = true;

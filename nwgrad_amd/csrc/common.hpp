@@ -88,6 +88,80 @@ __device__ __forceinline__ int wave_fetch(int32_t* counter) {
     return __builtin_amdgcn_readfirstlane(old);
 }
 
+// The walk's outputs besides the gradient, written by the one lane that walks a pair:
+//  - the guide (the reference's guide_j: gj[i] = the column after row i of A,
+//    aligner.hpp:3155-3160 affine, :2761-2773 linear): gj[0] = 0, gj[m] = n, then the end cell's
+//    gj[best_i] = best_j; each M move out of (i, j) sets gj[i-1] = j-1, each move in B (X, up)
+//    gj[i-1] = j; a Global / Glocal column-0 run sets its rows to 0.  Rows a Local path does not
+//    cross are interpolated as fill_guide_gaps does (aligner.hpp:2734-2749): between gj[0] and
+//    the start row, and between the end row and gj[m];
+//  - the stored path (keep_paths): run-length ops (type << 30) | length, type 0 both advance,
+//    1 a gap in B (A advances), 2 a gap in A, written backward from the end of the pair's
+//    buffer of m + n entries as the walk meets the moves last first (batch_engine.hpp:2056-2092,
+//    OpsSink), so they read forward in path order.
+struct WalkOut {
+    int32_t* gj = nullptr;    // the pair's m + 1 guide entries (null: no guide)
+    uint32_t* w = nullptr;    // the next op slot, one past it (null: no stored path)
+    uint32_t t = 3u, len = 0u;
+    int low = -1, lowv = 0;   // the lowest row a move's guide entry wrote, and its value
+
+    __device__ __forceinline__ void begin(int32_t* g, uint32_t* ops_end, int m, int n, int bi, int bj) {
+        gj = g;
+        w = ops_end;
+        if (gj) {
+            gj[0] = 0;
+            gj[m] = n;
+            gj[bi] = bj;
+        }
+    }
+    __device__ __forceinline__ void op(uint32_t k, uint32_t cnt) {
+        if (!w) return;
+        if (k == t) {
+            len += cnt;
+            return;
+        }
+        if (len) *--w = (t << 30) | len;
+        t = k;
+        len = cnt;
+    }
+    // a move out of cell (i, j): k 0 both advance, 1 a gap in B, 2 a gap in A
+    __device__ __forceinline__ void move(int k, int i, int j) {
+        if (gj && k != 2) {
+            const int v = k == 0 ? j - 1 : j;
+            gj[i - 1] = v;
+            low = i - 1;
+            lowv = v;
+        }
+        op((uint32_t)k, 1u);
+    }
+    // the Global / Glocal run down column 0 from row i, and along row 0 from column j
+    __device__ __forceinline__ void column0(int i) {
+        if (gj)
+            for (int r = 0; r < i; ++r) gj[r] = 0;
+        if (i > 0) {
+            low = 0;
+            lowv = 0;
+        }
+        op(1u, (uint32_t)i);
+    }
+    __device__ __forceinline__ void row0(int j) { op(2u, (uint32_t)j); }
+    // After the walk: Local's interpolated rows (i0 the row the walk stopped in); the number of
+    // ops written, counted from ops_end.
+    __device__ __forceinline__ void end_local(int i0, int m, int n, int bi, int bj) {
+        if (!gj) return;
+        const int64_t g0 = 0;
+        const int64_t gi0 = (low == i0) ? lowv : (i0 == bi ? bj : 0);
+        for (int k = 1; k < i0; ++k) gj[k] = (int32_t)(g0 + (gi0 - g0) * k / i0);
+        for (int k = bi + 1; k < m; ++k) gj[k] = (int32_t)(bj + (int64_t)(n - bj) * (k - bi) / (m - bi));
+    }
+    __device__ __forceinline__ int32_t finish(const uint32_t* ops_end) {
+        if (!w) return 0;
+        if (len) *--w = (t << 30) | len;
+        len = 0;
+        return (int32_t)(ops_end - w);
+    }
+};
+
 // Wavefront-local memory ordering for LDS handed between lanes of one wave.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");

@@ -62,7 +62,35 @@ struct HardArgs {
     double* grads;            // per pair, K * (na*na + 4) (hard only)
     uint32_t* table;          // pointer-code scratch (hard only)
     int64_t slot_dwords;      // dwords per table slot
+    // hard only, each may be null: guides (pair p's m + 1 entries at off_a[p] + p), stored paths
+    // (pair p's ops end at off_a[p+1] + off_b[p+1], m + n entries of room), their op counts and
+    // end cells (i, j)
+    int32_t* guide;
+    uint32_t* ops;
+    int32_t* ops_n;
+    int32_t* ends;
+    int32_t band;             // the BAND kernels: the half-width around each pair's guide (> 0)
 };
+
+// The guide band of banded re-alignment (the reference's GuideBanded fill, aligner.hpp:2644-2688):
+// row i (1..m) computes columns [band_lo, band_hi]; Global's row m reaches n.  gj is the pair's
+// guide (m + 1 entries, non-decreasing in [0, n]) and bw the band clamped to n.  Column 0 of row
+// i >= 1 is in the band (its Global / Glocal border value, else -inf) when gj[i] <= bw: the rows
+// border_rows() counts, as a guide never decreases; row 0 up to column band_hi(0).
+__device__ __forceinline__ int band_lo(const int32_t* gj, int i, int bw) {
+    const int v = gj[i] - bw;
+    return v > 1 ? v : 1;
+}
+template <int AM>
+__device__ __forceinline__ int band_hi(const int32_t* gj, int i, int m, int n, int bw) {
+    if (AM == kGlobal && i == m) return n;
+    int hi = gj[i] + bw;
+    if (i < m) {
+        const int h2 = gj[i + 1] - 1 + bw;
+        hi = h2 > hi ? h2 : hi;
+    }
+    return hi < n ? hi : n;
+}
 
 // Best-cell candidate, combined across the lanes of a slot: larger value wins, ties go to
 // the smaller (i, j) -- the reference's first-in-row-major-order rule.
@@ -104,10 +132,14 @@ __device__ __forceinline__ Cand<T> reduce_slot(Cand<T> c, int G) {
 // the left neighbour (column j0 of row i), (dM, dX, dY) the diagonal (column j0 of row i-1).
 // cw receives the predecessor codes, four cells to a dword; rowmax the row's maximum over
 // the lane's real columns (Local).  Affine: aligner.hpp:4020-4047; linear: aligner.hpp:3828-3849.
-template <typename T, int GM, int AM, int S, bool HARD>
+// BAND (banded re-alignment): only the strip's cells k in [klo, khi] are in the row's band; the
+// others are -inf in every state, as the reference's banded fill leaves them (aligner.hpp:
+// 5991-6098, band_fill then the row's [jlo(i), jhi(i)]), before any maximum or row best sees them.
+template <typename T, int GM, int AM, int S, bool HARD, bool BAND = false>
 __device__ __forceinline__ void strip_row(T (&M)[S], T (&X)[S], T (&Y)[S], T lM, T lX, T lY, T dM, T dX, T dY,
                                           const T* __restrict__ srow, const int (&coff)[S], int ncols, T go_a,
-                                          T ge_a, T go_b, T ge_b, uint32_t (&cw)[S / 4], T& rowmax) {
+                                          T ge_a, T go_b, T ge_b, uint32_t (&cw)[S / 4], T& rowmax, int klo = 0,
+                                          int khi = S) {
 #pragma unroll
     for (int k = 0; k < S / 4; ++k) cw[k] = 0u;
     rowmax = T(0);
@@ -129,16 +161,22 @@ __device__ __forceinline__ void strip_row(T (&M)[S], T (&X)[S], T (&Y)[S], T lM,
             if constexpr (AM == kLocal) {
                 mv = dmax(mv, T(0));
                 if constexpr (HARD) km = (mv <= T(0)) ? 3u : km;
-                if (k < ncols) rowmax = dmax(rowmax, dmax(mv, dmax(xv, yv)));
+            }
+            T mo = mv, xo = xv, yo = yv;
+            if constexpr (BAND) {
+                if (k < klo || k > khi) { mo = ninf<T>(); xo = ninf<T>(); yo = ninf<T>(); }
+            }
+            if constexpr (AM == kLocal) {
+                if (k < ncols) rowmax = dmax(rowmax, dmax(mo, dmax(xo, yo)));
             }
             if constexpr (HARD) {
                 const uint32_t kx = (ux >= tx) ? 0u : ((vx >= wx) ? 1u : 2u);
                 const uint32_t ky = (uy >= ty) ? 0u : ((vy >= wy) ? 1u : 2u);
                 cw[k >> 2] |= (km | (kx << 2) | (ky << 4)) << (8 * (k & 3));
             }
-            M[k] = mv; X[k] = xv; Y[k] = yv;
+            M[k] = mo; X[k] = xo; Y[k] = yo;
             dM = uM; dX = uX; dY = uY;
-            lM = mv; lX = xv; lY = yv;
+            lM = mo; lX = xo; lY = yo;
         } else {
             const T uH = M[k];
             const T d0 = dM + s;
@@ -151,6 +189,11 @@ __device__ __forceinline__ void strip_row(T (&M)[S], T (&X)[S], T (&Y)[S], T lM,
             if constexpr (AM == kLocal) {
                 v = dmax(v, T(0));
                 if constexpr (HARD) c = (v <= T(0)) ? 3u : c;
+            }
+            if constexpr (BAND) {
+                if (k < klo || k > khi) v = ninf<T>();
+            }
+            if constexpr (AM == kLocal) {
                 if (k < ncols) rowmax = dmax(rowmax, v);
             }
             if constexpr (HARD) cw[k >> 2] |= c << (8 * (k & 3));
@@ -243,7 +286,7 @@ __device__ __forceinline__ T col0_border(const TrackArgs<T>& tr, int64_t r0, int
     else return (-(T)i) * ge_b;
 }
 
-template <typename T, int GM, int AM, int S, bool HARD, bool TR>
+template <typename T, int GM, int AM, int S, bool HARD, bool TR, bool BAND>
 __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint16_t* __restrict__ hist,
                           int tile, int64_t slot, int lane) {
     static_assert(S % 4 == 0, "S is a multiple of 4");
@@ -270,6 +313,16 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
     const int j0 = g * S;
     const int ncols = n - j0 < 0 ? 0 : (n - j0 > S ? S : n - j0);
     T go_a = A.go_a, ge_a = A.ge_a, go_b = A.go_b, ge_b = A.ge_b;
+    // banded: the pair's guide from the previous pass, the band clamped to n, row 0's last column
+    const int32_t* bgj = nullptr;
+    int bw = 0, bj0 = 0;
+    if constexpr (BAND) {
+        if (p >= 0) {
+            bgj = A.guide + A.off_a[p] + p;
+            bw = A.band < n ? A.band : n;
+            bj0 = band_hi<AM>(bgj, 0, m, n, bw);
+        }
+    }
     const T* __restrict__ gT = ssub + knn;   // tracked: the K x 4 gap table in LDS
     if constexpr (TR) {   // row 0's gaps in A: set gtrk of row 0
         if (p >= 0) {
@@ -295,6 +348,9 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
             M[k] = (AM == kGlobal) ? (-jt) * ge_a : T(0);
             X[k] = NI; Y[k] = NI;
         }
+        if constexpr (BAND && AM == kGlobal) {   // row 0's border runs up to column band_hi(0)
+            if (j0 + k + 1 > bj0) { if (GM == kAffine) Y[k] = NI; else M[k] = NI; }
+        }
     }
     // The diagonal of row 1's first cell: row 0 at column j0.
     T pM, pX, pY;
@@ -303,9 +359,11 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
         if constexpr (GM == kAffine) {
             if (AM == kGlobal) {
                 pM = (j0 == 0) ? T(0) : NI; pX = NI; pY = (j0 == 0) ? NI : -(go_a + jt * ge_a);
+                if (BAND && j0 > bj0) pY = NI;
             } else { pM = T(0); pX = NI; pY = NI; }
         } else {
             pM = (AM == kGlobal) ? (-jt) * ge_a : T(0); pX = NI; pY = NI;
+            if (BAND && AM == kGlobal && j0 > bj0) pM = NI;
         }
     }
     T oM = M[S - 1], oX = X[S - 1], oY = Y[S - 1];
@@ -335,6 +393,9 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
             T border = NI;
             if constexpr (AM != kLocal) {
                 if (!TR || (p >= 0 && i >= 1 && i <= m)) border = col0_border<T, GM, TR>(A.tr, r0, i, go_b, ge_b);
+                if constexpr (BAND) {   // past border_rows(): outside the band
+                    if (!(p >= 0 && i >= 1 && i <= m && bgj[i] <= bw)) border = NI;
+                }
             }
             if constexpr (GM == kAffine) {
                 if (AM == kLocal) { iM = T(0); iX = NI; }
@@ -348,8 +409,13 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
             const T* __restrict__ srow = ssub + ((TR ? rs * na : 0) + (int)ap[i - 1]) * na;
             uint32_t cw[S / 4];
             T rowmax;
-            strip_row<T, GM, AM, S, HARD>(M, X, Y, iM, iX, iY, pM, pX, pY, srow, coff, ncols, go_a, ge_a, go_b, ge_b,
-                                          cw, rowmax);
+            int klo = 0, khi = S;
+            if constexpr (BAND) {
+                klo = band_lo(bgj, i, bw) - j0 - 1;
+                khi = band_hi<AM>(bgj, i, m, n, bw) - j0 - 1;
+            }
+            strip_row<T, GM, AM, S, HARD, BAND>(M, X, Y, iM, iX, iY, pM, pX, pY, srow, coff, ncols, go_a, ge_a, go_b,
+                                                ge_b, cw, rowmax, klo, khi);
             oM = M[S - 1];
             if constexpr (GM == kAffine) { oX = X[S - 1]; oY = Y[S - 1]; }
             if constexpr (HARD) {
@@ -365,7 +431,10 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
     Cand<T> c;
     c.v = T(0); c.i = -1; c.j = 0; c.st = 0;
     if (p >= 0) {
-        const T col0v = (m == 0 || AM == kLocal) ? T(0) : col0_border<T, GM, TR>(A.tr, r0, m, A.go_b, A.ge_b);
+        T col0v = (m == 0 || AM == kLocal) ? T(0) : col0_border<T, GM, TR>(A.tr, r0, m, A.go_b, A.ge_b);
+        if constexpr (BAND) {
+            if (AM != kLocal && m > 0 && bgj[m] > bw) col0v = NI;
+        }
         c = end_candidate<T, GM, AM, S>(M, X, Y, best, m, n, j0, ncols, g == 0, col0v);
     }
     c = reduce_slot(c, G);
@@ -382,6 +451,9 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
             uint16_t* h = hist + q * knn;
             int i = c.i, j = c.j, st = c.st;
             const uint32_t* tr = tb + q * G;
+            uint32_t* const oend = A.ops ? A.ops + A.off_a[p + 1] + A.off_b[p + 1] : nullptr;
+            WalkOut wo;
+            wo.begin(A.guide ? A.guide + A.off_a[p] + p : nullptr, oend, m, n, c.i, c.j);
             // count a gap move: f 0 go_a, 1 ge_a (set gtrk of row i), 2 go_b, 3 ge_b (set trk)
             auto gap = [&](int f, int row, int v) {
                 if constexpr (TR) {
@@ -402,6 +474,8 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
                         if (AM == kGlocal) break;
                         gap(1, 0, j);
                         if (GM == kAffine) gap(0, 0, 1);
+                        wo.row0(j);
+                        j = 0;
                         break;
                     }
                     if constexpr (TR) {
@@ -410,6 +484,8 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
                         gap(3, i, i);
                     }
                     if (GM == kAffine) gap(2, 1, 1);
+                    wo.column0(i);
+                    i = 0;
                     break;
                 }
                 const int jj = j - 1;
@@ -422,6 +498,7 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
                         if (AM == kLocal && (cc & 3u) == 3u) break;
                         const uint32_t kk = cc & 3u;
                         h[mrow] += 1;
+                        wo.move(0, i, j);
                         st = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
                         --i; --j;
                     } else if (st == 1) {
@@ -429,6 +506,7 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
                         const int pv = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
                         gap(3, i, 1);
                         if (pv != 1) gap(2, i, 1);
+                        wo.move(1, i, j);
                         st = pv;
                         --i;
                     } else {
@@ -436,15 +514,22 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
                         const int pv = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
                         gap(1, i, 1);
                         if (pv != 2) gap(0, i, 1);
+                        wo.move(2, i, j);
                         st = pv;
                         --j;
                     }
                 } else {
                     if (cc == 3u) break;
-                    if (cc == 0u) { h[mrow] += 1; --i; --j; }
-                    else if (cc == 1u) { gap(3, i, 1); --i; }
-                    else { gap(1, i, 1); --j; }
+                    if (cc == 0u) { h[mrow] += 1; wo.move(0, i, j); --i; --j; }
+                    else if (cc == 1u) { gap(3, i, 1); wo.move(1, i, j); --i; }
+                    else { gap(1, i, 1); wo.move(2, i, j); --j; }
                 }
+            }
+            if constexpr (AM == kLocal) wo.end_local(i, m, n, c.i, c.j);
+            if (oend) {
+                A.ops_n[p] = wo.finish(oend);
+                A.ends[2 * p] = c.i;
+                A.ends[2 * p + 1] = c.j;
             }
         }
     }
@@ -495,7 +580,7 @@ __host__ __device__ constexpr int hard_lds_hist_bytes(int na, int K, bool tr, in
 //    queue is drained (every wave reaches the exit test once the counter passes n_tiles),
 //    for batches whose codes would not fit a slot per tile.
 // The slot histograms start at zero and each tile's flush clears what it used.
-template <typename T, int GM, int AM, int S, bool HARD, bool TR>
+template <typename T, int GM, int AM, int S, bool HARD, bool TR, bool BAND = false>
 __global__ __launch_bounds__(64) void hard_fill_kernel(HardArgs<T> A) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int na = A.na, nn = na * na;
@@ -513,13 +598,13 @@ __global__ __launch_bounds__(64) void hard_fill_kernel(HardArgs<T> A) {
     __syncthreads();
     if (A.queue == nullptr) {
         const int tile = A.tile_base + (int)blockIdx.x;
-        if (tile < A.n_tiles) hard_tile<T, GM, AM, S, HARD, TR>(A, ssub, hist, tile, (int64_t)blockIdx.x, lane);
+        if (tile < A.n_tiles) hard_tile<T, GM, AM, S, HARD, TR, BAND>(A, ssub, hist, tile, (int64_t)blockIdx.x, lane);
         return;
     }
     for (;;) {
         const int tile = wave_fetch(A.queue);
         if (tile >= A.n_tiles) break;
-        hard_tile<T, GM, AM, S, HARD, TR>(A, ssub, hist, tile, (int64_t)blockIdx.x, lane);
+        hard_tile<T, GM, AM, S, HARD, TR, BAND>(A, ssub, hist, tile, (int64_t)blockIdx.x, lane);
     }
 }
 

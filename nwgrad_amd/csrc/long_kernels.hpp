@@ -84,9 +84,15 @@ struct LongArgs {
     double* grads;
     int32_t* err;         // bit 0: a wait passed its bound (or the walk met an invalid end cell)
     uint64_t wait_ticks;  // bound of one wait, in s_memrealtime ticks
+    // hard only, each may be null: guides, stored paths, op counts and end cells, as HardArgs
+    int32_t* guide;
+    uint32_t* ops;
+    int32_t* ops_n;
+    int32_t* ends;
+    int32_t band;         // the BAND kernels: the half-width around each pair's guide (> 0)
 };
 
-template <typename T, int GM, int AM, int S, bool HARD, int RB, bool TR>
+template <typename T, int GM, int AM, int S, bool HARD, int RB, bool TR, bool BAND>
 __device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int it, int lane) {
     static_assert(RB >= 1 && RB <= kWave, "a row block fits the lanes of a wave");
     const PanelItem item = A.items[it];
@@ -109,6 +115,14 @@ __device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int
     }
     const T NI = ninf<T>();
     const bool first = item.bnd_in < 0, last = item.bnd_out < 0;
+    // banded: the pair's guide, the band clamped to n, row 0's last column (as in hard_tile)
+    const int32_t* bgj = nullptr;
+    int bw = 0, bj0 = 0;
+    if constexpr (BAND) {
+        bgj = A.guide + a0 + p;
+        bw = A.band < n ? A.band : n;
+        bj0 = band_hi<AM>(bgj, 0, m, n, bw);
+    }
 
     int coff[S];
 #pragma unroll
@@ -126,6 +140,9 @@ __device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int
             M[k] = (AM == kGlobal) ? (-jt) * ge_a : T(0);
             X[k] = NI; Y[k] = NI;
         }
+        if constexpr (BAND && AM == kGlobal) {
+            if (j0 + k + 1 > bj0) { if (GM == kAffine) Y[k] = NI; else M[k] = NI; }
+        }
     }
     T pM, pX, pY;
     {
@@ -133,9 +150,11 @@ __device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int
         if constexpr (GM == kAffine) {
             if (AM == kGlobal) {
                 pM = (j0 == 0) ? T(0) : NI; pX = NI; pY = (j0 == 0) ? NI : -(go_a + jt * ge_a);
+                if (BAND && j0 > bj0) pY = NI;
             } else { pM = T(0); pX = NI; pY = NI; }
         } else {
             pM = (AM == kGlobal) ? (-jt) * ge_a : T(0); pX = NI; pY = NI;
+            if (BAND && AM == kGlobal && j0 > bj0) pM = NI;
         }
     }
     T oM = M[S - 1], oX = X[S - 1], oY = Y[S - 1];
@@ -202,6 +221,9 @@ __device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int
             T border = NI;
             if constexpr (AM != kLocal) {
                 if (!TR || (i >= 1 && i <= m)) border = col0_border<T, GM, TR>(A.tr, r0, i, go_b, ge_b);
+                if constexpr (BAND) {
+                    if (!(i >= 1 && i <= m && bgj[i] <= bw)) border = NI;
+                }
             }
             if constexpr (GM == kAffine) {
                 if (AM == kLocal) { iM = T(0); iX = NI; }
@@ -216,8 +238,13 @@ __device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int
             const T* __restrict__ srow = ssub + ((TR ? rs * na : 0) + (int)ap[i - 1]) * na;
             uint32_t cw[S / 4];
             T rowmax;
-            strip_row<T, GM, AM, S, HARD>(M, X, Y, iM, iX, iY, pM, pX, pY, srow, coff, ncols, go_a, ge_a, go_b, ge_b,
-                                          cw, rowmax);
+            int klo = 0, khi = S;
+            if constexpr (BAND) {
+                klo = band_lo(bgj, i, bw) - j0 - 1;
+                khi = band_hi<AM>(bgj, i, m, n, bw) - j0 - 1;
+            }
+            strip_row<T, GM, AM, S, HARD, BAND>(M, X, Y, iM, iX, iY, pM, pX, pY, srow, coff, ncols, go_a, ge_a, go_b,
+                                                ge_b, cw, rowmax, klo, khi);
             oM = M[S - 1];
             if constexpr (GM == kAffine) { oX = X[S - 1]; oY = Y[S - 1]; }
             if constexpr (HARD) {
@@ -236,7 +263,10 @@ __device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int
         pM = iM; pX = iX; pY = iY;
     }
 
-    const T col0v = (m == 0 || AM == kLocal) ? T(0) : col0_border<T, GM, TR>(A.tr, r0, m, A.go_b, A.ge_b);
+    T col0v = (m == 0 || AM == kLocal) ? T(0) : col0_border<T, GM, TR>(A.tr, r0, m, A.go_b, A.ge_b);
+    if constexpr (BAND) {
+        if (AM != kLocal && m > 0 && bgj[m] > bw) col0v = NI;
+    }
     Cand<T> c = end_candidate<T, GM, AM, S>(M, X, Y, best, m, n, j0, ncols, item.panel == 0 && g == 0, col0v);
     c = reduce_slot(c, kWave);
     if (lane == 0) {
@@ -248,7 +278,7 @@ __device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int
 
 // One wavefront per block, pulling (pair, panel) items until the queue is drained; every
 // wave reaches the exit test once the counter passes n_items.
-template <typename T, int GM, int AM, int S, bool HARD, int RB, bool TR>
+template <typename T, int GM, int AM, int S, bool HARD, int RB, bool TR, bool BAND = false>
 __global__ __launch_bounds__(64) void long_fill_kernel(LongArgs<T> A) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int nn = A.na * A.na, K = TR ? A.tr.K : 1;
@@ -261,7 +291,7 @@ __global__ __launch_bounds__(64) void long_fill_kernel(LongArgs<T> A) {
     for (;;) {
         const int it = wave_fetch(A.queue);
         if (it >= A.n_items) break;
-        long_panel<T, GM, AM, S, HARD, RB, TR>(A, ssub, it, lane);
+        long_panel<T, GM, AM, S, HARD, RB, TR, BAND>(A, ssub, it, lane);
     }
 }
 
@@ -416,6 +446,11 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
             if (tid == 0) __hip_atomic_fetch_or(A.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             i = 0; j = 0;
         }
+        // thread 0 writes the guide and the stored path (every thread walks the same moves)
+        uint32_t* const oend = (A.ops && tid == 0) ? A.ops + A.off_a[p + 1] + A.off_b[p + 1] : nullptr;
+        WalkOut wo;
+        const int bi = i, bj = j;
+        wo.begin((A.guide && tid == 0) ? A.guide + a0 + p : nullptr, oend, m, n, bi, bj);
         constexpr int WS = walk_steps<S>();   // table steps of the window
         uint8_t* sb = reinterpret_cast<uint8_t*>(win + WS * SD);   // B codes of the window's panel
         uint8_t* sa = sb + PW;                                               // A codes of rows alo ..
@@ -440,6 +475,8 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
                     if (AM == kGlocal) break;
                     gap(1, 0, j);
                     if (GM == kAffine) gap(0, 0, 1);
+                    wo.row0(j);
+                    j = 0;
                     break;
                 }
                 if constexpr (TR) {
@@ -448,6 +485,8 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
                     gap(3, i, i);
                 }
                 if (GM == kAffine) gap(2, 1, 1);
+                wo.column0(i);
+                i = 0;
                 break;
             }
             const int jj = j - 1;
@@ -502,6 +541,7 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
                     const uint32_t kk = cc & 3u;
                     if (tid == 0)
                         __hip_atomic_fetch_add(&hist[mrow], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    wo.move(0, i, j);
                     st = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
                     --i; --j;
                 } else if (st == 1) {
@@ -509,6 +549,7 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
                     const int pv = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
                     gapw(3);
                     if (pv != 1) gapw(2);
+                    wo.move(1, i, j);
                     st = pv;
                     --i;
                 } else {
@@ -516,6 +557,7 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
                     const int pv = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
                     gapw(1);
                     if (pv != 2) gapw(0);
+                    wo.move(2, i, j);
                     st = pv;
                     --j;
                 }
@@ -524,10 +566,17 @@ __global__ __launch_bounds__(kWalkThreads) void long_walk_kernel(LongArgs<T> A) 
                 if (cc == 0u) {
                     if (tid == 0)
                         __hip_atomic_fetch_add(&hist[mrow], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    wo.move(0, i, j);
                     --i; --j;
-                } else if (cc == 1u) { gapw(3); --i; }
-                else { gapw(1); --j; }
+                } else if (cc == 1u) { gapw(3); wo.move(1, i, j); --i; }
+                else { gapw(1); wo.move(2, i, j); --j; }
             }
+        }
+        if constexpr (AM == kLocal) wo.end_local(i, m, n, bi, bj);
+        if (oend) {
+            A.ops_n[p] = wo.finish(oend);
+            A.ends[2 * p] = bi;
+            A.ends[2 * p + 1] = bj;
         }
         __syncthreads();
         double* gr = A.grads + (int64_t)p * gs;

@@ -232,6 +232,15 @@ struct nwg_batch {
     int64_t table_budget = kHardTableBudget;                 // short-pair pointer codes of one fill
     int64_t hard_grid = 0;                                   // the last hard fill's persistent grid (0: plain)
     int soft_occ_cache[4] = {0, 0, 0, 0};   // resident blocks per CU: float, double; untracked, tracked
+    // hard: the guides of the last pass (pair p's m + 1 entries at off_a[p] + p), and its stored
+    // paths when kept (pair p's ops end at off_a[p+1] + off_b[p+1]; op counts; end cells)
+    DevPtr<int32_t> guide;
+    DevPtr<uint32_t> ops;
+    DevPtr<int32_t> ops_n, ends;
+    bool keep_paths = false;     // the next passes store their paths
+    bool guides_valid = false, paths_valid = false;
+    bool banded = false;         // while a banded pass is launched: its half-width
+    int band = 0;
     DevPtr<uint8_t> soft_ok;
     std::vector<uint8_t> host_ok;
     int64_t soft_reruns = 0;
@@ -842,6 +851,12 @@ int load_pairs(nwg_batch* b, int64_t N, const uint8_t* codes_a, const int64_t* o
     NWG_HIP(ctx, size(b->scores, (size_t)std::max<int64_t>(N, 1)));
     if (b->prob.grad_mode != 0)
         NWG_HIP(ctx, size(b->grads, (size_t)std::max<int64_t>(N, 1) * (size_t)b->K * (nn + 4)));
+    if (b->prob.grad_mode == 1) NWG_HIP(ctx, size(b->guide, (size_t)std::max<int64_t>(la + N, 1)));
+    if (b->prob.grad_mode == 1 && b->keep_paths) {
+        NWG_HIP(ctx, size(b->ops, (size_t)std::max<int64_t>(la + lb, 1)));
+        NWG_HIP(ctx, size(b->ops_n, (size_t)std::max<int64_t>(N, 1)));
+        NWG_HIP(ctx, size(b->ends, (size_t)std::max<int64_t>(2 * N, 1)));
+    }
     if (la) NWG_HIP(ctx, hipMemcpyAsync(b->ca.p, codes_a, (size_t)la, hipMemcpyHostToDevice, ctx->stream));
     if (lb) NWG_HIP(ctx, hipMemcpyAsync(b->cb.p, codes_b, (size_t)lb, hipMemcpyHostToDevice, ctx->stream));
     const int64_t zero[1] = {0};
@@ -869,6 +884,7 @@ int load_pairs(nwg_batch* b, int64_t N, const uint8_t* codes_a, const int64_t* o
         if (rc) return rc;
     }
     b->results_valid = false;
+    b->guides_valid = b->paths_valid = false;
     b->soft_reruns = 0;
     for (SoftPlan* sp : {&b->sp_main, &b->sp_wide, &b->sq_main, &b->sq_wide}) sp->built = false;
     if (b->prob.grad_mode != 2) {
@@ -1037,6 +1053,7 @@ int nwg_batch_set_params_k(nwg_batch* b, int K, const double* matrices, const do
     }
     b->have_params = true;
     b->results_valid = false;
+    b->paths_valid = false;   // guides stay: a banded pass re-aligns around them
     return NWG_OK;
 }
 
@@ -1126,10 +1143,15 @@ nwg::TrackArgs<T> track_args(nwg_batch* b) {
 
 // The hard / score-only fill kernel of a problem type and lane width (kS or kSN columns).
 template <typename T>
-void (*hard_kernel_for(int gm, int am, bool hard, bool tr, int S))(nwg::HardArgs<T>) {
+void (*hard_kernel_for(int gm, int am, bool hard, bool tr, int S, bool band = false))(nwg::HardArgs<T>) {
     auto pick = [&](auto gmc, auto amc, auto hardc, auto trc) -> void (*)(nwg::HardArgs<T>) {
         constexpr int G = decltype(gmc)::value, AM = decltype(amc)::value;
         constexpr bool H = decltype(hardc)::value, TRC = decltype(trc)::value;
+        if constexpr (H) {   // banded passes are hard passes
+            if (band)
+                return S == kSN ? nwg::hard_fill_kernel<T, G, AM, kSN, H, TRC, true>
+                                : nwg::hard_fill_kernel<T, G, AM, kS, H, TRC, true>;
+        }
         return S == kSN ? nwg::hard_fill_kernel<T, G, AM, kSN, H, TRC> : nwg::hard_fill_kernel<T, G, AM, kS, H, TRC>;
     };
     using I0 = std::integral_constant<int, 0>;
@@ -1156,7 +1178,7 @@ int launch_hard(nwg_batch* b, const Plan& pl) {
     if (lds > 65536)
         return fail(ctx, NWG_EINVAL, "nwgrad_amd: %d parameter sets of %d x %d do not fit the fill's local memory",
                     b->K, b->na, b->na);
-    void (*kern)(nwg::HardArgs<T>) = hard_kernel_for<T>(b->prob.gap_model, b->prob.mode, hard, tr, pl.S);
+    void (*kern)(nwg::HardArgs<T>) = hard_kernel_for<T>(b->prob.gap_model, b->prob.mode, hard, tr, pl.S, b->banded);
     if (!kern) return fail(ctx, NWG_EINVAL, "nwgrad_hip: unsupported problem type");
     // fill 0: one plain launch over the persistent order when a slot per tile fits the table
     // budget, else one persistent launch with as many slots (waves) as fit it, at most the
@@ -1192,6 +1214,15 @@ int launch_hard(nwg_batch* b, const Plan& pl) {
     A.scores = b->scores.p;
     A.grads = b->grads.p;
     A.table = b->table.p;
+    if (hard) {
+        A.guide = b->guide.p;
+        A.band = b->band;
+        if (b->keep_paths) {
+            A.ops = b->ops.p;
+            A.ops_n = b->ops_n.p;
+            A.ends = b->ends.p;
+        }
+    }
     b->hard_grid = grid;
     if (b->fill == 0) {
         A.tiles = pl.d_ptiles.p;   // most row steps first
@@ -1232,6 +1263,7 @@ int launch_long(nwg_batch* b) {
     const int gm = b->prob.gap_model, am = b->prob.mode;
     using FillK = void (*)(nwg::LongArgs<T>);
     FillK fill = nullptr, walk = nullptr;
+    const bool band = b->banded;
     auto pick = [&](auto gmc, auto amc, auto hardc) {
         constexpr int G = decltype(gmc)::value, AM = decltype(amc)::value;
         constexpr bool H = decltype(hardc)::value;
@@ -1241,6 +1273,11 @@ int launch_long(nwg_batch* b) {
         } else {
             fill = nwg::long_fill_kernel<T, G, AM, kLS, H, kRowBlock, false>;
             walk = nwg::long_walk_kernel<T, G, AM, kLS, H, false>;
+        }
+        if constexpr (H) {   // banded passes are hard passes
+            if (band)
+                fill = tr ? nwg::long_fill_kernel<T, G, AM, kLS, H, kRowBlock, true, true>
+                          : nwg::long_fill_kernel<T, G, AM, kLS, H, kRowBlock, false, true>;
         }
     };
     using I0 = std::integral_constant<int, 0>;
@@ -1297,6 +1334,15 @@ int launch_long(nwg_batch* b) {
     A.err = b->lerr.p;
     A.wait_ticks = kWaitTicks;
     A.queue = b->queue.p + 1;
+    if (hard) {
+        A.guide = b->guide.p;
+        A.band = b->band;
+        if (b->keep_paths) {
+            A.ops = b->ops.p;
+            A.ops_n = b->ops_n.p;
+            A.ends = b->ends.p;
+        }
+    }
     const int64_t resident = resident_blocks(ctx, fill, lds_fill, kBlocksPerCU);
     for (const LongGroup& gr : L.groups) {
         const int items = gr.item_end - gr.item_begin, pairs = gr.pair_end - gr.pair_begin;
@@ -1328,6 +1374,7 @@ int check_long_err(nwg_batch* b) {
     if (e == 0) return NWG_OK;
     NWG_HIP(ctx, hipMemsetAsync(b->lerr.p, 0, sizeof(int32_t), ctx->stream));
     b->results_valid = false;
+    b->guides_valid = b->paths_valid = false;
     return fail(ctx, NWG_EDEVICE,
                 "nwgrad_hip: the long-pair fill failed (error word %d: 1 = a column panel waited past its bound for "
                 "its left neighbour, 2 = a pair had no valid end cell); the results of this call are not valid",
@@ -1801,6 +1848,10 @@ int run_async(nwg_batch* b, bool record) {
     }
     if (timed) NWG_HIP(ctx, hipEventRecord(b->ev1, ctx->stream));
     b->results_valid = true;
+    if (b->prob.grad_mode == 1) {
+        b->guides_valid = true;
+        b->paths_valid = b->keep_paths;
+    }
     return NWG_OK;
 }
 
@@ -2040,6 +2091,124 @@ int nwg_batch_launch_info(nwg_batch* b, int64_t* grid, int64_t* wide_slots, int6
 int nwg_batch_soft_reruns(nwg_batch* b, int64_t* n) {
     if (!b || !n) return NWG_EINVAL;
     *n = b->soft_reruns;
+    return NWG_OK;
+}
+
+int nwg_batch_set_keep_paths(nwg_batch* b, int keep) {
+    if (!b) return NWG_EINVAL;
+    nwg_ctx* ctx = b->ctx;
+    if (keep && b->prob.grad_mode != 1)
+        return fail(ctx, NWG_EINVAL, "nwgrad_hip: stored paths come from hard passes (grad_mode=\"hard\")");
+    if (keep && b->ops.n == 0) {
+        int64_t cap = 0;
+        for (int64_t i = 0; i < b->N; ++i) cap += (int64_t)b->mlen[(size_t)i] + b->nlen[(size_t)i];
+        NWG_HIP(ctx, hipSetDevice(ctx->device));
+        NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        NWG_HIP(ctx, b->ops.alloc((size_t)std::max<int64_t>(cap, 1)));
+        NWG_HIP(ctx, b->ops_n.alloc((size_t)std::max<int64_t>(b->N, 1)));
+        NWG_HIP(ctx, b->ends.alloc((size_t)std::max<int64_t>(2 * b->N, 1)));
+    }
+    b->keep_paths = keep != 0;
+    return NWG_OK;
+}
+
+int nwg_batch_drop_paths(nwg_batch* b) {
+    if (!b) return NWG_EINVAL;
+    NWG_HIP(b->ctx, hipSetDevice(b->ctx->device));
+    NWG_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+    b->ops.reset();
+    b->ops_n.reset();
+    b->ends.reset();
+    b->keep_paths = false;
+    b->paths_valid = false;
+    return NWG_OK;
+}
+
+int nwg_batch_guides(nwg_batch* b, int32_t* out) {
+    if (!b || !out) return NWG_EINVAL;
+    nwg_ctx* ctx = b->ctx;
+    if (b->prob.grad_mode != 1)
+        return fail(ctx, NWG_EINVAL, "nwgrad_hip: guides come from hard passes (grad_mode=\"hard\")");
+    if (!b->guides_valid) return fail(ctx, NWG_ESTATE, "nwgrad: alignment not computed; call align() first");
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t n = b->N ? b->hoa[(size_t)b->N] + b->N : 0;
+    if (n) NWG_HIP(ctx, hipMemcpyAsync(out, b->guide.p, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return check_long_err(b);
+}
+
+int nwg_batch_pair_guide(nwg_batch* b, int64_t i, int32_t* out) {
+    if (!b || !out) return NWG_EINVAL;
+    nwg_ctx* ctx = b->ctx;
+    if (i < 0 || i >= b->N)
+        return fail(ctx, NWG_EINDEX, "nwgrad: pair index %lld out of range for %lld pairs", (long long)i, (long long)b->N);
+    if (b->prob.grad_mode != 1)
+        return fail(ctx, NWG_EINVAL, "nwgrad_hip: guides come from hard passes (grad_mode=\"hard\")");
+    if (!b->guides_valid) return fail(ctx, NWG_ESTATE, "nwgrad: alignment not computed; call align() first");
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    NWG_HIP(ctx, hipMemcpyAsync(out, b->guide.p + b->hoa[(size_t)i] + i, sizeof(int32_t) * (size_t)(b->mlen[(size_t)i] + 1),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return check_long_err(b);
+}
+
+int nwg_batch_paths(nwg_batch* b, uint32_t* ops, int32_t* n_ops, int32_t* ends) {
+    if (!b || !ops || !n_ops || !ends) return NWG_EINVAL;
+    nwg_ctx* ctx = b->ctx;
+    if (!b->paths_valid)
+        return fail(ctx, NWG_ESTATE,
+                    "nwgrad: stored path unavailable; use align(keep_paths=True) or banded_align(keep_paths=True)");
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    int64_t cap = 0;
+    for (int64_t i = 0; i < b->N; ++i) cap += (int64_t)b->mlen[(size_t)i] + b->nlen[(size_t)i];
+    if (cap) NWG_HIP(ctx, hipMemcpyAsync(ops, b->ops.p, sizeof(uint32_t) * (size_t)cap, hipMemcpyDeviceToHost, ctx->stream));
+    if (b->N) {
+        NWG_HIP(ctx, hipMemcpyAsync(n_ops, b->ops_n.p, sizeof(int32_t) * (size_t)b->N, hipMemcpyDeviceToHost, ctx->stream));
+        NWG_HIP(ctx, hipMemcpyAsync(ends, b->ends.p, sizeof(int32_t) * 2 * (size_t)b->N, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    }
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return check_long_err(b);
+}
+
+int nwg_batch_run_banded(nwg_batch* b, int bandwidth, double* total_out) {
+    if (!b) return NWG_EINVAL;
+    nwg_ctx* ctx = b->ctx;
+    if (b->prob.grad_mode != 1)
+        return fail(ctx, NWG_EINVAL, "nwgrad_amd: banded_grad() runs on hard batches (grad_mode=\"hard\") on the GPU path");
+    if (bandwidth < 0) return fail(ctx, NWG_EINVAL, "nwgrad: band must be >= 0, got %d", bandwidth);
+    if (!b->guides_valid)
+        return fail(ctx, NWG_ESTATE, "nwgrad: banded_grad() needs a guide path; run score_and_grad() first");
+    b->banded = true;
+    b->band = bandwidth;
+    const int rc = nwg_batch_run(b, total_out);
+    b->banded = false;
+    b->band = 0;
+    return rc;
+}
+
+int nwg_batch_set_guides(nwg_batch* b, const int32_t* guides) {
+    if (!b || !guides) return NWG_EINVAL;
+    nwg_ctx* ctx = b->ctx;
+    if (b->prob.grad_mode != 1)
+        return fail(ctx, NWG_EINVAL, "nwgrad_hip: guides belong to hard batches (grad_mode=\"hard\")");
+    for (int64_t p = 0; p < b->N; ++p) {   // the reference's validate_guide (aligner.hpp:2622-2642)
+        const int32_t* g = guides + b->hoa[(size_t)p] + p;
+        const int m = b->mlen[(size_t)p], n = b->nlen[(size_t)p];
+        for (int i = 0; i <= m; ++i) {
+            if (g[i] < 0 || g[i] > n)
+                return fail(ctx, NWG_EINVAL, "nwgrad: guide_j[%d] = %d is outside [0, %d] for sequence B of length %d", i,
+                            (int)g[i], n, n);
+            if (i > 0 && g[i] < g[i - 1])
+                return fail(ctx, NWG_EINVAL, "nwgrad: guide_j decreases at row %d (%d -> %d)", i, (int)g[i - 1], (int)g[i]);
+        }
+    }
+    NWG_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t n = b->N ? b->hoa[(size_t)b->N] + b->N : 0;
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (n) NWG_HIP(ctx, hipMemcpyAsync(b->guide.p, guides, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    b->guides_valid = true;
     return NWG_OK;
 }
 

@@ -304,6 +304,42 @@ void bind_view(py::module_& m, const char* name) {
         .def_property_readonly("mode", [](const PV& v) { return am_name(v.b().align_mode()); })
         .def_property_readonly("grad_mode", [](const PV& v) { return gd_name(v.b().grad_mode()); })
         .def_property_readonly("index", [](const PV& v) { return v.i; })
+        .def_property_readonly("guide_j", [](const PV& v) -> py::object {
+            if (!v.b().path_valid()) return py::none();
+            return py::cast(v.b().guide_j(v.i));
+        }, "Current alignment as a guide_j vector (length m+1), or None.")
+        .def_property_readonly("path_valid", [](const PV& v) { return v.b().path_valid(); })
+        .def_property_readonly("stored_path_valid", [](const PV& v) { return v.b().stored_path_valid(); },
+                               "Whether a stored path is available (aligned(), coordinates(), cigar()).")
+        .def("aligned", [](const PV& v) { return v.b().aligned(v.i); },
+             "The alignment as a pair of gapped strings (seq_a, seq_b); needs score_and_grad(keep_paths=True).")
+        .def("coordinates", [](const PV& v) {
+            auto cc = v.b().coordinates(v.i);
+            const py::ssize_t k = static_cast<py::ssize_t>(cc.first.size());
+            py::array_t<int64_t> out({static_cast<py::ssize_t>(2), k});
+            std::memcpy(out.mutable_data(), cc.first.data(), sizeof(int64_t) * cc.first.size());
+            std::memcpy(out.mutable_data() + k, cc.second.data(), sizeof(int64_t) * cc.second.size());
+            return out;
+        }, "The alignment as Biopython-style coordinates: int64 (2, k), row 0 positions in seq_a, row 1 in seq_b.")
+        .def("cigar", [](const PV& v, bool extended, bool soft_clip) { return v.b().cigar(v.i, extended, soft_clip); },
+             py::kw_only(), py::arg("extended") = false, py::arg("soft_clip") = false,
+             "The alignment as a SAM CIGAR string (seq_a the query, seq_b the reference).")
+        .def("formatted", [](const PV& v, int width) {
+            auto al = v.b().aligned(v.i);
+            const std::string& a = al.first;
+            const std::string& b = al.second;
+            std::string mid(a.size(), ' ');
+            for (size_t k = 0; k < a.size(); ++k)
+                mid[k] = (a[k] == '-' || b[k] == '-') ? ' ' : (a[k] == b[k] ? '|' : '.');
+            const size_t w = width > 0 ? static_cast<size_t>(width) : std::max<size_t>(a.size(), 1);
+            std::string out;
+            for (size_t at = 0; at < a.size(); at += w) {
+                const size_t len = std::min(w, a.size() - at);
+                out += a.substr(at, len) + "\n" + mid.substr(at, len) + "\n" + b.substr(at, len);
+                if (at + len < a.size()) out += "\n\n";
+            }
+            return out;
+        }, py::arg("width") = 60, "Pretty-printed alignment block, wrapped at `width` columns (0 = no wrap).")
         .def("__repr__", [name](const PV& v) {
             return std::string(name) + "(batch pair " + std::to_string(v.i) + ", len(a)=" +
                    std::to_string(v.b().len_a(v.i)) + ", len(b)=" + std::to_string(v.b().len_b(v.i)) + ")";
@@ -401,15 +437,75 @@ void bind_batch(py::module_& m, const char* name) {
         .def(
             "score_and_grad",
             [](PB& s, bool keep_paths) {
-                if (keep_paths)
-                    throw py::value_error("nwgrad_amd: keep_paths (stored alignment paths) is not part of this "
-                                          "build's GPU path");
                 py::gil_scoped_release nogil;
-                return s.b->score_and_grad();
+                return s.b->score_and_grad(keep_paths);
             },
             py::arg("keep_paths") = false,
-            "Full DP on every pair: score and (unless grad_mode \"none\") gradient, cached per pair on the\n"
-            "device.  Returns the sum of scores, in pair order.")
+            "Full DP on every pair: score, guide and (unless grad_mode \"none\") gradient, cached per pair on\n"
+            "the device.  keep_paths=True also stores each alignment path (batch[i].aligned() /\n"
+            ".coordinates() / .cigar(), cigars()).  Returns the sum of scores, in pair order.")
+        .def(
+            "banded_grad",
+            [](PB& s, int bandwidth, bool keep_paths) {
+                py::gil_scoped_release nogil;
+                return s.b->banded_grad(bandwidth, keep_paths);
+            },
+            py::arg("bandwidth"), py::arg("keep_paths") = false,
+            "Banded re-align + gradient around each pair's cached guide path: run score_and_grad() once,\n"
+            "then set_params() + banded_grad(bw) after each update.  Returns the sum of scores.  Hard\n"
+            "batches.")
+        .def("drop_paths", [](PB& s) { s.b->drop_paths(); },
+             "Free the stored paths; scores, gradients and guides survive.")
+        .def("cigars", [](const PB& s, bool extended, bool soft_clip) { return s.b->cigars(extended, soft_clip); },
+             py::kw_only(), py::arg("extended") = false, py::arg("soft_clip") = false,
+             "Every pair's cigar() as a list of str, in pair order; needs stored paths.")
+        .def(
+            "align",
+            [](const PB& s, const std::vector<std::string>& seqs_a, const std::vector<std::string>& seqs_b,
+               const AlignParams& params, int band, const std::vector<std::string>& aligned_a,
+               const std::vector<std::string>& aligned_b, const std::string& kernel, py::object tracks,
+               py::object gap_tracks) {
+                if (kernel != "auto" && kernel != "gpu" && kernel != "gfx950")
+                    throw py::value_error("nwgrad_amd: kernel \"" + kernel + "\": this build's only backend is the "
+                                          "gfx950 GPU (\"auto\")");
+                if (seqs_a.size() != seqs_b.size())
+                    throw py::value_error("nwgrad: align() needs seqs_a and seqs_b of equal length");
+                const bool guided = !aligned_a.empty() || !aligned_b.empty();
+                if (guided && (aligned_a.size() != seqs_a.size() || aligned_b.size() != seqs_a.size()))
+                    throw py::value_error("nwgrad: aligned_a / aligned_b need one entry per pair");
+                std::vector<std::vector<int>> guides;
+                if (guided)
+                    for (size_t i = 0; i < aligned_a.size(); ++i) guides.push_back(guide_from_aligned(aligned_a[i], aligned_b[i]));
+                std::vector<int64_t> la(seqs_a.size());
+                for (size_t i = 0; i < seqs_a.size(); ++i) la[i] = static_cast<int64_t>(seqs_a[i].size());
+                bool tg = false, gg = false;
+                std::vector<char> tpg, gpg;
+                std::vector<int64_t> lg(la);
+                for (auto& v : lg) v += 1;
+                const std::vector<uint16_t> t = flat_tracks(tracks, la, "tracks", tg, tpg);
+                std::vector<uint16_t> g = flat_tracks(gap_tracks, lg, "gap_tracks", gg, gpg);
+                if (gg) {   // a pair whose gap track is None takes the default
+                    int64_t at = 0, ag = 0;
+                    for (size_t p = 0; p < la.size(); ++p) {
+                        if (!gpg[p]) {
+                            g[static_cast<size_t>(ag)] = la[p] ? t[static_cast<size_t>(at)] : 0;
+                            for (int64_t i = 1; i <= la[p]; ++i) g[static_cast<size_t>(ag + i)] = t[static_cast<size_t>(at + i - 1)];
+                        }
+                        at += la[p];
+                        ag += la[p] + 1;
+                    }
+                }
+                auto va = views(seqs_a), vb = views(seqs_b);
+                py::gil_scoped_release nogil;
+                return s.b->align(va, vb, params, band, guides, tg || gg ? t.data() : nullptr, gg ? g.data() : nullptr);
+            },
+            py::arg("seqs_a"), py::arg("seqs_b"), py::arg("params"), py::arg("band") = 0,
+            py::arg("aligned_a") = std::vector<std::string>{}, py::arg("aligned_b") = std::vector<std::string>{},
+            py::arg("kernel") = "auto", py::kw_only(), py::arg("tracks") = py::none(), py::arg("gap_tracks") = py::none(),
+            "Align (seqs_a[i], seqs_b[i]) under `params` WITHOUT adding them to the batch: returns a\n"
+            "BatchResult (scores, gradient summed over the pairs), storing nothing per pair.  Uses the\n"
+            "batch's problem type, grad mode and settings.  band > 0 bands every pair (hard batches;\n"
+            "aligned_a / aligned_b, one gapped string per pair, give the guides; the diagonal otherwise).")
         .def("score_and_grad_async", [](PB& s) { s.b->score_and_grad_async(); },
              "Enqueue score_and_grad() on the batch's stream without waiting (no total).")
         .def("compute_grad", [](const PB& s) {
@@ -700,6 +796,15 @@ PYBIND11_MODULE(_nwgrad_amd, m) {
         .def("__isub__", [](AlignParams& a, const AlignParams& b) -> AlignParams& { a -= b; return a; },
              py::return_value_policy::reference);
 
+    using SR = BatchResult;
+    py::class_<SR>(m, "BatchResult")
+        .def_property_readonly("scores", [](const SR& r) {
+            py::array_t<double> out(static_cast<py::ssize_t>(r.scores.size()));
+            if (!r.scores.empty()) std::memcpy(out.mutable_data(), r.scores.data(), sizeof(double) * r.scores.size());
+            return out;
+        }, "Alignment scores, shape (N,).")
+        .def_property_readonly("grad", [](const SR& r) { return r.grad; },
+                               "Summed gradient over the batch as an AlignParams object.");
     bind_view<float>(m, "SeqPair");
     bind_view<double>(m, "SeqPairDouble");
     bind_batch<float>(m, "SeqPairBatch");

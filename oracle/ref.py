@@ -45,6 +45,15 @@ def lib():
         L.nwgref_grads.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_double)]
         L.nwgref_weighted_grad.argtypes = [vp, P(ctypes.c_double), ctypes.c_long, P(ctypes.c_double)]
         L.nwgref_compute_grad.argtypes = [vp, P(ctypes.c_double)]
+        L.nwgref_score_and_grad_keep.argtypes = [vp, ctypes.c_int, P(ctypes.c_double)]
+        L.nwgref_banded_grad.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_double)]
+        L.nwgref_guides.argtypes = [vp, P(ctypes.c_int32)]
+        L.nwgref_path.argtypes = [vp, ctypes.c_long, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_long, P(ctypes.c_long),
+                                  P(ctypes.c_int64), ctypes.c_long, P(ctypes.c_long), ctypes.c_char_p,
+                                  ctypes.c_char_p]
+        L.nwgref_align.argtypes = [vp, ctypes.c_char_p, P(ctypes.c_double), P(ctypes.c_double), ctypes.c_long,
+                                   ctypes.c_char_p, P(ctypes.c_long), ctypes.c_char_p, P(ctypes.c_long),
+                                   P(ctypes.c_double), P(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -162,3 +171,54 @@ class RefBatch:
         out = np.zeros(getattr(self, "K", 1) * (self.n_alpha ** 2 + 4), np.float64)
         _check(lib().nwgref_compute_grad(self._h, _d(out)))
         return out
+
+    def score_and_grad_keep(self, keep_paths):
+        t = ctypes.c_double()
+        _check(lib().nwgref_score_and_grad_keep(self._h, 1 if keep_paths else 0, ctypes.byref(t)))
+        return t.value
+
+    def banded_grad(self, bandwidth, keep_paths=False):
+        t = ctypes.c_double()
+        _check(lib().nwgref_banded_grad(self._h, int(bandwidth), 1 if keep_paths else 0, ctypes.byref(t)))
+        return t.value
+
+    def guides(self, off_a):
+        """Every pair's guide_j concatenated (pair i's m + 1 entries at off_a[i] + i)."""
+        out = np.zeros(int(off_a[-1]) + self.N, np.int32)
+        _check(lib().nwgref_guides(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return out
+
+    def path(self, i, cap):
+        """Pair i's stored path: (aligned_a, aligned_b, coordinates (2, k), cigar, cigar extended +
+        soft-clipped).  cap: room for the aligned strings (len(a) + len(b) suffices)."""
+        cap = max(int(cap), 4) + 1
+        a = ctypes.create_string_buffer(cap)
+        b = ctypes.create_string_buffer(cap)
+        c0 = ctypes.create_string_buffer(cap * 8)
+        c1 = ctypes.create_string_buffer(cap * 8)
+        coords = np.zeros((2, cap), np.int64)
+        cols = ctypes.c_long()
+        k = ctypes.c_long()
+        _check(lib().nwgref_path(self._h, int(i), a, b, cap, ctypes.byref(cols),
+                                 coords.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), cap, ctypes.byref(k), c0, c1))
+        n = cols.value
+        return (a.raw[:n].decode(), b.raw[:n].decode(), coords[:, :k.value].copy(), c0.value.decode(),
+                c1.value.decode())
+
+    def align(self, seqs_a, seqs_b, matrix, gaps):
+        """Streaming align() under (matrix, gaps), band 0: scores and the summed gradient."""
+        sa = "".join(seqs_a).encode()
+        sb = "".join(seqs_b).encode()
+        n = len(seqs_a)
+        oa = np.zeros(n + 1, np.int64)
+        ob = np.zeros(n + 1, np.int64)
+        oa[1:] = np.cumsum([len(x) for x in seqs_a])
+        ob[1:] = np.cumsum([len(x) for x in seqs_b])
+        mat = np.ascontiguousarray(matrix, np.float64)
+        g = np.ascontiguousarray(gaps, np.float64)
+        scores = np.zeros(max(n, 1), np.float64)
+        grad = np.zeros(self.n_alpha ** 2 + 4, np.float64)
+        _check(lib().nwgref_align(self._h, self.alphabet.encode(), _d(mat), _d(g), n, sa,
+                                  oa.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), sb,
+                                  ob.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), _d(scores), _d(grad)))
+        return scores[:n], grad

@@ -223,4 +223,87 @@ int nwgref_compute_grad(void* hp, double* out) {
     return guarded([&] { with_batch<void>(h, [&](auto& b) { params_out(b.compute_grad(), out); }); });
 }
 
+// score_and_grad(keep_paths) / banded_grad(bandwidth, keep_paths) (batch_engine.hpp:615-653)
+int nwgref_score_and_grad_keep(void* hp, int keep, double* total) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guarded([&] {
+        *total = with_batch<double>(h, [&](auto& b) -> double {
+            if (h->grad_mode == 2) return nwgrad_soft_score_and_grad(b, keep != 0);
+            return b.score_and_grad(keep != 0);
+        });
+    });
+}
+
+int nwgref_banded_grad(void* hp, int bandwidth, int keep, double* total) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guarded([&] {
+        *total = with_batch<double>(h, [&](auto& b) -> double { return b.banded_grad(bandwidth, keep != 0); });
+    });
+}
+
+// Every pair's guide_j (batch[i].guide_j), concatenated: pair i's m + 1 entries at off_a[i] + i.
+int nwgref_guides(void* hp, int32_t* out) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guarded([&] {
+        with_batch<void>(h, [&](auto& b) {
+            size_t at = 0;
+            for (size_t i = 0; i < b.size(); ++i) {
+                const std::vector<int> g = b[i].guide_j();
+                for (int v : g) out[at++] = v;
+            }
+        });
+    });
+}
+
+// Pair i's stored path as the view renders it: aligned() (two strings of `cols` characters into
+// a and b, cap bytes each), coordinates() (2 x k int64 into coords, cap_k columns), cigar() and
+// cigar(extended=True, soft_clip=True) (NUL-terminated, 8 * cap bytes each).
+int nwgref_path(void* hp, long i, char* a, char* b, long cap, long* cols, int64_t* coords, long cap_k, long* k,
+                char* cigar, char* cigar_ext) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guarded([&] {
+        with_batch<void>(h, [&](auto& bt) {
+            auto v = bt[static_cast<size_t>(i)];
+            const auto al = v.aligned();
+            if ((long)al.first.size() > cap) throw std::out_of_range("nwgref: aligned() longer than the buffer");
+            std::memcpy(a, al.first.data(), al.first.size());
+            std::memcpy(b, al.second.data(), al.second.size());
+            *cols = static_cast<long>(al.first.size());
+            const auto cc = v.coordinates();
+            if ((long)cc.first.size() > cap_k) throw std::out_of_range("nwgref: coordinates() wider than the buffer");
+            *k = static_cast<long>(cc.first.size());
+            for (size_t q = 0; q < cc.first.size(); ++q) {
+                coords[q] = cc.first[q];
+                coords[cap_k + static_cast<long>(q)] = cc.second[q];
+            }
+            const std::string c0 = v.cigar(false, false), c1 = v.cigar(true, true);
+            if ((long)c0.size() >= 8 * cap || (long)c1.size() >= 8 * cap) throw std::out_of_range("nwgref: cigar too long");
+            std::memcpy(cigar, c0.c_str(), c0.size() + 1);
+            std::memcpy(cigar_ext, c1.c_str(), c1.size() + 1);
+        });
+    });
+}
+
+// Streaming align() (batch_engine.hpp:748-1273) on the handle's problem type: pairs given as
+// concatenated characters with offsets, params over the handle's alphabet; band 0 (the full DP).
+// scores: n_pairs doubles; grad: n*n + 4 (the summed gradient, params_out's layout).
+int nwgref_align(void* hp, const char* alphabet, const double* matrix, const double* gaps4, long n_pairs,
+                 const char* seq_a, const long* off_a, const char* seq_b, const long* off_b, double* scores,
+                 double* grad) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guarded([&] {
+        std::vector<ProblemInstance> probs(static_cast<size_t>(n_pairs));
+        for (long q = 0; q < n_pairs; ++q) {
+            probs[static_cast<size_t>(q)].seq_a = std::string_view(seq_a + off_a[q], static_cast<size_t>(off_a[q + 1] - off_a[q]));
+            probs[static_cast<size_t>(q)].seq_b = std::string_view(seq_b + off_b[q], static_cast<size_t>(off_b[q + 1] - off_b[q]));
+        }
+        const AlignParams p = make_params(alphabet, matrix, gaps4);
+        with_batch<void>(h, [&](auto& b) {
+            const BatchResult r = b.align(probs, p, 0);
+            std::memcpy(scores, r.scores.data(), r.scores.size() * sizeof(double));
+            if (grad) params_out(r.grad, grad);
+        });
+    });
+}
+
 }  // extern "C"

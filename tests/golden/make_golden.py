@@ -5,6 +5,7 @@ sources (oracle/_ref/libnwgref.so; see oracle/Makefile.ref).  Run in the build c
     python tests/golden/make_golden.py long                                  # the long-pair sets only
     python tests/golden/make_golden.py tracks                                # the tracked sets only
     python tests/golden/make_golden.py soft_tracks                           # the tracked soft sets only
+    python tests/golden/make_golden.py paths                                 # guides, paths, banded, align()
 
 Each fixture is an .npz (no pickles) holding the inputs -- alphabet, matrix, gaps, problem
 type, encoded pairs -- and the reference's outputs: per-pair scores and gradients, and for the
@@ -213,7 +214,124 @@ def save_tracked(name, w, precision, grad_mode, res, extra):
     print("wrote", path, len(w.off_a) - 1, "pairs")
 
 
+def _perturbed(w, seed):
+    """A second parameter point near the first, as a training step would move it: matrix entries
+    and gaps shifted by multiples of 0.25."""
+    r = wl.rng(w.name, seed)
+    mat = w.matrix + 0.25 * r.integers(-3, 4, size=w.matrix.shape)
+    mat = (mat + mat.T) / 2
+    gaps = np.maximum(w.gaps + 0.25 * r.integers(-2, 3, size=4), 0.0)
+    if w.gap_model == "linear":
+        gaps[0] = gaps[2] = 0.0
+    return mat, gaps
+
+
+def _paths_of(b, w):
+    """Every pair's stored path as the reference renders it."""
+    al_a, al_b, cig, cig_x, coords, coff = [], [], [], [], [], [0]
+    for i in range(w.n_pairs):
+        cap = int(w.off_a[i + 1] - w.off_a[i] + w.off_b[i + 1] - w.off_b[i])
+        a, bb, cc, c0, c1 = b.path(i, cap)
+        al_a.append(a)
+        al_b.append(bb)
+        cig.append(c0)
+        cig_x.append(c1)
+        coords.append(cc)
+        coff.append(coff[-1] + cc.shape[1])
+    return {"aligned_a": np.array(al_a), "aligned_b": np.array(al_b), "cigars": np.array(cig),
+            "cigars_ext": np.array(cig_x), "coords": np.concatenate(coords, axis=1), "coords_off": np.array(coff)}
+
+
+def _hard_results(b, n):
+    scores = b.scores()
+    mats, gaps = b.grads()
+    return scores, np.concatenate([mats.reshape(n, -1), gaps.reshape(n, -1)], axis=1)
+
+
+def path_set(name, w, prec, bands=(3,), traceback="auto"):
+    """One hard batch: score_and_grad(keep_paths=True) under (matrix, gaps) -- scores, gradients,
+    guides and the stored paths -- then, for each bandwidth, set_params(the perturbed point) and
+    banded_grad(bw, keep_paths=True) from those guides: scores, gradients, new guides, cigars.
+
+    traceback: the full pass's.  The long sets name "pointers": their pairs of more than 512
+    rows would take Grid under the reference's `auto` on a SIMD build -- Pointers' result by
+    the reference's contract, guides included (aligner.hpp:89-100, guide_j_grid_) -- but this
+    scalar build has no Grid kernels, and its fallback writes guides in Hirschberg's convention
+    (the column a path enters a row at, and n at row m)."""
+    sa, sb = w.strings()
+    mat1, gaps1 = _perturbed(w, "p1")
+    out = {"matrix1": mat1, "gaps1": gaps1, "bands": np.array(bands)}
+    b = ref.RefBatch(sa, sb, w.alphabet, w.matrix, w.gaps, double=(prec == "float64"), gap_model=w.gap_model,
+                     mode=w.mode, grad_mode="hard", n_threads=1, traceback=traceback)
+    out["total"] = np.array(b.score_and_grad_keep(True))
+    out["scores"], out["grads"] = _hard_results(b, w.n_pairs)
+    out["guides"] = b.guides(w.off_a)
+    out.update(_paths_of(b, w))
+    for bw in bands:
+        b = ref.RefBatch(sa, sb, w.alphabet, w.matrix, w.gaps, double=(prec == "float64"), gap_model=w.gap_model,
+                         mode=w.mode, grad_mode="hard", n_threads=1, traceback=traceback)
+        b.score_and_grad_keep(False)
+        b.set_params(mat1, gaps1)
+        out[f"band{bw}_total"] = np.array(b.banded_grad(bw, True))
+        out[f"band{bw}_scores"], out[f"band{bw}_grads"] = _hard_results(b, w.n_pairs)
+        out[f"band{bw}_guides"] = b.guides(w.off_a)
+        out[f"band{bw}_cigars"] = _paths_of(b, w)["cigars"]
+    save(name, w, prec, "hard", out)
+
+
+def path_sets():
+    for gm in ("affine", "linear"):
+        for mode in ("global", "local", "glocal"):
+            for prec in ("float32", "float64"):
+                w = wl.random_pairs(f"golden_paths_dna_{gm}_{mode}", 40, "ACGT", (0, 40), (0, 80),
+                                    gaps=(2.0, 1.0, 3.0, 1.0) if gm == "affine" else (0.0, 1.0, 0.0, 2.0),
+                                    gap_model=gm, mode=mode, related=0.6)
+                path_set(f"paths_dna_{gm}_{mode}_{prec}", edge_pairs(w), prec, bands=(1, 4))
+                w = wl.random_pairs(f"golden_paths_prot_{gm}_{mode}", 24, wl.AA, (1, 150), (1, 220),
+                                    gaps=(10.5, 0.75, 9.25, 1.5) if gm == "affine" else (0.0, 1.25, 0.0, 1.75),
+                                    gap_model=gm, mode=mode, related=0.6)
+                path_set(f"paths_prot_{gm}_{mode}_{prec}", w, prec, bands=(2, 16))
+            # long pairs (column panels on the GPU)
+            w = wl.random_pairs(f"golden_paths_long_{gm}_{mode}", 4, wl.AA, (300, 900), (1025, 2400),
+                                gaps=(10.5, 0.75, 9.25, 1.5) if gm == "affine" else (0.0, 1.25, 0.0, 1.75),
+                                gap_model=gm, mode=mode, related=0.0)
+            path_set(f"paths_long_{gm}_{mode}_float32", w, "float32", bands=(8,), traceback="pointers")
+    # the guides of soft batches (the eager guide: the hard fill under the params scored)
+    for mode in ("global", "local", "glocal"):
+        w = wl.random_pairs(f"golden_softguide_{mode}", 16, wl.AA, (1, 90), (1, 120), gaps=(11.0, 1.0, 11.0, 1.0),
+                            mode=mode, related=0.5)
+        sa, sb = w.strings()
+        b = ref.RefBatch(sa, sb, w.alphabet, w.matrix, w.gaps, gap_model="affine", mode=mode, grad_mode="soft",
+                         n_threads=1)
+        b.set_soft(precision="float32", impl="scaled_or_log")
+        total = b.score_and_grad()
+        save(f"softguide_prot_affine_{mode}_float32", w, "float32", "soft",
+             {"total": np.array(total), "scores": b.scores(), "guides": b.guides(w.off_a)})
+    # streaming align(): every pair's score and the summed gradient, nothing kept
+    for gm in ("affine", "linear"):
+        for mode in ("global", "local", "glocal"):
+            w = wl.random_pairs(f"golden_align_{gm}_{mode}", 20, wl.AA, (1, 120), (1, 160),
+                                gaps=(10.5, 0.75, 9.25, 1.5) if gm == "affine" else (0.0, 1.25, 0.0, 1.75),
+                                gap_model=gm, mode=mode, related=0.5)
+            sa, sb = w.strings()
+            mat1, gaps1 = _perturbed(w, "align")
+            res = {"matrix1": mat1, "gaps1": gaps1}
+            for gd in ("hard", "none", "soft"):
+                if gd == "soft" and gm == "linear":
+                    continue
+                b = ref.RefBatch(sa[:2], sb[:2], w.alphabet, w.matrix, w.gaps, gap_model=gm, mode=mode, grad_mode=gd,
+                                 n_threads=1)
+                if gd == "soft":
+                    b.set_soft(precision="float32", impl="scaled_or_log")
+                sc, gr = b.align(sa, sb, mat1, gaps1)
+                res[f"{gd}_scores"], res[f"{gd}_grad"] = sc, gr
+            save(f"align_prot_{gm}_{mode}_float32", w, "float32", "hard", res)
+
+
 def main():
+    if sys.argv[1:] == ["paths"]:
+        path_sets()
+        return
     if sys.argv[1:] == ["long"]:
         long_sets()
         return
