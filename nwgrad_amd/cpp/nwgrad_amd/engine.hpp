@@ -427,6 +427,7 @@ public:
     double score_and_grad(bool keep_paths = false) {
         double total = 0.0;
         if (gd_ == GradMode::Hard) check_(nwg_batch_set_keep_paths(b_, keep_paths ? 1 : 0));
+        gbscores_ = false;
         check_(nwg_batch_run(b_, &total));
         valid_ = true;
         gparams_ = params_;
@@ -438,19 +439,29 @@ public:
         return total;
     }
     // batch_engine.hpp:633-653: a banded re-alignment and gradient around every pair's guide,
-    // the training-loop step after set_params; new guides.  Hard batches.
+    // the training-loop step after set_params; new guides.  Hard batches run it on their own
+    // device batch.  A score-only batch's banded pass is the banded Viterbi score with its guide
+    // and path (batch_engine.hpp:2188-2205, 2125-2134), which is the companion hard batch's
+    // banded pass around the same guides: its scores are this batch's until the next pass.
     double banded_grad(int bandwidth, bool keep_paths = false) {
         if (N_ == 0) return 0.0;
         if (bandwidth <= 0)
             throw std::invalid_argument("nwgrad: banded_grad() needs bandwidth > 0 (got " + std::to_string(bandwidth) +
                                         "); use score_and_grad() for full DP");
-        if (gd_ != GradMode::Hard)
-            throw std::invalid_argument("nwgrad_amd: banded_grad() runs on hard batches (grad_mode=\"hard\") on the "
-                                        "GPU path");
+        if (gd_ == GradMode::Soft)
+            throw std::invalid_argument("nwgrad_amd: banded_grad() of a soft batch (the banded forward-backward) is "
+                                        "not on the GPU path; run score_and_grad()");
         if (!guides_) throw std::logic_error("nwgrad: banded_grad() needs a guide path; run score_and_grad() first");
         double total = 0.0;
-        check_(nwg_batch_set_keep_paths(b_, keep_paths ? 1 : 0));
-        check_(nwg_batch_run_banded(b_, bandwidth, &total));
+        if (gd_ == GradMode::None) {
+            if (!gfresh_) resolve_guides_(false);   // the guides of the last pass, under its params
+            gb_->set_params(params_);
+            total = gb_->banded_grad(bandwidth, keep_paths);
+            gbscores_ = true;
+        } else {
+            check_(nwg_batch_set_keep_paths(b_, keep_paths ? 1 : 0));
+            check_(nwg_batch_run_banded(b_, bandwidth, &total));
+        }
         valid_ = true;
         gparams_ = params_;
         paths_cached_ = false;
@@ -549,6 +560,7 @@ public:
     }
     // Enqueue only (no total, no synchronisation): for device-side pipelines.
     void score_and_grad_async() {
+        gbscores_ = false;
         check_(nwg_batch_run_async(b_));
         valid_ = true;
     }
@@ -556,6 +568,7 @@ public:
     std::vector<double> scores() const {
         std::vector<double> out(static_cast<size_t>(N_));
         if (!valid_) throw std::logic_error("nwgrad: score not valid; compute an alignment first");
+        if (gbscores_) return gb_->scores();
         check_(nwg_batch_scores(b_, out.data()));
         return out;
     }
@@ -594,6 +607,9 @@ public:
             }
         }
         valid_ = false;
+        gbscores_ = false;
+        gb_.reset();   // the companion hard batch is rebuilt with these tracks when next needed
+        gfresh_ = false;
     }
     int num_sets() const {
         int K = 1, tr = 0;
@@ -626,6 +642,7 @@ public:
     bool results_valid() const noexcept { return valid_; }
     double pair_score(size_t i) const {
         double v = 0.0;
+        if (gbscores_) return gb_->pair_score(i);
         check_(nwg_batch_pair_result(b_, static_cast<int64_t>(i), &v, nullptr));
         return v;
     }
@@ -725,14 +742,15 @@ public:
                                         "\" in a batch over alphabet \"" + alpha_->symbols() + "\"");
         if (band < 0) throw std::invalid_argument("nwgrad: band must be >= 0, got " + std::to_string(band));
         const bool banded = band > 0 || !guides.empty();
-        if (banded && gd_ != GradMode::Hard)
-            throw std::invalid_argument("nwgrad_amd: banded align() runs on hard batches (grad_mode=\"hard\") on the GPU "
-                                        "path");
+        if (banded && gd_ == GradMode::Soft)
+            throw std::invalid_argument("nwgrad_amd: banded align() of a soft batch (the banded forward-backward) is not "
+                                        "on the GPU path");
         if (!guides.empty() && guides.size() != seqs_a.size())
             throw std::invalid_argument("nwgrad: aligned_a / aligned_b need one entry per pair");
         BatchResult r{std::vector<double>(seqs_a.size(), 0.0), AlignParams(params.alphabet(), params.num_sets())};
         if (seqs_a.empty()) return r;
-        SeqPairBatchT<T> tmp(seqs_a, seqs_b, params, gm_, am_, gd_, n_threads_, device_);
+        // a banded score-only pass is the banded Viterbi's scores: a hard batch's, gradient unread
+        SeqPairBatchT<T> tmp(seqs_a, seqs_b, params, gm_, am_, banded ? GradMode::Hard : gd_, n_threads_, device_);
         if (tracks || gap_tracks) tmp.set_tracks(tracks, gap_tracks);
         if (gd_ == GradMode::Soft) {
             tmp.set_soft_temperature(soft_temp_);
@@ -847,6 +865,7 @@ private:
         const std::vector<double> g = params_.all_gaps();
         check_(nwg_batch_set_params_k(b_, params_.num_sets(), params_.matrix.data(), g.data()));
         valid_ = false;
+        gbscores_ = false;
     }
     AlignParams reduce_(const Alphabet& a, const double* w) const {
         if (gd_ == GradMode::None) throw std::logic_error("nwgrad: grad_mode \"none\" computes no gradient");
@@ -878,6 +897,7 @@ private:
     // copy of the paths; for score-only and soft batches the hard batch that holds them and the
     // params it must run under
     bool guides_ = false, stored_ = false, gfresh_ = false;
+    bool gbscores_ = false;   // a score-only batch's banded scores: held by the companion
     mutable bool paths_cached_ = false;
     mutable std::vector<uint32_t> pops_;
     mutable std::vector<int32_t> pn_, pend_;

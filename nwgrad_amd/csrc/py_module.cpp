@@ -452,8 +452,8 @@ void bind_batch(py::module_& m, const char* name) {
             },
             py::arg("bandwidth"), py::arg("keep_paths") = false,
             "Banded re-align + gradient around each pair's cached guide path: run score_and_grad() once,\n"
-            "then set_params() + banded_grad(bw) after each update.  Returns the sum of scores.  Hard\n"
-            "batches.")
+            "then set_params() + banded_grad(bw) after each update.  Returns the sum of scores.  Hard and\n"
+            "score-only batches (a soft batch's banded forward-backward raises ValueError).")
         .def("drop_paths", [](PB& s) { s.b->drop_paths(); },
              "Free the stored paths; scores, gradients and guides survive.")
         .def("cigars", [](const PB& s, bool extended, bool soft_clip) { return s.b->cigars(extended, soft_clip); },

@@ -279,6 +279,29 @@ def path_set(name, w, prec, bands=(3,), traceback="auto"):
     save(name, w, prec, "hard", out)
 
 
+def bandscore_sets():
+    """Banded passes of score-only batches: the banded Viterbi's scores, guides and paths around
+    the eager guides of a full pass (batch_engine.hpp:2188-2205)."""
+    for gm in ("affine", "linear"):
+        for mode in ("global", "local", "glocal"):
+            w = wl.random_pairs(f"golden_bandscore_{gm}_{mode}", 24, wl.AA, (0, 150), (0, 220),
+                                gaps=(10.5, 0.75, 9.25, 1.5) if gm == "affine" else (0.0, 1.25, 0.0, 1.75),
+                                gap_model=gm, mode=mode, related=0.6)
+            sa, sb = w.strings()
+            mat1, gaps1 = _perturbed(w, "p1")
+            out = {"matrix1": mat1, "gaps1": gaps1, "bands": np.array((2, 16))}
+            for bw in (2, 16):
+                b = ref.RefBatch(sa, sb, w.alphabet, w.matrix, w.gaps, gap_model=gm, mode=mode, grad_mode="none",
+                                 n_threads=1)
+                b.score_and_grad()
+                b.set_params(mat1, gaps1)
+                out[f"band{bw}_total"] = np.array(b.banded_grad(bw, True))
+                out[f"band{bw}_scores"] = b.scores()
+                out[f"band{bw}_guides"] = b.guides(w.off_a)
+                out[f"band{bw}_cigars"] = _paths_of(b, w)["cigars"]
+            save(f"bandscore_prot_{gm}_{mode}_float32", w, "float32", "none", out)
+
+
 def path_sets():
     for gm in ("affine", "linear"):
         for mode in ("global", "local", "glocal"):
@@ -296,6 +319,7 @@ def path_sets():
                                 gaps=(10.5, 0.75, 9.25, 1.5) if gm == "affine" else (0.0, 1.25, 0.0, 1.75),
                                 gap_model=gm, mode=mode, related=0.0)
             path_set(f"paths_long_{gm}_{mode}_float32", w, "float32", bands=(8,), traceback="pointers")
+    bandscore_sets()
     # the guides of soft batches (the eager guide: the hard fill under the params scored)
     for mode in ("global", "local", "glocal"):
         w = wl.random_pairs(f"golden_softguide_{mode}", 16, wl.AA, (1, 90), (1, 120), gaps=(11.0, 1.0, 11.0, 1.0),
@@ -329,6 +353,9 @@ def path_sets():
 
 
 def main():
+    if sys.argv[1:] == ["bandscore"]:
+        bandscore_sets()
+        return
     if sys.argv[1:] == ["paths"]:
         path_sets()
         return

@@ -76,3 +76,27 @@ def test_path_fixture_consistency(path):
         assert np.all(np.diff(g) >= 0) and g.min() >= 0 and g.max() <= n
         assert g[x1] == y1 and (mode != "global" or g[m] == n)
     assert goff == len(z["guides"])
+
+
+BANDSCORE = sorted(glob.glob(os.path.join(HERE, "bandscore_*.npz")))
+
+
+@pytest.mark.skipif(not BANDSCORE, reason="no score-only banded fixtures")
+@pytest.mark.parametrize("path", BANDSCORE, ids=lambda p: os.path.basename(p)[:-4])
+def test_bandscore_fixture_is_the_hard_banded_pass(path):
+    """A score-only batch's banded pass is the hard banded pass's scores, guides and paths around
+    the same guides: the reference engine itself agrees (the GPU path runs it that way)."""
+    import oracle
+    if not oracle.have_ref():
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    from oracle import ref
+    z = np.load(path)
+    sa, sb = _seqs(z)
+    for bw in z["bands"]:
+        b = ref.RefBatch(sa, sb, str(z["alphabet"]), z["matrix"], z["gaps"], gap_model=str(z["gap_model"]),
+                         mode=str(z["mode"]), grad_mode="hard", n_threads=1)
+        b.score_and_grad()
+        b.set_params(z["matrix1"], z["gaps1"])
+        assert b.banded_grad(int(bw), False) == float(z[f"band{bw}_total"])
+        np.testing.assert_array_equal(b.scores(), z[f"band{bw}_scores"])
+        np.testing.assert_array_equal(b.guides(z["off_a"]), z[f"band{bw}_guides"])

@@ -26,6 +26,7 @@ HERE = os.path.join(os.path.dirname(__file__), "golden")
 PATHS = sorted(glob.glob(os.path.join(HERE, "paths_*.npz")))
 SOFTGUIDE = sorted(glob.glob(os.path.join(HERE, "softguide_*.npz")))
 ALIGN = sorted(glob.glob(os.path.join(HERE, "align_*.npz")))
+BANDSCORE = sorted(glob.glob(os.path.join(HERE, "bandscore_*.npz")))
 
 
 def _guides(b):
@@ -41,7 +42,8 @@ def _strings(z):
 
 def _batch(z):
     return batch(str(z["alphabet"]), z["matrix"], z["gaps"], z["codes_a"], z["off_a"], z["codes_b"], z["off_b"],
-                 precision=str(z["precision"]), gap_model=str(z["gap_model"]), mode=str(z["mode"]))
+                 precision=str(z["precision"]), gap_model=str(z["gap_model"]), mode=str(z["mode"]),
+                 grad_mode=str(z["grad_mode"]))
 
 
 @pytest.mark.parametrize("path", PATHS, ids=lambda p: os.path.basename(p)[:-4])
@@ -76,6 +78,31 @@ def test_paths_golden(gpu, path):
         np.testing.assert_array_equal(grads_rows(b), z[f"band{bw}_grads"])
         np.testing.assert_array_equal(_guides(b), z[f"band{bw}_guides"])
         assert b.cigars() == list(z[f"band{bw}_cigars"])
+
+
+@pytest.mark.parametrize("path", BANDSCORE, ids=lambda p: os.path.basename(p)[:-4])
+def test_banded_score_only_golden(gpu, path):
+    """banded_grad of a score-only batch: the banded Viterbi's scores, new guides and paths."""
+    z = np.load(path)
+    say("case", os.path.basename(path))
+    p1 = params_of(str(z["alphabet"]), z["matrix1"], z["gaps1"])
+    for bw in z["bands"]:
+        b = _batch(z)   # grad_mode "none"
+        b.score_and_grad()
+        b.set_params(p1)
+        with pytest.raises(RuntimeError):
+            b.scores()   # set_params cleared them
+        t = b.banded_grad(int(bw), keep_paths=True)
+        assert t == float(z[f"band{bw}_total"])
+        np.testing.assert_array_equal(b.scores(), z[f"band{bw}_scores"])
+        assert [b[i].score for i in range(len(b))] == list(z[f"band{bw}_scores"])
+        np.testing.assert_array_equal(_guides(b), z[f"band{bw}_guides"])
+        assert b.cigars() == list(z[f"band{bw}_cigars"])
+        with pytest.raises(RuntimeError, match="grad_mode"):
+            b.grads()
+        # a full pass afterwards serves its own scores again
+        b.score_and_grad()
+        assert b.scores()[0] == b[0].score
 
 
 @pytest.mark.parametrize("path", SOFTGUIDE, ids=lambda p: os.path.basename(p)[:-4])
@@ -181,5 +208,5 @@ def test_paths_errors(gpu):
     assert b[0].guide_j is not None and b[0].score is not None
     s = batch(w.alphabet, w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, grad_mode="soft")
     s.score_and_grad()
-    with pytest.raises(ValueError, match="hard batches"):
+    with pytest.raises(ValueError, match="soft batch"):
         s.banded_grad(4)
