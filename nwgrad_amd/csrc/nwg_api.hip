@@ -44,7 +44,7 @@ constexpr int kSS = 8;                 // columns per lane: soft pass (float32 a
 constexpr int kMaxG = 64;              // lanes per pair at most: one wavefront
 constexpr int kMaxN = kS * kMaxG;      // longest B a lane group holds (hard / score-only); wider: column panels
 constexpr int kHistBytes = 16384;      // LDS histogram budget per wave (hard walk, uint16 counts)
-constexpr int kSoftHistBytes = 10240;  // LDS histogram budget per wave (soft, double expected counts)
+constexpr int kSoftHistBytes = 13440;  // LDS histogram budget per wave (soft, double expected counts)
 constexpr size_t kSoftLdsMax = 65536;  // LDS of one soft wavefront, at most (tracked: K sets)
 constexpr int kBlocksPerCU = 8;        // persistent grid of the hard / score fills: waves per CU
 constexpr int kSoftBlocksPerCU = 8;    // fallback persistent grid of the soft pass

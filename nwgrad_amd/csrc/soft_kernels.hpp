@@ -730,11 +730,15 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
     const double izr = zbad ? 0.0 : 1.0 / zr;
 
     // ── backward ────────────────────────────────────────────────────────────
-    // The lane's running sums live in LDS (five doubles per lane, added without return): the
-    // gap counts oa, ea, ob, eb and the lost-mass bound.
+    // The lane's running sums: the gap counts oa, ea, ob, eb and the lost-mass bound.  Untracked
+    // they stay in registers; tracked, the K x 4 counts live in LDS (per lane, added without
+    // return), since the set a row's counts go to varies along the pair.
     double* __restrict__ acc = lacc + lane;
+    double racc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if constexpr (TR) {
 #pragma unroll
-    for (int c = 0; c < 4 * KS + 1; ++c) acc[c * kWave] = 0.0;
+        for (int c = 0; c < 4 * KS + 1; ++c) acc[c * kWave] = 0.0;
+    }
     const int kmin = (L || GL) ? 1 : 0;   // first row whose Y opens count
     const bool col0_opens = !L;           // column-0 X opens count (kmin0 == 0)
     const int hrow = na + 1;   // histogram rows carry a spill column for the no-column letter
@@ -1007,7 +1011,9 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             if (!zbad && (mx > SF(0) || mfc > SF(0))) {
                 const double lr = 32.0 * (double)(ncols + 1) * A.vmin * (3.0 * (double)mx + (double)mfc) * fnew +
                                   8.0 * (double)(ncols + 1) * A.vtiny * (fnew + fold);
-                atomicAdd(acc + 4 * KS * kWave, (lr <= 1.0e300) ? lr : 1.0e300);
+                const double lc = (lr <= 1.0e300) ? lr : 1.0e300;
+                if constexpr (TR) atomicAdd(acc + 4 * KS * kWave, lc);
+                else racc[4] += lc;
             }
             if (!zbad) {
                 // every term is a sum of non-negative products: one range check covers them
@@ -1017,10 +1023,17 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 const double c4 = (double)sob * (double)r_ob * fold;
                 if (!(c1 + c2 + c3 + c4 <= 1.0e300)) bad = true;
                 else {
-                    atomicAdd(acc + (4 * gi) * kWave, c3);
-                    atomicAdd(acc + (4 * gi + 1) * kWave, c2);
-                    atomicAdd(acc + (4 * tn + 2) * kWave, c4);
-                    atomicAdd(acc + (4 * ti + 3) * kWave, c1);
+                    if constexpr (TR) {
+                        atomicAdd(acc + (4 * gi) * kWave, c3);
+                        atomicAdd(acc + (4 * gi + 1) * kWave, c2);
+                        atomicAdd(acc + (4 * tn + 2) * kWave, c4);
+                        atomicAdd(acc + (4 * ti + 3) * kWave, c1);
+                    } else {
+                        racc[0] += c3;
+                        racc[1] += c2;
+                        racc[2] += c4;
+                        racc[3] += c1;
+                    }
                 }
             }
             if (!(mx <= R::fmax)) bad = true;
@@ -1089,7 +1102,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
     wave_sync();
     double g_oa = 0.0, g_ea = 0.0, g_ob = 0.0, g_eb = 0.0, loss = 0.0;
     if constexpr (!TR) {
-        g_oa = acc[0]; g_ea = acc[kWave]; g_ob = acc[2 * kWave]; g_eb = acc[3 * kWave]; loss = acc[4 * kWave];
+        g_oa = racc[0]; g_ea = racc[1]; g_ob = racc[2]; g_eb = racc[3]; loss = racc[4];
         for (int off = 1; off < kWave; off <<= 1) {
             const double o1 = __shfl_down(g_oa, off, kWave), o2 = __shfl_down(g_ea, off, kWave);
             const double o3 = __shfl_down(g_ob, off, kWave), o4 = __shfl_down(g_eb, off, kWave);
