@@ -36,14 +36,32 @@ __device__ __forceinline__ double ninf<double>() { return -__builtin_huge_val();
 __device__ __forceinline__ float dmax(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ __forceinline__ double dmax(double a, double b) { return __builtin_fmax(a, b); }
 
-// Lane-shift by one towards higher lanes across the whole wavefront (lane 0 keeps its own
-// value, which the caller overwrites).  ds_bpermute based: works for any group width.
-__device__ __forceinline__ float shift_up(float v) { return __shfl_up(v, 1, kWave); }
-__device__ __forceinline__ double shift_up(double v) { return __shfl_up(v, 1, kWave); }
-__device__ __forceinline__ int shift_up(int v) { return __shfl_up(v, 1, kWave); }
-__device__ __forceinline__ float shift_down(float v) { return __shfl_down(v, 1, kWave); }
-__device__ __forceinline__ double shift_down(double v) { return __shfl_down(v, 1, kWave); }
-__device__ __forceinline__ int shift_down(int v) { return __shfl_down(v, 1, kWave); }
+// One-lane wave shifts on the DPP path (a VALU move, no LDS round trip): up = from the lane
+// below (wave_shr:1), down = from the lane above (wave_shl:1).  Lane 0 (up) and lane 63
+// (down) receive 0; every caller replaces or ignores that value.
+__device__ __forceinline__ int dpp_up(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_down(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ float wshift_up(float v) { return __int_as_float(dpp_up(__float_as_int(v))); }
+__device__ __forceinline__ float wshift_down(float v) { return __int_as_float(dpp_down(__float_as_int(v))); }
+__device__ __forceinline__ int wshift_up(int v) { return dpp_up(v); }
+__device__ __forceinline__ int wshift_down(int v) { return dpp_down(v); }
+__device__ __forceinline__ double wshift_up(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_up((int)(b & 0xffffffff)), hi = dpp_up((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double wshift_down(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_down((int)(b & 0xffffffff)), hi = dpp_down((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Lane-shift by one towards higher lanes across the whole wavefront, as the DP's lane
+// pipelines use it: every lane of the wave active, lane 0 receiving 0, which the caller
+// overwrites (its column 0 or its panel boundary).
+__device__ __forceinline__ float shift_up(float v) { return wshift_up(v); }
+__device__ __forceinline__ double shift_up(double v) { return wshift_up(v); }
+__device__ __forceinline__ int shift_up(int v) { return wshift_up(v); }
 
 // The value of the wave's first active lane as a wave-uniform (scalar) value: branches on it
 // are uniform to the compiler.

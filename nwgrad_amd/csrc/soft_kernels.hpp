@@ -188,26 +188,6 @@ __device__ __forceinline__ double em_max(double v) {
     return x == -32768 ? 0.0 : __builtin_amdgcn_ldexp(1.0, x);
 }
 
-// One-lane wave shifts on the DPP path (a VALU move, no LDS round trip): up = from the lane
-// below (wave_shr:1), down = from the lane above (wave_shl:1).  Lane 0 (up) and lane 63
-// (down) receive 0; every caller replaces or ignores that value.
-__device__ __forceinline__ int dpp_up(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
-__device__ __forceinline__ int dpp_down(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
-__device__ __forceinline__ float wshift_up(float v) { return __int_as_float(dpp_up(__float_as_int(v))); }
-__device__ __forceinline__ float wshift_down(float v) { return __int_as_float(dpp_down(__float_as_int(v))); }
-__device__ __forceinline__ int wshift_up(int v) { return dpp_up(v); }
-__device__ __forceinline__ int wshift_down(int v) { return dpp_down(v); }
-__device__ __forceinline__ double wshift_up(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = dpp_up((int)(b & 0xffffffff)), hi = dpp_up((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ double wshift_down(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = dpp_down((int)(b & 0xffffffff)), hi = dpp_down((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
 // The value, hidden from loop-invariant code motion: a per-column comparison against it is
 // evaluated where it is used (one compare into VCC) instead of being hoisted out of the row
 // loop as S live 64-bit lane masks.
