@@ -253,6 +253,8 @@ def main():
     ap.add_argument("--workload", default=os.environ.get("NWG_BENCH_WORKLOAD", "protein_soft_homologs"),
                     choices=WORKLOADS)
     ap.add_argument("--fill", default="interpair", choices=("interpair", "striped"))
+    ap.add_argument("--columns", type=int, default=0, choices=(0, 8, 16),
+                    help="columns a lane of the hard / score-only tiles (0: the lane-width table's choice)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-launch comparison leg")
     ap.add_argument("--serve-seconds", type=float, default=None,
@@ -306,6 +308,8 @@ def main():
     w, grad_mode = make_workload(args.workload)
     precision = "float64" if args.workload.endswith("_double") else "float32"
     b = make_batch(w, grad_mode, args.fill, precision)
+    if args.columns and grad_mode != "soft":
+        b.columns_per_lane = args.columns
     cells = b.cells
     print(f"bench: {w.n_pairs} pairs, {cells} cells, {b.tiles} tiles; timing {args.steps} steps", file=sys.stderr,
           flush=True)
@@ -359,7 +363,8 @@ def main():
             "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": dtype, "data": "synthetic",
             "config": {"workload": args.workload, "pairs": w.n_pairs, "cells": cells, "grad_mode": grad_mode,
-                       "gap_model": w.gap_model, "mode": w.mode, "fill": args.fill},
+                       "gap_model": w.gap_model, "mode": w.mode, "fill": args.fill,
+                       "columns_per_lane": int(b.columns_per_lane)},
             "kernel_gcups": round(cells / (kms * 1e-3) / 1e9, 3),
             "plain_vs_batched": None if gcups_other is None else
                 {"fill": other, "gcups": round(gcups_other, 3), "kernel_gcups": round(cells / (kms2 * 1e-3) / 1e9, 3),

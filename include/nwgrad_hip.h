@@ -117,6 +117,23 @@ int nwg_batch_set_params_dev(nwg_batch *b, const double *theta_dev);
  * (the reference's `fill=` knob, py_exports.cpp:1180-1207). */
 int nwg_batch_set_fill(nwg_batch *b, int fill);
 
+/* Columns a lane of the hard and score-only tiles: 0 (default) chosen per batch -- the
+ * lane-width table's matching rule, else the built-in rule (8 when the tiles at 16 number
+ * fewer than two per SIMD and every pair fits a wave at 8, else 16) -- or 8 or 16 forced (16
+ * when a pair is wider than 512 columns).  The lane-group width follows: the power of two of
+ * ceil(len(B) / columns) lanes, so 64 / width pairs a wave.  Results do not depend on it.
+ * The reference's counterpart is its runtime gates and tuning table (simd_levels.hpp:704-751,
+ * _tuning.py), which choose its inter-pair group width per ISA.  nwg_batch_lanes gives the
+ * columns the batch's tiles are cut at (soft batches: the soft pass's 8). */
+int nwg_batch_set_lanes(nwg_batch *b, int columns);
+int nwg_batch_lanes(nwg_batch *b, int *columns);
+/* The process-wide lane-width table: n_rules rules of three int32 (n_max, tiles16_max,
+ * columns), first match wins -- a batch whose median len(B) is at most n_max and whose tiles
+ * at 16 columns a lane number at most tiles16_max / 16 per CU is cut at `columns` (8 or 16).
+ * n_rules 0 clears it.  Batches planned afterwards follow it (the tuner in
+ * nwgrad_amd/tuning.py measures and writes it). */
+int nwg_set_lane_table(int n_rules, const int32_t *rules);
+
 /* Soft settings (py_exports.cpp soft_temperature / soft_precision).  precision 0: float32
  * scaled pass (float32 batches only), 1: float64 scaled pass. */
 int nwg_batch_set_soft(nwg_batch *b, double temperature, int precision);

@@ -48,6 +48,10 @@ except ImportError as _e:  # the native module is the product: fail loudly
         "nwgrad_amd: the native module is not built (run `python nwgrad_amd/_build.py` or "
         "`python -c 'import __graft_entry__ as g; g.build()'`): " + str(_e)) from _e
 
+from . import tuning as _tuning
+
+_tuning.install()   # the measured lane-width table (nwgrad_amd/lane_table.json), when present
+
 AA_ORDER = "ACDEFGHIKLMNPQRSTVWY"
 DNA = "ACGT"
 

@@ -397,6 +397,14 @@ public:
         fill_ = f;
     }
     const std::string& fill() const noexcept { return fill_; }
+    // Columns a lane of the hard / score-only tiles: 0 chosen per batch (the lane-width table,
+    // else the built-in rule), 8 or 16 forced; results do not depend on it.
+    void set_columns_per_lane(int c) { check_(nwg_batch_set_lanes(b_, c)); }
+    int columns_per_lane() const {
+        int c = 0;
+        check_(nwg_batch_lanes(b_, &c));
+        return c;
+    }
 
     void set_soft_temperature(double t) {
         if (!(t > 0.0)) throw std::invalid_argument("nwgrad: soft_temperature must be > 0");

@@ -81,7 +81,7 @@ int noalloc_run(nwg_ctx* ctx, void* scratch_ptr, const nwg_problem& prob, int na
     int panels = 1;
     cut_tiles(mlen, nlen, pairs, kS, gmin, true, order, tiles, buckets, ptiles, max_steps, panels);
     int S = kS;
-    if (!tiles.empty() && (int64_t)tiles.size() < 2 * 4 * (int64_t)ctx->n_cu && widest <= kSN * kMaxG) {
+    if (choose_lanes(nlen, pairs, (int64_t)tiles.size(), ctx->n_cu, 0) == kSN) {   // as a batch of these pairs
         S = kSN;
         cut_tiles(mlen, nlen, pairs, kSN, gmin, true, order, tiles, buckets, ptiles, max_steps, panels);
     }

@@ -11,6 +11,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <string>
@@ -241,6 +242,7 @@ struct nwg_batch {
     bool guides_valid = false, paths_valid = false;
     bool banded = false;         // while a banded pass is launched: its half-width
     int band = 0;
+    int lanes_S = 0;             // columns per lane of the hard / score-only tiles: 0 = chosen per batch
     DevPtr<uint8_t> soft_ok;
     std::vector<uint8_t> host_ok;
     int64_t soft_reruns = 0;
@@ -480,6 +482,46 @@ int build_long_plan(nwg_batch* b, std::vector<int32_t> pairs, LongPlan& L) {
     return NWG_OK;
 }
 
+// The lane-width table of the hard / score-only fills (nwg_set_lane_table): rules (n_max,
+// tiles16_max, S), first match wins.  A batch whose median len(B) is at most n_max and whose
+// tile count at 16 columns a lane is at most tiles16_max / 16 per CU cuts its tiles at S
+// columns a lane.  Without a matching rule the built-in choice holds (choose_lanes).
+struct LaneRule {
+    int32_t n_max, tiles16_max, S;
+};
+std::mutex g_lane_mu;
+std::vector<LaneRule> g_lane_rules;
+
+// Columns a lane for the tiles of `sp`, given their tile count at kS.  Fewer tiles than two
+// per SIMD leave the fill latency-bound (one wave a SIMD has nothing to hide a step's latency
+// behind): kSN columns a lane gives about twice as many waves, when every pair fits one wave
+// at that width (protein_global_hard, 1,182 tiles at 16 columns: 266 -> 328 GCUPS;
+// dna_local_short, 65,536 tiles, keeps 16, where 8 measured 18 % slower).  A rule of the
+// lane-width table, or the batch's own setting, overrides that choice.
+int choose_lanes(const std::vector<int32_t>& nlen, const std::vector<int32_t>& sp, int64_t tiles_at_kS, int n_cu,
+                 int forced) {
+    int64_t widest = 0;
+    for (int32_t i : sp) widest = std::max<int64_t>(widest, nlen[(size_t)i]);
+    const bool narrow_fits = widest <= kSN * kMaxG;
+    if (forced == kS || (forced == kSN && !narrow_fits)) return kS;
+    if (forced == kSN) return kSN;
+    {
+        std::lock_guard<std::mutex> lk(g_lane_mu);
+        if (!g_lane_rules.empty() && !sp.empty()) {
+            std::vector<int32_t> ns;
+            ns.reserve(sp.size());
+            for (int32_t i : sp) ns.push_back(nlen[(size_t)i]);
+            std::nth_element(ns.begin(), ns.begin() + ns.size() / 2, ns.end());
+            const int32_t med = ns[ns.size() / 2];
+            const double per_cu16 = 16.0 * (double)tiles_at_kS / (double)std::max(1, n_cu);
+            for (const LaneRule& r : g_lane_rules)
+                if (med <= r.n_max && per_cu16 <= (double)r.tiles16_max)
+                    return (r.S == kSN && narrow_fits) ? kSN : kS;
+        }
+    }
+    return (tiles_at_kS > 0 && tiles_at_kS < 2 * 4 * (int64_t)n_cu && narrow_fits) ? kSN : kS;
+}
+
 // The pairs of `pairs`: those within one wavefront's columns into the tile plan, the wider
 // ones into the long-pair plan.
 int build_plans(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& main, LongPlan& wide) {
@@ -487,14 +529,7 @@ int build_plans(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& main, Lon
     for (int32_t i : pairs) (b->nlen[(size_t)i] > kMaxN ? lp : sp).push_back(i);
     int rc = build_plan(b, sp, main, kS, true);
     if (rc) return rc;
-    // Fewer tiles than two per SIMD leave the fill latency-bound (one wave a SIMD has nothing
-    // to hide a step's latency behind): cut the tiles at kSN columns a lane, about twice as
-    // many waves, when every pair fits one wave at that width.  protein_global_hard (1,182
-    // tiles at 16 columns): 266 -> 328 GCUPS; dna_local_short (65,536 tiles) keeps 16 columns,
-    // where 8 measured 18 % slower.
-    int64_t widest = 0;
-    for (int32_t i : sp) widest = std::max<int64_t>(widest, b->nlen[(size_t)i]);
-    if (!main.tiles.empty() && (int64_t)main.tiles.size() < 2 * 4 * (int64_t)b->ctx->n_cu && widest <= kSN * kMaxG) {
+    if (choose_lanes(b->nlen, sp, (int64_t)main.tiles.size(), b->ctx->n_cu, b->lanes_S) == kSN) {
         rc = build_plan(b, sp, main, kSN, true);
         if (rc) return rc;
     }
@@ -1097,6 +1132,40 @@ int nwg_batch_set_params_dev(nwg_batch* b, const double* theta_dev) {
 int nwg_batch_set_fill(nwg_batch* b, int fill) {
     if (!b || fill < 0 || fill > 1) return NWG_EINVAL;
     b->fill = fill;
+    return NWG_OK;
+}
+
+int nwg_batch_set_lanes(nwg_batch* b, int columns) {
+    if (!b) return NWG_EINVAL;
+    if (columns != 0 && columns != kS && columns != kSN)
+        return fail(b->ctx, NWG_EINVAL, "nwgrad_hip: columns per lane must be 0 (chosen per batch), %d or %d", kSN, kS);
+    b->lanes_S = columns;
+    if (b->prob.grad_mode != 2) {
+        NWG_HIP(b->ctx, hipSetDevice(b->ctx->device));
+        std::vector<int32_t> all((size_t)b->N);
+        std::iota(all.begin(), all.end(), 0);
+        const int rc = build_plans(b, all, b->plan, b->lpl);
+        if (rc) return rc;
+    }
+    return NWG_OK;
+}
+
+int nwg_batch_lanes(nwg_batch* b, int* columns) {
+    if (!b || !columns) return NWG_EINVAL;
+    *columns = b->prob.grad_mode == 2 ? kSS : b->plan.S;
+    return NWG_OK;
+}
+
+int nwg_set_lane_table(int n_rules, const int32_t* rules) {
+    if (n_rules < 0 || (n_rules > 0 && !rules)) return NWG_EINVAL;
+    std::vector<LaneRule> v;
+    for (int k = 0; k < n_rules; ++k) {
+        const LaneRule r{rules[3 * k], rules[3 * k + 1], rules[3 * k + 2]};
+        if (r.S != kS && r.S != kSN) return NWG_EINVAL;
+        v.push_back(r);
+    }
+    std::lock_guard<std::mutex> lk(g_lane_mu);
+    g_lane_rules.swap(v);
     return NWG_OK;
 }
 

@@ -558,6 +558,12 @@ void bind_batch(py::module_& m, const char* name) {
             "score tiles one per block in dispatch order, soft and long-pair work pulled from device queues\n"
             "by a persistent grid; \"striped\" / \"rowwise\": plain per-bucket launches.  Results are identical.")
         .def_property(
+            "columns_per_lane", [](const PB& s) { return s.b->columns_per_lane(); },
+            [](PB& s, int v) { s.b->set_columns_per_lane(v); },
+            "Columns a lane of the hard / score-only tiles (8 or 16; the soft pass's 8 for soft batches).\n"
+            "Set 0 to choose per batch (the lane-width table of nwgrad_amd.tuning, else the built-in rule),\n"
+            "8 or 16 to force.  Results are identical.  No counterpart in the reference.")
+        .def_property(
             "soft_temperature", [](const PB& s) { return s.b->soft_temperature(); },
             [](PB& s, double v) { s.b->set_soft_temperature(v); })
         .def_property(
@@ -811,6 +817,15 @@ PYBIND11_MODULE(_nwgrad_amd, m) {
     bind_batch<double>(m, "SeqPairBatchDouble");
     bind_server(m);
 
+    m.def(
+        "_set_lane_table",
+        [](const std::vector<std::array<int32_t, 3>>& rules) {
+            std::vector<int32_t> flat;
+            for (const auto& r : rules) flat.insert(flat.end(), r.begin(), r.end());
+            const int rc = nwg_set_lane_table(static_cast<int>(rules.size()), flat.data());
+            if (rc != NWG_OK) throw py::value_error("nwgrad_amd: lane-width rules are (n_max, tiles_per_cu_max, 8 or 16)");
+        },
+        py::arg("rules"), "Install the process-wide lane-width table (see nwgrad_amd.tuning).");
     m.def("device_available", []() {
         try {
             Device::ctx(0);
