@@ -450,18 +450,25 @@ public:
     // the training-loop step after set_params; new guides.  Hard batches run it on their own
     // device batch.  A score-only batch's banded pass is the banded Viterbi score with its guide
     // and path (batch_engine.hpp:2188-2205, 2125-2134), which is the companion hard batch's
-    // banded pass around the same guides: its scores are this batch's until the next pass.
+    // banded pass around the same guides: its scores are this batch's until the next pass.  A
+    // soft batch's is the banded forward-backward around the guides (score and gradient, on its
+    // own device batch), its new guides and paths the banded Viterbi's, the companion's.
     double banded_grad(int bandwidth, bool keep_paths = false) {
         if (N_ == 0) return 0.0;
         if (bandwidth <= 0)
             throw std::invalid_argument("nwgrad: banded_grad() needs bandwidth > 0 (got " + std::to_string(bandwidth) +
                                         "); use score_and_grad() for full DP");
-        if (gd_ == GradMode::Soft)
-            throw std::invalid_argument("nwgrad_amd: banded_grad() of a soft batch (the banded forward-backward) is "
-                                        "not on the GPU path; run score_and_grad()");
         if (!guides_) throw std::logic_error("nwgrad: banded_grad() needs a guide path; run score_and_grad() first");
         double total = 0.0;
-        if (gd_ == GradMode::None) {
+        if (gd_ == GradMode::Soft) {
+            if (!gfresh_) resolve_guides_(false);   // the guides of the last pass, under its params
+            const std::vector<int32_t> g = gb_->all_guides_();
+            check_(nwg_batch_set_guides(b_, g.data()));
+            gbscores_ = false;
+            check_(nwg_batch_run_banded(b_, bandwidth, &total));
+            gb_->set_params(params_);
+            gb_->banded_grad(bandwidth, keep_paths);   // the new guides (and paths): the banded Viterbi's
+        } else if (gd_ == GradMode::None) {
             if (!gfresh_) resolve_guides_(false);   // the guides of the last pass, under its params
             gb_->set_params(params_);
             total = gb_->banded_grad(bandwidth, keep_paths);
@@ -750,15 +757,13 @@ public:
                                         "\" in a batch over alphabet \"" + alpha_->symbols() + "\"");
         if (band < 0) throw std::invalid_argument("nwgrad: band must be >= 0, got " + std::to_string(band));
         const bool banded = band > 0 || !guides.empty();
-        if (banded && gd_ == GradMode::Soft)
-            throw std::invalid_argument("nwgrad_amd: banded align() of a soft batch (the banded forward-backward) is not "
-                                        "on the GPU path");
         if (!guides.empty() && guides.size() != seqs_a.size())
             throw std::invalid_argument("nwgrad: aligned_a / aligned_b need one entry per pair");
         BatchResult r{std::vector<double>(seqs_a.size(), 0.0), AlignParams(params.alphabet(), params.num_sets())};
         if (seqs_a.empty()) return r;
         // a banded score-only pass is the banded Viterbi's scores: a hard batch's, gradient unread
-        SeqPairBatchT<T> tmp(seqs_a, seqs_b, params, gm_, am_, banded ? GradMode::Hard : gd_, n_threads_, device_);
+        SeqPairBatchT<T> tmp(seqs_a, seqs_b, params, gm_, am_, banded && gd_ == GradMode::None ? GradMode::Hard : gd_,
+                             n_threads_, device_);
         if (tracks || gap_tracks) tmp.set_tracks(tracks, gap_tracks);
         if (gd_ == GradMode::Soft) {
             tmp.set_soft_temperature(soft_temp_);
@@ -842,6 +847,12 @@ private:
         }
         gb_->score_and_grad(keep);
         gfresh_ = true;
+    }
+    // Every pair's guide, pair p's m + 1 entries at off_a[p] + p (a hard batch's).
+    std::vector<int32_t> all_guides_() const {
+        std::vector<int32_t> g(static_cast<size_t>(std::max<int64_t>(oa_.back() + N_, 1)));
+        check_(nwg_batch_guides(b_, g.data()));
+        return g;
     }
     std::string decode_(const std::vector<uint8_t>& codes, const std::vector<int64_t>& off, size_t i) const {
         if (i >= static_cast<size_t>(N_)) throw std::out_of_range("SeqPairBatch index out of range");

@@ -232,7 +232,7 @@ struct nwg_batch {
     size_t soft_scratch = kScratchBudget;                    // row-segment scratch of one launch
     int64_t table_budget = kHardTableBudget;                 // short-pair pointer codes of one fill
     int64_t hard_grid = 0;                                   // the last hard fill's persistent grid (0: plain)
-    int soft_occ_cache[4] = {0, 0, 0, 0};   // resident blocks per CU: float, double; untracked, tracked
+    int soft_occ_cache[8] = {};   // resident blocks per CU: float, double; untracked, tracked; full, banded
     // hard: the guides of the last pass (pair p's m + 1 entries at off_a[p] + p), and its stored
     // paths when kept (pair p's ops end at off_a[p+1] + off_b[p+1]; op counts; end cells)
     DevPtr<int32_t> guide;
@@ -1473,7 +1473,11 @@ template <typename SF>
 using SoftKernel = void (*)(nwg::SoftArgs<SF>);
 
 template <typename SF>
-SoftKernel<SF> soft_kernel_for(int mode, bool tracked) {
+SoftKernel<SF> soft_kernel_for(int mode, bool tracked, bool band) {
+    if (band)   // banded passes: untracked batches (the caller refuses tracked ones)
+        return mode == 1 ? nwg::soft_kernel<SF, nwg::kLocal, kSS, false, true>
+                         : mode == 2 ? nwg::soft_kernel<SF, nwg::kGlocal, kSS, false, true>
+                                     : nwg::soft_kernel<SF, nwg::kGlobal, kSS, false, true>;
     if (tracked)
         return mode == 1 ? nwg::soft_kernel<SF, nwg::kLocal, kSS, true>
                          : mode == 2 ? nwg::soft_kernel<SF, nwg::kGlocal, kSS, true>
@@ -1595,14 +1599,15 @@ int launch_soft(nwg_batch* b, const SoftPlan& pl, SoftPlan& wl) {
     const int K = tr ? b->K : 1;
     NWG_HIP(ctx, dps.reserve((size_t)K * nn));
     NWG_HIP(ctx, SS::gw(b).reserve((size_t)4 * K));
-    SoftKernel<SF> kern = soft_kernel_for<SF>(b->prob.mode, tr);
+    const bool band = b->banded;
+    SoftKernel<SF> kern = soft_kernel_for<SF>(b->prob.mode, tr, band);
     const size_t lds = nwg::soft_lds_bytes<SF>(b->na, K, b->hist_slots);
     if (lds > kSoftLdsMax)
         return fail(ctx, NWG_EINVAL,
                     "nwgrad_amd: the soft pass with %d parameter sets over a %d-letter alphabet needs %zu bytes of LDS "
                     "per wavefront (at most %zu)", K, b->na, lds, kSoftLdsMax);
     // resident blocks per CU: queried once per kernel and LDS size (no query inside a captured step)
-    int& occ = b->soft_occ_cache[(std::is_same<SF, float>::value ? 0 : 1) + (tr ? 2 : 0)];
+    int& occ = b->soft_occ_cache[(std::is_same<SF, float>::value ? 0 : 1) + (tr ? 2 : 0) + (band ? 4 : 0)];
     if (occ <= 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64, lds) != hipSuccess || occ <= 0))
         occ = kSoftBlocksPerCU;
     b->soft_occ = occ;
@@ -1736,6 +1741,8 @@ int launch_soft(nwg_batch* b, const SoftPlan& pl, SoftPlan& wl) {
     A.tol = std::is_same<SF, float>::value ? 0x1p-20 : 0x1p-45;   // the reference's SoftRange<F>::tol
     A.vmin = (double)std::numeric_limits<SF>::min();
     A.vtiny = (double)std::numeric_limits<SF>::denorm_min();
+    A.guide = band ? b->guide.p : nullptr;
+    A.band = band ? b->band : 0;
     // the weights (and, persistent, the queue counters) from the device params
     hipLaunchKernelGGL(nwg::soft_weights_kernel<SF>, dim3(1), dim3(256), 0, ctx->stream, b->theta.p, nn, K,
                        1.0 / b->temperature, lin ? 1 : 0, dps.p, SS::gw(b).p, persistent ? b->queue.p : nullptr,
@@ -2243,8 +2250,11 @@ int nwg_batch_paths(nwg_batch* b, uint32_t* ops, int32_t* n_ops, int32_t* ends) 
 int nwg_batch_run_banded(nwg_batch* b, int bandwidth, double* total_out) {
     if (!b) return NWG_EINVAL;
     nwg_ctx* ctx = b->ctx;
-    if (b->prob.grad_mode != 1)
-        return fail(ctx, NWG_EINVAL, "nwgrad_amd: banded_grad() runs on hard batches (grad_mode=\"hard\") on the GPU path");
+    if (b->prob.grad_mode == 0)
+        return fail(ctx, NWG_EINVAL,
+                    "nwgrad_amd: a score-only banded pass is its hard batch's (grad_mode=\"hard\") banded scores");
+    if (b->prob.grad_mode == 2 && b->tracked)
+        return fail(ctx, NWG_EINVAL, "nwgrad_amd: the banded soft pass of tracked pairs is not on the GPU path");
     if (bandwidth < 0) return fail(ctx, NWG_EINVAL, "nwgrad: band must be >= 0, got %d", bandwidth);
     if (!b->guides_valid)
         return fail(ctx, NWG_ESTATE, "nwgrad: banded_grad() needs a guide path; run score_and_grad() first");
@@ -2259,8 +2269,8 @@ int nwg_batch_run_banded(nwg_batch* b, int bandwidth, double* total_out) {
 int nwg_batch_set_guides(nwg_batch* b, const int32_t* guides) {
     if (!b || !guides) return NWG_EINVAL;
     nwg_ctx* ctx = b->ctx;
-    if (b->prob.grad_mode != 1)
-        return fail(ctx, NWG_EINVAL, "nwgrad_hip: guides belong to hard batches (grad_mode=\"hard\")");
+    if (b->prob.grad_mode == 0)
+        return fail(ctx, NWG_EINVAL, "nwgrad_hip: guides belong to hard and soft batches");
     for (int64_t p = 0; p < b->N; ++p) {   // the reference's validate_guide (aligner.hpp:2622-2642)
         const int32_t* g = guides + b->hoa[(size_t)p] + p;
         const int m = b->mlen[(size_t)p], n = b->nlen[(size_t)p];
@@ -2275,6 +2285,7 @@ int nwg_batch_set_guides(nwg_batch* b, const int32_t* guides) {
     NWG_HIP(ctx, hipSetDevice(ctx->device));
     const int64_t n = b->N ? b->hoa[(size_t)b->N] + b->N : 0;
     NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    NWG_HIP(ctx, b->guide.reserve((size_t)std::max<int64_t>(n, 1)));   // soft batches: the band's guides
     if (n) NWG_HIP(ctx, hipMemcpyAsync(b->guide.p, guides, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
     NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     b->guides_valid = true;

@@ -135,6 +135,10 @@ struct SoftArgs {
     double tol;              // lost-mass tolerance (2^-20 float32, 2^-45 double)
     double vmin;             // the working format's smallest normal value
     double vtiny;            // its smallest subnormal value
+    // Banded passes (the BAND kernels): each pair's guide (m + 1 columns at off_a[p] + p) and
+    // the band's half-width.
+    const int32_t* guide;
+    int32_t band;
 };
 
 template <typename SF>
@@ -299,7 +303,7 @@ __device__ __forceinline__ void load_rec(const SF* __restrict__ base, int lane, 
 // A wave that holds an item is running, so every wait ends.  A tile in the pool is done when
 // its last item has made its writes visible at agent scope: another XCD's L2 then holds no
 // dirty line of the region that a later tile reuses.
-template <typename SF, int AM, int S, bool TR>
+template <typename SF, int AM, int S, bool TR, bool BAND>
 __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int32_t* __restrict__ order,
                           const SoftTile* __restrict__ tdp, int64_t slot, const SF* __restrict__ spsub,
                           const SF* __restrict__ sgw,
@@ -332,6 +336,35 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
     }
     const int NP = td.panels;
     const int PW = G * S;   // columns of one panel of this pair
+    // Banded (aligner.hpp:2644-2690): row r's cells are the columns [lo, hi] around the guide,
+    // lo = max(1, g[r] - bw), hi = min(n, max(g[r] + bw, g[r+1] - 1 + bw)) (Global's row m
+    // reaches n); column 0 of row r is in the band while g[r] <= bw (border_rows); the other
+    // cells hold zero, in both directions.  Local's column-0 starts and Local / Glocal's row 0
+    // stay as the full pass has them in the forward, but Local's Z counts a row's cells only
+    // inside [jlo0, jhi0] (column 0 while g[r] <= bw; soft_kernel_impl.inl:289, 487), and the
+    // backward holds zero outside every row's [jlo0, jhi0] (soft_kernel_impl.inl:826-828).
+    const int32_t* bgj = nullptr;
+    int bw = 0;
+    if constexpr (BAND) {
+        if (p >= 0) {
+            bgj = A.guide + A.off_a[p] + p;
+            bw = A.band < n ? A.band : n;
+        }
+    }
+    auto band_of = [&](int r, int& blo, int& bhi) {
+        const int g0 = bgj[r];
+        blo = g0 - bw > 1 ? g0 - bw : 1;
+        if (!L && !GL && r == m) {
+            bhi = n;
+            return;
+        }
+        int h = g0 + bw;
+        if (r < m) {
+            const int h1 = bgj[r + 1] - 1 + bw;
+            h = h1 > h ? h1 : h;
+        }
+        bhi = h < n ? h : n;
+    };
     // the panels whose forward and backward this call runs
     const bool whole = idir == -1 || idir == 2;
     const int pf0 = whole ? 0 : ipp, pf1 = whole ? NP : (idir == 0 ? ipp + 1 : ipp);
@@ -472,6 +505,16 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 M[k] = SF(0); X[k] = SF(0);
                 Y[k] = (k < ncols) ? (SF)__builtin_amdgcn_ldexp(ym[k], yex[k] - E) : SF(0);
             }
+            if constexpr (BAND) {   // row 0's border runs up to its band's end (border_cols)
+                if (p >= 0) {
+                    int blo, bhi;
+                    band_of(0, blo, bhi);
+#pragma unroll
+                    for (int k = 0; k < S; ++k)
+                        if (j0 + k + 1 > bhi) Y[k] = SF(0);
+                    if (j0 > bhi) cY = SF(0);
+                }
+            }
         }
         SF z = SF(0);
 #pragma unroll
@@ -483,8 +526,13 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         if (live) {
             if (L || (GL && m == 0)) {
                 SF s = j0 == 0 ? cM + cX + cY : SF(0);
+                int zlo = 1, zhi = n;   // banded: row 0's cells in [jlo0(0), jhi0(0)]
+                if constexpr (BAND) {
+                    band_of(0, zlo, zhi);
+                    if (!(bgj[0] <= bw)) s = SF(0);
+                }
 #pragma unroll
-                for (int k = 0; k < S; ++k) s += (k < ncols) ? T[k] : SF(0);
+                for (int k = 0; k < S; ++k) s += (k < ncols && j0 + k + 1 >= zlo && j0 + k + 1 <= zhi) ? T[k] : SF(0);
                 expacc_add(zacc, (double)s, E);
             }
             store_seg<SF, S>(seg + (int64_t)g * LY::row, lane, M, X, Y, cM, cX, cY, E, z);
@@ -492,7 +540,9 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         SF oM = M[S - 1], oX = X[S - 1], oY = Y[S - 1];
         int oE = E;
         int kpend = 0;                                                // next row's units: E + kpend
-        bool fzero = !L && !(z > SF(0)) && !(cM + cX + cY > SF(0));   // the lane's last row held no mass
+        // the lane's last row held no mass (Local lanes always hold their start term, except
+        // outside a band, where they take a neighbour's units as the other modes' do)
+        bool fzero = (!L || BAND) && !(z > SF(0)) && !(cM + cX + cY > SF(0));
         const bool pin = g == 0 && j0 > 0;                            // reads the previous panel's column
         const bool pout = g == G - 1 && pp + 1 < NP;                  // writes this panel's last column
         int acode = (live && m >= 1) ? (int)ap[0] : 0;                // code of the lane's next row
@@ -570,8 +620,15 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 const SF ebs = r_eb * s, obs = r_ob * s;
                 if (j0 == 0) {   // column 0 of row i (soft_kernel_impl.inl:669-674)
                     if constexpr (L) { nM = sldexp(SF(1), -Ec); nX = SF(0); nY = SF(0); }
-                    else { nM = SF(0); nY = SF(0); nX = sfma(cX, ebs, (cM + cY) * obs); }
+                    else {
+                        nM = SF(0); nY = SF(0); nX = sfma(cX, ebs, (cM + cY) * obs);
+                        if constexpr (BAND) {
+                            if (!(bgj[i] <= bw)) nX = SF(0);
+                        }
+                    }
                 }
+                int blo = 1, bhi = n;
+                if constexpr (BAND) band_of(i, blo, bhi);
                 const SF fr = L ? sldexp(SF(1), -Ec) : SF(0);
                 const SF* __restrict__ prow = spsub + ac * (na + 1);
                 SF pw[S];
@@ -584,9 +641,12 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
 #pragma unroll
                 for (int k = 0; k < S; ++k) {
                     const SF uM = M[k], uX = X[k], uY = Y[k];
-                    const SF mv = sfma(dT, pw[k] * s, fr);
-                    const SF xv = sfma(uX, ebs, (uM + uY) * obs);
-                    const SF yv = sfma(ly, r_ea, (lm + lx) * r_oa);
+                    SF mv = sfma(dT, pw[k] * s, fr);
+                    SF xv = sfma(uX, ebs, (uM + uY) * obs);
+                    SF yv = sfma(ly, r_ea, (lm + lx) * r_oa);
+                    if constexpr (BAND) {
+                        if (j0 + k + 1 < blo || j0 + k + 1 > bhi) { mv = SF(0); xv = SF(0); yv = SF(0); }
+                    }
                     dT = T[k];
                     const SF tv = mv + xv + yv;
                     M[k] = mv; X[k] = xv; Y[k] = yv; T[k] = tv;
@@ -602,9 +662,12 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                     const int k2 = silogb(mx);
                     kpend = k2 > R::kmax ? R::kmax : (k2 < -R::kmax ? -R::kmax : k2);
                 }
-                fzero = !L && !(mx > SF(0)) && !(cM + cX + cY > SF(0));
+                fzero = (!L || BAND) && !(mx > SF(0)) && !(cM + cX + cY > SF(0));
                 if (L || (GL && i == m)) {
                     SF sr = j0 == 0 ? cM + cX + cY : SF(0);
+                    if constexpr (BAND) {   // column 0 counts inside the band only (jlo0)
+                        if (j0 == 0 && !(bgj[i] <= bw)) sr = SF(0);
+                    }
 #pragma unroll
                     for (int k = 0; k < S; ++k) sr += (k < nc) ? T[k] : SF(0);
                     expacc_add(zacc, (double)sr, E);
@@ -828,7 +891,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
 
         // Recompute forward row i (i >= 1) into Fd from row i-1 in Fs and row i's record: the
         // forward's row step (same operations, same order) in the record's units.
-        auto recompute = [&](const SF(&Fs)[LY::vals], SF(&Fd)[LY::vals], const SF(&rc)[4], int code, int sets) {
+        auto recompute = [&](const SF(&Fs)[LY::vals], SF(&Fd)[LY::vals], const SF(&rc)[4], int code, int sets, int r) {
             const int Ec = em_E(rc[3]);
             const SF s = sldexp(SF(1), em_E(Fs[LY::iE]) - Ec);   // row i-1's values into row i's units
             SF r_oa = oa, r_ea = ea, r_ob = ob, r_eb = eb;
@@ -847,12 +910,17 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             SF lm = rc[0], lx = rc[1], ly = rc[2];
             SF mx = (j0 == 0) ? rc[0] + rc[1] + rc[2] : SF(0);
             const int nc = opaque(ncols);
+            int blo = 1, bhi = n;
+            if constexpr (BAND) band_of(r, blo, bhi);
 #pragma unroll
             for (int k = 0; k < S; ++k) {
                 const SF uM = Fs[k], uX = Fs[LY::iX + k], uY = Fs[LY::iY + k];
-                const SF mv = sfma(dT, pw[k] * s, fr);
-                const SF xv = sfma(uX, ebs, (uM + uY) * obs);
-                const SF yv = sfma(ly, r_ea, (lm + lx) * r_oa);
+                SF mv = sfma(dT, pw[k] * s, fr);
+                SF xv = sfma(uX, ebs, (uM + uY) * obs);
+                SF yv = sfma(ly, r_ea, (lm + lx) * r_oa);
+                if constexpr (BAND) {
+                    if (j0 + k + 1 < blo || j0 + k + 1 > bhi) { mv = SF(0); xv = SF(0); yv = SF(0); }
+                }
                 dT = uM + uX + uY;
                 const SF tv = mv + xv + yv;
                 Fd[k] = mv; Fd[LY::iX + k] = xv; Fd[LY::iY + k] = yv;
@@ -900,14 +968,18 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             // This row's units.  A segment without mass of its own (Global's row m left of
             // column n, or rows whose mass has not arrived) takes the neighbour's exponent; one
             // whose neighbour's values would not fit moves up to them minus the headroom.
-            const bool own_zero = (i == m) ? (!L && !GL && !owns_end) : (!L && bzero);
+            const bool own_zero = (i == m) ? (!L && !GL && !owns_end) : ((!L || BAND) && bzero);
             int Ec = (i == m) ? 0 : EB + kpend;
             SF s = (i == m) ? SF(1) : sldexp(SF(1), -kpend);   // row i+1's values into them
+            bool adopt = false;
             if (has_right && own_zero) {
                 // no mass yet: take the neighbour's units, normalised to its boundary values
+                // (the lane's own row i+1 values are zero; the one it carries from the right
+                // neighbour, BM(i+1, j0+S+1), is converted on its own below)
                 const SF mr = dmax(rM, rY);
                 Ec = mr > SF(0) ? rE + silogb(mr) : rE;
                 s = SF(1);
+                adopt = true;
             } else if (has_right) {
                 // headroom, as the forward's (tracked: on the boundary's magnitude)
                 int rmag = rE;
@@ -926,12 +998,24 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             const SF obs = r_ob * s, ebs = r_eb * s;
             // posterior factors: of row i's units, and of row i+1's (the late sums); none of
             // the late sums exist at row m
-            const double fnew = zbad ? 0.0 : __builtin_amdgcn_ldexp(izr, efc + Ec - ze);
-            const double fold = (zbad || i == m) ? 0.0 : __builtin_amdgcn_ldexp(izr, efc + EB - ze);
+            // Banded: a lane whose forward or backward segment is empty (outside the band) keeps
+            // borrowed units in that direction, whose sum with the other's can exceed the double
+            // range while every product it scales is an exact zero; the factors are held to
+            // 2^1000 there (a product that is not zero has values in range and a factor far
+            // below it), so that no 0 x inf appears.
+            int xnew = efc + Ec - ze, xold = efc + EB - ze;
+            if constexpr (BAND) {
+                xnew = xnew < 1000 ? xnew : 1000;
+                xold = xold < 1000 ? xold : 1000;
+            }
+            const double fnew = zbad ? 0.0 : __builtin_amdgcn_ldexp(izr, xnew);
+            const double fold = (zbad || i == m) ? 0.0 : __builtin_amdgcn_ldexp(izr, xold);
             const double fmatch = i == m ? 0.0 : fnew;
             const int nc = opaque(ncols);
+            int blo = 1, bhi = n;
+            if constexpr (BAND) band_of(i, blo, bhi);
             SF byr = rYc;   // BY(i, j+1), right to left
-            SF dn = rdM;    // BM(i+1, j+1), row i+1's units
+            SF dn = adopt ? sldexp(rdM, EB - Ec) : rdM;   // BM(i+1, j+1), row i+1's units (adopting: row i's)
             SF mx = SF(0);
             SF sx = SF(0), sy = SF(0), so = SF(0), sob = SF(0);
             // match posteriors go into the pair's histogram row a_{i+1} (row m: none, added as 0)
@@ -953,7 +1037,8 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 // Columns past len(B) hold exact zeros in Global and Glocal: their profile entry is
                 // 0 and no end weight lies there.  Local's start term reaches every cell, so its
                 // columns past len(B) are cleared.
-                const bool in = !L || k < nc;
+                bool in = !L || k < nc;
+                if constexpr (BAND) in = in && j0 + k + 1 >= blo && j0 + k + 1 <= bhi;
                 const SF bmn = in ? b1 + hh : SF(0);
                 const SF bxn = in ? b2 + hh : SF(0);
                 const SF byv = in ? byn : SF(0);
@@ -979,6 +1064,9 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 const SF hh = byr * r_oa;
                 c0M = b1 + hh;
                 c0X = b2 + hh;
+                if constexpr (BAND) {   // column 0 outside the row's span (jlo0(i) > 0)
+                    if (!(bgj[i] <= bw)) { c0M = SF(0); c0X = SF(0); }
+                }
                 mx = dmax(mx, dmax(c0M, c0X));
                 sx = sfma(Fc[LY::iC + 1], c0X, sx);
             }
@@ -988,7 +1076,11 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             // M + X + Y) times the row's posterior factor, charged for every operation of the
             // cell.  A product of two stored values that leaves the normal range loses at most
             // the smallest subnormal (soft_kernel_impl.inl:933-948).
-            if (!zbad && (mx > SF(0) || mfc > SF(0))) {
+            // Banded: a lane whose cells all lie outside the row's span holds exact zeros backward
+            // (no mass can be lost there), and its units are borrowed, so it charges nothing.
+            bool span = true;
+            if constexpr (BAND) span = (j0 + ncols >= blo && j0 + 1 <= bhi) || (j0 == 0 && bgj[i] <= bw);
+            if (!zbad && span && (mx > SF(0) || mfc > SF(0))) {
                 const double lr = 32.0 * (double)(ncols + 1) * A.vmin * (3.0 * (double)mx + (double)mfc) * fnew +
                                   8.0 * (double)(ncols + 1) * A.vtiny * (fnew + fold);
                 const double lc = (lr <= 1.0e300) ? lr : 1.0e300;
@@ -1043,11 +1135,12 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             if (!(live && i >= 0 && i <= m)) return;
             if (p < kCK - 1) {
                 const int c = i - (kCK - 1 - p);
-                const int nr = i - (c > 0 ? c : 0);   // rows to recompute, ending at row i
-                if (nr >= 1) recompute(Sb, Wb, rec[0], rcode[0], rsets[0]);
+                const int c0 = c > 0 ? c : 0;
+                const int nr = i - c0;   // rows to recompute, ending at row i
+                if (nr >= 1) recompute(Sb, Wb, rec[0], rcode[0], rsets[0], c0 + 1);
 #pragma unroll
                 for (int j = 1; j < kCK - 1 - p; ++j)
-                    if (nr > j) recompute(Wb, Wb, rec[j], rcode[j], rsets[j]);
+                    if (nr > j) recompute(Wb, Wb, rec[j], rcode[j], rsets[j], c0 + 1 + j);
                 if (nr == 0) copy_row(Sb, Wb);
                 step(i, rM, rY, rE, Wb);
             } else {
@@ -1246,7 +1339,7 @@ __host__ __device__ constexpr size_t soft_lds_bytes(int na, int K, int hist_slot
            (size_t)K * (na + 1) * (na + 1) * sizeof(SF) + (size_t)4 * K * sizeof(SF);
 }
 
-template <typename SF, int AM, int S, bool TR>
+template <typename SF, int AM, int S, bool TR, bool BAND = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(SF) == 4 ? 2 : 1)))
 void soft_kernel(SoftArgs<SF> A) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -1311,7 +1404,7 @@ void soft_kernel(SoftArgs<SF> A) {
                 if (tile >= A.n_tiles) break;
             }
         }
-        soft_tile<SF, AM, S, TR>(A, wide ? A.lview : A.view, wide ? A.lorder : A.order,
+        soft_tile<SF, AM, S, TR, BAND>(A, wide ? A.lview : A.view, wide ? A.lorder : A.order,
                                  (wide ? A.ltiles : A.tiles) + tile, (int64_t)blk, spsub, sgw, hist, lacc, lane, tile,
                                  ipp, idir);
     }

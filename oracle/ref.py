@@ -53,7 +53,7 @@ def lib():
                                   ctypes.c_char_p]
         L.nwgref_align.argtypes = [vp, ctypes.c_char_p, P(ctypes.c_double), P(ctypes.c_double), ctypes.c_long,
                                    ctypes.c_char_p, P(ctypes.c_long), ctypes.c_char_p, P(ctypes.c_long),
-                                   P(ctypes.c_double), P(ctypes.c_double)]
+                                   ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -205,8 +205,9 @@ class RefBatch:
         return (a.raw[:n].decode(), b.raw[:n].decode(), coords[:, :k.value].copy(), c0.value.decode(),
                 c1.value.decode())
 
-    def align(self, seqs_a, seqs_b, matrix, gaps):
-        """Streaming align() under (matrix, gaps), band 0: scores and the summed gradient."""
+    def align(self, seqs_a, seqs_b, matrix, gaps, band=0):
+        """Streaming align() under (matrix, gaps): scores and the summed gradient; band > 0
+        bands every pair around the diagonal guide."""
         sa = "".join(seqs_a).encode()
         sb = "".join(seqs_b).encode()
         n = len(seqs_a)
@@ -220,5 +221,5 @@ class RefBatch:
         grad = np.zeros(self.n_alpha ** 2 + 4, np.float64)
         _check(lib().nwgref_align(self._h, self.alphabet.encode(), _d(mat), _d(g), n, sa,
                                   oa.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), sb,
-                                  ob.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), _d(scores), _d(grad)))
+                                  ob.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), int(band), _d(scores), _d(grad)))
         return scores[:n], grad

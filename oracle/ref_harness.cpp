@@ -288,7 +288,7 @@ int nwgref_path(void* hp, long i, char* a, char* b, long cap, long* cols, int64_
 // concatenated characters with offsets, params over the handle's alphabet; band 0 (the full DP).
 // scores: n_pairs doubles; grad: n*n + 4 (the summed gradient, params_out's layout).
 int nwgref_align(void* hp, const char* alphabet, const double* matrix, const double* gaps4, long n_pairs,
-                 const char* seq_a, const long* off_a, const char* seq_b, const long* off_b, double* scores,
+                 const char* seq_a, const long* off_a, const char* seq_b, const long* off_b, int band, double* scores,
                  double* grad) {
     Handle* h = static_cast<Handle*>(hp);
     return guarded([&] {
@@ -299,7 +299,7 @@ int nwgref_align(void* hp, const char* alphabet, const double* matrix, const dou
         }
         const AlignParams p = make_params(alphabet, matrix, gaps4);
         with_batch<void>(h, [&](auto& b) {
-            const BatchResult r = b.align(probs, p, 0);
+            const BatchResult r = b.align(probs, p, band);
             std::memcpy(scores, r.scores.data(), r.scores.size() * sizeof(double));
             if (grad) params_out(r.grad, grad);
         });

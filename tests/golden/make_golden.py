@@ -302,6 +302,38 @@ def bandscore_sets():
             save(f"bandscore_prot_{gm}_{mode}_float32", w, "float32", "none", out)
 
 
+def bandsoft_sets():
+    """Banded passes of soft batches (scaled_or_log, float32): after a full pass,
+    set_params(the perturbed point) and banded_grad(bw, keep_paths=True) -- the banded
+    forward-backward's scores and gradients around the eager guides, and the banded Viterbi's
+    new guides and paths (batch_engine.hpp:2188-2205, 2125-2134) -- and streaming align() with a
+    band around the diagonal (scores and the summed gradient)."""
+    for gm in ("affine", "linear"):
+        for mode in ("global", "local", "glocal"):
+            w = wl.random_pairs(f"golden_bandsoft_{gm}_{mode}", 20, wl.AA, (1, 120), (1, 160),
+                                gaps=(11.0, 1.0, 11.0, 1.0) if gm == "affine" else (0.0, 1.5, 0.0, 1.5),
+                                gap_model=gm, mode=mode, related=0.6)
+            sa, sb = w.strings()
+            mat1, gaps1 = _perturbed(w, "p1")
+            out = {"matrix1": mat1, "gaps1": gaps1, "bands": np.array((2, 16))}
+            for bw in (2, 16):
+                b = ref.RefBatch(sa, sb, w.alphabet, w.matrix, w.gaps, gap_model=gm, mode=mode, grad_mode="soft",
+                                 n_threads=1)
+                b.set_soft(precision="float32", impl="scaled_or_log")
+                b.score_and_grad()
+                out["guides0"] = b.guides(w.off_a)   # the eager guides the band follows
+                b.set_params(mat1, gaps1)
+                out[f"band{bw}_total"] = np.array(b.banded_grad(bw, True))
+                out[f"band{bw}_scores"], out[f"band{bw}_grads"] = _hard_results(b, w.n_pairs)
+                out[f"band{bw}_guides"] = b.guides(w.off_a)
+                out[f"band{bw}_cigars"] = _paths_of(b, w)["cigars"]
+            b = ref.RefBatch(sa[:2], sb[:2], w.alphabet, w.matrix, w.gaps, gap_model=gm, mode=mode, grad_mode="soft",
+                             n_threads=1)
+            b.set_soft(precision="float32", impl="scaled_or_log")
+            out["align8_scores"], out["align8_grad"] = b.align(sa, sb, mat1, gaps1, band=8)
+            save(f"bandsoft_prot_{gm}_{mode}_float32", w, "float32", "soft", out)
+
+
 def path_sets():
     for gm in ("affine", "linear"):
         for mode in ("global", "local", "glocal"):
@@ -320,6 +352,7 @@ def path_sets():
                                 gap_model=gm, mode=mode, related=0.0)
             path_set(f"paths_long_{gm}_{mode}_float32", w, "float32", bands=(8,), traceback="pointers")
     bandscore_sets()
+    bandsoft_sets()
     # the guides of soft batches (the eager guide: the hard fill under the params scored)
     for mode in ("global", "local", "glocal"):
         w = wl.random_pairs(f"golden_softguide_{mode}", 16, wl.AA, (1, 90), (1, 120), gaps=(11.0, 1.0, 11.0, 1.0),
@@ -353,6 +386,9 @@ def path_sets():
 
 
 def main():
+    if sys.argv[1:] == ["bandsoft"]:
+        bandsoft_sets()
+        return
     if sys.argv[1:] == ["bandscore"]:
         bandscore_sets()
         return

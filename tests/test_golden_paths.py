@@ -100,3 +100,55 @@ def test_bandscore_fixture_is_the_hard_banded_pass(path):
         assert b.banded_grad(int(bw), False) == float(z[f"band{bw}_total"])
         np.testing.assert_array_equal(b.scores(), z[f"band{bw}_scores"])
         np.testing.assert_array_equal(b.guides(z["off_a"]), z[f"band{bw}_guides"])
+
+
+BANDSOFT = sorted(glob.glob(os.path.join(HERE, "bandsoft_*.npz")))
+
+
+@pytest.mark.skipif(not BANDSOFT, reason="no soft banded fixtures")
+@pytest.mark.parametrize("path", BANDSOFT, ids=lambda p: os.path.basename(p)[:-4])
+def test_bandsoft_guides_are_the_hard_banded_pass(path):
+    """A soft batch's banded pass leaves the guides and paths of the banded Viterbi around the
+    same (eager) guides: the reference engine's hard banded pass gives them bit for bit."""
+    import oracle
+    if not oracle.have_ref():
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    from oracle import ref
+    z = np.load(path)
+    sa, sb = _seqs(z)
+    for bw in z["bands"]:
+        b = ref.RefBatch(sa, sb, str(z["alphabet"]), z["matrix"], z["gaps"], gap_model=str(z["gap_model"]),
+                         mode=str(z["mode"]), grad_mode="hard", n_threads=1)
+        b.score_and_grad()
+        b.set_params(z["matrix1"], z["gaps1"])
+        b.banded_grad(int(bw), False)
+        np.testing.assert_array_equal(b.guides(z["off_a"]), z[f"band{bw}_guides"])
+
+
+@pytest.mark.skipif(not BANDSOFT, reason="no soft banded fixtures")
+@pytest.mark.parametrize("path", BANDSOFT, ids=lambda p: os.path.basename(p)[:-4])
+def test_bandsoft_fixture_against_brute_force(path):
+    """The reference's banded soft results against an independent brute-force forward-backward
+    over the band (tests/banded_soft_model.py), on the fixture's smaller pairs; the model itself
+    is checked against the oracle's log-space full pass first."""
+    import oracle
+    from tests.banded_soft_model import banded_soft
+    z = np.load(path)
+    gm, mode = str(z["gap_model"]), str(z["mode"])
+    oa, ob = z["off_a"], z["off_b"]
+    sizes = (oa[1:] - oa[:-1]) * (ob[1:] - ob[:-1])
+    pairs = [int(p) for p in np.argsort(sizes, kind="stable")[:4]]
+    for p in pairs:
+        a, b = z["codes_a"][oa[p]:oa[p + 1]], z["codes_b"][ob[p]:ob[p + 1]]
+        lz, cnt = banded_soft(a, b, z["matrix"], z["gaps"], mode, None, None, gap_model=gm)
+        s_o, g_o = oracle.soft_log(z["matrix"], z["gaps"], a, np.array([0, len(a)]), b, np.array([0, len(b)]),
+                                   gap_model=gm, mode=mode)
+        assert abs(lz - s_o[0]) <= 1e-9 * max(1.0, abs(s_o[0]))
+        np.testing.assert_allclose(cnt, g_o[0], rtol=1e-8, atol=1e-9)
+        g0 = z["guides0"][oa[p] + p:oa[p + 1] + p + 1]
+        for bw in z["bands"]:
+            lz, cnt = banded_soft(a, b, z["matrix1"], z["gaps1"], mode, g0, int(bw), gap_model=gm)
+            want_s, want_g = float(z[f"band{bw}_scores"][p]), z[f"band{bw}_grads"][p]
+            assert abs(lz - want_s) <= 2e-6 * max(1.0, abs(want_s))   # the reference runs float32
+            scale = np.max(np.abs(want_g))
+            assert np.all(np.abs(cnt - want_g) <= 1e-5 * (np.abs(want_g) + 1e-7 * scale) + 1e-12), (p, bw)

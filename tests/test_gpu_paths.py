@@ -27,6 +27,7 @@ PATHS = sorted(glob.glob(os.path.join(HERE, "paths_*.npz")))
 SOFTGUIDE = sorted(glob.glob(os.path.join(HERE, "softguide_*.npz")))
 ALIGN = sorted(glob.glob(os.path.join(HERE, "align_*.npz")))
 BANDSCORE = sorted(glob.glob(os.path.join(HERE, "bandscore_*.npz")))
+BANDSOFT = sorted(glob.glob(os.path.join(HERE, "bandsoft_*.npz")))
 
 
 def _guides(b):
@@ -103,6 +104,62 @@ def test_banded_score_only_golden(gpu, path):
         # a full pass afterwards serves its own scores again
         b.score_and_grad()
         assert b.scores()[0] == b[0].score
+
+
+@pytest.mark.parametrize("path", BANDSOFT, ids=lambda p: os.path.basename(p)[:-4])
+def test_banded_soft_golden(gpu, path):
+    """banded_grad of a soft batch: the banded forward-backward's scores and expected counts
+    around the eager guides (within the soft tolerance of the reference's float32), the banded
+    Viterbi's new guides and paths (bit for bit); streaming align() with a band."""
+    z = np.load(path)
+    say("case", os.path.basename(path))
+    p1 = params_of(str(z["alphabet"]), z["matrix1"], z["gaps1"])
+    for bw in z["bands"]:
+        b = _batch(z)   # grad_mode "soft"
+        b.soft_impl = "scaled_or_log"
+        b.score_and_grad()
+        b.set_params(p1)
+        t = b.banded_grad(int(bw), keep_paths=True)
+        assert_soft_close(b.scores(), grads_rows(b), z[f"band{bw}_scores"], z[f"band{bw}_grads"],
+                          label=f"band {bw}: ")
+        assert abs(t - float(z[f"band{bw}_total"])) <= 1e-5 * max(1.0, abs(float(z[f"band{bw}_total"])))
+        np.testing.assert_array_equal(_guides(b), z[f"band{bw}_guides"])
+        assert b.cigars() == list(z[f"band{bw}_cigars"])
+    sa, sb = _strings(z)
+    host = nw.SeqPairBatch(sa[:2], sb[:2], params_of(str(z["alphabet"]), z["matrix"], z["gaps"]),
+                           gap_model=str(z["gap_model"]), mode=str(z["mode"]), grad_mode="soft")
+    r = host.align(sa, sb, p1, band=8)
+    d = r.grad.to_dict()
+    g = np.concatenate([d["matrix"].ravel(), [d["gap_open_a"], d["gap_extend_a"], d["gap_open_b"], d["gap_extend_b"]]])
+    assert_soft_close(r.scores, g[None, :], z["align8_scores"], z["align8_grad"][None, :], label="align band 8: ")
+
+
+@pytest.mark.parametrize("mode", ["global", "local", "glocal"])
+def test_banded_soft_properties(gpu, mode):
+    """At any size: a band as wide as B is the full pass; a narrower band sums over fewer paths,
+    so its log Z never exceeds the full one's and grows with the band."""
+    # homolog pairs (B a mutated copy of A): unrelated pairs of very different lengths do not fit
+    # the scaled pass at narrow bands, in the reference either (its scaled pass refuses 13 of 800
+    # such pairs at band 2, in double)
+    w = wl.random_pairs(f"banded_soft_props_{mode}", 800, wl.AA, (1, 600), (1, 600), mode=mode, related=1.0,
+                        gaps=(11.0, 1.0, 11.0, 1.0))
+    b = batch(w.alphabet, w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, mode=mode, grad_mode="soft",
+              precision="float64")
+    b.score_and_grad()
+    s0, g0 = b.scores(), grads_rows(b)
+    wide = int(np.max(np.diff(w.off_b))) + 1
+    b.banded_grad(wide)
+    assert_soft_close(b.scores(), grads_rows(b), s0, g0, rtol=1e-12, floor=1e-12, label="wide band vs full: ")
+    prev = None
+    for bw in (2, 8, 32):
+        b.set_params(params_of(w.alphabet, w.matrix, w.gaps))
+        b.score_and_grad()   # the guides of the full pass, each time
+        b.banded_grad(bw)
+        s = b.scores()
+        assert np.all(s <= s0 + 1e-9 * np.maximum(1.0, np.abs(s0)))
+        if prev is not None:
+            assert np.all(s >= prev - 1e-9 * np.maximum(1.0, np.abs(prev)))
+        prev = s
 
 
 @pytest.mark.parametrize("path", SOFTGUIDE, ids=lambda p: os.path.basename(p)[:-4])
@@ -207,6 +264,14 @@ def test_paths_errors(gpu):
         b[0].aligned()
     assert b[0].guide_j is not None and b[0].score is not None
     s = batch(w.alphabet, w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, grad_mode="soft")
-    s.score_and_grad()
-    with pytest.raises(ValueError, match="soft batch"):
+    with pytest.raises(RuntimeError, match="guide path"):
         s.banded_grad(4)
+    s.score_and_grad()
+    s.banded_grad(4)   # soft batches band too
+    K = 2
+    pk = nw.AlignParams.stack([params_of(w.alphabet, w.matrix, w.gaps)] * K)
+    t = nw.SeqPairBatch.from_codes(w.codes_a, w.off_a, w.codes_b, w.off_b, pk, grad_mode="soft",
+                                   tracks=np.zeros(len(w.codes_a), np.uint16))
+    t.score_and_grad()
+    with pytest.raises(ValueError, match="tracked"):
+        t.banded_grad(4)
