@@ -545,7 +545,11 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         bool fzero = (!L || BAND) && !(z > SF(0)) && !(cM + cX + cY > SF(0));
         const bool pin = g == 0 && j0 > 0;                            // reads the previous panel's column
         const bool pout = g == G - 1 && pp + 1 < NP;                  // writes this panel's last column
-        int acode = (live && m >= 1) ? (int)ap[0] : 0;                // code of the lane's next row
+        // the codes of the lane's next three rows (loaded three rows ahead: waiting for a load
+        // also waits for the stores issued before it, and a row step stores its row)
+        int acode = (live && m >= 1) ? (int)ap[0] : 0;
+        int acode2 = (live && m >= 2) ? (int)ap[1] : 0;
+        int acode3 = (live && m >= 3) ? (int)ap[2] : 0;
         int tnext = 0, gnext = 0;                                     // and (tracks) its sets
         if constexpr (TR) {
             if (live && m >= 1) {
@@ -563,13 +567,21 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         // One wavefront step (row i = t - g + 1 on lane g); the loop runs it twice per trip,
         // which lets the compiler keep the step's values in place instead of copying them
         // at the loop's back edge.
-        auto fstep = [&](const int t) {
+        // chk false: a step at which every lane of every pair is on one of its rows (the tile's
+        // middle steps), run without the per-lane branch; the lanes beyond a pair's columns and
+        // the lanes of no pair then compute too, on their own registers and their own rows of
+        // the slot, and only their writes others read (the panel column, Z, the range flag)
+        // are held back.
+        auto fstep = [&](const int t, auto chk) {
+            constexpr bool CHECK = decltype(chk)::value;
             const int i = t - g + 1;
             SF lM = wshift_up(oM), lX = wshift_up(oX), lY = wshift_up(oY);
             int lE = wshift_up(oE);
-            if (live && i >= 1 && i <= m) {
+            if (!CHECK || (live && i >= 1 && i <= m)) {
                 int ac = acode;
-                if (i < m) acode = (int)ap[i];   // one row ahead
+                acode = acode2;
+                acode2 = acode3;
+                if (i + 2 < m) acode3 = (int)ap[i + 2];   // three rows ahead
                 // the row's weights: its sets' (tracks), else the constants
                 SF r_oa = oa, r_ea = ea, r_ob = ob, r_eb = eb;
                 if constexpr (TR) {
@@ -654,8 +666,8 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                     mx = dmax(mx, tv);
                 }
                 cM = nM; cX = nX; cY = nY;
-                if (!(mx <= R::fmax)) bad = true;
-                if (sizeof(SF) == 4 && (Ec >= kPackEMax || Ec <= -kPackEMax)) bad = true;   // no room to pack E
+                if (live && !(mx <= R::fmax)) bad = true;
+                if (live && sizeof(SF) == 4 && (Ec >= kPackEMax || Ec <= -kPackEMax)) bad = true;   // no room to pack E
                 E = Ec;
                 kpend = 0;
                 if (mx > R::hi || (mx < R::lo && mx > SF(0))) {
@@ -663,7 +675,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                     kpend = k2 > R::kmax ? R::kmax : (k2 < -R::kmax ? -R::kmax : k2);
                 }
                 fzero = (!L || BAND) && !(mx > SF(0)) && !(cM + cX + cY > SF(0));
-                if (L || (GL && i == m)) {
+                if (live && (L || (GL && i == m))) {
                     SF sr = j0 == 0 ? cM + cX + cY : SF(0);
                     if constexpr (BAND) {   // column 0 counts inside the band only (jlo0)
                         if (j0 == 0 && !(bgj[i] <= bw)) sr = SF(0);
@@ -678,7 +690,7 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
                 if ((m + G - 1 - g - i) % kCK == kCK - 1) store_seg<SF, S>(seg + (int64_t)(i + g) * LY::row, lane, M, X, Y, cM, cX, cY, E, mx);
                 else store_rec<SF, S>(seg + (int64_t)(i + g) * LY::row, lane, cM, cX, cY, E);
                 oM = M[S - 1]; oX = X[S - 1]; oY = Y[S - 1]; oE = E;
-                if (pout) {
+                if (pout && live) {
                     SF* bo = bnd(pp, i);
                     bo[0] = oM; bo[1] = oX; bo[2] = oY;
                     bex(pp, i)[0] = oE;
@@ -693,15 +705,43 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             if (fin != nullptr) bounded_wait(fin, min(t + kPipeChunk + kWave, fsteps), serr, A.wait_ticks, lane, 1);
             sync_at += kPipeChunk;
         };
+        // The middle steps [t0, t1): every pair of the tile on a row of every lane, from the
+        // widest pair's last lane's row 1 to the shortest pair's first lane's row m (both ends
+        // even, so the sync test, which meets only even steps, sees every chunk boundary).
+        // Banded tiles keep the per-lane branch throughout (a lane of no pair has no guide).
+        int t0 = 0, t1 = 0;
+        if constexpr (!BAND) {
+            int gq = p >= 0 ? G : 0, mq = p >= 0 ? m : 0x7fffffff;
+            for (int off = 32; off > 0; off >>= 1) {
+                gq = max(gq, __shfl_xor(gq, off, kWave));
+                mq = min(mq, __shfl_xor(mq, off, kWave));
+            }
+            t0 = (wave_uniform(gq) - 1 + 1) & ~1;
+            t1 = wave_uniform(mq) & ~1;
+            if (t1 > fsteps) t1 = fsteps & ~1;
+            if (t1 < t0) t1 = t0;
+        }
+        const std::true_type checked;
+        const std::false_type unchecked;
         int t = 0;
+        for (; t < t0; t += 2) {
+            if (t == sync_at) sync(t);
+            fstep(t, checked);
+            if (t + 1 < fsteps) fstep(t + 1, checked);
+        }
+        for (; t < t1; t += 2) {
+            if (t == sync_at) sync(t);
+            fstep(t, unchecked);
+            fstep(t + 1, unchecked);
+        }
         for (; t + 1 < fsteps; t += 2) {
             if (t == sync_at) sync(t);
-            fstep(t);
-            fstep(t + 1);
+            fstep(t, checked);
+            fstep(t + 1, checked);
         }
         if (t < fsteps) {
             if (t == sync_at) sync(t);
-            fstep(t);
+            fstep(t, checked);
         }
         if (fout != nullptr) publish(fout, fsteps, lane);
         // Global's end cell: row m of column n, in the panel that holds column n (a narrow
@@ -878,6 +918,8 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
             qM = bq[3]; qY = bq[4]; qE = bex(pp + 1, m)[1];
         }
         int acode = na;                // code of row i+1 (row m: the virtual row)
+        int acode2 = (live && m >= 1) ? (int)ap[m - 1] : na;   // and of rows i, i-1 (loaded ahead)
+        int acode3 = (live && m >= 2) ? (int)ap[m - 2] : na;
         // (tracks) row i+1's set, and row i's sets, loaded one row ahead
         int btn = 0, bti = 0, bgi = 0;
         if constexpr (TR) {
@@ -937,7 +979,9 @@ __device__ void soft_tile(const SoftArgs<SF>& A, const SoftView<SF> V, const int
         // One backward step on row i: Fc holds forward row i.
         auto step = [&](int i, SF rM, SF rY, int rE, SF(&Fc)[LY::vals]) {
             const int ac = acode;
-            if (i >= 1) acode = (int)ap[i - 1];   // one row ahead
+            acode = acode2;
+            acode2 = acode3;
+            if (i >= 3) acode3 = (int)ap[i - 3];   // three rows ahead
             // The step's weights.  Tracks: row i+1's set tn (its profile, its gaps in B, its
             // match counts and X opens), row i's sets ti (its X moves) and gi (its gaps in A).
             int tn = 0, ti = 0, gi = 0;
