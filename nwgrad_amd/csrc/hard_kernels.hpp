@@ -372,6 +372,17 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
     best.v = T(0); best.i = (AM == kLocal) ? 0 : -1; best.j = 0; best.st = 0;
 
     uint32_t* tb = A.table + slot * A.slot_dwords;
+    // Score-only fills: the codes of the lane's rows i, i+1, i+2, loaded three rows ahead (a row's
+    // code is otherwise loaded in the step that uses it; measured: dna_local_short score-only
+    // 1,956 -> 2,019 GCUPS).  The hard fills load it in the step: their three more registers
+    // cost dna_local_short a wave per SIMD (96 -> 98 registers), and the hard fills measured
+    // slower with them.
+    int ac0 = 0, ac1 = 0, ac2 = 0;
+    if (!HARD && p >= 0) {
+        if (m >= 1) ac0 = ap[0];
+        if (m >= 2) ac1 = ap[1];
+        if (m >= 3) ac2 = ap[2];
+    }
     const int steps = td.max_m + G - 1;
     for (int t = 0; t < steps; ++t) {
         const int i = t - g + 1;
@@ -406,7 +417,7 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
             }
         }
         if (active) {
-            const T* __restrict__ srow = ssub + ((TR ? rs * na : 0) + (int)ap[i - 1]) * na;
+            const T* __restrict__ srow = ssub + ((TR ? rs * na : 0) + (HARD ? (int)ap[i - 1] : ac0)) * na;
             uint32_t cw[S / 4];
             T rowmax;
             int klo = 0, khi = S;
@@ -426,6 +437,11 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
             if constexpr (AM == kLocal) local_update<T, GM, S>(best, M, X, Y, rowmax, i, j0, ncols);
         }
         pM = iM; pX = iX; pY = iY;
+        if (!HARD && p >= 0 && i >= 1 && i <= m) {   // the next row's code, and one more ahead
+            ac0 = ac1;
+            ac1 = ac2;
+            if (i + 2 < m) ac2 = ap[i + 2];
+        }
     }
 
     Cand<T> c;
