@@ -5,9 +5,10 @@ package, and only as the checker.  The product (``nwgrad_amd``) never imports it
 
 Two oracles live here:
 
-* ``liboracle.so`` (built from ``nwg_oracle.c``): a plain-C restatement of the reference's
-  hard fill / walk / counts, score-only fill, log-space soft forward-backward and blocked
-  gradient sum.  Each function cites the reference lines it follows.
+* ``liboracle.so`` (built from ``nwg_oracle.c`` and ``nwg_oracle_hb.c``): a plain-C restatement
+  of the reference's hard fill / walk / counts, score-only fill, log-space soft
+  forward-backward, blocked gradient sum and the Hirschberg (linear-memory) traceback of
+  Global affine pairs.  Each function cites the reference lines it follows.
 * ``_ref/libnwgref.so`` (built by ``Makefile.ref`` from the reference's own sources, when
   they are present): the reference engine itself behind a small C ABI (``ref_harness.cpp``).
 
@@ -33,10 +34,10 @@ MODES = {"global": 0, "local": 1, "glocal": 2}
 def build(force: bool = False) -> str:
     """Compile the C restatement (seconds).  -ffp-contract=off keeps every hard-path
     operation a separate rounding, as the reference's build does (CMakeLists.txt:75)."""
-    src = os.path.join(HERE, "nwg_oracle.c")
-    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
+    srcs = [os.path.join(HERE, "nwg_oracle.c"), os.path.join(HERE, "nwg_oracle_hb.c")]
+    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < max(map(os.path.getmtime, srcs)):
         subprocess.check_call(["gcc", "-O2", "-std=c11", "-ffp-contract=off", "-fPIC", "-shared",
-                               "-o", LIB_PATH, src, "-lm"])
+                               "-o", LIB_PATH, *srcs, "-lm"])
     return LIB_PATH
 
 
@@ -91,6 +92,11 @@ def lib():
                                            P(ctypes.c_uint16), P(ctypes.c_uint16), P(ctypes.c_double),
                                            P(ctypes.c_double)]
         L.nwo_soft_log_tracked.restype = ctypes.c_int
+        L.nwo_hirschberg.argtypes = [P(_Params), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                     P(ctypes.c_uint8), P(ctypes.c_int64), P(ctypes.c_uint8), P(ctypes.c_int64),
+                                     P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_int32), P(ctypes.c_uint8),
+                                     P(ctypes.c_int32), P(ctypes.c_uint8)]
+        L.nwo_hirschberg.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -240,3 +246,31 @@ def soft_log_tracked(matrices, gaps, codes_a, off_a, codes_b, off_b, tracks, gap
     if rc:
         raise MemoryError("oracle: allocation failed")
     return scores, grads
+
+
+def hirschberg(matrix, gaps, codes_a, off_a, codes_b, off_b, *, precision="float32", pmax=False, cutoff=512):
+    """The Hirschberg traceback of Global affine hard pairs (traceback "hirschberg", or
+    "hirschberg_pmax" with pmax=True): every pair of more than `cutoff` rows through the
+    recursion.  Returns a dict: split (N,) bool -- the pairs that split (the others run as
+    Pointers in the reference and are left at zero here) -- scores (N,), grads (N, na*na+4),
+    guides (pair i's m + 1 entries at off_a[i] + i), moves (pair i's at off_a[i] + off_b[i],
+    0 diagonal / 1 a gap in B / 2 a gap in A) and n_moves (N,)."""
+    p, mat = _params(matrix, gaps, "affine", "global")
+    ca, oa, cb, ob = _pairs(codes_a, off_a, codes_b, off_b)
+    N = len(oa) - 1
+    na = mat.shape[0]
+    scores = np.zeros(N, np.float64)
+    grads = np.zeros((N, na * na + 4), np.float64)
+    guides = np.zeros(int(oa[-1]) + N, np.int32)
+    moves = np.zeros(max(int(oa[-1] + ob[-1]), 1), np.uint8)
+    n_moves = np.zeros(N, np.int32)
+    split = np.zeros(max(N, 1), np.uint8)
+    rc = lib().nwo_hirschberg(ctypes.byref(p), 1 if precision in ("float64", "double") else 0, 1 if pmax else 0,
+                              int(cutoff), N, _ptr(ca, ctypes.c_uint8), _ptr(oa, ctypes.c_int64),
+                              _ptr(cb, ctypes.c_uint8), _ptr(ob, ctypes.c_int64), _ptr(scores, ctypes.c_double),
+                              _ptr(grads, ctypes.c_double), _ptr(guides, ctypes.c_int32),
+                              _ptr(moves, ctypes.c_uint8), _ptr(n_moves, ctypes.c_int32), _ptr(split, ctypes.c_uint8))
+    if rc:
+        raise MemoryError("oracle: allocation failed")
+    return {"split": split[:N].astype(bool), "scores": scores, "grads": grads, "guides": guides, "moves": moves,
+            "n_moves": n_moves}

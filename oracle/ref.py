@@ -38,6 +38,7 @@ def lib():
                                             ctypes.c_int, P(vp)]
         L.nwgref_destroy.argtypes = [vp]
         L.nwgref_set_fill.argtypes = [vp, ctypes.c_int]
+        L.nwgref_set_hb_cutoff.argtypes = [vp, ctypes.c_int]
         L.nwgref_set_soft.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.nwgref_set_params.argtypes = [vp, ctypes.c_char_p, P(ctypes.c_double), P(ctypes.c_double)]
         L.nwgref_score_and_grad.argtypes = [vp, P(ctypes.c_double)]
@@ -132,6 +133,9 @@ class RefBatch:
 
     def set_fill(self, fill):
         _check(lib().nwgref_set_fill(self._h, _FILL[fill]))
+
+    def set_hb_cutoff(self, rows):
+        _check(lib().nwgref_set_hb_cutoff(self._h, int(rows)))
 
     def set_soft(self, temperature=1.0, precision="float32", impl="scaled"):
         _check(lib().nwgref_set_soft(self._h, float(temperature), 0 if precision == "float32" else 1,

@@ -157,6 +157,12 @@ int nwgref_set_fill(void* hp, int fill) {
     return guarded([&] { with_batch<void>(h, [&](auto& b) { b.set_fill(fill == 2, fill == 0); }); });
 }
 
+// Rows of the Hirschberg base case (SeqPairBatch.hb_cutoff, seq_pair_batch.hpp:153).
+int nwgref_set_hb_cutoff(void* hp, int rows) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guarded([&] { with_batch<void>(h, [&](auto& b) { b.set_hb_cutoff(rows); }); });
+}
+
 // precision 0 float32 / 1 double; impl 0 scaled / 1 scaled_or_log / 2 log
 int nwgref_set_soft(void* hp, double temperature, int precision, int impl) {
     Handle* h = static_cast<Handle*>(hp);

@@ -6,6 +6,7 @@ sources (oracle/_ref/libnwgref.so; see oracle/Makefile.ref).  Run in the build c
     python tests/golden/make_golden.py tracks                                # the tracked sets only
     python tests/golden/make_golden.py soft_tracks                           # the tracked soft sets only
     python tests/golden/make_golden.py paths                                 # guides, paths, banded, align()
+    python tests/golden/make_golden.py hirschberg                            # traceback="hirschberg*" sets
 
 Each fixture is an .npz (no pickles) holding the inputs -- alphabet, matrix, gaps, problem
 type, encoded pairs -- and the reference's outputs: per-pair scores and gradients, and for the
@@ -248,7 +249,7 @@ def _hard_results(b, n):
     return scores, np.concatenate([mats.reshape(n, -1), gaps.reshape(n, -1)], axis=1)
 
 
-def path_set(name, w, prec, bands=(3,), traceback="auto"):
+def path_set(name, w, prec, bands=(3,), traceback="auto", hb_cutoff=None):
     """One hard batch: score_and_grad(keep_paths=True) under (matrix, gaps) -- scores, gradients,
     guides and the stored paths -- then, for each bandwidth, set_params(the perturbed point) and
     banded_grad(bw, keep_paths=True) from those guides: scores, gradients, new guides, cigars.
@@ -261,8 +262,13 @@ def path_set(name, w, prec, bands=(3,), traceback="auto"):
     sa, sb = w.strings()
     mat1, gaps1 = _perturbed(w, "p1")
     out = {"matrix1": mat1, "gaps1": gaps1, "bands": np.array(bands)}
+    if hb_cutoff is not None:
+        out["hb_cutoff"] = np.array(hb_cutoff)
+        out["traceback"] = np.array(traceback)
     b = ref.RefBatch(sa, sb, w.alphabet, w.matrix, w.gaps, double=(prec == "float64"), gap_model=w.gap_model,
                      mode=w.mode, grad_mode="hard", n_threads=1, traceback=traceback)
+    if hb_cutoff is not None:
+        b.set_hb_cutoff(hb_cutoff)
     out["total"] = np.array(b.score_and_grad_keep(True))
     out["scores"], out["grads"] = _hard_results(b, w.n_pairs)
     out["guides"] = b.guides(w.off_a)
@@ -270,6 +276,8 @@ def path_set(name, w, prec, bands=(3,), traceback="auto"):
     for bw in bands:
         b = ref.RefBatch(sa, sb, w.alphabet, w.matrix, w.gaps, double=(prec == "float64"), gap_model=w.gap_model,
                          mode=w.mode, grad_mode="hard", n_threads=1, traceback=traceback)
+        if hb_cutoff is not None:
+            b.set_hb_cutoff(hb_cutoff)
         b.score_and_grad_keep(False)
         b.set_params(mat1, gaps1)
         out[f"band{bw}_total"] = np.array(b.banded_grad(bw, True))
@@ -385,7 +393,27 @@ def path_sets():
             save(f"align_prot_{gm}_{mode}_float32", w, "float32", "hard", res)
 
 
+def hb_sets():
+    """Global affine hard batches under traceback="hirschberg" (the exact carry) and
+    "hirschberg_pmax" (the prefix-max carry) at small base-case cutoffs, so that pairs of a
+    few hundred rows recurse several levels (aligner.hpp:4456-4518) while the shortest run as
+    Pointers: scores, gradients, the guides in Hirschberg's convention, stored paths, and a
+    banded pass that follows those guides.  Integer gaps (11/1) make ties common, where the
+    recursion's path differs from Pointers'; fractional gaps make the two carries round
+    differently."""
+    for prec in ("float32", "float64"):
+        for tb, cutoff in (("hirschberg", 5), ("hirschberg_pmax", 24)):
+            for tag, gaps in (("int", (11.0, 1.0, 11.0, 1.0)), ("frac", (10.5, 0.7, 9.3, 1.1))):
+                w = wl.random_pairs(f"golden_hb_{tb}_{tag}", 18, wl.AA, (1, 420), (1, 460), gaps=gaps,
+                                    related=0.6)
+                w = edge_pairs(w)
+                path_set(f"hb_{tb}_{tag}_{prec}", w, prec, bands=(4,), traceback=tb, hb_cutoff=cutoff)
+
+
 def main():
+    if sys.argv[1:] == ["hirschberg"]:
+        hb_sets()
+        return
     if sys.argv[1:] == ["bandsoft"]:
         bandsoft_sets()
         return
