@@ -232,6 +232,17 @@ int nwg_batch_launch_info(nwg_batch *b, int64_t *grid, int64_t *wide_slots, int6
  * reference, whose CPU pass keeps one pair's rows per worker thread (soft_kernel_impl.inl
  * row storage); it bounds the HBM a serving process spends on soft batches. */
 int nwg_batch_set_soft_scratch(nwg_batch *b, int64_t bytes);
+/* Traceback of the hard passes: 0 auto, 1 pointers, 2 scores, 3 hirschberg, 4 hirschberg_pmax
+ * (the reference's TracebackMode, aligner.hpp:30-111, fixed at construction by
+ * SeqPairBatch(traceback=), py_exports.cpp:796-831), and the Hirschberg base case in rows
+ * (SeqPairBatch.hb_cutoff, seq_pair_batch.hpp:136-153).  Under 3 / 4 a hard pass takes every
+ * pair of more than hb_cutoff rows through the linear-memory recursion (aligner.hpp:4118-4560),
+ * Global affine untracked pairs only; the others run as Pointers, which the other values all
+ * give (auto's Grid returns Pointers' path, aligner.hpp:89-100). */
+int nwg_batch_set_traceback(nwg_batch *b, int traceback, int hb_cutoff);
+/* The last hard pass's recursion: pairs that split, levels, base-case blocks and the device
+ * scratch it held in bytes.  No reference counterpart (diagnostics of the GPU path). */
+int nwg_batch_hb_info(nwg_batch *b, int64_t *pairs, int64_t *levels, int64_t *base_blocks, int64_t *scratch_bytes);
 /* Device memory the hard fill of pairs up to 1,024 columns may take for its pointer codes
  * (0: the default, 16 GiB).  Fill 0 runs one plain launch, a code slot per tile, when that
  * fits, else one persistent launch whose waves (one slot each) fit it; fill 1 splits each

@@ -21,8 +21,10 @@ CSRC = os.path.join(HERE, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["nwg_api.hip"]
-HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+# Translation units, compiled in parallel and linked into one library (each holds its own
+# kernels; no device code crosses units).
+HIP_SOURCES = ["nwg_api.hip", "hirschberg.hip"]
+HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
@@ -37,9 +39,29 @@ def ext_path() -> str:
 
 def build_hip(verbose: bool = False) -> str:
     out = lib_path()
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    cmd = [HIPCC, *HIP_FLAGS, "-o", out + ".tmp", *srcs]
-    _run(cmd, verbose)
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs, procs = [], []
+    for src in HIP_SOURCES:
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        cmd = [HIPCC, *HIP_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd, cwd=CSRC, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                            text=True)))
+        objs.append(obj)
+    failed = None
+    for cmd, p in procs:
+        text = p.communicate()[0]
+        lines = [l for l in text.splitlines() if "packed-fp32-ops" not in l]
+        if p.returncode != 0:
+            sys.stderr.write("\n".join(lines) + "\n")
+            failed = failed or cmd
+        elif verbose and lines:
+            print("\n".join(lines))
+    if failed:
+        raise RuntimeError("build failed: " + " ".join(failed))
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", out + ".tmp", *objs], verbose)
     os.replace(out + ".tmp", out)
     return out
 

@@ -711,6 +711,15 @@ public:
     // Pointer-code memory of the hard fill (0: default); its launches, whether fill 0 runs
     // persistent, the last fill's persistent grid (0: plain).
     void set_table_budget(int64_t bytes) { check_(nwg_batch_set_table_budget(b_, bytes)); }
+    // traceback: 0 auto, 1 pointers, 2 scores, 3 hirschberg, 4 hirschberg_pmax (the
+    // reference's TracebackMode; auto, pointers, scores and grid all give Pointers' result)
+    void set_traceback(int traceback, int hb_cutoff) { check_(nwg_batch_set_traceback(b_, traceback, hb_cutoff)); }
+    // the last hard pass's recursion: pairs that split, levels, base blocks, scratch bytes
+    std::array<int64_t, 4> hb_info() const {
+        std::array<int64_t, 4> r{};
+        check_(nwg_batch_hb_info(b_, &r[0], &r[1], &r[2], &r[3]));
+        return r;
+    }
     std::array<int64_t, 3> hard_launches() const {
         std::array<int64_t, 3> r{0, 0, 0};
         check_(nwg_batch_hard_launches(b_, &r[0], &r[1], &r[2]));

@@ -19,6 +19,7 @@
 
 #include "../../include/nwgrad_hip.h"
 #include "hard_kernels.hpp"
+#include "hb_host.hpp"
 #include "long_kernels.hpp"
 #include "reduce_kernels.hpp"
 #include "soft_kernels.hpp"
@@ -275,6 +276,15 @@ struct nwg_batch {
     bool results_valid = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
+    // traceback (0 auto, 1 pointers, 2 scores, 3 hirschberg, 4 hirschberg_pmax) and the
+    // Hirschberg base case; the pairs a hard pass runs through the recursion, and whether the
+    // current plans leave them out
+    int traceback = 0;
+    int hb_cutoff = 512;
+    std::vector<int32_t> hb_pairs;
+    bool plan_excl_hb = false;
+    std::vector<int64_t> hob;             // off_b, host copy
+    std::unique_ptr<nwg_hb::Work, void (*)(nwg_hb::Work*)> hbw{nullptr, nwg_hb::work_free};
 
     ~nwg_batch() {
         if (host_scores) (void)hipHostFree(host_scores);
@@ -534,6 +544,36 @@ int build_plans(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& main, Lon
         if (rc) return rc;
     }
     return build_long_plan(b, lp, wide);
+}
+
+// The pairs a hard pass takes through the Hirschberg recursion: under traceback "hirschberg"
+// or "hirschberg_pmax", those of more than hb_cutoff rows (hb_splits, aligner.hpp:136-141:
+// an explicit Hirschberg mode has no pointer budget); the shorter ones run as Pointers.
+void find_split_pairs(nwg_batch* b) {
+    b->hb_pairs.clear();
+    if (b->prob.grad_mode != 1 || (b->traceback != 3 && b->traceback != 4)) return;
+    for (int64_t p = 0; p < b->N; ++p)
+        if (b->mlen[(size_t)p] > b->hb_cutoff) b->hb_pairs.push_back((int32_t)p);
+}
+
+// (Re)builds the hard / score-only plans: every pair for a banded pass (the band bounds memory
+// already, and the reference's banded fill takes no Hirschberg mode), the others without the
+// pairs the recursion takes.
+int replan(nwg_batch* b) {
+    find_split_pairs(b);
+    const bool excl = !b->hb_pairs.empty() && !b->banded;
+    std::vector<int32_t> pairs;
+    pairs.reserve((size_t)b->N);
+    size_t k = 0;
+    for (int64_t p = 0; p < b->N; ++p) {
+        if (excl && k < b->hb_pairs.size() && b->hb_pairs[k] == (int32_t)p) {
+            ++k;
+            continue;
+        }
+        pairs.push_back((int32_t)p);
+    }
+    b->plan_excl_hb = excl;
+    return build_plans(b, pairs, b->plan, b->lpl);
 }
 
 // Soft tiles.  A pair needs ceil(n / (64 S)) column panels at least; in a tile of NP panels
@@ -914,6 +954,7 @@ int load_pairs(nwg_batch* b, int64_t N, const uint8_t* codes_a, const int64_t* o
     b->nlen.swap(nlen);
     b->cells = cells;
     b->hoa.assign(off_a ? off_a : zero, (off_a ? off_a : zero) + (N ? N + 1 : 1));
+    b->hob.assign(off_b ? off_b : zero, (off_b ? off_b : zero) + (N ? N + 1 : 1));
     if (b->tracked) {   // new pairs: the default tracks (set 0) until the caller gives others
         int rc = set_tracks_host(b, nullptr, nullptr);
         if (rc) return rc;
@@ -923,9 +964,7 @@ int load_pairs(nwg_batch* b, int64_t N, const uint8_t* codes_a, const int64_t* o
     b->soft_reruns = 0;
     for (SoftPlan* sp : {&b->sp_main, &b->sp_wide, &b->sq_main, &b->sq_wide}) sp->built = false;
     if (b->prob.grad_mode != 2) {
-        std::vector<int32_t> all((size_t)N);
-        std::iota(all.begin(), all.end(), 0);
-        const int rc = build_plans(b, all, b->plan, b->lpl);
+        const int rc = replan(b);
         if (rc) return rc;
     }
     return NWG_OK;
@@ -1081,9 +1120,7 @@ int nwg_batch_set_params_k(nwg_batch* b, int K, const double* matrices, const do
     }
     b->bx_dirty = true;
     if (set_slots(b) && b->prob.grad_mode != 2) {
-        std::vector<int32_t> all((size_t)b->N);
-        std::iota(all.begin(), all.end(), 0);
-        const int rc = build_plans(b, all, b->plan, b->lpl);
+        const int rc = replan(b);
         if (rc) return rc;
     }
     b->have_params = true;
@@ -1099,9 +1136,7 @@ int nwg_batch_set_tracks(nwg_batch* b, const uint16_t* tracks, const uint16_t* g
     const int rc = set_tracks_host(b, tracks, gap_tracks);
     if (rc) return rc;
     if (set_slots(b) && b->prob.grad_mode != 2) {
-        std::vector<int32_t> all((size_t)b->N);
-        std::iota(all.begin(), all.end(), 0);
-        const int rc2 = build_plans(b, all, b->plan, b->lpl);
+        const int rc2 = replan(b);
         if (rc2) return rc2;
     }
     b->results_valid = false;
@@ -1142,9 +1177,7 @@ int nwg_batch_set_lanes(nwg_batch* b, int columns) {
     b->lanes_S = columns;
     if (b->prob.grad_mode != 2) {
         NWG_HIP(b->ctx, hipSetDevice(b->ctx->device));
-        std::vector<int32_t> all((size_t)b->N);
-        std::iota(all.begin(), all.end(), 0);
-        const int rc = build_plans(b, all, b->plan, b->lpl);
+        const int rc = replan(b);
         if (rc) return rc;
     }
     return NWG_OK;
@@ -1896,6 +1929,64 @@ int soft_finish(nwg_batch* b) {
                 (long long)refused.size(), (long long)refused[0]);
 }
 
+// What the Hirschberg path needs of a pass (aligner.hpp:3295-3330): Global affine untracked
+// pairs, non-negative gap penalties (the reference raises for an explicit Hirschberg mode
+// otherwise), and a pass that may synchronise (the host cuts each recursion level).
+int check_hb(nwg_batch* b, bool capturing) {
+    nwg_ctx* ctx = b->ctx;
+    if (b->prob.gap_model != 1 || b->prob.mode != 0)
+        return fail(ctx, NWG_EINVAL, "nwgrad_amd: traceback=\"hirschberg\" / \"hirschberg_pmax\" is on the GPU path for "
+                    "Global affine pairs only");
+    if (b->tracked)
+        return fail(ctx, NWG_EINVAL, "nwgrad_amd: traceback=\"hirschberg\" / \"hirschberg_pmax\" of tracked pairs is not "
+                    "on the GPU path");
+    for (int k = 0; k < 4; ++k)
+        if (b->gaps[(size_t)k] < 0.0)
+            return fail(ctx, NWG_EINVAL, "nwgrad: traceback=\"hirschberg\" / \"hirschberg_pmax\" needs non-negative gap "
+                        "penalties (got a negative one); use traceback=\"pointers\" or \"auto\"");
+    if (capturing)
+        return fail(ctx, NWG_EINVAL, "nwgrad_amd: a pass with Hirschberg pairs synchronises once per recursion level and "
+                    "cannot be captured into a graph");
+    return NWG_OK;
+}
+
+// The pairs of b->hb_pairs through the recursion (hirschberg.hip), results into the batch's
+// scores, gradient rows, guides and stored paths.
+int launch_hb(nwg_batch* b) {
+    nwg_ctx* ctx = b->ctx;
+    if (!b->hbw) b->hbw.reset(nwg_hb::work_new());
+    if (!b->hbw) return fail(ctx, NWG_ENOMEM, "nwgrad_hip: out of host memory");
+    nwg_hb::Job J;
+    J.stream = ctx->stream;
+    J.precision = b->prob.precision;
+    J.pmax = b->traceback == 4 ? 1 : 0;
+    J.cutoff = b->hb_cutoff;
+    J.na = b->na;
+    J.codes_a = b->ca.p;
+    J.codes_b = b->cb.p;
+    J.sub = b->prob.precision ? static_cast<const void*>(b->sub_f64.p) : static_cast<const void*>(b->sub_f32.p);
+    J.go_a = b->gaps[0];
+    J.ge_a = b->gaps[1];
+    J.go_b = b->gaps[2];
+    J.ge_b = b->gaps[3];
+    J.scores = b->scores.p;
+    J.grads = b->grads.p;
+    J.guide = b->guide.p;
+    if (b->keep_paths) {
+        J.ops = b->ops.p;
+        J.ops_n = b->ops_n.p;
+        J.ends = b->ends.p;
+    }
+    J.pairs = &b->hb_pairs;
+    J.off_a = &b->hoa;
+    J.off_b = &b->hob;
+    J.table_budget = b->table_budget;
+    std::string err;
+    const int rc = nwg_hb::run(b->hbw.get(), J, err);
+    if (rc) return fail(ctx, rc, "%s", err.c_str());
+    return NWG_OK;
+}
+
 // record: the pass is asynchronous (its caller may consume results before any rerun), so its
 // float32 refusals go into the batch's refusal record.
 int run_async(nwg_batch* b, bool record) {
@@ -1913,12 +2004,25 @@ int run_async(nwg_batch* b, bool record) {
     } else {
         rc = prepare_tracks(b);
         if (rc) return rc;
+        const bool hb = !b->hb_pairs.empty() && !b->banded;
+        if (hb != b->plan_excl_hb) {
+            rc = replan(b);
+            if (rc) return rc;
+        }
+        if (hb) {
+            rc = check_hb(b, !timed);
+            if (rc) return rc;
+        }
         if (!b->plan.tiles.empty()) {
             rc = b->prob.precision == 0 ? launch_hard<float>(b, b->plan) : launch_hard<double>(b, b->plan);
             if (rc) return rc;
         }
         if (!b->lpl.items.empty()) {
             rc = b->prob.precision == 0 ? launch_long<float>(b) : launch_long<double>(b);
+            if (rc) return rc;
+        }
+        if (hb) {
+            rc = launch_hb(b);
             if (rc) return rc;
         }
     }
@@ -2308,6 +2412,36 @@ int nwg_batch_soft_refusals(nwg_batch* b, int64_t* pair_passes, int64_t* first_p
         NWG_HIP(ctx, hipMemcpyAsync(b->srec.p, r, 2 * sizeof(unsigned long long), hipMemcpyHostToDevice, ctx->stream));
         NWG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
+    return NWG_OK;
+}
+
+int nwg_batch_set_traceback(nwg_batch* b, int traceback, int hb_cutoff) {
+    if (!b) return NWG_EINVAL;
+    nwg_ctx* ctx = b->ctx;
+    if (traceback < 0 || traceback > 4) return fail(ctx, NWG_EINVAL, "nwgrad: unknown traceback %d", traceback);
+    if (hb_cutoff < 1) return fail(ctx, NWG_EINVAL, "nwgrad: hb_cutoff must be >= 1");
+    if ((traceback == 3 || traceback == 4) && b->prob.grad_mode == 1 && (b->prob.gap_model != 1 || b->prob.mode != 0))
+        return fail(ctx, NWG_EINVAL, "nwgrad_amd: traceback=\"hirschberg\" / \"hirschberg_pmax\" is on the GPU path for "
+                    "Global affine pairs only");
+    const bool changed = traceback != b->traceback || hb_cutoff != b->hb_cutoff;
+    b->traceback = traceback;
+    b->hb_cutoff = hb_cutoff;
+    if (changed && b->prob.grad_mode != 2) {
+        NWG_HIP(ctx, hipSetDevice(ctx->device));
+        const int rc = replan(b);
+        if (rc) return rc;
+    }
+    return NWG_OK;
+}
+
+int nwg_batch_hb_info(nwg_batch* b, int64_t* pairs, int64_t* levels, int64_t* base_blocks, int64_t* scratch_bytes) {
+    if (!b || !pairs || !levels || !base_blocks || !scratch_bytes) return NWG_EINVAL;
+    *pairs = (int64_t)b->hb_pairs.size();
+    const nwg_hb::Stats z{};
+    const nwg_hb::Stats& st = b->hbw ? nwg_hb::work_stats(b->hbw.get()) : z;
+    *levels = st.levels;
+    *base_blocks = st.base_blocks;
+    *scratch_bytes = st.peak_bytes;
     return NWG_OK;
 }
 

@@ -42,12 +42,17 @@ GradMode parse_grad_mode(const std::string& s) {
     if (s == "none") return GradMode::None;
     throw py::value_error("nwgrad: unknown grad_mode \"" + s + "\" (expected \"hard\", \"soft\" or \"none\")");
 }
-void check_traceback(const std::string& s) {
-    if (s == "auto" || s == "pointers" || s == "scores") return;
-    if (s == "hirschberg" || s == "hirschberg_pmax")
-        throw py::value_error("nwgrad_amd: traceback \"" + s + "\" is not available on the GPU path; \"auto\", "
-                              "\"pointers\" and \"scores\" (identical results) are");
-    throw py::value_error("nwgrad: unknown traceback \"" + s + "\"");
+// The reference's TracebackMode names (py_exports.cpp:355-366) as the ABI's codes: auto,
+// pointers, scores and grid give Pointers' result (0-2); hirschberg 3, hirschberg_pmax 4.
+int parse_traceback(const std::string& s) {
+    if (s == "auto" || s == "grid") return 0;
+    if (s == "pointers") return 1;
+    if (s == "scores") return 2;
+    if (s == "hirschberg") return 3;
+    if (s == "hirschberg_pmax") return 4;
+    throw py::value_error("nwgrad: unknown traceback \"" + s +
+                          "\" (expected \"auto\", \"pointers\", \"scores\", \"hirschberg\", \"hirschberg_pmax\" or "
+                          "\"grid\")");
 }
 JobKind parse_kind(const std::string& s) {
     if (s == "score" || s == "none") return JobKind::Score;
@@ -358,7 +363,7 @@ void bind_batch(py::module_& m, const char* name) {
                  const GapModel gm = parse_gap_model(gap_model);
                  const AlignMode am = parse_mode(mode);
                  const GradMode gd = parse_grad_mode(grad_mode);
-                 check_traceback(traceback);
+                 const int tb = parse_traceback(traceback);
                  if (n_threads < 0) throw py::value_error("nwgrad: n_threads must be >= 0");
                  if (kernel != "auto" && kernel != "gpu" && kernel != "gfx950")
                      throw py::value_error("nwgrad_amd: kernel \"" + kernel + "\": this build's only backend is the "
@@ -375,6 +380,7 @@ void bind_batch(py::module_& m, const char* name) {
                  for (size_t i = 0; i < seqs_a.size(); ++i) la[i] = static_cast<int64_t>(seqs_a[i].size());
                  apply_tracks(*pb->b, la, tracks, gap_tracks);
                  pb->o.traceback = traceback;
+                 pb->b->set_traceback(tb, pb->o.hb_cutoff);
                  return pb;
              }),
              py::arg("seqs_a"), py::arg("seqs_b"), py::arg("params"), py::kw_only(), py::arg("n_threads") = 0,
@@ -391,10 +397,11 @@ void bind_batch(py::module_& m, const char* name) {
                py::array_t<uint8_t, py::array::c_style | py::array::forcecast> codes_b,
                py::array_t<int64_t, py::array::c_style | py::array::forcecast> off_b, const AlignParams& params,
                const std::string& gap_model, const std::string& mode, const std::string& grad_mode, int device,
-               py::object tracks, py::object gap_tracks) {
+               py::object tracks, py::object gap_tracks, const std::string& traceback) {
                 const GapModel gm = parse_gap_model(gap_model);
                 const AlignMode am = parse_mode(mode);
                 const GradMode gd = parse_grad_mode(grad_mode);
+                const int tb = parse_traceback(traceback);
                 if (off_a.ndim() != 1 || off_b.ndim() != 1 || off_a.shape(0) != off_b.shape(0) || off_a.shape(0) < 1)
                     throw py::value_error("nwgrad_amd: offsets must be 1-D arrays of N+1 entries");
                 const int64_t N = off_a.shape(0) - 1;
@@ -411,11 +418,14 @@ void bind_batch(py::module_& m, const char* name) {
                 std::vector<int64_t> la(static_cast<size_t>(N));
                 for (int64_t i = 0; i < N; ++i) la[static_cast<size_t>(i)] = oa[i + 1] - oa[i];
                 apply_tracks(*pb->b, la, tracks, gap_tracks);
+                pb->o.traceback = traceback;
+                pb->b->set_traceback(tb, pb->o.hb_cutoff);
                 return pb;
             },
             py::arg("codes_a"), py::arg("off_a"), py::arg("codes_b"), py::arg("off_b"), py::arg("params"),
             py::kw_only(), py::arg("gap_model") = "affine", py::arg("mode") = "global", py::arg("grad_mode") = "hard",
             py::arg("device") = 0, py::arg("tracks") = py::none(), py::arg("gap_tracks") = py::none(),
+            py::arg("traceback") = "auto",
             "A batch from pre-encoded pairs: uint8 codes in alphabet order and int64 offsets (N+1), pair i\n"
             "being codes[off[i]:off[i+1]].")
         .def("__len__", [](const PB& s) { return s.b->size(); })
@@ -596,8 +606,15 @@ void bind_batch(py::module_& m, const char* name) {
             "hb_cutoff", [](const PB& s) { return s.o.hb_cutoff; },
             [](PB& s, int v) {
                 if (v < 1) throw py::value_error("nwgrad: hb_cutoff must be >= 1");
+                s.b->set_traceback(parse_traceback(s.o.traceback), v);
                 s.o.hb_cutoff = v;
-            })
+            },
+            "Hirschberg base case in rows: under traceback \"hirschberg\" / \"hirschberg_pmax\" a hard pass\n"
+            "takes every pair of more than hb_cutoff rows through the linear-memory recursion on the GPU.")
+        .def_property_readonly(
+            "hb_info", [](const PB& s) { auto r = s.b->hb_info(); return py::make_tuple(r[0], r[1], r[2], r[3]); },
+            "(pairs split, recursion levels, base-case blocks, device scratch bytes) of the last hard pass's\n"
+            "Hirschberg recursion.")
         .def_property(
             "n_threads", [](const PB& s) { return s.b->n_threads(); }, [](PB& s, int v) { s.b->set_n_threads(v); })
         .def_property_readonly("alphabet", [](const PB& s) { return s.b->alphabet().symbols(); })
