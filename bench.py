@@ -35,8 +35,8 @@ HBM_PEAK_GBS = 8000.0           # MI355X HBM3E
 SOFT_BYTES_PER_CELL = 24
 
 WORKLOADS = ("protein_soft_homologs", "dna_local_short", "dna_local_short_score", "protein_global_hard",
-             "long_hard_36000", "long_hard_36000_double", "long_hard_4096", "long_soft_8000", "serving",
-             "protein_soft_train")
+             "long_hard_36000", "long_hard_36000_double", "long_hard_36000_hb", "long_hard_36000_hb_double",
+             "long_hard_4096", "long_soft_8000", "serving", "protein_soft_train")
 
 
 def measured_traffic(workload, fill):
@@ -64,7 +64,7 @@ def make_workload(name):
     if name == "protein_soft_homologs":
         return wl.protein_soft_homologs(), "soft"
     # long pairs: B a mutated copy of A (substitution 0.1, indels 0.01, mean length 5), global affine 11/1
-    if name in ("long_hard_36000", "long_hard_36000_double"):
+    if name in ("long_hard_36000", "long_hard_36000_double", "long_hard_36000_hb", "long_hard_36000_hb_double"):
         return wl.long_pairs(16, 36000), "hard"
     if name == "long_hard_4096":
         return wl.long_pairs(256, 4096), "hard"
@@ -73,13 +73,19 @@ def make_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def make_batch(w, grad_mode, fill, precision="float32"):
+def traceback_of(workload):
+    """The long-pair configuration's linear-memory arm runs the Hirschberg recursion (base case
+    of 512 rows); every other workload the default traceback."""
+    return "hirschberg" if "_hb" in workload else "auto"
+
+
+def make_batch(w, grad_mode, fill, precision="float32", traceback="auto"):
     import nwgrad_amd as nw
     p = nw.AlignParams(w.matrix, alphabet=w.alphabet, gap_open_a=w.gaps[0], gap_extend_a=w.gaps[1],
                        gap_open_b=w.gaps[2], gap_extend_b=w.gaps[3])
     cls = nw.SeqPairBatch if precision == "float32" else nw.SeqPairBatchDouble
     b = cls.from_codes(w.codes_a, w.off_a, w.codes_b, w.off_b, p, gap_model=w.gap_model, mode=w.mode,
-                                   grad_mode=grad_mode)
+                       grad_mode=grad_mode, traceback=traceback)
     b.fill = fill
     return b
 
@@ -98,7 +104,7 @@ def time_steps(b, steps, warmup, sync):
     return dt, float(np.mean(kms))
 
 
-def cpu_baseline(w, grad_mode, precision="float32", budget_s=8.0):
+def cpu_baseline(w, grad_mode, precision="float32", budget_s=8.0, traceback="auto"):
     """The reference engine (oracle/_ref) on all host cores, else the C oracle on one: a
     bounded sample of the workload's pairs, best of 2 timed passes after a warm-up."""
     import oracle
@@ -111,7 +117,8 @@ def cpu_baseline(w, grad_mode, precision="float32", budget_s=8.0):
         def run(sub):
             sa, sb = sub.strings()
             rb = ref.RefBatch(sa, sb, sub.alphabet, sub.matrix, sub.gaps, gap_model=sub.gap_model, mode=sub.mode,
-                              grad_mode=grad_mode, n_threads=cores, double=precision == "float64")
+                              grad_mode=grad_mode, n_threads=cores, double=precision == "float64",
+                              traceback=traceback)
             if grad_mode == "soft":
                 rb.set_soft(precision="float32", impl="scaled_or_log")
             rb.score_and_grad()  # warm-up
@@ -307,7 +314,8 @@ def main():
         return 0
     w, grad_mode = make_workload(args.workload)
     precision = "float64" if args.workload.endswith("_double") else "float32"
-    b = make_batch(w, grad_mode, args.fill, precision)
+    tb = traceback_of(args.workload)
+    b = make_batch(w, grad_mode, args.fill, precision, tb)
     if args.columns and grad_mode != "soft":
         b.columns_per_lane = args.columns
     cells = b.cells
@@ -332,7 +340,10 @@ def main():
               file=sys.stderr, flush=True)
     b.fill = args.fill
     ops = OPS_PER_CELL[(w.gap_model, grad_mode)] + (1 if (w.mode == "local" and w.gap_model == "linear") else 0)
-    achieved = ops * cells / (kms * 1e-3) / 1e12
+    # the recursion fills about twice the pairs' cells (each level re-sweeps its half): the
+    # roofline counts the cells filled, GCUPS the pairs' cells
+    work_cells = int(b.hb_info[4]) if tb != "auto" else cells
+    achieved = ops * work_cells / (kms * 1e-3) / 1e12
     valu = {"achieved": round(achieved, 3), "unit": "T lane-ops/s", "ops_per_cell": ops,
             "issue_peak": VALU_ISSUE_PEAK_TOPS, "issue_frac": round(achieved / VALU_ISSUE_PEAK_TOPS, 4),
             "fp32_peak_tflops": FP32_PEAK_TFLOPS, "fp32_frac": round(achieved / FP32_PEAK_TFLOPS, 4)}
@@ -342,7 +353,7 @@ def main():
     # FP32 vector peak, against which `frac` is quoted; `valu` adds the non-packed issue fraction.
     roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic, "traffic_from": traffic_from,
-            "ops_per_cell": ops, "kernel_ms": round(kms, 4), "valu": valu}
+            "ops_per_cell": ops, "cells": work_cells, "kernel_ms": round(kms, 4), "valu": valu}
     if grad_mode == "soft":
         # secondary: the table traffic the pass would have without recomputation (3 states x 4 B
         # written, then read back), as an HBM rate -- a model, not a measurement
@@ -354,7 +365,7 @@ def main():
                                     "peak": HBM_PEAK_GBS, "frac": round(traffic / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     if not args.no_cpu:
         print("bench: CPU baseline (bounded sample)", file=sys.stderr, flush=True)
-    cpu = None if args.no_cpu else cpu_baseline(w, grad_mode, precision)
+    cpu = None if args.no_cpu else cpu_baseline(w, grad_mode, precision, traceback=tb)
     serve_s = 10.0 if args.serve_seconds is None else args.serve_seconds
     serving = serving_leg(serve_s) if serve_s > 0 else None
     dtype = "f32" if precision == "float32" else "f64"
@@ -363,13 +374,18 @@ def main():
             "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": dtype, "data": "synthetic",
             "config": {"workload": args.workload, "pairs": w.n_pairs, "cells": cells, "grad_mode": grad_mode,
-                       "gap_model": w.gap_model, "mode": w.mode, "fill": args.fill,
+                       "gap_model": w.gap_model, "mode": w.mode, "fill": args.fill, "traceback": tb,
                        "columns_per_lane": int(b.columns_per_lane)},
             "kernel_gcups": round(cells / (kms * 1e-3) / 1e9, 3),
             "plain_vs_batched": None if gcups_other is None else
                 {"fill": other, "gcups": round(gcups_other, 3), "kernel_gcups": round(cells / (kms2 * 1e-3) / 1e9, 3),
                  "batched_over_plain": round(gcups / gcups_other, 3)},
             "roofline": roof, "cpu_baseline": cpu, "serving": serving}
+    if tb != "auto":
+        split, levels, blocks, scratch, filled = b.hb_info
+        line["hirschberg"] = {"pairs_split": int(split), "levels": int(levels), "base_blocks": int(blocks),
+                              "scratch_bytes": int(scratch), "hb_cutoff": int(b.hb_cutoff), "cells_filled": int(filled),
+                              "pointer_table_bytes": int(cells)}
     if grad_mode == "soft":
         line["soft_pairs_rerun_in_double"] = int(reruns)
         line["launch"] = {"grid": int(launch[0]), "wide_queue_blocks": int(launch[1]), "resident_per_cu": int(launch[2])}

@@ -240,9 +240,11 @@ int nwg_batch_set_soft_scratch(nwg_batch *b, int64_t bytes);
  * Global affine untracked pairs only; the others run as Pointers, which the other values all
  * give (auto's Grid returns Pointers' path, aligner.hpp:89-100). */
 int nwg_batch_set_traceback(nwg_batch *b, int traceback, int hb_cutoff);
-/* The last hard pass's recursion: pairs that split, levels, base-case blocks and the device
- * scratch it held in bytes.  No reference counterpart (diagnostics of the GPU path). */
-int nwg_batch_hb_info(nwg_batch *b, int64_t *pairs, int64_t *levels, int64_t *base_blocks, int64_t *scratch_bytes);
+/* The last hard pass's recursion: pairs that split, levels, base-case blocks, the device
+ * scratch it held in bytes, and the cells its half sweeps and base cases filled (about twice
+ * the pairs' cells).  No reference counterpart (diagnostics of the GPU path). */
+int nwg_batch_hb_info(nwg_batch *b, int64_t *pairs, int64_t *levels, int64_t *base_blocks, int64_t *scratch_bytes,
+                      int64_t *filled_cells);
 /* Device memory the hard fill of pairs up to 1,024 columns may take for its pointer codes
  * (0: the default, 16 GiB).  Fill 0 runs one plain launch, a code slot per tile, when that
  * fits, else one persistent launch whose waves (one slot each) fit it; fill 1 splits each

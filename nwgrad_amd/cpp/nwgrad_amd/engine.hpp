@@ -714,10 +714,11 @@ public:
     // traceback: 0 auto, 1 pointers, 2 scores, 3 hirschberg, 4 hirschberg_pmax (the
     // reference's TracebackMode; auto, pointers, scores and grid all give Pointers' result)
     void set_traceback(int traceback, int hb_cutoff) { check_(nwg_batch_set_traceback(b_, traceback, hb_cutoff)); }
-    // the last hard pass's recursion: pairs that split, levels, base blocks, scratch bytes
-    std::array<int64_t, 4> hb_info() const {
-        std::array<int64_t, 4> r{};
-        check_(nwg_batch_hb_info(b_, &r[0], &r[1], &r[2], &r[3]));
+    // the last hard pass's recursion: pairs that split, levels, base blocks, scratch bytes,
+    // cells filled by its sweeps and base cases
+    std::array<int64_t, 5> hb_info() const {
+        std::array<int64_t, 5> r{};
+        check_(nwg_batch_hb_info(b_, &r[0], &r[1], &r[2], &r[3], &r[4]));
         return r;
     }
     std::array<int64_t, 3> hard_launches() const {

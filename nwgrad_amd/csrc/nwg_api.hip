@@ -2434,14 +2434,16 @@ int nwg_batch_set_traceback(nwg_batch* b, int traceback, int hb_cutoff) {
     return NWG_OK;
 }
 
-int nwg_batch_hb_info(nwg_batch* b, int64_t* pairs, int64_t* levels, int64_t* base_blocks, int64_t* scratch_bytes) {
-    if (!b || !pairs || !levels || !base_blocks || !scratch_bytes) return NWG_EINVAL;
+int nwg_batch_hb_info(nwg_batch* b, int64_t* pairs, int64_t* levels, int64_t* base_blocks, int64_t* scratch_bytes,
+                      int64_t* filled_cells) {
+    if (!b || !pairs || !levels || !base_blocks || !scratch_bytes || !filled_cells) return NWG_EINVAL;
     *pairs = (int64_t)b->hb_pairs.size();
     const nwg_hb::Stats z{};
     const nwg_hb::Stats& st = b->hbw ? nwg_hb::work_stats(b->hbw.get()) : z;
     *levels = st.levels;
     *base_blocks = st.base_blocks;
     *scratch_bytes = st.peak_bytes;
+    *filled_cells = st.sweep_cells + st.base_cells;
     return NWG_OK;
 }
 

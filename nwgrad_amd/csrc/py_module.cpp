@@ -612,9 +612,9 @@ void bind_batch(py::module_& m, const char* name) {
             "Hirschberg base case in rows: under traceback \"hirschberg\" / \"hirschberg_pmax\" a hard pass\n"
             "takes every pair of more than hb_cutoff rows through the linear-memory recursion on the GPU.")
         .def_property_readonly(
-            "hb_info", [](const PB& s) { auto r = s.b->hb_info(); return py::make_tuple(r[0], r[1], r[2], r[3]); },
-            "(pairs split, recursion levels, base-case blocks, device scratch bytes) of the last hard pass's\n"
-            "Hirschberg recursion.")
+            "hb_info", [](const PB& s) { auto r = s.b->hb_info(); return py::make_tuple(r[0], r[1], r[2], r[3], r[4]); },
+            "(pairs split, recursion levels, base-case blocks, device scratch bytes, cells filled) of the last\n"
+            "hard pass's Hirschberg recursion.")
         .def_property(
             "n_threads", [](const PB& s) { return s.b->n_threads(); }, [](PB& s, int v) { s.b->set_n_threads(v); })
         .def_property_readonly("alphabet", [](const PB& s) { return s.b->alphabet().symbols(); })

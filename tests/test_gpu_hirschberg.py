@@ -51,7 +51,7 @@ def test_golden(gpu, path):
     say("case", os.path.basename(path))
     b = fixture_batch(z)
     total = b.score_and_grad(keep_paths=True)
-    split, levels, blocks, scratch = b.hb_info
+    split, levels, blocks, scratch, _ = b.hb_info
     say(f"  split {split} pairs, {levels} levels, {blocks} base blocks, {scratch} B scratch")
     assert split >= 10 and levels >= 1
     assert total == float(z["total"])
@@ -146,8 +146,8 @@ def test_long_36000_pair(gpu, precision):
     say("case 36000", precision, w.n_pairs, "pair", int(w.off_a[1]), "x", int(w.off_b[1]))
     b = hb_batch(w.alphabet, w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, precision, "hirschberg", 512)
     b.score_and_grad(keep_paths=True)
-    split, levels, blocks, scratch = b.hb_info
-    say(f"  {levels} levels, {blocks} base blocks, scratch {scratch / 2**20:.1f} MiB")
+    split, levels, blocks, scratch, filled = b.hb_info
+    say(f"  {filled} cells filled, {levels} levels, {blocks} base blocks, scratch {scratch / 2**20:.1f} MiB")
     o = oracle.hirschberg(w.matrix, w.gaps, w.codes_a, w.off_a, w.codes_b, w.off_b, precision=precision, cutoff=512)
     assert split == 1 and o["split"][0]
     assert b.scores()[0] == o["scores"][0]
