@@ -24,6 +24,11 @@ ARCH = "gfx950"
 # Translation units, compiled in parallel and linked into one library (each holds its own
 # kernels; no device code crosses units).
 HIP_SOURCES = ["nwg_api.hip", "hirschberg.hip"]
+# Per-unit flags.  The Hirschberg unit's values are finite or -inf and never NaN (every
+# operation is a max, or a sum / difference with a finite operand), so fmax needs no NaN
+# quieting of its operands: -fno-honor-nans drops those extra instructions (143 VALU a row
+# step instead of 170).  Infinities keep their IEEE meaning.
+UNIT_FLAGS = {"hirschberg.hip": ["-fno-honor-nans"]}
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
@@ -37,19 +42,39 @@ def ext_path() -> str:
     return os.path.join(HERE, "_nwgrad_amd" + suffix)
 
 
-def build_hip(verbose: bool = False) -> str:
+def _deps(path, seen=None):
+    """The file and every file it includes with #include "...", transitively."""
+    seen = set() if seen is None else seen
+    path = os.path.normpath(path)
+    if path in seen or not os.path.exists(path):
+        return seen
+    seen.add(path)
+    with open(path) as f:
+        for line in f:
+            line = line.strip()
+            if line.startswith("#include \""):
+                _deps(os.path.join(os.path.dirname(path), line.split('"')[1]), seen)
+    return seen
+
+
+def build_hip(verbose: bool = False, force: bool = True) -> str:
+    """Compile every HIP unit and link the library.  force=False recompiles only the units
+    whose object is older than the unit or a header it includes (a developer's rebuild)."""
     out = lib_path()
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs, procs = [], []
     for src in HIP_SOURCES:
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC, *HIP_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)]
+        objs.append(obj)
+        if not force and os.path.exists(obj) and os.path.getmtime(obj) > max(
+                os.path.getmtime(d) for d in _deps(os.path.join(CSRC, src))):
+            continue
+        cmd = [HIPCC, *HIP_FLAGS, *UNIT_FLAGS.get(src, []), "-c", "-o", obj, os.path.join(CSRC, src)]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((cmd, subprocess.Popen(cmd, cwd=CSRC, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                                             text=True)))
-        objs.append(obj)
     failed = None
     for cmd, p in procs:
         text = p.communicate()[0]
@@ -92,10 +117,10 @@ def _run(cmd, verbose):
         print("\n".join(lines))
 
 
-def build(verbose: bool = False) -> None:
-    build_hip(verbose)
+def build(verbose: bool = False, force: bool = True) -> None:
+    build_hip(verbose, force)
     build_ext(verbose)
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, force="--changed" not in sys.argv)

@@ -141,19 +141,69 @@ struct HbReplayArgs {
 template <typename T>
 __device__ __forceinline__ T max3(T a, T b, T c) { return dmax(dmax(a, b), c); }
 
-// One row of a lane's strip under the prefix-max carry (aligner.hpp:4176-4190): M and X as
-// the exact cell, Y from the running maximum P of g[k] + k * ge_a along the row.  c0 is the
-// block column of the strip's first cell minus one.  Score only.
-template <typename T, int S>
-__device__ __forceinline__ void hb_strip_pmax(T (&M)[S], T (&X)[S], T (&Y)[S], T lM, T lX, T& P, T dM, T dX,
-                                              T dY, const T* __restrict__ srow, const int (&coff)[S], int c0,
-                                              T go_a, T ge_a, T go_b, T ge_b) {
+// One row of a lane's strip, cells c0 + 1 .. c0 + S, updated in place: the reference's affine
+// Global cell (aligner.hpp:4174-4176, 4366-4374) in its order, as strip_row computes it, with the
+// row's substitution values sv already loaded.  (lM, lX, lY) is the left neighbour, (dM, dX, dY)
+// the diagonal; HARD: the cells' codes M | X<<2 | Y<<4 (the >= chains of the reference's pick)
+// into cw, four to a dword.
+//
+// Score only, X and Y take a shorter form of the same values: x -> (x - go) - ge and x -> x - ge
+// are non-decreasing in x (every rounding is), so the largest of the three candidates is
+// ((max(uM, uY) - go_b) max uX) - ge_b: a maximum of images under a non-decreasing map is the
+// image of the maximum.  4 operations a state instead of 6; the values are the reference's
+// (at most the sign of a zero differs, which no comparison sees).
+template <typename T, int S, bool HARD>
+__device__ __forceinline__ void hb_strip(T (&M)[S], T (&X)[S], T (&Y)[S], T lM, T lX, T lY, T dM, T dX, T dY,
+                                         const T (&sv)[S], T go_a, T ge_a, T go_b, T ge_b, uint32_t (&cw)[S / 4]) {
+#pragma unroll
+    for (int k = 0; k < S / 4; ++k) cw[k] = 0u;
+    if constexpr (!HARD) {
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const T uM = M[k], uX = X[k], uY = Y[k];
+            const T mv = dmax(dM, dmax(dX, dY)) + sv[k];
+            const T xv = dmax(dmax(uM, uY) - go_b, uX) - ge_b;
+            const T yv = dmax(dmax(lM, lX) - go_a, lY) - ge_a;
+            M[k] = mv; X[k] = xv; Y[k] = yv;
+            dM = uM; dX = uX; dY = uY;
+            lM = mv; lX = xv; lY = yv;
+        }
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < S; ++k) {
-        const T s = srow[coff[k]];
         const T uM = M[k], uX = X[k], uY = Y[k];
-        const T mv = dmax(dM, dmax(dX, dY)) + s;
-        const T xv = dmax((uM - go_b) - ge_b, dmax(uX - ge_b, (uY - go_b) - ge_b));
+        const T tXY = dmax(dX, dY);
+        const T mv = dmax(dM, tXY) + sv[k];
+        const T ux = (uM - go_b) - ge_b, vx = uX - ge_b, wx = (uY - go_b) - ge_b;
+        const T tx = dmax(vx, wx);
+        const T xv = dmax(ux, tx);
+        const T uy = (lM - go_a) - ge_a, vy = (lX - go_a) - ge_a, wy = lY - ge_a;
+        const T ty = dmax(vy, wy);
+        const T yv = dmax(uy, ty);
+        if constexpr (HARD) {
+            const uint32_t km = (dM >= tXY) ? 0u : ((dX >= dY) ? 1u : 2u);
+            const uint32_t kx = (ux >= tx) ? 0u : ((vx >= wx) ? 1u : 2u);
+            const uint32_t ky = (uy >= ty) ? 0u : ((vy >= wy) ? 1u : 2u);
+            cw[k >> 2] |= (km | (kx << 2) | (ky << 4)) << (8 * (k & 3));
+        }
+        M[k] = mv; X[k] = xv; Y[k] = yv;
+        dM = uM; dX = uX; dY = uY;
+        lM = mv; lX = xv; lY = yv;
+    }
+}
+
+// The same row under the prefix-max carry (aligner.hpp:4176-4190): M and X as the exact cell,
+// Y from the running maximum P of g[k] + k * ge_a along the row.  c0 is the block column of
+// the strip's first cell minus one.  Score only.
+template <typename T, int S>
+__device__ __forceinline__ void hb_strip_pmax(T (&M)[S], T (&X)[S], T (&Y)[S], T lM, T lX, T& P, T dM, T dX,
+                                              T dY, const T (&sv)[S], int c0, T go_a, T ge_a, T go_b, T ge_b) {
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const T uM = M[k], uX = X[k], uY = Y[k];
+        const T mv = dmax(dM, dmax(dX, dY)) + sv[k];
+        const T xv = dmax(dmax(uM, uY) - go_b, uX) - ge_b;   // as hb_strip's score-only form
         const T g = (dmax(lM, lX) - go_a) - ge_a;
         const T ramp = (T)(c0 + k) * ge_a;   // rmp[c - 1], c = c0 + k + 1
         P = dmax(g + ramp, P);
@@ -162,6 +212,17 @@ __device__ __forceinline__ void hb_strip_pmax(T (&M)[S], T (&X)[S], T (&Y)[S], T
         dM = uM; dX = uX; dY = uY;
         lM = mv; lX = xv;
     }
+}
+
+// A lane's value for the whole wave, the lane given by a wave-uniform index.
+__device__ __forceinline__ float bcast(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double bcast(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
 // Fill of one panel of a block, rows 0..H: the sweep's (OUT: the last row into `out`; WAIT:
@@ -173,7 +234,7 @@ __device__ void hb_panel(const T* __restrict__ ssub, int na, const uint8_t* __re
                          const uint8_t* __restrict__ codes_b, int64_t a_base, int a_step, int64_t b_base, int b_step,
                          int H, int W, int panel, int flag, T go_a, T ge_a, T go_b, T ge_b, const T* bin, T* bout,
                          const int32_t* prog_in, int32_t* prog_out, int32_t* err, uint64_t ticks, T* out,
-                         uint32_t* tbl, T* fin, int lane) {
+                         uint32_t* tbl, T* fin, uint8_t* acodes, int lane) {
     constexpr int NB = PMAX ? 4 : 3;
     const T NI = ninf<T>();
     const int g = lane;
@@ -191,8 +252,25 @@ __device__ void hb_panel(const T* __restrict__ ssub, int na, const uint8_t* __re
     T bM = NI, bX = NI, bY = NI, bP = NI;     // lane r: the left column of row t0 + r of the block
     const int rows = H + 1;
     const int steps = rows + kWave - 1;
+    // The codes of A pass through a 256-row ring in LDS (acodes): at step t0 = 64 k the wave
+    // stores rows t0 + 1 .. t0 + 64, loaded one block earlier, and loads the next block; lane g
+    // then reads row t - g from LDS.  A step never waits on a global load.
+    auto arow = [&](int rr) -> int {
+        return (rr >= 1 && rr <= H) ? (int)codes_a[a_base + (int64_t)a_step * (rr - 1)] : 0;
+    };
+    int anext = arow(lane + 1);
+    // The substitution values of the lane's next row, loaded one step ahead (the row's code
+    // from the ring, then its S values from the matrix in LDS), so no step waits on LDS.
+    T sv[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) sv[k] = ssub[coff[k]];
     for (int t = 0; t < steps; ++t) {
         const int r = t - g;
+        if (t % kWave == 0) {
+            acodes[(t + 1 + lane) & 255] = (uint8_t)anext;
+            anext = arow(t + kWave + 1 + lane);
+            wave_sync();
+        }
         if (!first && t < rows && t % kHbRB == 0) {
             const int need = t + kHbRB < rows ? t + kHbRB : rows;
             if constexpr (WAIT) (void)bounded_wait(prog_in, need, err, ticks, lane, 1);
@@ -208,9 +286,9 @@ __device__ void hb_panel(const T* __restrict__ ssub, int na, const uint8_t* __re
         T iP = PMAX ? shift_up(oP) : NI;
         if (!first) {
             const int sl = t % kHbRB;
-            const T lM = __shfl(bM, sl, kWave), lX = __shfl(bX, sl, kWave), lY = __shfl(bY, sl, kWave);
+            const T lM = bcast(bM, sl), lX = bcast(bX, sl), lY = bcast(bY, sl);
             T lP = NI;
-            if constexpr (PMAX) lP = __shfl(bP, sl, kWave);
+            if constexpr (PMAX) lP = bcast(bP, sl);
             if (g == 0) { iM = lM; iX = lX; iY = lY; iP = lP; }
         } else if (g == 0) {   // column 0 of row t (aligner.hpp:4161-4165, 4169-4171)
             if (t == 0) {
@@ -224,21 +302,23 @@ __device__ void hb_panel(const T* __restrict__ ssub, int na, const uint8_t* __re
             iY = NI;
             iP = NI;   // the carry starts at cY[0] - ge_a = -inf
         }
-        if (r >= 0 && r <= H && ncols > 0) {
-            const int ac = r >= 1 ? (int)codes_a[a_base + (int64_t)a_step * (r - 1)] : 0;
-            const T* __restrict__ srow = ssub + ac * na;
+        // Every lane computes every step, which keeps the loop free of divergent branches: a lane
+        // before its row 0 (r < 0) holds -inf in every state and receives -inf from its left
+        // neighbour (which has not started either), so it stays at -inf; after its row H, or
+        // past the block's last column, what it computes is never stored nor passed on to a
+        // lane that stores.
+        {
             T P = iP;
             uint32_t cw[S / 4];
             if (PMAX && r >= 1) {
-                hb_strip_pmax<T, S>(M, X, Y, iM, iX, P, pM, pX, pY, srow, coff, jl, go_a, ge_a, go_b, ge_b);
+                hb_strip_pmax<T, S>(M, X, Y, iM, iX, P, pM, pX, pY, sv, jl, go_a, ge_a, go_b, ge_b);
             } else {
-                T rowmax;
-                strip_row<T, kAffine, kGlobal, S, HARD>(M, X, Y, iM, iX, iY, pM, pX, pY, srow, coff, ncols, go_a,
-                                                         ge_a, go_b, ge_b, cw, rowmax);
+                hb_strip<T, S, HARD>(M, X, Y, iM, iX, iY, pM, pX, pY, sv, go_a, ge_a, go_b, ge_b, cw);
                 P = NI;
             }
             oM = M[S - 1]; oX = X[S - 1]; oY = Y[S - 1];
             if constexpr (PMAX) oP = P;
+          if (r >= 0 && r <= H && ncols > 0) {
             if constexpr (HARD) {
 #pragma unroll
                 for (int k = 0; k < S / 4; ++k) tbl[((int64_t)t * (S / 4) + k) * kWave + lane] = cw[k];
@@ -270,12 +350,20 @@ __device__ void hb_panel(const T* __restrict__ ssub, int na, const uint8_t* __re
                     }
                 }
             }
+          }
         }
         if (first && g == 0 && t == H) {   // column 0 of the last row
             if (out) { out[0] = iM; out[1] = iX; out[2] = iY; }
             if (fin && W == 0) { fin[0] = iM; fin[1] = iX; fin[2] = iY; }
         }
         pM = iM; pX = iX; pY = iY;
+        {   // the next step's row, r + 1: stored in the ring by now (rows up to t + 1 are)
+            const int rn = r + 1;
+            const int ac = (rn >= 1 && rn <= H) ? (int)acodes[rn & 255] : 0;
+            const T* __restrict__ srow = ssub + ac * na;
+#pragma unroll
+            for (int k = 0; k < S; ++k) sv[k] = srow[coff[k]];
+        }
     }
 }
 
@@ -284,7 +372,8 @@ __device__ void hb_panel(const T* __restrict__ ssub, int na, const uint8_t* __re
 template <typename T, bool PMAX>
 __global__ __launch_bounds__(64) void hb_sweep_kernel(HbSweepArgs<T> A) {
     extern __shared__ __align__(16) unsigned char lds[];
-    T* ssub = reinterpret_cast<T*>(lds);
+    uint8_t* acodes = lds;   // 256 bytes: the ring of A's codes
+    T* ssub = reinterpret_cast<T*>(lds + 256);
     const int lane = threadIdx.x;
     for (int k = lane; k < A.na * A.na; k += kWave) ssub[k] = A.sub[k];
     __syncthreads();
@@ -297,7 +386,7 @@ __global__ __launch_bounds__(64) void hb_sweep_kernel(HbSweepArgs<T> A) {
             ssub, A.na, A.codes_a, A.codes_b, item.a_base, item.a_step, item.b_base, item.b_step, item.H, item.W,
             item.panel, item.flag, A.go_a, A.ge_a, A.go_b, A.ge_b, first ? nullptr : A.bnd + item.bnd_in,
             last ? nullptr : A.bnd + item.bnd_out, A.progress + (first ? it : it - 1), A.progress + it, A.err,
-            A.wait_ticks, A.out + item.out, nullptr, nullptr, lane);
+            A.wait_ticks, A.out + item.out, nullptr, nullptr, acodes, lane);
     }
 }
 
@@ -367,7 +456,8 @@ __device__ __forceinline__ int64_t hb_code_at(int H, int r, int c) {
 template <typename T>
 __global__ __launch_bounds__(64) void hb_base_kernel(HbBaseArgs<T> A) {
     extern __shared__ __align__(16) unsigned char lds[];
-    T* ssub = reinterpret_cast<T*>(lds);
+    uint8_t* acodes = lds;   // 256 bytes: the ring of A's codes
+    T* ssub = reinterpret_cast<T*>(lds + 256);
     T* fin = ssub + A.na * A.na;
     const int lane = threadIdx.x;
     for (int k = lane; k < A.na * A.na; k += kWave) ssub[k] = A.sub[k];
@@ -383,7 +473,8 @@ __global__ __launch_bounds__(64) void hb_base_kernel(HbBaseArgs<T> A) {
         T* bout = pp + 1 < P ? bnd + 3 * (int64_t)pp * (B.H + 1) : nullptr;
         hb_panel<T, kHbS, false, true, false>(ssub, A.na, A.codes_a, A.codes_b, B.a_base, 1, B.b_base, 1, B.H, B.W,
                                               pp, B.in_x, A.go_a, A.ge_a, A.go_b, A.ge_b, bin, bout, nullptr,
-                                              nullptr, nullptr, 0, nullptr, tb + pp * panel_dwords, fin, lane);
+                                              nullptr, nullptr, 0, nullptr, tb + pp * panel_dwords, fin, acodes,
+                                              lane);
         // this wave's right column (and, at the end, its codes) are read back by this wave
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         __builtin_amdgcn_wave_barrier();
@@ -420,35 +511,66 @@ __device__ __forceinline__ int wave_incl_sum(int v, int lane) {
     return v;
 }
 
-// LDS of the replay: the matrix (T), then match counts (uint32), then 64 move records.
+// LDS of the replay: the matrix (T), then match counts (uint32).
 template <typename T>
 __host__ __device__ constexpr int hb_replay_lds_bytes(int na) {
-    return ((na * na * (int)sizeof(T) + 15) & ~15) + ((na * na * 4 + 15) & ~15) + kWave * (int)sizeof(T) + kWave * 4;
+    return ((na * na * (int)sizeof(T) + 15) & ~15) + ((na * na * 4 + 15) & ~15);
+}
+
+// A lane's value, read by the whole wave (q a compile-time lane).
+template <int Q>
+__device__ __forceinline__ float lane_value(float v) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), Q));
+}
+template <int Q>
+__device__ __forceinline__ double lane_value(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), Q);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), Q);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// s = (s + t1[q]) + t2[q] for the wave's lanes q = Q .. 63, in lane order.
+template <typename T, int Q>
+__device__ __forceinline__ T add_in_lane_order(T s, T t1, T t2) {
+    if constexpr (Q == kWave) {
+        return s;
+    } else {
+        s = (s + lane_value<Q>(t1)) + lane_value<Q>(t2);
+        return add_in_lane_order<T, Q + 1>(s, t1, t2);
+    }
+}
+
+// Global memory handed between lanes of one wave.
+__device__ __forceinline__ void wg_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
 }
 
 // The replay of one split pair: its blocks' moves in order, 64 at a time.  Positions come
-// from prefix sums; every lane counts its move; lane 0 adds the moves' terms to the score in
-// path order, in T (aligner.hpp:4530-4553), and run-length encodes the stored path.
+// from prefix sums; every lane counts its move and forms its terms of the score -- a match's
+// substitution value, or a gap's -go (at a run's first move, else 0) and -ge; the wave then
+// adds them in path order, (s + t1) + t2 move by move, which is the reference's own sequence
+// of roundings (aligner.hpp:4530-4553): s never becomes -0 (it starts at +0, and x + -x is
+// +0), so a zero term changes nothing.  A stored path is run-length encoded from the runs'
+// first moves (their type and position), then converted to lengths at the region's end.
 template <typename T>
 __global__ __launch_bounds__(64) void hb_replay_kernel(HbReplayArgs<T> A) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int nn = A.na * A.na;
     T* ssub = reinterpret_cast<T*>(lds);
     uint32_t* hist = reinterpret_cast<uint32_t*>(lds + ((nn * (int)sizeof(T) + 15) & ~15));
-    T* sterm = reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(hist) + ((nn * 4 + 15) & ~15));
-    int32_t* sop = reinterpret_cast<int32_t*>(sterm + kWave);
     const int lane = threadIdx.x;
     for (int k = lane; k < nn; k += kWave) { ssub[k] = A.sub[k]; hist[k] = 0u; }
     __syncthreads();
     const HbPair pr = A.pairs[blockIdx.x];
     const int p = pr.pair;
     int32_t* gj = A.guide ? A.guide + pr.a0 + p : nullptr;
-    uint32_t* ops = A.ops ? A.ops + pr.a0 + pr.b0 : nullptr;   // written forward, moved to the end below
-    int i = 0, j = 0, prev = 3;
+    uint32_t* ops = A.ops ? A.ops + pr.a0 + pr.b0 : nullptr;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+    int i = 0, j = 0, prev = 3, done = 0, nruns = 0;
     int open_a = 0, ext_a = 0, open_b = 0, ext_b = 0;
     T s = T(0);
-    uint32_t rt = 3u, rlen = 0u;
-    int nruns = 0;
     for (int bk = 0; bk < pr.count; ++bk) {
         const HbBase B = A.blocks[pr.first + bk];
         const int cnt = A.hcount[pr.first + bk];
@@ -461,49 +583,39 @@ __global__ __launch_bounds__(64) void hb_replay_kernel(HbReplayArgs<T> A) {
             const int jj = j + wave_incl_sum(valid && op != 1 ? 1 : 0, lane);
             int pop = __shfl_up(op, 1, kWave);
             if (lane == 0) pop = prev;
-            const bool opn = op != pop;
-            T term = T(0);
+            const bool opn = valid && op != pop;
+            T t1 = T(0), t2 = T(0);
             if (valid) {
                 if (op == 0) {
                     const int ca = A.codes_a[pr.a0 + ii - 1], cbv = A.codes_b[pr.b0 + jj - 1];
-                    term = ssub[ca * A.na + cbv];
+                    t1 = ssub[ca * A.na + cbv];
                     atomicAdd(&hist[ca * A.na + cbv], 1u);
                     if (gj) gj[ii] = jj;
                 } else if (op == 1) {
                     ++ext_b;
                     open_b += opn;
+                    t1 = opn ? -A.go_b : T(0);
+                    t2 = -A.ge_b;
                     if (gj) gj[ii] = jj;
                 } else {
                     ++ext_a;
                     open_a += opn;
+                    t1 = opn ? -A.go_a : T(0);
+                    t2 = -A.ge_a;
                 }
             }
-            sterm[lane] = term;
-            sop[lane] = op | (opn ? 4 : 0);
-            wave_sync();
-            if (lane == 0) {
-                const int lim = cnt - base < kWave ? cnt - base : kWave;
-                for (int q = 0; q < lim; ++q) {
-                    const int o = sop[q], t = o & 3;
-                    if (t == 0) s += sterm[q];
-                    else if (t == 1) { if (o & 4) s -= A.go_b; s -= A.ge_b; }
-                    else { if (o & 4) s -= A.go_a; s -= A.ge_a; }
-                    if (ops) {
-                        if ((uint32_t)t == rt) ++rlen;
-                        else {
-                            if (rlen) ops[nruns++] = (rt << 30) | rlen;
-                            rt = (uint32_t)t;
-                            rlen = 1u;
-                        }
-                    }
-                }
+            s = add_in_lane_order<T, 0>(s, t1, t2);
+            if (ops) {
+                const uint64_t starts = __ballot(opn);
+                if (opn) ops[nruns + __popcll(starts & below)] = ((uint32_t)op << 30) | (uint32_t)(done + k);
+                nruns += __popcll(starts);
             }
-            wave_sync();
             const int lastl = (cnt - base < kWave ? cnt - base : kWave) - 1;
             i = __shfl(ii, kWave - 1, kWave);
             j = __shfl(jj, kWave - 1, kWave);
             prev = __shfl(op, lastl, kWave);
         }
+        done += cnt;
     }
     // counts: matches from LDS, gaps summed over the wave (aligner.hpp:5191-5207)
     for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -523,24 +635,28 @@ __global__ __launch_bounds__(64) void hb_replay_kernel(HbReplayArgs<T> A) {
         A.scores[p] = (double)s;
         if (gj) { gj[0] = 0; gj[pr.m] = pr.n; }
         if (ops) {
-            if (rlen) ops[nruns++] = (rt << 30) | rlen;
             A.ops_n[p] = nruns;
             A.ends[2 * p] = pr.m;
             A.ends[2 * p + 1] = pr.n;
         }
     }
     if (ops) {
-        // the stored path ends at the region's end (the batch's layout): move the runs up
+        // run r starts at move pos[r] (low 30 bits) and ends where run r + 1 starts (the last
+        // at the path's end); its op (type << 30 | length) goes to the region's end, which is
+        // at or above its record, so chunks are converted from the top down, each chunk's
+        // records read before it is written, the next chunk's first position carried down
+        wg_sync();
         const int cap = pr.m + pr.n;
-        const int nr = __shfl(nruns, 0, kWave);
-        wave_sync();
-        for (int hi = nr; hi > 0; hi -= kWave) {
-            const int k = hi - kWave + lane;
-            uint32_t v = 0u;
-            if (k >= 0) v = ops[k];
-            wave_sync();
-            if (k >= 0) ops[cap - nr + k] = v;
-            wave_sync();
+        uint32_t carry = (uint32_t)done;
+        for (int hi = nruns; hi > 0; hi -= kWave) {
+            const int r = hi - kWave + lane;
+            const uint32_t v = r >= 0 ? ops[r] : 0u;
+            uint32_t nx = (uint32_t)__shfl_down((int)v, 1, kWave);
+            if (lane == kWave - 1) nx = carry;
+            carry = (uint32_t)__shfl((int)v, 0, kWave);
+            wg_sync();
+            if (r >= 0) ops[cap - nruns + r] = (v & 0xc0000000u) | ((nx & 0x3fffffffu) - (v & 0x3fffffffu));
+            wg_sync();
         }
     }
 }

@@ -131,7 +131,7 @@ int run_t(Work* w, const Job& J, std::string& err) {
     int dev = 0, n_cu = 256;
     HB_HIP(hipGetDevice(&dev));
     HB_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-    const size_t lds_sub = (size_t)J.na * J.na * sizeof(T);
+    const size_t lds_sub = 256 + (size_t)J.na * J.na * sizeof(T);   // the A-code ring, then the matrix
     int per_cu = 0;
     HB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, nwg::hb_sweep_kernel<T, PMAX>, 64, lds_sub));
     const int64_t resident = (int64_t)std::max(per_cu, 1) * n_cu;
