@@ -36,6 +36,15 @@ __device__ __forceinline__ double ninf<double>() { return -__builtin_huge_val();
 __device__ __forceinline__ float dmax(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ __forceinline__ double dmax(double a, double b) { return __builtin_fmax(a, b); }
 
+// The same maximum of two finite-or-(-inf) float32 values as one median, med3(a, b, FLT_MAX)
+// (no value here exceeds the largest finite float): fmax's operands must be NaN-quieted first
+// wherever the compiler cannot prove them results of arithmetic (values carried around a
+// loop, lane shifts), one more instruction each; the median takes them as they are.  (With
+// +inf as the third operand the compiler folds the median back into fmax.)  float64 has no
+// median instruction and keeps fmax.
+__device__ __forceinline__ float vmax(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, 3.40282347e+38f); }
+__device__ __forceinline__ double vmax(double a, double b) { return dmax(a, b); }
+
 // One-lane wave shifts on the DPP path (a VALU move, no LDS round trip): up = from the lane
 // below (wave_shr:1), down = from the lane above (wave_shl:1).  Lane 0 (up) and lane 63
 // (down) receive 0; every caller replaces or ignores that value.
@@ -62,6 +71,18 @@ __device__ __forceinline__ double wshift_down(double v) {
 __device__ __forceinline__ float shift_up(float v) { return wshift_up(v); }
 __device__ __forceinline__ double shift_up(double v) { return wshift_up(v); }
 __device__ __forceinline__ int shift_up(int v) { return wshift_up(v); }
+
+// A lane's value for the whole wave, the lane given by a wave-uniform index (a lane read into
+// a scalar register, not a permute through LDS).
+__device__ __forceinline__ float bcast(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double bcast(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
 
 // The value of the wave's first active lane as a wave-uniform (scalar) value: branches on it
 // are uniform to the compiler.

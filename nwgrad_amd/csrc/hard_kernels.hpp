@@ -148,18 +148,18 @@ __device__ __forceinline__ void strip_row(T (&M)[S], T (&X)[S], T (&Y)[S], T lM,
         const T s = srow[coff[k]];
         if constexpr (GM == kAffine) {
             const T uM = M[k], uX = X[k], uY = Y[k];
-            const T tXY = dmax(dX, dY);
-            T mv = dmax(dM, tXY) + s;
+            const T tXY = vmax(dX, dY);
+            T mv = vmax(dM, tXY) + s;
             uint32_t km = 0;
             if constexpr (HARD) km = (dM >= tXY) ? 0u : ((dX >= dY) ? 1u : 2u);
             const T ux = (uM - go_b) - ge_b, vx = uX - ge_b, wx = (uY - go_b) - ge_b;
-            const T tx = dmax(vx, wx);
-            const T xv = dmax(ux, tx);
+            const T tx = vmax(vx, wx);
+            const T xv = vmax(ux, tx);
             const T uy = (lM - go_a) - ge_a, vy = (lX - go_a) - ge_a, wy = lY - ge_a;
-            const T ty = dmax(vy, wy);
-            const T yv = dmax(uy, ty);
+            const T ty = vmax(vy, wy);
+            const T yv = vmax(uy, ty);
             if constexpr (AM == kLocal) {
-                mv = dmax(mv, T(0));
+                mv = vmax(mv, T(0));
                 if constexpr (HARD) km = (mv <= T(0)) ? 3u : km;
             }
             T mo = mv, xo = xv, yo = yv;

@@ -214,17 +214,6 @@ __device__ __forceinline__ void hb_strip_pmax(T (&M)[S], T (&X)[S], T (&Y)[S], T
     }
 }
 
-// A lane's value for the whole wave, the lane given by a wave-uniform index.
-__device__ __forceinline__ float bcast(float v, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-__device__ __forceinline__ double bcast(double v, int l) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
 // Fill of one panel of a block, rows 0..H: the sweep's (OUT: the last row into `out`; WAIT:
 // the left column from another wave, published by `prog_in`) and the base case's (HARD: codes
 // into `tbl`; the left column written earlier by this same wave).  bin / bout hold NB values
