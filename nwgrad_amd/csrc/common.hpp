@@ -84,6 +84,20 @@ __device__ __forceinline__ double bcast(double v, int l) {
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+// Loaded values made ordinary register values: waits here for the loads that produce them (the
+// empty asm reads and writes each register), so that later uses -- in later trips of a loop,
+// where the compiler can no longer count the loads in flight and waits for every outstanding
+// vector memory operation, stores included -- need no wait at all.
+template <typename V>
+__device__ __forceinline__ void settle(V& v) {
+    asm volatile("" : "+v"(v));
+}
+template <typename V, typename... Vs>
+__device__ __forceinline__ void settle(V& v, Vs&... vs) {
+    settle(v);
+    settle(vs...);
+}
+
 // The value of the wave's first active lane as a wave-uniform (scalar) value: branches on it
 // are uniform to the compiler.
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }

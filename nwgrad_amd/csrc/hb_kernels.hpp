@@ -38,8 +38,13 @@
 
 namespace nwg {
 
-constexpr int kHbS = 8;                    // columns per lane of the sweeps and base fills
-constexpr int kHbPanel = kHbS * kWave;     // columns per panel
+// Columns per lane: the sweeps 16 (1,024-column panels: half the panels of 8, so half the
+// pipeline's hand-offs along a level-0 sweep, and about one wave a SIMD, each step's work
+// twice as long); the base cases 8 (their codes and walk; 16 would spill in float64).
+constexpr int kHbSweepS = 16;
+constexpr int kHbSweepPanel = kHbSweepS * kWave;   // columns per sweep panel
+constexpr int kHbS = 8;
+constexpr int kHbPanel = kHbS * kWave;             // columns per base-case panel
 constexpr int kHbRB = 64;                  // rows a panel publishes at a time
 
 // One panel of one half sweep.
@@ -260,6 +265,10 @@ __device__ void hb_panel(const T* __restrict__ ssub, int na, const uint8_t* __re
             anext = arow(t + kWave + 1 + lane);
             wave_sync();
         }
+        // (the block's loads are waited for here, once, by settle(): a value still pending from
+        // a load issued in an earlier trip of the loop makes the compiler wait for every vector
+        // memory operation outstanding at each of its uses -- this wave's boundary and code
+        // stores of the step before included, a store's full round trip every step)
         if (!first && t < rows && t % kHbRB == 0) {
             const int need = t + kHbRB < rows ? t + kHbRB : rows;
             if constexpr (WAIT) (void)bounded_wait(prog_in, need, err, ticks, lane, 1);
@@ -270,6 +279,7 @@ __device__ void hb_panel(const T* __restrict__ ssub, int na, const uint8_t* __re
                 bY = bin[(int64_t)NB * rr + 2];
                 if constexpr (PMAX) bP = bin[(int64_t)NB * rr + 3];
             }
+            settle(bM, bX, bY, bP);
         }
         T iM = shift_up(oM), iX = shift_up(oX), iY = shift_up(oY);
         T iP = PMAX ? shift_up(oP) : NI;
@@ -371,7 +381,7 @@ __global__ __launch_bounds__(64) void hb_sweep_kernel(HbSweepArgs<T> A) {
         if (it >= A.n_items) break;
         const HbItem item = A.items[it];
         const bool first = item.bnd_in < 0, last = item.bnd_out < 0;
-        hb_panel<T, kHbS, PMAX, false, true>(
+        hb_panel<T, kHbSweepS, PMAX, false, true>(
             ssub, A.na, A.codes_a, A.codes_b, item.a_base, item.a_step, item.b_base, item.b_step, item.H, item.W,
             item.panel, item.flag, A.go_a, A.ge_a, A.go_b, A.ge_b, first ? nullptr : A.bnd + item.bnd_in,
             last ? nullptr : A.bnd + item.bnd_out, A.progress + (first ? it : it - 1), A.progress + it, A.err,

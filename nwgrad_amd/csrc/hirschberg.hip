@@ -160,7 +160,7 @@ int run_t(Work* w, const Job& J, std::string& err) {
                 const int64_t outoff = out_elems;
                 out_elems += 3 * ((int64_t)W + 1);
                 (half ? jn.rev : jn.fwd) = outoff;
-                const int P = (int)std::max<int64_t>(1, ceil_div(W, nwg::kHbPanel));
+                const int P = (int)std::max<int64_t>(1, ceil_div(W, nwg::kHbSweepPanel));
                 for (int pp = 0; pp < P; ++pp) {
                     nwg::HbItem it{};
                     it.node = (int32_t)k;

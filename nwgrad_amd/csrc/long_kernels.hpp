@@ -93,7 +93,8 @@ struct LongArgs {
 };
 
 template <typename T, int GM, int AM, int S, bool HARD, int RB, bool TR, bool BAND>
-__device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int it, int lane) {
+__device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, uint8_t* __restrict__ acodes, int it,
+                           int lane) {
     static_assert(RB >= 1 && RB <= kWave, "a row block fits the lanes of a wave");
     const PanelItem item = A.items[it];
     const int p = item.pair, na = A.na;
@@ -170,8 +171,20 @@ __device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int
     T bM = NI, bX = NI, bY = NI;   // lane r: the left boundary of row t0 + 1 + r of the current block
 
     const int steps = m + kWave - 1;
+    // The codes of A pass through a 256-row ring in LDS: at step t0 = 64 k the wave stores rows
+    // t0 + 1 .. t0 + 64, loaded one block earlier, and loads the next block; lane g reads row
+    // t - g + 1 from LDS.  A load in the step itself would make the step wait for every vector
+    // memory operation in flight (the loop's trips cannot be counted apart), the previous
+    // step's code stores included.
+    auto arow = [&](int r) -> int { return (r >= 1 && r <= m) ? (int)ap[r - 1] : 0; };
+    int anext = arow(lane + 1);
     for (int t = 0; t < steps; ++t) {
         const int i = t - g + 1;
+        if (t % kWave == 0) {
+            acodes[(t + 1 + lane) & 255] = (uint8_t)anext;
+            anext = arow(t + kWave + 1 + lane);
+            wave_sync();
+        }
         int rs = 0;   // tracked: the row's set (matrix, gaps in B)
         if constexpr (TR) {
             if (i >= 1 && i <= m) {
@@ -206,16 +219,17 @@ __device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int
                 bM = bin[3 * (int64_t)r];
                 if constexpr (GM == kAffine) { bX = bin[3 * (int64_t)r + 1]; bY = bin[3 * (int64_t)r + 2]; }
             }
+            settle(bM, bX, bY);   // waited for here, once a block, not at every step's use
         }
         T iM = shift_up(oM);
         T iX, iY;
         if constexpr (GM == kAffine) { iX = shift_up(oX); iY = shift_up(oY); }
         else { iX = NI; iY = NI; }
         if (!first) {
-            const int sl = t % RB;
-            const T lM = __shfl(bM, sl, kWave);
+            const int sl = t % RB;   // wave-uniform: a lane read, not a permute
+            const T lM = bcast(bM, sl);
             T lX = NI, lY = NI;
-            if constexpr (GM == kAffine) { lX = __shfl(bX, sl, kWave); lY = __shfl(bY, sl, kWave); }
+            if constexpr (GM == kAffine) { lX = bcast(bX, sl); lY = bcast(bY, sl); }
             if (g == 0) { iM = lM; iX = lX; iY = lY; }
         } else if (g == 0) {   // column 0 of row i (aligner.hpp:4009-4018, 3823-3826)
             T border = NI;
@@ -235,7 +249,7 @@ __device__ void long_panel(const LongArgs<T>& A, const T* __restrict__ ssub, int
         }
         const bool active = i >= 1 && i <= m && ncols > 0;
         if (active) {
-            const T* __restrict__ srow = ssub + ((TR ? rs * na : 0) + (int)ap[i - 1]) * na;
+            const T* __restrict__ srow = ssub + ((TR ? rs * na : 0) + (int)acodes[i & 255]) * na;
             uint32_t cw[S / 4];
             T rowmax;
             int klo = 0, khi = S;
@@ -283,6 +297,7 @@ __global__ __launch_bounds__(64) void long_fill_kernel(LongArgs<T> A) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int nn = A.na * A.na, K = TR ? A.tr.K : 1;
     T* ssub = reinterpret_cast<T*>(lds);
+    uint8_t* acodes = lds + hard_lds_sub_bytes<T>(A.na, K, TR);   // 256 bytes: the ring of A's codes
     const int lane = threadIdx.x;
     for (int k = lane; k < K * nn; k += kWave) ssub[k] = A.sub[k];
     if constexpr (TR)
@@ -291,7 +306,7 @@ __global__ __launch_bounds__(64) void long_fill_kernel(LongArgs<T> A) {
     for (;;) {
         const int it = wave_fetch(A.queue);
         if (it >= A.n_items) break;
-        long_panel<T, GM, AM, S, HARD, RB, TR, BAND>(A, ssub, it, lane);
+        long_panel<T, GM, AM, S, HARD, RB, TR, BAND>(A, ssub, acodes, it, lane);
     }
 }
 

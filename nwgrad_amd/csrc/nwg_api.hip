@@ -1392,7 +1392,7 @@ int launch_long(nwg_batch* b) {
     NWG_PICK(I0, I0) NWG_PICK(I0, I1) NWG_PICK(I0, I2) NWG_PICK(I1, I0) NWG_PICK(I1, I1) NWG_PICK(I1, I2)
 #undef NWG_PICK
     if (!fill) return fail(ctx, NWG_EINVAL, "nwgrad_hip: unsupported problem type");
-    const size_t lds_fill = (size_t)nwg::hard_lds_sub_bytes<T>(b->na, b->K, tr);
+    const size_t lds_fill = (size_t)nwg::hard_lds_sub_bytes<T>(b->na, b->K, tr) + 256;   // + the A-code ring
     const size_t lds_walk = hard ? (size_t)nwg::long_walk_lds_bytes(b->na, b->K, tr, kLS)
                                  : 16;
     if (lds_fill > 65536 || lds_walk > 65536)
