@@ -200,12 +200,28 @@ struct WalkOut {
     __device__ __forceinline__ void row0(int j) { op(2u, (uint32_t)j); }
     // After the walk: Local's interpolated rows (i0 the row the walk stopped in); the number of
     // ops written, counted from ops_end.
+    // Each interpolated entry is the exact floor of base + span * k / rows, stepped as
+    // q = span / rows and a remainder carried against rows: two divisions a run, none a row.
     __device__ __forceinline__ void end_local(int i0, int m, int n, int bi, int bj) {
         if (!gj) return;
-        const int64_t g0 = 0;
-        const int64_t gi0 = (low == i0) ? lowv : (i0 == bi ? bj : 0);
-        for (int k = 1; k < i0; ++k) gj[k] = (int32_t)(g0 + (gi0 - g0) * k / i0);
-        for (int k = bi + 1; k < m; ++k) gj[k] = (int32_t)(bj + (int64_t)(n - bj) * (k - bi) / (m - bi));
+        const int gi0 = (low == i0) ? lowv : (i0 == bi ? bj : 0);
+        ramp(1, i0, 0, gi0, i0);                 // rows 1 .. i0-1: gi0 * k / i0
+        ramp(bi + 1, m, bj, n - bj, m - bi);     // rows bi+1 .. m-1: bj + (n - bj)(k - bi) / (m - bi)
+    }
+    // gj[k] = base + floor(span * (k - k0 + 1) / rows) for k in [k0, k1); span >= 0, rows >= 1
+    __device__ __forceinline__ void ramp(int k0, int k1, int base, int span, int rows) {
+        if (k0 >= k1) return;
+        const int q = span / rows, r = span - q * rows;
+        int v = base, e = 0;
+        for (int k = k0; k < k1; ++k) {
+            v += q;
+            e += r;
+            if (e >= rows) {
+                e -= rows;
+                ++v;
+            }
+            gj[k] = v;
+        }
     }
     __device__ __forceinline__ int32_t finish(const uint32_t* ops_end) {
         if (!w) return 0;

@@ -462,11 +462,20 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
                                              ((A.hist_slots * knn * 2 + 15) & ~15));
     int* gcnt = gall + q * 4 * K;
     if (g == 0 && p >= 0) {
-        A.scores[p] = (double)c.v;
         if constexpr (HARD) {
             uint16_t* h = hist + q * knn;
             int i = c.i, j = c.j, st = c.st;
             const uint32_t* tr = tb + q * G;
+            // The pointer-code word of cell (i, j), i, j >= 1.  Each step loads the next step's
+            // word before it stores this step's guide entry and path op (and the first before
+            // the score and the guide's fixed entries): a load issued after a store would wait
+            // for the store too (one counter covers both).
+            auto word_at = [&](int ci, int cj) -> uint32_t {
+                const int cg = (cj - 1) / S, ck = (cj - 1) - cg * S;
+                return tr[((int64_t)(ci - 1 + cg) * (S / 4) + (ck >> 2)) * kWave + cg];
+            };
+            uint32_t wnext = (i > 0 && j > 0) ? word_at(i, j) : 0u;
+            A.scores[p] = (double)c.v;
             uint32_t* const oend = A.ops ? A.ops + A.off_a[p + 1] + A.off_b[p + 1] : nullptr;
             WalkOut wo;
             wo.begin(A.guide ? A.guide + A.off_a[p] + p : nullptr, oend, m, n, c.i, c.j);
@@ -504,42 +513,36 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
                     i = 0;
                     break;
                 }
-                const int jj = j - 1;
-                const int go = jj / S, k = jj - go * S;
-                const uint32_t w = tr[((int64_t)(i - 1 + go) * (S / 4) + (k >> 2)) * kWave + go];
-                const uint32_t cc = (w >> (8 * (k & 3))) & 0xffu;
-                const int mrow = ((TR ? (int)tk[i - 1] * na : 0) + (int)ap[i - 1]) * na + (int)bp[j - 1];
+                const int k = (j - 1) % S;
+                const uint32_t cc = (wnext >> (8 * (k & 3))) & 0xffu;
+                // the move out of (i, j): 0 diagonal, 1 up (a gap in B), 2 left (a gap in A),
+                // and the state it enters (affine)
+                int mv, nst = 0;
                 if constexpr (GM == kAffine) {
-                    if (st == 0) {
-                        if (AM == kLocal && (cc & 3u) == 3u) break;
-                        const uint32_t kk = cc & 3u;
-                        h[mrow] += 1;
-                        wo.move(0, i, j);
-                        st = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
-                        --i; --j;
-                    } else if (st == 1) {
-                        const uint32_t kk = (cc >> 2) & 3u;
-                        const int pv = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
-                        gap(3, i, 1);
-                        if (pv != 1) gap(2, i, 1);
-                        wo.move(1, i, j);
-                        st = pv;
-                        --i;
-                    } else {
-                        const uint32_t kk = cc >> 4;
-                        const int pv = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
-                        gap(1, i, 1);
-                        if (pv != 2) gap(0, i, 1);
-                        wo.move(2, i, j);
-                        st = pv;
-                        --j;
-                    }
+                    const uint32_t kk = st == 0 ? (cc & 3u) : (st == 1 ? ((cc >> 2) & 3u) : (cc >> 4));
+                    if (AM == kLocal && st == 0 && kk == 3u) break;
+                    mv = st;
+                    nst = kk == 1u ? 1 : (kk == 2u ? 2 : 0);
                 } else {
                     if (cc == 3u) break;
-                    if (cc == 0u) { h[mrow] += 1; wo.move(0, i, j); --i; --j; }
-                    else if (cc == 1u) { gap(3, i, 1); wo.move(1, i, j); --i; }
-                    else { gap(1, i, 1); wo.move(2, i, j); --j; }
+                    mv = cc == 0u ? 0 : (cc == 1u ? 1 : 2);
                 }
+                const int ni = mv == 2 ? i : i - 1, nj = mv == 1 ? j : j - 1;
+                if (ni > 0 && nj > 0) wnext = word_at(ni, nj);
+                if (mv == 0) {
+                    const int mrow = ((TR ? (int)tk[i - 1] * na : 0) + (int)ap[i - 1]) * na + (int)bp[j - 1];
+                    h[mrow] += 1;
+                } else if (mv == 1) {
+                    gap(3, i, 1);
+                    if (GM == kAffine && nst != 1) gap(2, i, 1);
+                } else {
+                    gap(1, i, 1);
+                    if (GM == kAffine && nst != 2) gap(0, i, 1);
+                }
+                wo.move(mv, i, j);
+                if constexpr (GM == kAffine) st = nst;
+                i = ni;
+                j = nj;
             }
             if constexpr (AM == kLocal) wo.end_local(i, m, n, c.i, c.j);
             if (oend) {
@@ -547,6 +550,8 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
                 A.ends[2 * p] = c.i;
                 A.ends[2 * p + 1] = c.j;
             }
+        } else {
+            A.scores[p] = (double)c.v;
         }
     }
     if constexpr (HARD) {
