@@ -41,7 +41,10 @@ namespace {
 
 constexpr int kS = 16;                 // columns per lane: hard and score-only fills
 constexpr int kSN = 8;                 // ... narrow: batches with too few tiles at kS to fill the SIMDs
-constexpr int kLS = 8;                 // columns per lane: long pairs (column panels of kLS * 64)
+// Columns per lane of long pairs (column panels of 64 lanes): 8 in float32 (twice the waves of
+// 16, at 103 VGPRs), 16 in float64 (16 pairs of 36,000^2: 270 against 225 GCUPS at 8, half
+// the panel hand-offs at the same one to two waves a SIMD).
+constexpr int long_cols(bool f64) { return f64 ? 16 : 8; }
 constexpr int kSS = 8;                 // columns per lane: soft pass (float32 and double)
 constexpr int kMaxG = 64;              // lanes per pair at most: one wavefront
 constexpr int kMaxN = kS * kMaxG;      // longest B a lane group holds (hard / score-only); wider: column panels
@@ -430,8 +433,9 @@ int build_plan(nwg_batch* b, const std::vector<int32_t>& pairs, Plan& pl, int S,
 int build_long_plan(nwg_batch* b, std::vector<int32_t> pairs, LongPlan& L) {
     nwg_ctx* ctx = b->ctx;
     const bool hard = b->prob.grad_mode == 1;
-    constexpr int PW = kLS * kMaxG;
-    constexpr int64_t SD = (kLS / 4) * kMaxG;   // table dwords per step
+    const int LS = long_cols(b->prob.precision == 1);
+    const int PW = LS * kMaxG;
+    const int64_t SD = (LS / 4) * kMaxG;   // table dwords per step
     L.items.clear();
     L.lpairs.clear();
     L.groups.clear();
@@ -1366,6 +1370,7 @@ int launch_long(nwg_batch* b) {
     using FillK = void (*)(nwg::LongArgs<T>);
     FillK fill = nullptr, walk = nullptr;
     const bool band = b->banded;
+    constexpr int kLS = long_cols(sizeof(T) == 8);
     auto pick = [&](auto gmc, auto amc, auto hardc) {
         constexpr int G = decltype(gmc)::value, AM = decltype(amc)::value;
         constexpr bool H = decltype(hardc)::value;
