@@ -19,8 +19,9 @@
 //
 // Hard mode stores one byte of predecessor codes per cell (affine M | X<<2 | Y<<4,
 // linear 0 diag / 1 up / 2 left / 3 stop, the reference's encoding) into a per-wave
-// slot of HBM scratch, four cells to a dword, laid out so the 64 lanes of a step write
-// 64 consecutive dwords.  After the fill, lane 0 of each slot walks its pair's codes
+// slot of HBM scratch, four cells to a dword (code_word gives the layout: at 8 columns a lane
+// a lane's two dwords of a step are contiguous, one vector store a lane; at 16 the 64 lanes'
+// k-th dwords of a step are, four stores a lane -- each the faster of the two at its width).  After the fill, lane 0 of each slot walks its pair's codes
 // back from the end cell, counting matches into a per-slot LDS histogram and gap moves
 // in registers, and the wave writes the per-pair gradient rows.
 #pragma once
@@ -277,6 +278,15 @@ __device__ __forceinline__ Cand<T> end_candidate(const T (&M)[S], const T (&X)[S
     return c;
 }
 
+// The dword of a wave's pointer-code slot that holds step t's codes k*4 .. k*4+3 of lane `lane`.
+// Measured: lane-major at 8 columns a lane (protein_global_hard 300 -> 305 GCUPS, dwordx2
+// stores) and step-major at 16 (dna_local_short 835 against 812 lane-major).
+template <int S>
+__device__ __forceinline__ int64_t code_word(int t, int lane, int k) {
+    if constexpr (S == 8) return ((int64_t)t * kWave + lane) * (S / 4) + k;
+    else return ((int64_t)t * (S / 4) + k) * kWave + lane;
+}
+
 // The column-0 cell of row i >= 1 (Global / Glocal; aligner.hpp:4009-4018, 3823-3826): the
 // closed form untracked, the row's precomputed border tracked.
 template <typename T, int GM, bool TR>
@@ -431,8 +441,7 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
             if constexpr (GM == kAffine) { oX = X[S - 1]; oY = Y[S - 1]; }
             if constexpr (HARD) {
 #pragma unroll
-                for (int k = 0; k < S / 4; ++k)
-                    tb[((int64_t)t * (S / 4) + k) * kWave + lane] = cw[k];
+                for (int k = 0; k < S / 4; ++k) tb[code_word<S>(t, lane, k)] = cw[k];
             }
             if constexpr (AM == kLocal) local_update<T, GM, S>(best, M, X, Y, rowmax, i, j0, ncols);
         }
@@ -465,14 +474,14 @@ __device__ void hard_tile(const HardArgs<T>& A, const T* __restrict__ ssub, uint
         if constexpr (HARD) {
             uint16_t* h = hist + q * knn;
             int i = c.i, j = c.j, st = c.st;
-            const uint32_t* tr = tb + q * G;
+            const uint32_t* tr = tb;
             // The pointer-code word of cell (i, j), i, j >= 1.  Each step loads the next step's
             // word before it stores this step's guide entry and path op (and the first before
             // the score and the guide's fixed entries): a load issued after a store would wait
             // for the store too (one counter covers both).
             auto word_at = [&](int ci, int cj) -> uint32_t {
                 const int cg = (cj - 1) / S, ck = (cj - 1) - cg * S;
-                return tr[((int64_t)(ci - 1 + cg) * (S / 4) + (ck >> 2)) * kWave + cg];
+                return tr[code_word<S>(ci - 1 + cg, q * G + cg, ck >> 2)];
             };
             uint32_t wnext = (i > 0 && j > 0) ? word_at(i, j) : 0u;
             A.scores[p] = (double)c.v;
